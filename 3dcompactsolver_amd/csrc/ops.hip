@@ -1,0 +1,222 @@
+// ops.hip -- plan upload and the standalone compact operator kernel
+// (one field in, one field out, any direction, any layout).  This is the
+// GPU body of Derivatives::calc1stDerivField / calc2ndDerivField
+// (Derivatives.cpp:3-96) and Filter::filterField (Filter.cpp:129-171).
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "plan_dev.hpp"
+
+namespace cs3d {
+
+std::string hip_msg(hipError_t e, const char *what)
+{
+    return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+static void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(hip_msg(e, what));
+}
+
+template <typename real>
+void PlanDev<real>::release()
+{
+    if (dev) (void)hipFree(dev);
+    if (tables) (void)hipFree(tables);
+    dev = nullptr;
+    tables = nullptr;
+}
+
+OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32)
+{
+    // cross-segment terms below this weight are dropped (far below the
+    // rounding of the working precision)
+    const double tol = fp32 ? 1e-10 : 1e-20;
+    int M = kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
+    if (!periodic || n % M != 0) M = SEG_TABLE;
+    return build_plan(kind, periodic, n, dh, alpha_f, M, tol);
+}
+
+template <typename real>
+void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f)
+{
+    pl.release();
+    OpHost h = plan_for(kind, periodic, n, dh, alpha_f, sizeof(real) == 4);
+    pl.host = h;
+    OpDev<real> d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = h.n;
+    d.M = h.M;
+    d.P = h.P;
+    d.uniform = h.uniform;
+    d.cyclic = h.cyclic;
+    d.kind = h.kind;
+    d.H = h.H;
+    d.NB = h.NB;
+    d.K = h.K;
+    d.scale = (real)h.scale;
+    for (int i = 0; i < 5; i++) d.tap[i] = (real)h.tap[i];
+    for (int r = 0; r < 4; r++)
+        for (int u = 0; u < 7; u++) {
+            d.bcl[r][u] = (real)h.bcl[r][u];
+            d.bcr[r][u] = (real)h.bcr[r][u];
+        }
+    d.um = (real)h.um;
+    d.umu = (real)h.umu;
+    d.uiw = (real)h.uiw;
+    if (h.uniform) {
+        for (int t = 0; t < h.M; t++) {
+            d.upf[t] = (real)h.pf[t];
+            d.upb[t] = (real)h.pb[t];
+        }
+        for (int k = 0; k < h.K; k++) {
+            d.uphi[k] = (real)h.phi[k];
+            d.upsi[k] = (real)h.psi[k];
+        }
+    }
+    // tables: m iw mu pf pb (n each), phi psi (P*K each), sstart slen (P)
+    size_t nreal = 5 * (size_t)h.n + 2 * (size_t)h.P * h.K;
+    size_t bytes = nreal * sizeof(real) + 2 * (size_t)h.P * sizeof(int);
+    std::vector<char> buf(bytes);
+    real *rp = (real *)buf.data();
+    auto put = [&](const std::vector<double> &v) {
+        for (double x : v) *rp++ = (real)x;
+    };
+    put(h.m);
+    put(h.iw);
+    put(h.mu);
+    put(h.pf);
+    put(h.pb);
+    put(h.phi);
+    put(h.psi);
+    int *ip = (int *)rp;
+    for (int g = 0; g < h.P; g++) ip[g] = h.sstart[g];
+    for (int g = 0; g < h.P; g++) ip[h.P + g] = h.slen[g];
+    hip_check(hipMalloc(&pl.tables, bytes), "hipMalloc(plan tables)");
+    hip_check(hipMemcpy(pl.tables, buf.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(plan)");
+    real *dp = (real *)pl.tables;
+    d.m = dp;
+    d.iw = dp + h.n;
+    d.mu = dp + 2 * h.n;
+    d.pf = dp + 3 * h.n;
+    d.pb = dp + 4 * h.n;
+    d.phi = dp + 5 * h.n;
+    d.psi = d.phi + (size_t)h.P * h.K;
+    d.sstart = (const int *)(d.psi + (size_t)h.P * h.K);
+    d.slen = d.sstart + h.P;
+    pl.hd = d;
+    hip_check(hipMalloc(&pl.dev, sizeof(OpDev<real>)), "hipMalloc(plan)");
+    hip_check(hipMemcpy(pl.dev, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy(plan)");
+    // one workgroup: up to 1024 lanes, at most 64 lines
+    int LX = 1024 / h.P;
+    if (LX > 64) LX = 64;
+    if (LX < 1) throw std::invalid_argument("too many segments per line");
+    pl.LX = LX;
+    pl.threads = LX * h.P;
+}
+
+LineMap xyz_lines(int dir, int Nx, int Ny, int Nz)
+{
+    LineMap m;
+    long sxy = (long)Nx * Ny;
+    if (dir == 0) {  // lines (j, k): L = j + Ny k
+        m.n1 = Ny; m.s1 = Nx; m.s2 = sxy; m.sp = 1; m.nlines = Ny * Nz;
+    } else if (dir == 1) {  // lines (i, k): L = i + Nx k
+        m.n1 = Nx; m.s1 = 1; m.s2 = sxy; m.sp = Nx; m.nlines = Nx * Nz;
+    } else {  // lines (i, j): L = i + Nx j
+        m.n1 = Nx; m.s1 = 1; m.s2 = Nx; m.sp = sxy; m.nlines = Nx * Ny;
+    }
+    return m;
+}
+
+LineMap ref_lines(int dir, int Nx, int Ny, int Nz)
+{
+    if (dir == 0) return xyz_lines(0, Nx, Ny, Nz);
+    LineMap m;
+    if (dir == 1) {  // YZX: i*Nz*Ny + k*Ny + j; L = k + Nz i
+        m.n1 = Nz; m.s1 = Ny; m.s2 = (long)Nz * Ny; m.sp = 1; m.nlines = Nx * Nz;
+    } else {  // ZXY: j*Nx*Nz + i*Nz + k; L = i + Nx j
+        m.n1 = Nx; m.s1 = Nz; m.s2 = (long)Nx * Nz; m.sp = 1; m.nlines = Nx * Ny;
+    }
+    return m;
+}
+
+template <int KIND, typename real, int M, bool UNI, int MAP>
+__global__ __launch_bounds__(1024) void k_field_op(const OpDev<real> *__restrict__ op,
+                                                   LineMap lm, const real *__restrict__ in,
+                                                   real *__restrict__ out)
+{
+    extern __shared__ char smem_raw[];
+    real *xb = (real *)smem_raw;
+    const int P = op->P;
+    const int LX = blockDim.x / P;
+    Lane<real, MAP> c = make_lane<real, MAP>(op, P, LX, xb);
+    const int L = blockIdx.x * LX + c.line;
+    const bool active = L < lm.nlines;
+    const long base = lm.base(active ? L : lm.nlines - 1);
+    const long sp = lm.sp;
+    constexpr int HW = KIND == OP_FILTER ? 4 : 2;
+    real w[M + 2 * HW];
+    load_window<real, M, HW>([&](int p) { return in[base + p * sp]; }, c.s, op->n,
+                             op->cyclic, w);
+    real r[M];
+    compact_op<KIND, real, M, HW, UNI, MAP>(op, c, w, r);
+    if (active) {
+#pragma unroll
+        for (int t = 0; t < M; t++)
+            if (t < c.len) out[base + (long)(c.s + t) * sp] = r[t];
+    }
+}
+
+template <int KIND, typename real, int M, bool UNI>
+static hipError_t launch_kmu(const PlanDev<real> &pl, int map, const LineMap &lm,
+                             const real *in, real *out, hipStream_t st)
+{
+    dim3 grid((lm.nlines + pl.LX - 1) / pl.LX), block(pl.threads);
+    size_t shm = 2 * (size_t)pl.threads * sizeof(real);
+    if (map == MAP_ROW)
+        hipLaunchKernelGGL((k_field_op<KIND, real, M, UNI, MAP_ROW>), grid, block, shm, st,
+                           pl.dev, lm, in, out);
+    else
+        hipLaunchKernelGGL((k_field_op<KIND, real, M, UNI, MAP_COL>), grid, block, shm, st,
+                           pl.dev, lm, in, out);
+    return hipGetLastError();
+}
+
+template <int KIND, typename real>
+static hipError_t launch_k(const PlanDev<real> &pl, int map, const LineMap &lm,
+                           const real *in, real *out, hipStream_t st)
+{
+    constexpr int MU = KIND == OP_FILTER ? SEG_FILTER : SEG_DERIV;
+    if (pl.host.uniform) {
+        if (pl.host.M != MU) return hipErrorInvalidValue;
+        return launch_kmu<KIND, real, MU, true>(pl, map, lm, in, out, st);
+    }
+    if (pl.host.M != SEG_TABLE) return hipErrorInvalidValue;
+    return launch_kmu<KIND, real, SEG_TABLE, false>(pl, map, lm, in, out, st);
+}
+
+template <typename real>
+hipError_t launch_field_op(const PlanDev<real> &pl, int map, const LineMap &lm,
+                           const real *in, real *out, hipStream_t st)
+{
+    if (lm.nlines <= 0) return hipSuccess;
+    switch (pl.host.kind) {
+    case OP_D1: return launch_k<OP_D1, real>(pl, map, lm, in, out, st);
+    case OP_D2: return launch_k<OP_D2, real>(pl, map, lm, in, out, st);
+    default: return launch_k<OP_FILTER, real>(pl, map, lm, in, out, st);
+    }
+}
+
+template struct PlanDev<float>;
+template struct PlanDev<double>;
+template void make_plan<float>(PlanDev<float> &, int, int, int, double, double);
+template void make_plan<double>(PlanDev<double> &, int, int, int, double, double);
+template hipError_t launch_field_op<float>(const PlanDev<float> &, int, const LineMap &,
+                                           const float *, float *, hipStream_t);
+template hipError_t launch_field_op<double>(const PlanDev<double> &, int, const LineMap &,
+                                            const double *, double *, hipStream_t);
+
+}  // namespace cs3d

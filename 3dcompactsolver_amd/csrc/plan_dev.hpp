@@ -1,0 +1,56 @@
+// plan_dev.hpp -- device residency of operator plans and the launchers of the
+// standalone per-field operator (the Derivatives / Filter drop-in path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "line_engine.hpp"
+#include "plan.hpp"
+
+namespace cs3d {
+
+OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f, int M, double tol);
+// the plan the library uses for an operator in a precision (segment length
+// and truncation chosen as the kernels expect)
+OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32);
+
+// Points per lane.  Derivatives keep 4 points per lane in both precisions
+// (the fused sweeps hold many windows at once); the periodic filter keeps 8;
+// non-periodic and ragged lines (table path) keep 4.
+constexpr int SEG_DERIV = 4;
+constexpr int SEG_FILTER = 8;
+constexpr int SEG_TABLE = 4;
+
+template <typename real>
+struct PlanDev {
+    OpHost host;
+    OpDev<real> hd{};           // host copy holding device table pointers
+    OpDev<real> *dev = nullptr; // device copy read by kernels
+    void *tables = nullptr;
+    int LX = 0;                 // lines per workgroup
+    int threads = 0;            // workgroup size = P * LX
+
+    void release();
+};
+
+// Builds (FP64 host factorisation) and uploads a plan.  Throws
+// std::invalid_argument for unsupported sizes, std::runtime_error on HIP
+// failures.
+template <typename real>
+void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f);
+
+// out = op(in) on every line of lm; map is MAP_ROW for contiguous lines.
+template <typename real>
+hipError_t launch_field_op(const PlanDev<real> &pl, int map, const LineMap &lm,
+                           const real *in, real *out, hipStream_t st);
+
+// line maps of an Nx*Ny*Nz field in XYZ order (Macros.hpp:8), and of the
+// reference's per-direction layouts YZX / ZXY (Macros.hpp:9-10)
+LineMap xyz_lines(int dir, int Nx, int Ny, int Nz);
+LineMap ref_lines(int dir, int Nx, int Ny, int Nz);
+
+std::string hip_msg(hipError_t e, const char *what);
+
+}  // namespace cs3d

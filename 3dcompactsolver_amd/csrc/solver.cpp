@@ -1,0 +1,515 @@
+// solver.cpp -- host side of the GPU solver behind the C ABI: device state,
+// the stage/step orchestration of CSolver_AWS + RK4/TVDRK3
+// (CSolver_AWS.cpp:2897-2946, RK4.hpp:120-170, TVDRK3.hpp:98-139), and
+// host transfers.  Everything stays on one HIP stream; the only calls that
+// wait for the device are the ones that return host data.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cs3d.h"
+#include "capi_common.hpp"
+#include "plan_dev.hpp"
+#include "solver.hpp"
+#include "solver_kernels.hpp"
+
+namespace cs3d {
+
+static void hc(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(hip_msg(e, what));
+}
+
+template <typename real>
+struct SolverT : SolverBase {
+    cs3d_config cfg;
+    int Nx, Ny, Nz;
+    long N;
+    double dxyz[3];
+    PlanDev<real> D1[3], D2[3], FL[3];
+    int sweepLX[3], sweepThreads[3];
+    LineMap lm[3];
+    Fields<real> F;
+    Phys<real> ph;
+    StageArgs base;
+    hipStream_t st = nullptr;
+    void *mem = nullptr;
+    StepScalars *dsc = nullptr;
+    double *dpart = nullptr, *dout = nullptr;
+    int nred = 0;
+    bool sponge = false;
+    bool ic_done = false;
+    bool diag_on = false, diag_valid = false;
+    bool stale_pending = false;  // a sponge update happened since the last stage 1
+    // optional per-launch timing with HIP events on the solver stream
+    struct EvPair {
+        hipEvent_t a, b;
+        int cls;
+    };
+    bool prof = false;
+    std::vector<EvPair> evpool;
+    size_t evused = 0;
+    int timeStep = 0, filterTimeStep = 0;
+    double rho_ref = 1.0, rhoE_ref = 0.0;
+    double R_gas = 1.0, cp = 1.0;
+
+    explicit SolverT(const cs3d_config &c) : cfg(c)
+    {
+        Nx = c.Nx;
+        Ny = c.Ny;
+        Nz = c.Nz;
+        N = (long)Nx * Ny * Nz;
+        const int dims[3] = {Nx, Ny, Nz};
+        const double Ls[3] = {c.Lx, c.Ly, c.Lz};
+        for (int d = 0; d < 3; d++) {
+            // Domain.hpp:30-45
+            dxyz[d] = (1.0 / ((double)dims[d] - 1.0)) * Ls[d];
+            int per = c.bc_type[d] == CS3D_PERIODIC_SOLVE;
+            make_plan(D1[d], OP_D1, per, dims[d], dxyz[d], c.alpha_f);
+            make_plan(D2[d], OP_D2, per, dims[d], dxyz[d], c.alpha_f);
+            make_plan(FL[d], OP_FILTER, per, dims[d], dxyz[d], c.alpha_f);
+            if (D1[d].host.P != D2[d].host.P || D1[d].host.uniform != D2[d].host.uniform)
+                throw std::invalid_argument("inconsistent derivative plans");
+            int P = D1[d].host.P;
+            int LXs = (sizeof(real) == 4 ? 1024 : 512) / P;
+            if (LXs > 64) LXs = 64;
+            if (LXs < 1) throw std::invalid_argument("line too long for the fused sweeps");
+            sweepLX[d] = LXs;
+            sweepThreads[d] = LXs * P;
+            lm[d] = xyz_lines(d, Nx, Ny, Nz);
+        }
+        // IdealGas.hpp:35-47: p_ref = 1/gamma, rho_ref = T_ref = 1
+        R_gas = (1.0 / c.gamma) / 1.0 / 1.0;
+        cp = R_gas * c.gamma / (c.gamma - 1.0);
+        for (int f = 0; f < 6; f++) sponge |= c.bc_kind[f] == CS3D_BC_SPONGE;
+
+        std::memset(&F, 0, sizeof(F));
+        int nf = 15 + 4 + 4 + 5 + 5 + 4 + (sponge ? 11 : 0);
+        hc(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        hc(hipMalloc(&mem, (size_t)nf * N * sizeof(real)), "hipMalloc(fields)");
+        hc(hipMemsetAsync(mem, 0, (size_t)nf * N * sizeof(real), st), "hipMemset(fields)");
+        real *p = (real *)mem;
+        auto take = [&]() {
+            real *r = p;
+            p += N;
+            return r;
+        };
+        for (int v = 0; v < 5; v++) F.q[v] = take();
+        for (int v = 0; v < 5; v++) F.qk[v] = take();
+        for (int v = 0; v < 5; v++) F.acc[v] = take();
+        for (int v = 0; v < 4; v++) F.dX[v] = take();
+        for (int v = 0; v < 4; v++) F.dY[v] = take();
+        for (int v = 0; v < 5; v++) F.R[v] = take();
+        for (int v = 0; v < 5; v++) F.k2[v] = take();
+        for (int v = 0; v < 4; v++) F.diag[v] = take();
+        if (sponge) {
+            F.sigma = take();
+            for (int v = 0; v < 5; v++) F.avg[v] = take();
+            for (int v = 0; v < 5; v++) F.qstale[v] = take();
+        }
+        hc(hipMalloc(&dsc, sizeof(StepScalars)), "hipMalloc(scalars)");
+        hc(hipMemsetAsync(dsc, 0, sizeof(StepScalars), st), "hipMemset(scalars)");
+        nred = (int)std::min<long>(1024, (N + 255) / 256);
+        hc(hipMalloc(&dpart, (size_t)nred * RED_SLOTS_HOST * sizeof(double)), "hipMalloc(partials)");
+        hc(hipMalloc(&dout, 2 * RED_SLOTS_HOST * sizeof(double)), "hipMalloc(reductions)");
+
+        std::memset(&base, 0, sizeof(base));
+        base.rk = c.rk;
+        base.Nx = Nx;
+        base.Ny = Ny;
+        base.Nz = Nz;
+        base.compensated = 0;
+        for (int f = 0; f < 6; f++) base.sponge_face[f] = c.bc_kind[f] == CS3D_BC_SPONGE;
+        if (sponge) upload_sigma();
+        hc(hipStreamSynchronize(st), "init");
+    }
+
+    ~SolverT() override
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto &e : evpool) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        for (int d = 0; d < 3; d++) {
+            D1[d].release();
+            D2[d].release();
+            FL[d].release();
+        }
+        if (mem) (void)hipFree(mem);
+        if (dsc) (void)hipFree(dsc);
+        if (dpart) (void)hipFree(dpart);
+        if (dout) (void)hipFree(dout);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    // SpongeBC.hpp:35-154 (sigma starts at zero; see the sponge note there)
+    void upload_sigma()
+    {
+        std::vector<double> x(Nx), y(Ny), z(Nz);
+        for (int i = 0; i < Nx; i++) x[i] = ((double)i / ((double)Nx - 1.0)) * cfg.Lx;
+        for (int j = 0; j < Ny; j++) y[j] = ((double)j / ((double)Ny - 1.0)) * cfg.Ly;
+        for (int k = 0; k < Nz; k++) z[k] = ((double)k / ((double)Nz - 1.0)) * cfg.Lz;
+        double LX = cfg.sponge_frac[0] * cfg.Lx, LY = cfg.sponge_frac[1] * cfg.Ly,
+               LZ = cfg.sponge_frac[2] * cfg.Lz, st_ = cfg.sponge_strength;
+        auto ramp = [&](double s) { return st_ * (0.068 * std::pow(s, 2.0) + 0.845 * std::pow(s, 8.0)); };
+        std::vector<real> sg((size_t)N);
+        for (int k = 0; k < Nz; k++)
+            for (int j = 0; j < Ny; j++)
+                for (int i = 0; i < Nx; i++) {
+                    double v = 0.0;
+                    if (cfg.bc_kind[0] == CS3D_BC_SPONGE && x[i] < LX) v = std::fmax(ramp((LX - x[i]) / LX), v);
+                    if (cfg.bc_kind[1] == CS3D_BC_SPONGE && x[i] > cfg.Lx - LX)
+                        v = std::fmax(ramp((x[i] - (cfg.Lx - LX)) / LX), v);
+                    if (cfg.bc_kind[2] == CS3D_BC_SPONGE && y[j] < LY) v = std::fmax(ramp((LY - y[j]) / LY), v);
+                    if (cfg.bc_kind[3] == CS3D_BC_SPONGE && y[j] > cfg.Ly - LY)
+                        v = std::fmax(ramp((y[j] - (cfg.Ly - LY)) / LY), v);
+                    if (cfg.bc_kind[4] == CS3D_BC_SPONGE && z[k] < LZ) v = std::fmax(ramp((LZ - z[k]) / LZ), v);
+                    if (cfg.bc_kind[5] == CS3D_BC_SPONGE && z[k] > cfg.Lz - LZ)
+                        v = std::fmax(ramp((z[k] - (cfg.Lz - LZ)) / LZ), v);
+                    sg[(size_t)k * Nx * Ny + (size_t)j * Nx + i] = (real)v;
+                }
+        hc(hipMemcpy(F.sigma, sg.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "sigma");
+    }
+
+    void build_phys()
+    {
+        ph.gm1 = (real)(cfg.gamma - 1.0);
+        ph.gamma = (real)cfg.gamma;
+        ph.iR = (real)(1.0 / R_gas);
+        double p_ref = (cfg.gamma - 1.0) * rhoE_ref;
+        double T0s = p_ref / (rho_ref * R_gas);
+        ph.rho_ref = (real)rho_ref;
+        ph.rhoE_ref = (real)rhoE_ref;
+        ph.p_ref = (real)p_ref;
+        ph.T0s = (real)T0s;
+        ph.RT0s = (real)(R_gas * T0s);
+        ph.h_ref = (real)(rhoE_ref + p_ref);
+        ph.mu_ref = (real)cfg.mu_ref;
+        ph.iTref_ig = (real)1.0;
+        ph.cpPr = (real)(cp / cfg.Pr);
+        ph.idx = (real)(1.0 / dxyz[0]);
+        ph.idy = (real)(1.0 / dxyz[1]);
+        ph.idz = (real)(1.0 / dxyz[2]);
+    }
+
+    // ---------------------------------------------------------------- IC
+    void set_ic(const double *rho0, const double *U0, const double *V0, const double *W0,
+                const double *p0) override
+    {
+        // setInitialConditions (CSolver_AWS.cpp:207-234)
+        std::vector<double> q[5];
+        for (int v = 0; v < 5; v++) q[v].resize((size_t)N);
+        double srho = 0.0, sE = 0.0;
+        for (long i = 0; i < N; i++) {
+            double rho = rho0[i], U = U0[i], V = V0[i], W = W0[i], p = p0[i];
+            q[0][i] = rho;
+            q[1][i] = rho * U;
+            q[2][i] = rho * V;
+            q[3][i] = rho * W;
+            q[4][i] = p / (cfg.gamma - 1.0) + 0.5 * rho * (U * U + V * V + W * W);
+            srho += rho;
+            sE += q[4][i];
+        }
+        rho_ref = srho / (double)N;
+        rhoE_ref = sE / (double)N;
+        build_phys();
+        upload_state(q, 0);
+        if (sponge)
+            for (int v = 0; v < 5; v++)
+                hc(hipMemcpyAsync(F.avg[v], F.q[v], (size_t)N * sizeof(real), hipMemcpyDeviceToDevice, st),
+                   "sponge averages");
+        StepScalars z;
+        std::memset(&z, 0, sizeof(z));
+        hc(hipMemcpyAsync(dsc, &z, sizeof(z), hipMemcpyHostToDevice, st), "scalars");
+        timeStep = 0;
+        filterTimeStep = 0;
+        diag_valid = false;
+        stale_pending = false;
+        hc(launch_reduce<real>(F, ph, N, 0, dpart, nred, dout, dsc, st), "reduce");
+        hc(hipStreamSynchronize(st), "set_ic");
+        ic_done = true;
+    }
+
+    // q holds full conserved variables (FP64); stored in perturbation form
+    void upload_state(const std::vector<double> *q, int which)
+    {
+        std::vector<real> buf((size_t)N);
+        real *const *dst = which == 0 ? F.q : which == 1 ? F.qk : which == 2 ? F.k2 : F.acc;
+        for (int v = 0; v < 5; v++) {
+            double off = which == 2 ? 0.0 : (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0);
+            for (long i = 0; i < N; i++) buf[i] = (real)(q[v][i] - off);
+            hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
+        }
+    }
+
+    // ---------------------------------------------------------------- timing
+    template <typename Fn>
+    void timed(int cls, Fn launch)
+    {
+        if (!prof) {
+            launch();
+            return;
+        }
+        if (evused == evpool.size()) {
+            EvPair e;
+            hc(hipEventCreate(&e.a), "hipEventCreate");
+            hc(hipEventCreate(&e.b), "hipEventCreate");
+            evpool.push_back(e);
+        }
+        EvPair &e = evpool[evused++];
+        e.cls = cls;
+        hc(hipEventRecord(e.a, st), "hipEventRecord");
+        launch();
+        hc(hipEventRecord(e.b, st), "hipEventRecord");
+    }
+
+    void profile_enable(int on) override
+    {
+        prof = on != 0;
+        evused = 0;
+    }
+
+    void profile_report(double *ms, int *counts) override
+    {
+        for (int c = 0; c < PROF_CLASSES; c++) {
+            ms[c] = 0.0;
+            counts[c] = 0;
+        }
+        hc(hipStreamSynchronize(st), "sync");
+        for (size_t i = 0; i < evused; i++) {
+            float t = 0.0f;
+            hc(hipEventElapsedTime(&t, evpool[i].a, evpool[i].b), "hipEventElapsedTime");
+            ms[evpool[i].cls] += t;
+            counts[evpool[i].cls]++;
+        }
+        evused = 0;
+    }
+
+    // ---------------------------------------------------------------- steps
+    void pre_step() override
+    {
+        need_ic();
+        timed(5, [&] { hc(launch_dt(dsc, cfg.cfl, cfg.dt_fixed, st), "dt"); });
+        timeStep++;
+    }
+
+    void sweeps(int rkStep, int mode, int diag)
+    {
+        StageArgs sa = base;
+        sa.rkStep = rkStep;
+        sa.mode = mode;
+        sa.diag = diag;
+        sa.stale = (rkStep == 1 && stale_pending) ? 1 : 0;
+        for (int d = 0; d < 3; d++)
+            timed(d, [&] {
+                hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, D1[d].host.uniform, D1[d].host.M,
+                                      sweepThreads[d], sweepLX[d], lm[d], F, ph, sa, dsc, st),
+                   "sweep");
+            });
+    }
+
+    void filter_sweep(int dir, real *const *in, real *const *out)
+    {
+        Ptr5<real> io;
+        for (int v = 0; v < 5; v++) {
+            io.in[v] = in[v];
+            io.out[v] = out[v];
+        }
+        const PlanDev<real> &pl = FL[dir];
+        timed(3, [&] {
+            hc(launch_filter5<real>(pl.dev, pl.host.uniform, pl.host.M, pl.threads, pl.LX,
+                                    dir == 0 ? MAP_ROW : MAP_COL, lm[dir], io, st),
+               "filter");
+        });
+    }
+
+    // filterConservedData (CSolver_AWS.cpp:1682-1893): *2 -> *1
+    void filter_conserved()
+    {
+        if (timeStep % cfg.filter_step == 0) {
+            filterTimeStep++;
+            int m = filterTimeStep % 3;
+            int o[3];
+            if (m == 1) { o[0] = 0; o[1] = 1; o[2] = 2; }
+            else if (m == 2) { o[0] = 1; o[1] = 2; o[2] = 0; }
+            else { o[0] = 2; o[1] = 0; o[2] = 1; }
+            filter_sweep(o[0], F.acc, F.q);
+            filter_sweep(o[1], F.q, F.acc);
+            filter_sweep(o[2], F.acc, F.q);
+        } else {
+            // the memcpy of *2 into *1 becomes a swap of the two buffers
+            for (int v = 0; v < 5; v++) std::swap(F.q[v], F.acc[v]);
+        }
+    }
+
+    void sub_step(int rkStep, int rkLast) override
+    {
+        need_ic();
+        int diag = (diag_on && rkLast) ? 1 : 0;
+        sweeps(rkStep, 0, diag);
+        if (rkStep == 1) stale_pending = false;
+        if (rkLast) {
+            filter_conserved();
+            // calcDtFromCFL's maximum for the next step, and the step's
+            // diagnostics, from the filtered state before the sponge update
+            timed(4, [&] {
+                hc(launch_reduce<real>(F, ph, N, diag, dpart, nred, dout, dsc, st), "reduce");
+            });
+            diag_valid = diag != 0;
+        }
+    }
+
+    void post_step() override
+    {
+        need_ic();
+        if (sponge) {
+            StageArgs sa = base;
+            timed(5, [&] {
+                hc(launch_sponge<real>(F, ph, sa, dsc, cfg.sponge_avgT, cfg.sponge_epsP,
+                                       1.0 / cfg.gamma, st),
+                   "sponge");
+            });
+            stale_pending = true;
+        }
+    }
+
+    void step(int nsteps) override
+    {
+        int ns = cfg.rk == CS3D_RK4 ? 4 : 3;
+        for (int n = 0; n < nsteps; n++) {
+            pre_step();
+            for (int s = 1; s <= ns; s++) sub_step(s, s == ns);
+            post_step();
+        }
+    }
+
+    void rhs(int rkStep) override
+    {
+        need_ic();
+        sweeps(rkStep, 1, 0);
+    }
+
+    int check_finite() override
+    {
+        StepScalars h;
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        return h.nonfinite ? CS3D_ERR_NONFINITE : CS3D_OK;
+    }
+
+    void need_ic() const
+    {
+        if (!ic_done) throw std::logic_error("initial conditions not set");
+    }
+
+    // ---------------------------------------------------------------- host data
+    void get_conserved(int set, double *const *out) override
+    {
+        need_ic();
+        real *const *src = set == 1 ? F.q : set == 2 ? F.qk : set == 3 ? F.k2 : F.acc;
+        std::vector<real> buf((size_t)N);
+        hc(hipStreamSynchronize(st), "sync");
+        for (int v = 0; v < 5; v++) {
+            if (!out[v]) continue;
+            hc(hipMemcpy(buf.data(), src[v], (size_t)N * sizeof(real), hipMemcpyDeviceToHost), "download");
+            double off = set == 3 ? 0.0 : (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0);
+            for (long i = 0; i < N; i++) out[v][i] = (double)buf[i] + off;
+        }
+    }
+
+    void set_conserved(int set, const double *const *in) override
+    {
+        need_ic();
+        hc(hipStreamSynchronize(st), "sync");
+        std::vector<double> q[5];
+        for (int v = 0; v < 5; v++) {
+            if (!in[v]) throw std::invalid_argument("set_conserved needs all five fields");
+            q[v].assign(in[v], in[v] + N);
+        }
+        int which = set == 1 ? 0 : set == 2 ? 1 : set == 3 ? 2 : 3;
+        upload_state(q, which);
+        if (set == 1) hc(launch_reduce<real>(F, ph, N, 0, dpart, nred, dout, dsc, st), "reduce");
+    }
+
+    void get_primitives(double *const *out) override
+    {
+        need_ic();
+        std::vector<double> q[5];
+        double *o[5];
+        for (int v = 0; v < 5; v++) {
+            q[v].resize((size_t)N);
+            o[v] = q[v].data();
+        }
+        get_conserved(1, o);
+        // updateNonConservedData (CSolver_AWS.cpp:1895-1935)
+        for (long i = 0; i < N; i++) {
+            double rho = q[0][i], U = q[1][i] / rho, V = q[2][i] / rho, W = q[3][i] / rho;
+            double p = (cfg.gamma - 1.0) * (q[4][i] - 0.5 * rho * (U * U + V * V + W * W));
+            double T = p / (rho * R_gas);
+            double mu = cfg.mu_ref * std::pow(T / 1.0, 0.76);
+            double amu = 0.76 * (cfg.mu_ref / std::pow(1.0, 0.76)) * std::pow(T, 0.76 - 1.0);
+            double sos = std::sqrt(cfg.gamma * p / rho);
+            const double vals[8] = {U, V, W, p, T, mu, amu, sos};
+            for (int k = 0; k < 8; k++)
+                if (out[k]) out[k][i] = vals[k];
+        }
+    }
+
+    void get_time(double *time, double *dt, int *ts) override
+    {
+        StepScalars h;
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        if (time) *time = h.time;
+        if (dt) *dt = h.dt;
+        if (ts) *ts = h.timeStep;
+    }
+
+    void set_diag(int on) override { diag_on = on != 0; }
+
+    int diag_tgv(double out[7]) override
+    {
+        if (!diag_valid) return CS3D_ERR_STATE;
+        double r[RED_SLOTS_HOST];
+        StepScalars h;
+        hc(hipMemcpyAsync(r, dout, sizeof(r), hipMemcpyDeviceToHost, st), "diag");
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        out[0] = h.time;
+        out[1] = r[1];
+        out[2] = r[2];
+        out[3] = r[3];
+        out[4] = r[4];
+        out[5] = r[6] / (double)N;
+        out[6] = std::sqrt(r[5] / (double)N);
+        return h.nonfinite ? CS3D_ERR_NONFINITE : CS3D_OK;
+    }
+
+    int kinetic_energy(double *ke) override
+    {
+        need_ic();
+        double r[RED_SLOTS_HOST];
+        hc(launch_reduce<real>(F, ph, N, 2, dpart, nred, dout + RED_SLOTS_HOST, dsc, st), "reduce");
+        hc(hipMemcpyAsync(r, dout + RED_SLOTS_HOST, sizeof(r), hipMemcpyDeviceToHost, st), "ke");
+        hc(hipStreamSynchronize(st), "sync");
+        *ke = r[1] / (double)N;
+        return std::isfinite(*ke) ? CS3D_OK : CS3D_ERR_NONFINITE;
+    }
+
+    void synchronize() override { hc(hipStreamSynchronize(st), "sync"); }
+    void *stream() override { return (void *)st; }
+};
+
+SolverBase *make_solver(const cs3d_config &c)
+{
+    if (c.precision == CS3D_FP32) return new SolverT<float>(c);
+    return new SolverT<double>(c);
+}
+
+}  // namespace cs3d

@@ -1,0 +1,38 @@
+// solver.hpp -- precision-independent interface of the GPU solver used by
+// the C ABI (capi_solver.cpp).  Methods throw std::invalid_argument,
+// std::logic_error (call out of order) or std::runtime_error (HIP failure).
+#pragma once
+
+#include "../../include/cs3d.h"
+
+namespace cs3d {
+
+// launch classes of the per-launch timer
+constexpr int PROF_CLASSES = 6;  // X sweep, Y sweep, Z sweep, filter, reductions, other
+
+struct SolverBase {
+    virtual ~SolverBase() {}
+    virtual void set_ic(const double *rho0, const double *U0, const double *V0,
+                        const double *W0, const double *p0) = 0;
+    virtual void pre_step() = 0;
+    virtual void sub_step(int rkStep, int rkLast) = 0;
+    virtual void post_step() = 0;
+    virtual void step(int nsteps) = 0;
+    virtual void rhs(int rkStep) = 0;
+    virtual int check_finite() = 0;
+    virtual void get_conserved(int set, double *const *out) = 0;
+    virtual void set_conserved(int set, const double *const *in) = 0;
+    virtual void get_primitives(double *const *out) = 0;
+    virtual void get_time(double *time, double *dt, int *ts) = 0;
+    virtual void set_diag(int on) = 0;
+    virtual int diag_tgv(double out[7]) = 0;
+    virtual int kinetic_energy(double *ke) = 0;
+    virtual void synchronize() = 0;
+    virtual void *stream() = 0;
+    virtual void profile_enable(int on) = 0;
+    virtual void profile_report(double *ms, int *counts) = 0;
+};
+
+SolverBase *make_solver(const cs3d_config &c);
+
+}  // namespace cs3d

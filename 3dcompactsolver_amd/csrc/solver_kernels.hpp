@@ -1,0 +1,103 @@
+// solver_kernels.hpp -- data shared by the fused stage kernels and the host
+// solver: field pointers, physical constants, per-stage arguments.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "line_engine.hpp"
+
+namespace cs3d {
+
+// Conserved variables are stored as perturbations of a constant reference
+// state for density and total energy (rho' = rho - rho_ref,
+// rhoE' = rhoE - rhoE_ref); momenta are stored as they are.  Compact
+// derivatives of constants vanish and the filter preserves constants, so the
+// scheme is unchanged; the point is that FP32 keeps the pressure and
+// temperature fluctuations to full relative precision.
+template <typename real>
+struct Fields {
+    real *q[5];     // state (*1)
+    real *qk[5];    // stage state (*k)
+    real *acc[5];   // RK accumulator (*2)
+    real *qlo[5];   // compensation of q (FP32 compensated mode), else null
+    real *acclo[5]; // compensation of acc
+    real *dX[4];    // Ux Vx Wx Tx of the stage
+    real *dY[4];    // Uy Vy Wy Ty
+    real *R[5];     // partial right-hand side between sweeps
+    real *k2[5];    // dt*RHS (only written in RHS-only mode)
+    real *sigma;    // sponge strength (null without sponge)
+    real *avg[5];   // sponge running averages (perturbation form)
+    real *diag[4];  // per-point diagnostic inputs of the last stage
+    real *qstale[5];// state on the sponge planes before the sponge reset
+};
+
+template <typename real>
+struct Phys {
+    real gm1, gamma;
+    real iR;           // 1 / R_gas
+    real rho_ref, rhoE_ref, p_ref;
+    real T0s, RT0s;    // reference temperature p_ref/(rho_ref R), R*T0s
+    real h_ref;        // rhoE_ref + p_ref
+    real mu_ref, iTref_ig;  // IdealGas mu_ref and 1/T_ref (T_ref = 1)
+    real cpPr;         // cp / Pr
+    real idx, idy, idz;
+};
+
+// device-resident scalars of the time loop
+struct StepScalars {
+    double dt;
+    double time;
+    double cfl_max;    // max over the state of sum_i (|u_i| + c)/dx_i
+    int timeStep;
+    int filterTimeStep;
+    int nonfinite;     // set when a reduction meets NaN/Inf
+    int pad_;
+};
+
+struct StageArgs {
+    int rk;            // 4 or 3
+    int rkStep;        // 1..4
+    int mode;          // 0: RK update, 1: write dt*RHS into k2 only
+    int diag;          // store per-point diagnostic inputs
+    int sponge_face[6];// 1 where a face is a sponge (its plane's RHS is zeroed)
+    int Nx, Ny, Nz;
+    int compensated;
+    int stale;         // X sweep takes primitives on sponge planes from qstale
+};
+
+// XCD-aware workgroup order: the hardware hands consecutive workgroups to
+// the 8 XCDs in turn; remapping makes workgroups that share 128-byte lines
+// (neighbouring line batches) run on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int nx = 8;
+    if (nb < nx || nb % nx) return b;
+    return (b % nx) * (nb / nx) + b / nx;
+}
+
+template <typename real>
+struct Ptr5 {
+    const real *in[5];
+    real *out[5];
+};
+
+// launchers (stage_kernels.hip)
+template <typename real>
+hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                        int threads, int LX, const LineMap &lm, const Fields<real> &F,
+                        const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
+                        hipStream_t st);
+template <typename real>
+hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, int threads, int LX, int map,
+                          const LineMap &lm, const Ptr5<real> &io, hipStream_t st);
+constexpr int RED_SLOTS_HOST = 8;
+template <typename real>
+hipError_t launch_reduce(const Fields<real> &F, const Phys<real> &ph, long N, int diag,
+                         double *part, int nb, double *out, StepScalars *sc, hipStream_t st);
+hipError_t launch_dt(StepScalars *sc, double cfl, double dt_fixed, hipStream_t st);
+template <typename real>
+hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,
+                         const StepScalars *sc, double avgT, double epsP, double spongeP,
+                         hipStream_t st);
+
+}  // namespace cs3d

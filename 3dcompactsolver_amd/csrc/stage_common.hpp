@@ -1,0 +1,94 @@
+// stage_common.hpp -- device helpers shared by the stage sweeps: ideal-gas
+// relations on the perturbation-form state, and the per-lane set-up of a
+// sweep workgroup.
+#pragma once
+
+#include "plan_dev.hpp"
+#include "solver_kernels.hpp"
+
+namespace cs3d {
+
+template <typename real>
+__device__ __forceinline__ real pow076(real x);
+template <>
+__device__ __forceinline__ float pow076<float>(float x)
+{
+    return powf(x, 0.76f);
+}
+template <>
+__device__ __forceinline__ double pow076<double>(double x)
+{
+    return pow(x, 0.76);
+}
+
+// IdealGas.hpp:49-55 on the temperature: mu and Amu = dmu/dT
+template <typename real>
+__device__ __forceinline__ void visc(const Phys<real> &ph, real T, real &mu, real &amu)
+{
+    mu = ph.mu_ref * pow076(T * ph.iTref_ig);
+    amu = real(0.76) * mu / T;
+}
+
+// conserved (perturbation form) -> 1/rho, velocity, p' = p - p_ref,
+// T' = T - T0s  (IdealGas.hpp:57-71)
+template <typename real>
+__device__ __forceinline__ void prim(const Phys<real> &ph, real rp, real ru, real rv, real rw,
+                                     real re, real &irho, real &U, real &V, real &W, real &pp,
+                                     real &Tp)
+{
+    real rho = ph.rho_ref + rp;
+    irho = real(1) / rho;
+    U = ru * irho;
+    V = rv * irho;
+    W = rw * irho;
+    real ke = real(0.5) * (ru * U + rv * V + rw * W);
+    pp = ph.gm1 * (re - ke);
+    Tp = (pp - rp * ph.RT0s) * irho * ph.iR;
+}
+
+#define CS3D_LANE_SETUP(MAPV, OPP)                                                  \
+    extern __shared__ char smem_raw[];                                              \
+    real *xb = (real *)smem_raw;                                                    \
+    const int P = (OPP)->P;                                                         \
+    const int LX = blockDim.x / P;                                                  \
+    Lane<real, MAPV> c = make_lane<real, MAPV>((OPP), P, LX, xb);                   \
+    const int L = xcd_block(blockIdx.x, gridDim.x) * LX + c.line;                   \
+    const bool active = L < lm.nlines;                                              \
+    const long base = lm.base(active ? L : lm.nlines - 1);                          \
+    const long sp = lm.sp;                                                          \
+    const int n = (OPP)->n, per = (OPP)->cyclic;
+
+
+// q windows a0..a4 (perturbation form) and ir = 1/rho at window index t
+#define CS3D_U(t) (a1[t] * ir[t])
+#define CS3D_V(t) (a2[t] * ir[t])
+#define CS3D_W(t) (a3[t] * ir[t])
+
+template <typename real, int W>
+__device__ __forceinline__ real pprime(const Phys<real> &ph, const real (&a1)[W],
+                                       const real (&a2)[W], const real (&a3)[W],
+                                       const real (&a4)[W], const real (&ir)[W], int t)
+{
+    const real U = a1[t] * ir[t], V = a2[t] * ir[t], Wv = a3[t] * ir[t];
+    return ph.gm1 * (a4[t] - real(0.5) * (a1[t] * U + a2[t] * V + a3[t] * Wv));
+}
+
+// workgroup size limit of the sweeps: FP32 fits 4 waves/SIMD (128 VGPRs),
+// FP64 holds twice the registers and runs 2 waves/SIMD
+template <typename real>
+constexpr int sweep_threads() { return sizeof(real) == 4 ? 1024 : 512; }
+
+template <typename real, bool UNI>
+hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
+                        int use_k, const StageArgs &sa, hipStream_t st);
+template <typename real, bool UNI>
+hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
+                        int use_k, hipStream_t st);
+template <typename real, bool UNI>
+hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
+                        const StageArgs &sa, const StepScalars *sc, hipStream_t st);
+
+}  // namespace cs3d

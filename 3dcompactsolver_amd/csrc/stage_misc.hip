@@ -1,0 +1,257 @@
+// stage_misc.hip -- the step-level kernels around the stage sweeps: the
+// filter sweeps, the FP64 reductions (CFL maximum, diagnostics), the dt
+// update and the sponge update; and the sweep dispatcher.
+#include "stage_common.hpp"
+
+namespace cs3d {
+
+// -------------------------------------------------------------- filter
+// One filter sweep over the five conserved fields (Filter.cpp:129-171 as
+// called by filterConservedData, CSolver_AWS.cpp:1682-1893).
+
+template <typename real, int M, bool UNI, int MAP>
+__global__ __launch_bounds__(1024) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
+                                                  Ptr5<real> io)
+{
+    CS3D_LANE_SETUP(MAP, fo)
+    constexpr int HW = 4;
+    for (int v = 0; v < 5; v++) {
+        const real *src = io.in[v];
+        real w[M + 2 * HW], r[M];
+        load_window<real, M, HW>([&](int p) { return src[base + p * sp]; }, c.s, n, per, w);
+        compact_op<OP_FILTER, real, M, HW, UNI, MAP>(fo, c, w, r);
+        if (active) {
+            real *dst = io.out[v];
+#pragma unroll
+            for (int t = 0; t < M; t++)
+                if (t < c.len) dst[base + (long)(c.s + t) * sp] = r[t];
+        }
+    }
+}
+
+// -------------------------------------------------------------- reductions
+// Per-workgroup FP64 partial results over the state after the step:
+// [0] max of the CFL wave-speed sum (calcDtFromCFL, CSolver_AWS.cpp:513-525)
+// and, when diag is set, the sums of calcTaylorGreenQuantities
+// (CSolver_AWS.cpp:2188-2228): [1] sum rho|u|^2/2 [2] sum 2 mu S:grad u
+// [3] sum rho|w|^2/2 [4] sum mu rho |w|^2 [5] sum div^2 [6] sum mu.
+constexpr int RED_THREADS = 256;
+constexpr int RED_SLOTS = 8;
+
+template <typename real>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Phys<real> ph,
+                                                               long N, int diag,
+                                                               double *__restrict__ part)
+{
+    __shared__ double sh[RED_SLOTS][RED_THREADS];
+    double acc[RED_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long i = (long)blockIdx.x * RED_THREADS + threadIdx.x; i < N;
+         i += (long)gridDim.x * RED_THREADS) {
+        real irho, U, V, W, pp, Tp;
+        prim(ph, F.q[0][i], F.q[1][i], F.q[2][i], F.q[3][i], F.q[4][i], irho, U, V, W, pp, Tp);
+        double rho = (double)ph.rho_ref + (double)F.q[0][i];
+        double p = (double)ph.p_ref + (double)pp;
+        double cs = sqrt((double)ph.gamma * p / rho);
+        double u = (fabs((double)U) + cs) * (double)ph.idx +
+                   (fabs((double)V) + cs) * (double)ph.idy +
+                   (fabs((double)W) + cs) * (double)ph.idz;
+        if (u != u)
+            acc[7] += 1.0;  // NaN met: counted, reported as non-finite
+        else
+            acc[0] = fmax(acc[0], u);
+        if (diag) {
+            // diag 2: kinetic energy and mean viscosity only (no stage inputs)
+            real mu, amu;
+            visc(ph, ph.T0s + Tp, mu, amu);
+            double m = mu;
+            double uv = (double)U * U + (double)V * V + (double)W * W;
+            acc[1] += rho * uv / 2.0;
+            acc[6] += m;
+            if (diag == 1) {
+                double td = F.diag[0][i], w2 = F.diag[1][i], dv = F.diag[2][i];
+                acc[2] += 2.0 * m * td;
+                acc[3] += 0.5 * rho * w2;
+                acc[4] += m * rho * w2;
+                acc[5] += dv * dv;
+            }
+        }
+    }
+    for (int s = 0; s < RED_SLOTS; s++) sh[s][threadIdx.x] = acc[s];
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + w]);
+            for (int s = 1; s < RED_SLOTS; s++) sh[s][threadIdx.x] += sh[s][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < RED_SLOTS) part[(long)blockIdx.x * RED_SLOTS + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// fixed-order reduction of the partials into out[RED_SLOTS]
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ part,
+                                                               int nb, double *__restrict__ out,
+                                                               StepScalars *sc)
+{
+    __shared__ double sh[RED_SLOTS][RED_THREADS];
+    double acc[RED_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += RED_THREADS) {
+        acc[0] = fmax(acc[0], part[(long)b * RED_SLOTS]);
+        for (int s = 1; s < RED_SLOTS; s++) acc[s] += part[(long)b * RED_SLOTS + s];
+    }
+    for (int s = 0; s < RED_SLOTS; s++) sh[s][threadIdx.x] = acc[s];
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + w]);
+            for (int s = 1; s < RED_SLOTS; s++) sh[s][threadIdx.x] += sh[s][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < RED_SLOTS; s++) out[s] = sh[s][0];
+        sc->cfl_max = sh[0][0];
+        if (sh[7][0] > 0.0 || sh[0][0] > 1e300) sc->nonfinite = 1;
+    }
+}
+
+// calcDtFromCFL (CSolver_AWS.cpp:530-543), on the device so that the time
+// loop never waits for the host.  Q1: the CFL branch is the one taken.
+__global__ void k_dt(StepScalars *sc, double cfl, double dt_fixed)
+{
+    double dt = dt_fixed > 0.0 ? dt_fixed : cfl / sc->cfl_max;
+    if (!(dt > 0.0) || dt != dt || dt > 1e300) sc->nonfinite = 1;
+    sc->dt = dt;
+    if (sc->timeStep == 0) {
+        sc->timeStep++;
+        sc->time = 0.0;
+    } else {
+        sc->timeStep++;
+        sc->time += dt;
+    }
+}
+
+// updateSponge (CSolver_AWS.cpp:1937-2031): running averages, then the
+// sponge face planes of the state are reset to them.
+template <typename real>
+__global__ void k_sponge(Fields<real> F, Phys<real> ph, StageArgs sa, const StepScalars *sc,
+                         double avgT, double epsP, double spongeP)
+{
+    const long N = (long)sa.Nx * sa.Ny * sa.Nz;
+    const double eps = 1.0 / (avgT / sc->dt + 1.0);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real a[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            a[v] = F.avg[v][i];
+            a[v] += (real)eps * (F.q[v][i] - a[v]);
+        }
+        const real rho = ph.rho_ref + a[0];
+        const real target = (real)(spongeP / (double)ph.gm1) - ph.rhoE_ref +
+                            real(0.5) * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / rho;
+        a[4] = (real)epsP * a[4] + (real)(1.0 - epsP) * target;
+#pragma unroll
+        for (int v = 0; v < 5; v++) F.avg[v][i] = a[v];
+        const int ii = (int)(i % sa.Nx), jj = (int)((i / sa.Nx) % sa.Ny),
+                  kk = (int)(i / ((long)sa.Nx * sa.Ny));
+        const bool plane = (sa.sponge_face[0] && ii == 0) || (sa.sponge_face[1] && ii == sa.Nx - 1) ||
+                           (sa.sponge_face[2] && jj == 0) || (sa.sponge_face[3] && jj == sa.Ny - 1) ||
+                           (sa.sponge_face[4] && kk == 0) || (sa.sponge_face[5] && kk == sa.Nz - 1);
+        if (plane) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                F.qstale[v][i] = F.q[v][i];
+                F.q[v][i] = a[v];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- host launchers
+
+template <typename real>
+hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                        int threads, int LX, const LineMap &lm, const Fields<real> &F,
+                        const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
+                        hipStream_t st)
+{
+    if (M != SEG_DERIV) return hipErrorInvalidValue;
+    int use_k = sa.rkStep > 1;
+    if (which == 0)
+        return uni ? launch_xpass<real, true>(d1, d2, threads, LX, lm, F, ph, use_k, sa, st)
+                   : launch_xpass<real, false>(d1, d2, threads, LX, lm, F, ph, use_k, sa, st);
+    if (which == 1)
+        return uni ? launch_ypass<real, true>(d1, d2, threads, LX, lm, F, ph, use_k, st)
+                   : launch_ypass<real, false>(d1, d2, threads, LX, lm, F, ph, use_k, st);
+    return uni ? launch_zpass<real, true>(d1, d2, threads, LX, lm, F, ph, sa, sc, st)
+               : launch_zpass<real, false>(d1, d2, threads, LX, lm, F, ph, sa, sc, st);
+}
+
+template <typename real>
+hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, int threads, int LX, int map,
+                          const LineMap &lm, const Ptr5<real> &io, hipStream_t st)
+{
+    dim3 grid((lm.nlines + LX - 1) / LX), block(threads);
+    size_t shm = 2 * (size_t)threads * sizeof(real);
+    if (uni) {
+        if (M != SEG_FILTER) return hipErrorInvalidValue;
+        if (map == MAP_ROW)
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW>), grid, block, shm, st, fo, lm, io);
+        else
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL>), grid, block, shm, st, fo, lm, io);
+    } else {
+        if (M != SEG_TABLE) return hipErrorInvalidValue;
+        if (map == MAP_ROW)
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW>), grid, block, shm, st, fo, lm, io);
+        else
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL>), grid, block, shm, st, fo, lm, io);
+    }
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_reduce(const Fields<real> &F, const Phys<real> &ph, long N, int diag,
+                         double *part, int nb, double *out, StepScalars *sc, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_reduce_state<real>), dim3(nb), dim3(RED_THREADS), 0, st, F, ph, N, diag,
+                       part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st, part, nb, out, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt(StepScalars *sc, double cfl, double dt_fixed, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, sc, cfl, dt_fixed);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,
+                         const StepScalars *sc, double avgT, double epsP, double spongeP,
+                         hipStream_t st)
+{
+    long N = (long)sa.Nx * sa.Ny * sa.Nz;
+    int nb = (int)((N + 255) / 256);
+    if (nb > 65535) nb = 65535;
+    hipLaunchKernelGGL((k_sponge<real>), dim3(nb), dim3(256), 0, st, F, ph, sa, sc, avgT, epsP,
+                       spongeP);
+    return hipGetLastError();
+}
+
+#define CS3D_INST(real)                                                                          \
+    template hipError_t launch_sweep<real>(int, const OpDev<real> *, const OpDev<real> *, int, int, \
+                                           int, int, const LineMap &, const Fields<real> &,       \
+                                           const Phys<real> &, const StageArgs &,               \
+                                           const StepScalars *, hipStream_t);                   \
+    template hipError_t launch_filter5<real>(const OpDev<real> *, int, int, int, int, int,       \
+                                             const LineMap &, const Ptr5<real> &, hipStream_t);  \
+    template hipError_t launch_reduce<real>(const Fields<real> &, const Phys<real> &, long, int, \
+                                            double *, int, double *, StepScalars *, hipStream_t); \
+    template hipError_t launch_sponge<real>(const Fields<real> &, const Phys<real> &,            \
+                                            const StageArgs &, const StepScalars *, double,     \
+                                            double, double, hipStream_t);
+CS3D_INST(float)
+CS3D_INST(double)
+
+}  // namespace cs3d

@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the compact RHS pipeline on one MI355X.
+
+Workload (one "step" = one RK4 time step = 4 RHS stages + the 3-sweep filter):
+Taylor-Green vortex, 256^3 periodic on [0, 2pi)^3, RK4, 8th-order filter
+every step (alpha_f 0.49), CFL 0.5, Re 1600, Mach 0.1, gamma 1.4, Pr 0.71,
+isothermal density; FP32 bulk arithmetic.  The same workload in FP64 is
+timed beside it.  Inputs are resident in HBM before the timed region.
+
+value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
+% roof = value * 322 B / 8 TB/s (322 B = algorithmic bytes per point-stage
+         of RK4 + filter in FP32, 644 B in FP64).
+roofline = the dominant kernel class, timed with HIP events recorded on the
+         solver's own stream around every launch of the timed region.
+cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
+         with the reference's flags) timed on this host's cores on a
+         bounded sample.
+
+Only one GPU: --gpus N with N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3dcompactsolver_amd"))
+
+import cs3d  # noqa: E402
+
+METRIC = "Mpoint-RK-stages/s at 256^3 FP32 vs FP64 (% HBM roof); adjoint/forward ratio"
+HBM_PEAK = 8.0e12
+# algorithmic field accesses per point, per launch of each kernel class
+# (X sweep 5 reads + 9 writes, Y 12 + 9, Z 28 + 10, filter sweep 5 + 5,
+# post-step reduction 5 reads)
+ACCESSES = {"x_sweep": 14, "y_sweep": 21, "z_sweep": 38, "filter": 10, "reduce": 5}
+BYTES_PER_POINT_STAGE = {cs3d.FP32: 322, cs3d.FP64: 644}
+
+
+def tgv_ic(N, L, mach=0.1):
+    x = np.arange(N) * (L / (N - 1))
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    U = np.sin(X) * np.cos(Y) * np.cos(Z)
+    V = -np.cos(X) * np.sin(Y) * np.cos(Z)
+    W = np.zeros_like(U)
+    p0 = 1.0 / (1.4 * mach ** 2)
+    p = p0 + (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0) / 16.0
+    rho = p / p0
+    return rho, U, V, W, p
+
+
+def tgv_kw(N):
+    L = 2 * np.pi - 2 * np.pi / N
+    # mu(T0) = 1/Re with T0 = 100 and mu = mu_ref T^0.76 (IdealGas.hpp:49-51)
+    return dict(Lx=L, Ly=L, Lz=L, alpha_f=0.49, mu_ref=(1 / 1600) / 100 ** 0.76, Pr=0.71,
+                cfl=0.5, filter_step=1, rk=4), L
+
+
+def run_gpu(N, prec, steps, warmup, ic):
+    kw, _ = tgv_kw(N)
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=prec, **kw))
+    g.set_initial_conditions(*ic)
+    g.step(warmup)
+    g.synchronize()
+    g.profile(True)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    prof = g.profile_report()
+    g.profile(False)
+    ke = g.kinetic_energy()
+    g.close()
+    ms = (t1 - t0) / steps * 1e3
+    mpts = N ** 3 * 4 / (ms * 1e-3) / 1e6
+    return ms, mpts, prof, ke
+
+
+def roofline(prof, N, prec):
+    s = 4 if prec == cs3d.FP32 else 8
+    best = max((k for k in ACCESSES), key=lambda k: prof[k][0])
+    tot_ms, cnt = prof[best]
+    avg_s = tot_ms / cnt * 1e-3
+    alg_bytes = ACCESSES[best] * s * N ** 3
+    achieved = alg_bytes / avg_s / 1e9
+    return best, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
+                  "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
+                  "traffic": None, "kernel": best,
+                  "algorithmic_bytes_per_launch": alg_bytes,
+                  "avg_launch_ms": round(tot_ms / cnt, 4)}
+
+
+def cpu_baseline(N, steps):
+    """FP64 OpenMP restatement of the reference (liboracle_fast.so, reference
+    compiler flags), TGV N^3, one warm-up step then `steps` timed steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    kw, L = tgv_kw(N)
+    o = po.Solver(po.make_config(N, N, N, **kw), fast=True)
+    o.set_ic(*tgv_ic(N, L))
+    o.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step()
+    t1 = time.perf_counter()
+    o.close()
+    mpts = N ** 3 * 4 * steps / (t1 - t0) / 1e6
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(mpts, 3), "unit": "Mpoint-RK-stages/s", "cores": threads,
+            "kind": "port",
+            "sample": f"Taylor-Green {N}^3 FP64 RK4 + filter, {steps} steps after 1 warm-up "
+                      f"({t1 - t0:.1f} s of CPU time on {threads} OpenMP threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-grid", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "decomposition is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    N = args.grid
+    kw, L = tgv_kw(N)
+    ic = tgv_ic(N, L)
+    ms32, v32, prof32, ke32 = run_gpu(N, cs3d.FP32, args.steps, args.warmup, ic)
+    best, roof = roofline(prof32, N, cs3d.FP32)
+    fp64 = None
+    if not args.no_fp64:
+        ms64, v64, prof64, ke64 = run_gpu(N, cs3d.FP64, args.steps, args.warmup, ic)
+        _, roof64 = roofline(prof64, N, cs3d.FP64)
+        fp64 = {"value": round(v64, 1), "ms_per_step": round(ms64, 3),
+                "hbm_roof_frac": round(v64 * 1e6 * BYTES_PER_POINT_STAGE[cs3d.FP64] / HBM_PEAK, 4),
+                "roofline": roof64,
+                "kinetic_energy": ke64,
+                "fp32_ke_rel_err": abs(ke32 - ke64) / abs(ke64)}
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_grid, args.cpu_steps)
+    line = {
+        "metric": METRIC,
+        "value": round(v32, 1),
+        "unit": "Mpoint-RK-stages/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Taylor-Green initial field)",
+        "config": {"workload": f"Taylor-Green vortex {N}^3 periodic, RK4, filter every step "
+                               "(alpha_f 0.49), CFL 0.5, Re 1600, Mach 0.1, FP32 bulk",
+                   "grid": [N, N, N], "rk": "RK4", "filter_step": 1},
+        "hbm_roof_frac": round(v32 * 1e6 * BYTES_PER_POINT_STAGE[cs3d.FP32] / HBM_PEAK, 4),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof32.items()},
+        "fp64": fp64,
+        "adjoint_forward_ratio": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

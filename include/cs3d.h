@@ -1,0 +1,194 @@
+/*
+ * cs3d.h -- C ABI of the MI355X compact-RHS library (lib3dcompact_hip.so).
+ *
+ * Drop-in boundary for the per-stage right-hand-side pipeline of
+ * emathew1/3DCompactSolver.  Each entry point names the reference interface
+ * it replaces (<file>:<line>).  All functions return CS3D_OK (0) or an error
+ * code; cs3d_last_error() gives the message of the calling thread's last
+ * failure.  Plain pointers and sizes only.
+ *
+ * Layout: fields are Nx*Ny*Nz arrays in the reference's XYZ order,
+ * index = k*Nx*Ny + j*Nx + i (Macros.hpp:8), unless a call says otherwise.
+ * Threading: one solver handle per thread at a time (one HIP stream each);
+ * calls on one handle are not re-entrant.
+ */
+#ifndef CS3D_H
+#define CS3D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CS3D_OK = 0,
+    CS3D_ERR_ARG = 1,          /* invalid argument or unsupported size */
+    CS3D_ERR_HIP = 2,          /* HIP runtime failure (allocation, launch) */
+    CS3D_ERR_UNSUPPORTED = 3,  /* configuration outside the GPU path */
+    CS3D_ERR_STATE = 4,        /* call out of order (e.g. step before ICs) */
+    CS3D_ERR_NONFINITE = 5     /* NaN/Inf met in a reduction (dt, diagnostics) */
+};
+
+/* BC.hpp:10 BCType */
+enum { CS3D_PERIODIC_SOLVE = 0, CS3D_DIRICHLET_SOLVE = 1 };
+/* BC.hpp:11 BCKind */
+enum {
+    CS3D_BC_PERIODIC = 0,
+    CS3D_BC_SPONGE = 1,
+    CS3D_BC_ADIABATIC_WALL = 2,
+    CS3D_BC_MOVING_ADIABATIC_WALL = 3,
+    CS3D_BC_INLET = 4
+};
+/* Derivatives.hpp:48 Direct */
+enum { CS3D_DIRX = 0, CS3D_DIRY = 1, CS3D_DIRZ = 2 };
+/* operator kinds */
+enum { CS3D_FILTER = 0, CS3D_DERIV1 = 1, CS3D_DERIV2 = 2 };
+/* arithmetic of the bulk path */
+enum { CS3D_FP64 = 0, CS3D_FP32 = 1 };
+/* time integrator (RK4.hpp, TVDRK3.hpp) */
+enum { CS3D_TVDRK3 = 3, CS3D_RK4 = 4 };
+/* pointer kinds */
+enum { CS3D_HOST = 0, CS3D_DEVICE = 1 };
+/* field layouts for the standalone operators */
+enum { CS3D_LAYOUT_XYZ = 0, CS3D_LAYOUT_REFERENCE = 1 };
+
+/*
+ * Solver configuration: the values the reference takes from Domain
+ * (Domain.hpp:16), BC (BC.hpp:18), TimeStepping (TimeStepping.hpp:21),
+ * the CSolver_AWS constructor (CSolver_AWS.hpp:119) and IdealGas
+ * (IdealGas.hpp:35), plus the GPU path's own choices.
+ */
+typedef struct {
+    int Nx, Ny, Nz;
+    double Lx, Ly, Lz;        /* x_i = i*L/(N-1) (Domain.hpp:30-45) */
+    int bc_type[3];           /* CS3D_PERIODIC_SOLVE / CS3D_DIRICHLET_SOLVE per direction */
+    int bc_kind[6];           /* X0, X1, Y0, Y1, Z0, Z1 */
+    double cfl;               /* TimeStepping::CFL */
+    int filter_step;          /* TimeStepping::filterStep */
+    int max_steps;            /* TimeStepping::maxTimeStep */
+    double max_time;          /* TimeStepping::maxTime */
+    double alpha_f;           /* filter parameter */
+    double mu_ref;            /* IdealGas::mu_ref */
+    double gamma;             /* IdealGas::gamma (1.4) */
+    double Pr;                /* IdealGas::Pr (reference default 0.7) */
+    int rk;                   /* CS3D_RK4 or CS3D_TVDRK3 */
+    int precision;            /* CS3D_FP64 or CS3D_FP32 */
+    int compensated;          /* FP32 only: compensated (Kahan) state and RK sums */
+    double sponge_frac[3];    /* sponge length / domain length (SpongeBC.hpp:57-59) */
+    double sponge_strength;   /* 6.0 */
+    double sponge_avgT;       /* 15.0 */
+    double sponge_epsP;       /* 0.005 */
+    double dt_fixed;          /* > 0: constant dt instead of the CFL dt */
+} cs3d_config;
+
+typedef struct cs3d_solver cs3d_solver;
+typedef struct cs3d_ops cs3d_ops;
+
+/* message of this thread's last failed call ("" if none) */
+const char *cs3d_last_error(void);
+/* library version string */
+const char *cs3d_version(void);
+
+/* Reference defaults (3DCompact.cpp:45-102 / IdealGas.hpp:39-40) for a grid. */
+int cs3d_config_default(cs3d_config *cfg, int Nx, int Ny, int Nz);
+
+/* new CSolver_AWS(dom, bc, ts, alphaF, mu_ref, blocksize, useTiming)
+ * (CSolver_AWS.hpp:119-185) + initializeSolverData (CSolver_AWS.cpp:3-196):
+ * allocates the device state on the current HIP device. */
+int cs3d_create(const cs3d_config *cfg, cs3d_solver **out);
+void cs3d_destroy(cs3d_solver *s);
+
+/* setInitialConditions (CSolver_AWS.cpp:199-504): host arrays rho0, U0, V0,
+ * W0, p0 in XYZ order. */
+int cs3d_set_initial_conditions(cs3d_solver *s, const double *rho0, const double *U0,
+                                const double *V0, const double *W0, const double *p0);
+
+/* preStep -> calcDtFromCFL (CSolver_AWS.cpp:507-551, 2897-2904) */
+int cs3d_pre_step(cs3d_solver *s);
+/* one RK stage: preSubStep + solveEqnSet + postSubStep (CSolver_AWS.cpp:2906-2921),
+ * AbstractRK::updateConservedData (RK4.hpp:35-118, TVDRK3.hpp:38-96),
+ * updateData (CSolver_AWS.cpp:2923-2928). */
+int cs3d_sub_step(cs3d_solver *s, int rkStep, int rkLast);
+/* postStep -> updateSponge (CSolver_AWS.cpp:1937-2031, 2930-2946) */
+int cs3d_post_step(cs3d_solver *s);
+/* nsteps of the executeSolverLoop body (RK4.hpp:120-170, TVDRK3.hpp:98-139) */
+int cs3d_step(cs3d_solver *s, int nsteps);
+/* the stage right-hand side alone, dt*RHS into the *k2 set (solveEqnSet) */
+int cs3d_rhs(cs3d_solver *s, int rkStep);
+
+/* Host copies of a conserved set: set 1 = *1 (state), 2 = *k, 3 = *k2, 4 = *2.
+ * Any output pointer may be NULL. */
+int cs3d_get_conserved(cs3d_solver *s, int set, double *rho, double *rhoU, double *rhoV,
+                       double *rhoW, double *rhoE);
+int cs3d_set_conserved(cs3d_solver *s, int set, const double *rho, const double *rhoU,
+                       const double *rhoV, const double *rhoW, const double *rhoE);
+/* U, V, W, p, T, mu, Amu, sos of the state (updateNonConservedData,
+ * CSolver_AWS.cpp:1895-1935); any pointer may be NULL */
+int cs3d_get_primitives(cs3d_solver *s, double *U, double *V, double *W, double *p,
+                        double *T, double *mu, double *Amu, double *sos);
+int cs3d_get_time(cs3d_solver *s, double *time, double *dt, int *timeStep);
+
+/* When enabled, the last stage of every step keeps the per-point inputs of
+ * the step diagnostics (the reference computes them in postStep,
+ * CSolver_AWS.cpp:2930-2934) and the post-step reduction forms them. */
+int cs3d_set_diagnostics(cs3d_solver *s, int enabled);
+
+/* calcTaylorGreenQuantities (CSolver_AWS.cpp:2175-2257), FP64 reductions:
+ * out = time, kineticEngSum, turbDissSum, enstrophySum, enstrophySum2Mu,
+ * meanMu, dilprime.  Valid after a completed step. */
+int cs3d_diag_taylor_green(cs3d_solver *s, double out[7]);
+/* calcTurbulenceQuantities (CSolver_AWS.cpp:2033-2173): out = time, tau,
+ * taylorReyn3, turbMach, meanTurbDiss, uprime, kolNu, rhoprime, dilprime,
+ * vortprime, meanKineticEng */
+int cs3d_diag_turbulence(cs3d_solver *s, double out[11]);
+/* volume-averaged kinetic energy of the current state, FP64 */
+int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
+
+int cs3d_synchronize(cs3d_solver *s);
+/* HIP stream the solver launches on (hipStream_t) */
+void *cs3d_stream(cs3d_solver *s);
+
+/* Per-launch timing with HIP events recorded on the solver's stream around
+ * every kernel launch (no reference counterpart: the reference's useTiming
+ * tic/toc, Utils.cpp:377-386, times host phases).  Classes: 0 X sweep,
+ * 1 Y sweep, 2 Z sweep, 3 filter sweeps, 4 reductions, 5 other (dt,
+ * sponge).  The report sums milliseconds and counts launches since the
+ * last enable or report, then restarts. */
+int cs3d_profile_enable(cs3d_solver *s, int enabled);
+int cs3d_profile_report(cs3d_solver *s, double ms[6], int counts[6]);
+
+/*
+ * Standalone compact operators: Derivatives(dom, bcType, dir)
+ * (Derivatives.hpp:54) + calc1stDerivField / calc2ndDerivField
+ * (Derivatives.cpp:3-96), and Filter(alphaF, dom, bcType, dir)
+ * (Filter.hpp:31) + filterField (Filter.cpp:129-171).
+ * CS3D_LAYOUT_REFERENCE uses the reference's per-direction layout
+ * (DIRX: XYZ, DIRY: YZX, DIRZ: ZXY, Derivatives.cpp:13,27,40).
+ * precision CS3D_FP64: in/out are double*; CS3D_FP32: float*.
+ * ptr_kind CS3D_HOST stages through device buffers; CS3D_DEVICE runs in
+ * place on device memory, ordered on the ops stream.
+ */
+int cs3d_ops_create(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int bc_type,
+                    int dir, double alpha_f, int precision, cs3d_ops **out);
+int cs3d_ops_apply(cs3d_ops *o, int kind, int layout, const void *in, void *out, int ptr_kind);
+int cs3d_ops_synchronize(cs3d_ops *o);
+void cs3d_ops_destroy(cs3d_ops *o);
+
+/*
+ * The host FP64 factorisation of one compact operator (no reference
+ * counterpart: the reference refactorises every line inside solveTri /
+ * cyclic, Utils.cpp:4-46).  Returns the tables the kernels run on, for
+ * inspection and for CPU checks of the factorisation:
+ * ints = {n, M, P, K, uniform, cyclic, NB, nreals}; reals = {scale, tap[5],
+ * bcl[4][7], bcr[4][7], m[n], iw[n], mu[n], pf[n], pb[n], phi[P*K],
+ * psi[P*K]}; segs = {sstart[P], slen[P]}.  With reals == NULL only ints is
+ * filled.
+ */
+int cs3d_plan_info(int kind, int bc_type, int n, double dh, double alpha_f, int precision,
+                   int ints[8], double *reals, size_t nreals, int *segs, size_t nsegs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -820,29 +820,29 @@ void or_rhs(or_solver *s, int rkStep)
     post_step_bc(s);
 }
 
-/* RK4.hpp:35-118 and TVDRK3.hpp:38-96 */
-static void update_conserved(or_solver *s)
+/* RK4.hpp:35-118 and TVDRK3.hpp:38-96 on plain arrays:
+ * a[0..4] = *1, a[5..9] = *k, a[10..14] = *k2, a[15..19] = *2 */
+void or_rk_update_arrays(int rk, int rkStep, long N, double **a)
 {
-    long N = s->N;
-    double **q1 = s->q[0], **qk = s->q[1], **k2 = s->q[2], **q2 = s->q[3];
-    if (s->c.rk == 4) {
-        const double a[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-        const double b[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
-        int st = s->rkStep;
+    double **q1 = a, **qk = a + 5, **k2 = a + 10, **q2 = a + 15;
+    if (rk == 4) {
+        const double ca[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+        const double cb[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
+        int st = rkStep;
         for (int v = 0; v < 5; v++) {
 #pragma omp parallel for
             for (long ip = 0; ip < N; ip++) {
                 if (st == 1)
-                    q2[v][ip] = q1[v][ip] + a[0] * k2[v][ip];
+                    q2[v][ip] = q1[v][ip] + ca[0] * k2[v][ip];
                 else
-                    q2[v][ip] += a[st - 1] * k2[v][ip];
-                if (st < 4) qk[v][ip] = q1[v][ip] + b[st - 1] * k2[v][ip];
+                    q2[v][ip] += ca[st - 1] * k2[v][ip];
+                if (st < 4) qk[v][ip] = q1[v][ip] + cb[st - 1] * k2[v][ip];
             }
         }
     } else {
         const double a2 = 3.0 / 4.0, a3 = 1.0 / 3.0, b2 = 1.0 / 4.0,
                      b3 = 2.0 / 3.0, c1 = 1.0, c2 = 1.0 / 4.0, c3 = 2.0 / 3.0;
-        int st = s->rkStep;
+        int st = rkStep;
         for (int v = 0; v < 5; v++) {
 #pragma omp parallel for
             for (long ip = 0; ip < N; ip++) {
@@ -855,6 +855,14 @@ static void update_conserved(or_solver *s)
             }
         }
     }
+}
+
+static void update_conserved(or_solver *s)
+{
+    double *a[20];
+    for (int set = 0; set < 4; set++)
+        for (int v = 0; v < 5; v++) a[set * 5 + v] = s->q[set][v];
+    or_rk_update_arrays(s->c.rk, s->rkStep, s->N, a);
 }
 
 /* filterConservedData, CSolver_AWS.cpp:1682-1893: round-robin direction

@@ -101,6 +101,10 @@ double or_time(const or_solver *s);
 int or_timestep(const or_solver *s);
 void or_set_dt_override(or_solver *s, double dt); /* <0: CFL (default) */
 
+/* RK4.hpp:35-118 / TVDRK3.hpp:38-96 on plain arrays:
+ * a[0..4] = *1, a[5..9] = *k, a[10..14] = *k2, a[15..19] = *2 */
+void or_rk_update_arrays(int rk, int rkStep, long N, double **a);
+
 /* accessors: which = 0..4 -> rho, rhoU, rhoV, rhoW, rhoE;
  * set: 1 = *1 (state), 2 = *k, 3 = *k2 (dt*RHS), 4 = *2 */
 double *or_field(or_solver *s, int set, int which);

@@ -89,6 +89,8 @@ def _load(path):
     L.or_line_rhs.argtypes = L.or_line_op.argtypes
     L.or_line_lhs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_double, _dp, _dp, _dp]
+    L.or_rk_update_arrays.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long,
+                                      ctypes.POINTER(_dp)]
     L.or_field_op.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_double,
                                                    ctypes.c_int, _dp, _dp]
     return L
@@ -139,6 +141,13 @@ def field_op(kind, direction, bctype, field, dh=1.0, alpha_f=0.49, ref_layout=Fa
     lib().or_field_op(kind, direction, bctype, Nx, Ny, Nz, dh, alpha_f,
                       1 if ref_layout else 0, _ptr(f), _ptr(out))
     return out
+
+
+def rk_update(rk, rk_step, arrays):
+    """arrays: list of 20 float64 arrays (*1, *k, *k2, *2 x 5 variables),
+    updated in place."""
+    ptrs = (_dp * 20)(*[_ptr(a) for a in arrays])
+    lib().or_rk_update_arrays(rk, rk_step, arrays[0].size, ptrs)
 
 
 # --------------------------------------------------------------- solver

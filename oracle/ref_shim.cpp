@@ -130,21 +130,22 @@ void ref_idealgas(int n, double mu_ref, const double *rho, const double *rhoU,
 
 // SpongeBC.hpp:35-154: sigma for the given face kinds (0 periodic,
 // 1 sponge).  The reference allocates sigma with new double[] and reads it
-// (fmax with the formula) before writing it.  The mmap threshold is pinned at
-// 128 KiB here, so an array of at least that size comes back as fresh zero
-// pages and the result is defined; callers keep N*8 >= 128 KiB.
+// (fmax with the formula) before writing it.  While the constructor runs,
+// glibc's M_PERTURB makes every allocation come back zero-filled (the value a
+// fresh page holds), so the result is defined.
 void ref_sponge_sigma(int Nx, int Ny, int Nz, double Lx, double Ly,
                       double Lz, const int *kinds, double *sigma_out)
 {
     QuietCout q;
-    mallopt(M_MMAP_THRESHOLD, 128 * 1024);
     Domain dom(Nx, Ny, Nz, Lx, Ly, Lz);
     IdealGas ig(&dom, 1.0);
     BC::BCKind kk[6];
     for (int f = 0; f < 6; f++) kk[f] = kinds[f] == 1 ? BC::SPONGE : BC::PERIODIC;
     BC bc(BC::PERIODIC_SOLVE, kk[0], kk[1], BC::PERIODIC_SOLVE, kk[2], kk[3],
           BC::PERIODIC_SOLVE, kk[4], kk[5]);
+    mallopt(M_PERTURB, 0xff);  // allocations filled with ~0xff = 0x00
     SpongeBC spg(&dom, &ig, &bc);
+    mallopt(M_PERTURB, 0);
     long N = (long)Nx * Ny * Nz;
     for (long i = 0; i < N; i++) sigma_out[i] = spg.sigma[i];
 }

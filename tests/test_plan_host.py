@@ -1,0 +1,124 @@
+"""The host factorisation the kernels run on (cs3d_plan_info), checked on the
+CPU: a numpy emulation of the kernels' segmented two-recurrence solve with the
+library's own tables must reproduce the oracle's Thomas / Sherman-Morrison
+operators (Utils.cpp:4-46, Derivatives.cpp:250-284, Filter.cpp:98-117).
+This pins the factorisation, the segmentation, the cross-segment carries and
+their truncation without a GPU."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import cs3d
+import pyoracle as po
+from helpers import relerr
+
+
+def plan(kind, bc, n, dh, af, prec):
+    L = cs3d.lib()
+    ints = (ctypes.c_int * 8)()
+    cs3d.check(L.cs3d_plan_info(kind, bc, n, dh, af, prec, ints, None, 0, None, 0))
+    nr = ints[7]
+    P = ints[2]
+    reals = np.zeros(nr)
+    segs = np.zeros(2 * P, dtype=np.int32)
+    cs3d.check(L.cs3d_plan_info(kind, bc, n, dh, af, prec, ints,
+                                reals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nr,
+                                segs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 2 * P))
+    n, M, P, K, uni, cyc, NB = list(ints)[:7]
+    r = reals
+    d = dict(n=n, M=M, P=P, K=K, uniform=uni, cyclic=cyc, NB=NB, scale=r[0], tap=r[1:6],
+             bcl=r[6:34].reshape(4, 7), bcr=r[34:62].reshape(4, 7))
+    o = 62
+    for name in ("m", "iw", "mu", "pf", "pb"):
+        d[name] = r[o:o + n]
+        o += n
+    d["phi"] = r[o:o + P * K].reshape(P, K)
+    o += P * K
+    d["psi"] = r[o:o + P * K].reshape(P, K)
+    d["sstart"] = segs[:P]
+    d["slen"] = segs[P:]
+    return d
+
+
+def emulate(p, kind, f):
+    """The kernels' arithmetic, lane by lane, in FP64."""
+    n, P, K = p["n"], p["P"], p["K"]
+    per = p["cyclic"]
+    t = p["tap"]
+    H = 4 if kind == 0 else 2
+    g = np.pad(f, H, mode="wrap") if per else np.pad(f, H, mode="edge")
+    c = lambda o: g[H + o:H + o + n]
+    if kind == 1:
+        r = t[1] * (c(1) - c(-1)) + t[2] * (c(2) - c(-2))
+    elif kind == 2:
+        r = t[0] * c(0) + t[1] * (c(-1) + c(1)) + t[2] * (c(-2) + c(2))
+    else:
+        r = t[0] * c(0) + sum(t[k] * (c(-k) + c(k)) for k in range(1, 5))
+    for i in range(p["NB"]):
+        r[i] = p["bcl"][i] @ f[0:7]
+        r[n - p["NB"] + i] = p["bcr"][i] @ f[n - 7:n]
+    y = r.copy()
+    ends = np.zeros(P)
+    for gi in range(P):
+        s, ln = p["sstart"][gi], p["slen"][gi]
+        for i in range(s + 1, s + ln):
+            y[i] -= p["m"][i] * y[i - 1]
+        ends[gi] = y[s + ln - 1]
+    for gi in range(P):
+        s, ln = p["sstart"][gi], p["slen"][gi]
+        C = 0.0
+        for k in range(1, K + 1):
+            gg = gi - k
+            if per:
+                gg %= P
+            elif gg < 0:
+                break
+            C += p["phi"][gi, k - 1] * ends[gg]
+        y[s:s + ln] += p["pf"][s:s + ln] * C
+    x = np.zeros(n)
+    starts = np.zeros(P)
+    for gi in range(P):
+        s, ln = p["sstart"][gi], p["slen"][gi]
+        carry = 0.0
+        for i in range(s + ln - 1, s - 1, -1):
+            carry = p["iw"][i] * y[i] + p["mu"][i] * carry
+            x[i] = carry
+        starts[gi] = x[s]
+    out = np.zeros(n)
+    for gi in range(P):
+        s, ln = p["sstart"][gi], p["slen"][gi]
+        D = 0.0
+        for k in range(1, K + 1):
+            gg = gi + k
+            if per:
+                gg %= P
+            elif gg >= P:
+                break
+            D += p["psi"][gi, k - 1] * starts[gg]
+        out[s:s + ln] = p["scale"] * (x[s:s + ln] + p["pb"][s:s + ln] * D)
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 9, 12, 42, 64, 128, 256])
+@pytest.mark.parametrize("bc", [0, 1])
+@pytest.mark.parametrize("kind", [1, 2, 0])
+@pytest.mark.parametrize("prec", [cs3d.FP64, cs3d.FP32])
+def test_emulated_plan_matches_oracle(n, bc, kind, prec):
+    rng = np.random.default_rng(n + 100 * bc + 10 * kind)
+    dh = 0.05
+    p = plan(kind, bc, n, dh, 0.49, prec)
+    f = rng.standard_normal(n)
+    ref = po.line_op(kind, bc, f, dh=dh, alpha_f=0.49)
+    got = emulate(p, kind, f)
+    # FP32 plans drop cross-segment terms below 1e-10; FP64 plans below 1e-20
+    tol = 1e-9 if prec == cs3d.FP32 else 1e-13
+    assert relerr(got, ref) < tol
+
+
+def test_periodic_plans_are_uniform_and_short():
+    for n in (64, 128, 256):
+        p = plan(1, 0, n, 0.1, 0.49, cs3d.FP32)
+        assert p["uniform"] and p["M"] == 4 and p["P"] == n // 4 and p["K"] <= 8
+        p = plan(0, 0, n, 0.1, 0.49, cs3d.FP64)
+        assert p["uniform"] and p["M"] == 8
