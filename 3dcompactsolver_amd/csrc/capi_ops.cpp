@@ -98,12 +98,22 @@ int cs3d_ops_apply(cs3d_ops *o, int kind, int layout, const void *in, void *out,
     } else if (ptr_kind != CS3D_DEVICE) {
         return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: bad ptr_kind");
     }
+    // an earlier asynchronous failure is reported as such, not as this launch's
+    e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(CS3D_ERR_HIP, hip_msg(e, "cs3d_ops_apply: earlier HIP failure"));
     if (fp32)
         e = launch_field_op<float>(o->f[kind], map, lm, (const float *)din, (float *)dout, o->stream);
     else
         e = launch_field_op<double>(o->d[kind], map, lm, (const double *)din, (double *)dout,
                                     o->stream);
-    if (e != hipSuccess) return set_error(CS3D_ERR_HIP, hip_msg(e, "cs3d_ops_apply: launch"));
+    if (e != hipSuccess)
+        return set_error(CS3D_ERR_HIP,
+                         hip_msg(e, "cs3d_ops_apply: launch") + " (n=" +
+                             std::to_string(fp32 ? o->f[kind].host.n : o->d[kind].host.n) +
+                             " P=" + std::to_string(fp32 ? o->f[kind].host.P : o->d[kind].host.P) +
+                             " threads=" +
+                             std::to_string(fp32 ? o->f[kind].threads : o->d[kind].threads) + ")");
     if (ptr_kind == CS3D_HOST) {
         e = hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, o->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(o->stream);

@@ -34,9 +34,11 @@ OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool f
     // cross-segment terms below this weight are dropped (far below the
     // rounding of the working precision)
     const double tol = fp32 ? 1e-10 : 1e-20;
-    int M = kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
-    if (!periodic || n % M != 0) M = SEG_TABLE;
-    return build_plan(kind, periodic, n, dh, alpha_f, M, tol);
+    // uniform kernels only for periodic lines that split into equal
+    // segments of the uniform length; everything else takes the table kernels
+    const int MU = kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
+    if (periodic && n % MU == 0) return build_plan(kind, periodic, n, dh, alpha_f, MU, tol, true);
+    return build_plan(kind, periodic, n, dh, alpha_f, SEG_TABLE, tol, false);
 }
 
 template <typename real>
@@ -174,6 +176,7 @@ template <int KIND, typename real, int M, bool UNI>
 static hipError_t launch_kmu(const PlanDev<real> &pl, int map, const LineMap &lm,
                              const real *in, real *out, hipStream_t st)
 {
+    if (pl.LX < 1 || pl.threads < 1 || pl.threads > 1024) return hipErrorInvalidConfiguration;
     dim3 grid((lm.nlines + pl.LX - 1) / pl.LX), block(pl.threads);
     size_t shm = 2 * (size_t)pl.threads * sizeof(real);
     if (map == MAP_ROW)
@@ -191,10 +194,10 @@ static hipError_t launch_k(const PlanDev<real> &pl, int map, const LineMap &lm,
 {
     constexpr int MU = KIND == OP_FILTER ? SEG_FILTER : SEG_DERIV;
     if (pl.host.uniform) {
-        if (pl.host.M != MU) return hipErrorInvalidValue;
+        if (pl.host.M != MU) return hipErrorInvalidConfiguration;
         return launch_kmu<KIND, real, MU, true>(pl, map, lm, in, out, st);
     }
-    if (pl.host.M != SEG_TABLE) return hipErrorInvalidValue;
+    if (pl.host.M != SEG_TABLE) return hipErrorInvalidConfiguration;
     return launch_kmu<KIND, real, SEG_TABLE, false>(pl, map, lm, in, out, st);
 }
 

@@ -61,7 +61,7 @@ static void segment(int n, int M, OpHost &h, bool want_uniform)
 }
 
 OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f,
-                  int M, double tol)
+                  int M, double tol, bool allow_uniform)
 {
     OpHost h;
     h.n = n;
@@ -212,7 +212,7 @@ OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f,
         h.scale = 1.0;
     }
 
-    segment(n, M, h, periodic != 0);
+    segment(n, M, h, allow_uniform && periodic != 0);
     int P = h.P;
     if (!periodic) {
         if (h.slen[0] < h.NB || h.slen[P - 1] != M)

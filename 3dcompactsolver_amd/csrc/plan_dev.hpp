@@ -11,7 +11,10 @@
 
 namespace cs3d {
 
-OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f, int M, double tol);
+// allow_uniform = false forces the per-point table path even when the
+// segments come out equal (the caller picked M for the table kernels).
+OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f, int M, double tol,
+                  bool allow_uniform);
 // the plan the library uses for an operator in a precision (segment length
 // and truncation chosen as the kernels expect)
 OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32);

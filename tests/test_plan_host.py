@@ -122,3 +122,21 @@ def test_periodic_plans_are_uniform_and_short():
         assert p["uniform"] and p["M"] == 4 and p["P"] == n // 4 and p["K"] <= 8
         p = plan(0, 0, n, 0.1, 0.49, cs3d.FP64)
         assert p["uniform"] and p["M"] == 8
+
+
+@pytest.mark.parametrize("kind", [1, 2, 0])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_plan_segment_length_matches_a_compiled_kernel(kind, bc):
+    """Uniform plans use the kernels' uniform segment length (4 for the
+    derivatives, 8 for the filter); every other plan uses the table kernels'
+    length 4 and is never marked uniform (a ragged periodic filter line of 12
+    points splits into equal 4-point segments but has no uniform 4-point
+    filter kernel)."""
+    mu = 8 if kind == 0 else 4
+    for n in (8, 9, 12, 16, 20, 24, 36, 40, 64, 100, 256):
+        p = plan(kind, bc, n, 0.1, 0.49, cs3d.FP32)
+        if p["uniform"]:
+            assert bc == 0 and p["M"] == mu and n % mu == 0
+        else:
+            assert p["M"] == 4
+            assert not (bc == 0 and n % mu == 0)
