@@ -77,6 +77,8 @@ int cs3d_ops_apply(cs3d_ops *o, int kind, int layout, const void *in, void *out,
     int map = lm.sp == 1 ? MAP_ROW : MAP_COL;
     size_t N = (size_t)o->Nx * o->Ny * o->Nz;
     size_t bytes = N * (fp32 ? sizeof(float) : sizeof(double));
+    if (bytes > 0xFFFFFFFFull)
+        return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: field above 4 GiB (32-bit offsets)");
     const void *din = in;
     void *dout = out;
     hipError_t e;

@@ -3,9 +3,9 @@
 // A workgroup holds LX lines x P segments; each lane owns one segment of at
 // most M points of one line, in registers.  The two first-order recurrences of
 // the factorised compact matrix (plan.hpp) run segment-locally, and the
-// carries between segments go through a small LDS exchange buffer (one
-// barrier per recurrence, double-buffered so that the next write never races
-// the previous reads).
+// carries between segments go through an LDS exchange ring: one barrier per
+// recurrence, shared by every solve of a batch (solve_batch), double-buffered
+// so that the next write never races the previous reads.
 //
 // Lane layouts:
 //   MAP_COL: tid = g * LX + line.  Consecutive lanes are consecutive lines;
@@ -14,6 +14,22 @@
 //            consecutive addresses.
 //   MAP_ROW: tid = line * P + g.  Consecutive lanes are consecutive segments
 //            of one line; used where lines are contiguous (X sweeps).
+//
+// Addressing: every field of a sweep has the same layout, so a lane computes
+// the byte offsets of its window once (window_offsets) and every load and
+// store after that is one instruction with a 32-bit offset from the field's
+// base address.  Fields must therefore stay below 4 GiB each (checked on the
+// host).
+//
+// Exchange ring: per line, a row of P + KR positions (KR >= the K of every
+// operator using the ring, row padded to spread the lines of a wavefront over
+// the LDS banks); each position holds the values of the whole batch side by
+// side.  The forward carry of segment g
+// reads ring[KR + g - k] for k = 1..K: the primary values sit at KR + g and
+// the last KR segments also write a copy at KR + g - P (the cyclic wrap; zero
+// for non-periodic lines).  The backward carry reads ring[g + k]: primaries
+// at g, and the first KR segments copy to P + g.  No index ever wraps at run
+// time, so the reads are plain LDS loads at constant offsets.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,31 +46,96 @@ struct LineMap {
     int n1;
     int nlines;
     long s1, s2, sp;
-    __device__ __forceinline__ long base(int L) const
+    __host__ __device__ __forceinline__ long base(int L) const
     {
         return (long)(L % n1) * s1 + (long)(L / n1) * s2;
     }
 };
 
+// Multiplicative inverse of an odd number modulo 32.
+__host__ __device__ constexpr int inv_mod32(int s)
+{
+    int x = 1;
+    for (int i = 0; i < 32; i++) {
+        if (((s * x) & 31) == 1) return x;
+        x += 2;
+    }
+    return 1;
+}
+
+// Ring row length (in positions) for P segments, margin KR, LX lines per
+// workgroup and an odd position stride S: at least P + KR, and chosen so that
+// row * S = 32 / LX (mod 32).  Lanes of a half-wavefront then hit distinct
+// LDS banks: in MAP_COL they are 32 / LX consecutive segments of LX lines, at
+// offsets line * row * S + g * S; in MAP_ROW consecutive segments of one
+// line, at offsets g * S.
+__host__ __device__ inline int ring_row(int P, int KR, int LX, int S)
+{
+    const int want = LX >= 32 ? 1 : 32 / LX;
+    const int r = P + KR;
+    const int target = (want * inv_mod32(S)) & 31;
+    return r + (((target - r) % 32) + 32) % 32;
+}
+
+// Elements of one parity of the exchange buffer, for batches of up to
+// `planes` values per position.
+__host__ __device__ inline int ring_cap(int P, int KR, int LX, int planes)
+{
+    int best = 0;
+    for (int S = 1; S <= (planes | 1); S += 2) {
+        const int e = ring_row(P, KR, LX, S) * S;
+        best = e > best ? e : best;
+    }
+    return LX * best;
+}
+
+// LDS bytes of a workgroup's exchange buffers (two parities).
+__host__ __device__ inline size_t ring_bytes(int P, int KR, int LX, int planes, int esize)
+{
+    return 2 * (size_t)ring_cap(P, KR, LX, planes) * esize;
+}
+
+// Workgroup shape of a line-solve launch: LX lines x P segments per
+// workgroup, and the LDS bytes of its exchange ring.
+struct LaunchGeom {
+    int threads;
+    int LX;
+    size_t shm;
+};
+
+// LDS of one workgroup on gfx950
+constexpr size_t LDS_BYTES = 160 * 1024;
+
+// LX = max_threads / P lines (at most 64, fewer when the exchange ring would
+// not fit the LDS) per workgroup, ring margin kr, `planes` values per ring
+// position; LX = 0 when a line does not fit.
+inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esize)
+{
+    LaunchGeom g;
+    g.LX = P > 0 ? max_threads / P : 0;
+    if (g.LX > 64) g.LX = 64;
+    while (g.LX > 0 && ring_bytes(P, kr, g.LX, planes, esize) > LDS_BYTES) g.LX--;
+    g.threads = g.LX * P;
+    g.shm = g.LX > 0 ? ring_bytes(P, kr, g.LX, planes, esize) : 0;
+    return g;
+}
+
 template <typename real, int MAP>
 struct Lane {
     int g, line, P, LX;
-    int s, len;  // first point and length of this lane's segment
-    int par;     // exchange buffer parity
-    real *xb;    // LDS exchange buffer, 2 * P * LX entries
-
-    __device__ __forceinline__ int slot(int gg, int ln) const
-    {
-        return MAP == MAP_COL ? gg * LX + ln : ln * P + gg;
-    }
+    int s, len;   // first point and length of this lane's segment
+    int KR;       // ring margin (>= every K read through the ring)
+    int cap;      // elements of one parity of the exchange buffer
+    int par;      // exchange buffer parity
+    real *xb;     // LDS exchange buffers
 };
 
 template <typename real, int MAP>
-__device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restrict__ op,
-                                                     int P, int LX, real *xb)
+__device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restrict__ op, int P,
+                                                     int LX, int KR, int planes, real *xb)
 {
     Lane<real, MAP> c;
-    int tid = threadIdx.x;
+    const int tid = threadIdx.x;
     if (MAP == MAP_COL) {
         c.g = tid / LX;
         c.line = tid - c.g * LX;
@@ -64,6 +145,8 @@ __device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restri
     }
     c.P = P;
     c.LX = LX;
+    c.KR = KR;
+    c.cap = ring_cap(P, KR, LX, planes);
     c.par = 0;
     c.xb = xb;
     if (op->uniform) {
@@ -76,35 +159,14 @@ __device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restri
     return c;
 }
 
-// Publish v for this lane's segment; after the barrier return
-// sum_{k=1..K} w[k-1] * v(segment g - dir*k), wrapping when cyclic.
-template <typename real, int MAP>
-__device__ __forceinline__ real exchange(Lane<real, MAP> &c, real v, const real *__restrict__ w,
-                                         int K, int dir, int cyclic)
-{
-    real *b = c.xb + c.par * c.P * c.LX;
-    c.par ^= 1;
-    b[c.slot(c.g, c.line)] = v;
-    __syncthreads();
-    real acc = 0;
-    for (int k = 1; k <= K; k++) {
-        int gg = c.g - dir * k;
-        if (cyclic) {
-            gg += gg < 0 ? c.P : 0;
-            gg -= gg >= c.P ? c.P : 0;
-        } else if (gg < 0 || gg >= c.P) {
-            break;
-        }
-        acc += w[k - 1] * b[c.slot(gg, c.line)];
-    }
-    return acc;
-}
+// ------------------------------------------------------------ addressing
 
-// Load the window f[s-HW .. s+M+HW) of one line; periodic lines wrap,
-// non-periodic ones clamp (clamped values are never used by a stencil).
-template <typename real, int M, int HW, typename Src>
-__device__ __forceinline__ void load_window(Src src, int s, int n, int periodic,
-                                            real (&w)[M + 2 * HW])
+// Byte offsets of the window f[s-HW .. s+M+HW) of the line at element
+// `base` with point stride `sp`; periodic lines wrap, non-periodic ones clamp
+// (clamped entries are never used by a stencil).
+template <typename real, int M, int HW>
+__device__ __forceinline__ void window_offsets(unsigned base, unsigned sp, int s, int n,
+                                               int periodic, unsigned (&off)[M + 2 * HW])
 {
 #pragma unroll
     for (int t = 0; t < M + 2 * HW; t++) {
@@ -116,9 +178,31 @@ __device__ __forceinline__ void load_window(Src src, int s, int n, int periodic,
             p = p < 0 ? 0 : p;
             p = p >= n ? n - 1 : p;
         }
-        w[t] = src(p);
+        off[t] = (base + (unsigned)p * sp) * (unsigned)sizeof(real);
     }
 }
+
+template <typename real>
+__device__ __forceinline__ real ldo(const real *__restrict__ f, unsigned boff)
+{
+    return *(const real *)((const char *)f + boff);
+}
+
+template <typename real>
+__device__ __forceinline__ void sto(real *__restrict__ f, unsigned boff, real v)
+{
+    *(real *)((char *)f + boff) = v;
+}
+
+template <typename real, int W>
+__device__ __forceinline__ void load_w(const real *__restrict__ f, const unsigned (&off)[W],
+                                       real (&w)[W])
+{
+#pragma unroll
+    for (int t = 0; t < W; t++) w[t] = ldo(f, off[t]);
+}
+
+// ------------------------------------------------------------ stencils
 
 // Interior stencil B f for the M points of the segment.  The window w holds
 // f[s-HW+t]; the stencil needs HW >= its own half width.
@@ -184,67 +268,185 @@ __device__ __forceinline__ void boundary_rows(const OpDev<real> *__restrict__ op
     }
 }
 
-// Solve A x = r in place for the segment (r holds B f on entry).
-template <typename real, int M, bool UNI, int MAP>
-__device__ __forceinline__ void tri_solve(const OpDev<real> *__restrict__ op,
-                                          Lane<real, MAP> &c, real (&r)[M])
+// B f on the segment, boundary rows included.
+template <int KIND, typename real, int M, int HW, bool UNI, int MAP>
+__device__ __forceinline__ void rhs(const OpDev<real> *__restrict__ op, const Lane<real, MAP> &c,
+                                    const real (&w)[M + 2 * HW], real (&r)[M])
 {
-    const int K = op->K, cyc = op->cyclic;
+    stencil<KIND, real, M, HW>(op, w, r);
+    if (!UNI) boundary_rows<KIND, real, M, HW>(op, c.s, c.len, w, r);
+}
+
+// ------------------------------------------------------------ recurrences
+
+// forward recurrence y_t = r_t - m_t y_{t-1} with zero carry; returns the
+// segment's last value
+template <typename real, int M, bool UNI, int MAP>
+__device__ __forceinline__ real fwd_local(const OpDev<real> *__restrict__ op,
+                                          const Lane<real, MAP> &c, real (&r)[M])
+{
     if (UNI) {
-        const real m = op->um, mu = op->umu;
+        const real m = op->um;
 #pragma unroll
         for (int t = 1; t < M; t++) r[t] = r[t] - m * r[t - 1];
-        real C = exchange(c, r[M - 1], op->uphi, K, 1, cyc);
-#pragma unroll
-        for (int t = 0; t < M; t++) r[t] = r[t] + op->upf[t] * C;
-#pragma unroll
-        for (int t = M - 2; t >= 0; t--) r[t] = r[t] + mu * r[t + 1];
-        real D = exchange(c, r[0], op->upsi, K, -1, cyc);
-        const real sc = op->scale;
-#pragma unroll
-        for (int t = 0; t < M; t++) r[t] = sc * (r[t] + op->upb[t] * D);
+        return r[M - 1];
     } else {
-        const int s = c.s, len = c.len;
-        const real *__restrict__ tm = op->m + s;
-        const real *__restrict__ tiw = op->iw + s;
-        const real *__restrict__ tmu = op->mu + s;
-        const real *__restrict__ tpf = op->pf + s;
-        const real *__restrict__ tpb = op->pb + s;
+        const real *__restrict__ tm = op->m + c.s;
         real e = r[0];
 #pragma unroll
         for (int t = 1; t < M; t++) {
-            if (t < len) {
+            if (t < c.len) {
                 r[t] = r[t] - tm[t] * r[t - 1];
                 e = r[t];
             }
         }
-        real C = exchange(c, e, op->phi + (long)c.g * K, K, 1, cyc);
+        return e;
+    }
+}
+
+// forward fix-up y_t += pf_t C, then the backward recurrence
+// x_t = iw_t y_t + mu_t x_{t+1} with zero carry; returns the first value
+template <typename real, int M, bool UNI, int MAP>
+__device__ __forceinline__ real bwd_local(const OpDev<real> *__restrict__ op,
+                                          const Lane<real, MAP> &c, real (&r)[M], real C)
+{
+    if (UNI) {
+        // periodic factorisation: iw = 1
+        const real mu = op->umu;
+#pragma unroll
+        for (int t = 0; t < M; t++) r[t] = r[t] + op->upf[t] * C;
+#pragma unroll
+        for (int t = M - 2; t >= 0; t--) r[t] = r[t] + mu * r[t + 1];
+        return r[0];
+    } else {
+        const real *__restrict__ tiw = op->iw + c.s;
+        const real *__restrict__ tmu = op->mu + c.s;
+        const real *__restrict__ tpf = op->pf + c.s;
         real carry = 0;
 #pragma unroll
         for (int t = M - 1; t >= 0; t--) {
-            if (t < len) {
-                real y = r[t] + tpf[t] * C;
+            if (t < c.len) {
+                const real y = r[t] + tpf[t] * C;
                 carry = tiw[t] * y + tmu[t] * carry;
                 r[t] = carry;
             }
         }
-        real D = exchange(c, r[0], op->psi + (long)c.g * K, K, -1, cyc);
-        const real sc = op->scale;
-#pragma unroll
-        for (int t = 0; t < M; t++)
-            if (t < len) r[t] = sc * (r[t] + tpb[t] * D);
+        return r[0];
     }
 }
 
-// The whole operator on one segment: window -> out.
-template <int KIND, typename real, int M, int HW, bool UNI, int MAP>
-__device__ __forceinline__ void compact_op(const OpDev<real> *__restrict__ op,
-                                           Lane<real, MAP> &c, const real (&w)[M + 2 * HW],
-                                           real (&r)[M])
+// backward fix-up and output scale: out_t = scale (x_t + pb_t D)
+template <typename real, int M, bool UNI, int MAP>
+__device__ __forceinline__ void finish(const OpDev<real> *__restrict__ op,
+                                       const Lane<real, MAP> &c, real (&r)[M], real D)
 {
-    stencil<KIND, real, M, HW>(op, w, r);
-    if (!UNI) boundary_rows<KIND, real, M, HW>(op, c.s, c.len, w, r);
-    tri_solve<real, M, UNI, MAP>(op, c, r);
+    const real sc = op->scale;
+    if (UNI) {
+#pragma unroll
+        for (int t = 0; t < M; t++) r[t] = sc * (r[t] + op->upb[t] * D);
+    } else {
+        const real *__restrict__ tpb = op->pb + c.s;
+#pragma unroll
+        for (int t = 0; t < M; t++)
+            if (t < c.len) r[t] = sc * (r[t] + tpb[t] * D);
+    }
+}
+
+// ------------------------------------------------------------ exchange
+
+// Carry weights of one operator for this lane: uniform plans share one row,
+// table plans hold one row of K per segment.
+template <typename real, bool UNI, bool FWD, int MAP>
+__device__ __forceinline__ const real *carry_weights(const OpDev<real> *__restrict__ op,
+                                                     const Lane<real, MAP> &c)
+{
+    if (UNI) return FWD ? op->uphi : op->upsi;
+    return (FWD ? op->phi : op->psi) + (long)c.g * op->K;
+}
+
+// Publish v[0..N1+N2) for this lane's segment; after the barrier,
+// out[i] = sum_{k=1..K} w[k-1] v_i(segment g -+ k), with the weights and K of
+// operator 1 for i < N1 and of operator 2 after that.  The values of one
+// ring position sit side by side at an odd stride S (entry (q, i) at
+// q * S + i), so every read of a lane is at a constant offset from one
+// address.  Uniform plans store zero weights past K, so on lines of at least
+// KR >= KC segments their first KC terms run unrolled without branches;
+// terms past KC, shorter lines and table plans run in a plain loop.
+template <int N1, int N2, bool FWD, int KC, bool UNI, typename real, int MAP>
+__device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 + N2],
+                                         real (&out)[N1 + N2], const real *__restrict__ w1,
+                                         int K1, const real *__restrict__ w2, int K2, int cyc)
+{
+    constexpr int N = N1 + N2, S = N | 1;
+    const int row = ring_row(c.P, c.KR, c.LX, S);
+    real *b = c.xb + c.par * c.cap;
+    c.par ^= 1;
+    const int prim = (c.line * row + (FWD ? c.KR + c.g : c.g)) * S;
+    const bool copy = FWD ? c.g >= c.P - c.KR : c.g < c.KR;
+    const int cpos = FWD ? prim - c.P * S : prim + c.P * S;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        b[prim + i] = v[i];
+        if (copy) b[cpos + i] = cyc ? v[i] : real(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; i++) out[i] = 0;
+    constexpr int dir = FWD ? -S : S;
+    const real *p = b + prim;
+    // the unrolled reads reach KC positions back/forward; every one of them
+    // was written only when the line has at least KR >= KC segments
+    const bool unrolled = UNI && c.P >= c.KR;
+    const int k0 = unrolled ? KC : 0;
+    if (unrolled) {
+#pragma unroll
+        for (int k = 1; k <= KC; k++) {
+            const real wa = w1[k - 1], wb = w2[k - 1];
+#pragma unroll
+            for (int i = 0; i < N; i++) out[i] += (i < N1 ? wa : wb) * p[dir * k + i];
+        }
+    }
+    for (int k = k0 + 1; k <= K1; k++) {
+        const real wk = w1[k - 1];
+#pragma unroll
+        for (int i = 0; i < N1; i++) out[i] += wk * p[dir * k + i];
+    }
+    for (int k = k0 + 1; k <= K2; k++) {
+        const real wk = w2[k - 1];
+#pragma unroll
+        for (int i = N1; i < N; i++) out[i] += wk * p[dir * k + i];
+    }
+}
+
+// N1 solves with operator 1 and N2 with operator 2, r holding B f on entry
+// and A^-1 B f on exit; two barriers for the whole batch.  KC: carry terms
+// unrolled on uniform plans (the lane's ring margin must be at least KC).
+template <int N1, int N2, int KC, typename real, int M, bool UNI, int MAP>
+__device__ __forceinline__ void solve_batch2(const OpDev<real> *__restrict__ op1,
+                                             const OpDev<real> *__restrict__ op2,
+                                             Lane<real, MAP> &c, real (&r)[N1 + N2][M])
+{
+    constexpr int N = N1 + N2;
+    real e[N], C[N];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        e[i] = fwd_local<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i]);
+    exchange<N1, N2, true, KC, UNI>(c, e, C, carry_weights<real, UNI, true>(op1, c), op1->K,
+                           carry_weights<real, UNI, true>(op2, c), op2->K, op1->cyclic);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        e[i] = bwd_local<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i], C[i]);
+    exchange<N1, N2, false, KC, UNI>(c, e, C, carry_weights<real, UNI, false>(op1, c), op1->K,
+                            carry_weights<real, UNI, false>(op2, c), op2->K, op1->cyclic);
+#pragma unroll
+    for (int i = 0; i < N; i++) finish<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i], C[i]);
+}
+
+template <int N, int KC, typename real, int M, bool UNI, int MAP>
+__device__ __forceinline__ void solve_batch(const OpDev<real> *__restrict__ op,
+                                            Lane<real, MAP> &c, real (&r)[N][M])
+{
+    solve_batch2<N, 0, KC, real, M, UNI, MAP>(op, op, c, r);
 }
 
 }  // namespace cs3d

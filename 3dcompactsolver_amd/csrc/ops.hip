@@ -2,6 +2,7 @@
 // (one field in, one field out, any direction, any layout).  This is the
 // GPU body of Derivatives::calc1stDerivField / calc2ndDerivField
 // (Derivatives.cpp:3-96) and Filter::filterField (Filter.cpp:129-171).
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -112,11 +113,12 @@ void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, doub
     hip_check(hipMalloc(&pl.dev, sizeof(OpDev<real>)), "hipMalloc(plan)");
     hip_check(hipMemcpy(pl.dev, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy(plan)");
     // one workgroup: up to 1024 lanes, at most 64 lines
-    int LX = 1024 / h.P;
-    if (LX > 64) LX = 64;
-    if (LX < 1) throw std::invalid_argument("too many segments per line");
-    pl.LX = LX;
-    pl.threads = LX * h.P;
+    const int kc = kind == OP_FILTER ? kc_filter<real>() : kc_deriv<real>();
+    LaunchGeom g = line_geom(h.P, std::max(h.K, kc), 1024, 1, sizeof(real));
+    if (g.LX < 1) throw std::invalid_argument("too many segments per line");
+    pl.LX = g.LX;
+    pl.threads = g.threads;
+    pl.shm = g.shm;
 }
 
 LineMap xyz_lines(int dir, int Nx, int Ny, int Nz)
@@ -152,23 +154,24 @@ __global__ __launch_bounds__(1024) void k_field_op(const OpDev<real> *__restrict
 {
     extern __shared__ char smem_raw[];
     real *xb = (real *)smem_raw;
+    constexpr int HW = KIND == OP_FILTER ? 4 : 2, W = M + 2 * HW;
+    constexpr int KC = KIND == OP_FILTER ? kc_filter<real>() : kc_deriv<real>();
     const int P = op->P;
     const int LX = blockDim.x / P;
-    Lane<real, MAP> c = make_lane<real, MAP>(op, P, LX, xb);
+    Lane<real, MAP> c = make_lane<real, MAP>(op, P, LX, op->K > KC ? op->K : KC, 1, xb);
     const int L = blockIdx.x * LX + c.line;
     const bool active = L < lm.nlines;
-    const long base = lm.base(active ? L : lm.nlines - 1);
-    const long sp = lm.sp;
-    constexpr int HW = KIND == OP_FILTER ? 4 : 2;
-    real w[M + 2 * HW];
-    load_window<real, M, HW>([&](int p) { return in[base + p * sp]; }, c.s, op->n,
-                             op->cyclic, w);
-    real r[M];
-    compact_op<KIND, real, M, HW, UNI, MAP>(op, c, w, r);
+    unsigned off[W];
+    window_offsets<real, M, HW>((unsigned)lm.base(active ? L : lm.nlines - 1), (unsigned)lm.sp,
+                                c.s, op->n, op->cyclic, off);
+    real w[W], r[1][M];
+    load_w(in, off, w);
+    rhs<KIND, real, M, HW, UNI>(op, c, w, r[0]);
+    solve_batch<1, KC, real, M, UNI>(op, c, r);
     if (active) {
 #pragma unroll
         for (int t = 0; t < M; t++)
-            if (t < c.len) out[base + (long)(c.s + t) * sp] = r[t];
+            if (UNI || t < c.len) sto(out, off[HW + t], r[0][t]);
     }
 }
 
@@ -178,12 +181,11 @@ static hipError_t launch_kmu(const PlanDev<real> &pl, int map, const LineMap &lm
 {
     if (pl.LX < 1 || pl.threads < 1 || pl.threads > 1024) return hipErrorInvalidConfiguration;
     dim3 grid((lm.nlines + pl.LX - 1) / pl.LX), block(pl.threads);
-    size_t shm = 2 * (size_t)pl.threads * sizeof(real);
     if (map == MAP_ROW)
-        hipLaunchKernelGGL((k_field_op<KIND, real, M, UNI, MAP_ROW>), grid, block, shm, st,
+        hipLaunchKernelGGL((k_field_op<KIND, real, M, UNI, MAP_ROW>), grid, block, pl.shm, st,
                            pl.dev, lm, in, out);
     else
-        hipLaunchKernelGGL((k_field_op<KIND, real, M, UNI, MAP_COL>), grid, block, shm, st,
+        hipLaunchKernelGGL((k_field_op<KIND, real, M, UNI, MAP_COL>), grid, block, pl.shm, st,
                            pl.dev, lm, in, out);
     return hipGetLastError();
 }

@@ -26,14 +26,24 @@ constexpr int SEG_DERIV = 4;
 constexpr int SEG_FILTER = 8;
 constexpr int SEG_TABLE = 4;
 
+// Carry terms unrolled by the kernels on uniform plans (any further terms
+// run in a loop).  The derivative plans keep 6 (FP32) / 12 (FP64) terms; the
+// filter's grow as alpha_f approaches 1/2 (15 / 29 at 0.49 with 8-point
+// segments).  A lane's ring margin is max(K, KC).
+template <typename real>
+constexpr int kc_deriv() { return sizeof(real) == 4 ? 6 : 12; }
+template <typename real>
+constexpr int kc_filter() { return sizeof(real) == 4 ? 16 : 32; }
+
 template <typename real>
 struct PlanDev {
     OpHost host;
     OpDev<real> hd{};           // host copy holding device table pointers
     OpDev<real> *dev = nullptr; // device copy read by kernels
     void *tables = nullptr;
-    int LX = 0;                 // lines per workgroup
+    int LX = 0;                 // lines per workgroup (standalone operator)
     int threads = 0;            // workgroup size = P * LX
+    size_t shm = 0;             // LDS bytes of the exchange ring
 
     void release();
 };

@@ -5,7 +5,9 @@
 // wait for the device are the ones that return host data.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -25,14 +27,28 @@ static void hc(hipError_t e, const char *what)
     if (e != hipSuccess) throw std::runtime_error(hip_msg(e, what));
 }
 
+// Workgroup size caps of the sweeps and the filter sweeps.  The environment
+// variables CS3D_SWEEP_THREADS / CS3D_FILTER_THREADS lower them for tuning;
+// values outside [64, cap] are ignored.
+static int env_cap(const char *name, int cap)
+{
+    const char *v = std::getenv(name);
+    if (!v) return cap;
+    int x = std::atoi(v);
+    return (x >= 64 && x <= cap) ? x : cap;
+}
+
 template <typename real>
 struct SolverT : SolverBase {
+    static int sweep_max_threads() { return env_cap("CS3D_SWEEP_THREADS", sizeof(real) == 4 ? 1024 : 512); }
+    static int filter_max_threads() { return env_cap("CS3D_FILTER_THREADS", 512); }
+
     cs3d_config cfg;
     int Nx, Ny, Nz;
     long N;
     double dxyz[3];
     PlanDev<real> D1[3], D2[3], FL[3];
-    int sweepLX[3], sweepThreads[3];
+    LaunchGeom sweepG[3], filterG[3];
     LineMap lm[3];
     Fields<real> F;
     Phys<real> ph;
@@ -64,6 +80,9 @@ struct SolverT : SolverBase {
         Ny = c.Ny;
         Nz = c.Nz;
         N = (long)Nx * Ny * Nz;
+        // kernels address a field with 32-bit byte offsets
+        if ((unsigned long long)N * sizeof(real) > 0xFFFFFFFFull)
+            throw std::invalid_argument("grid too large: every field must stay below 4 GiB");
         const int dims[3] = {Nx, Ny, Nz};
         const double Ls[3] = {c.Lx, c.Ly, c.Lz};
         for (int d = 0; d < 3; d++) {
@@ -75,12 +94,13 @@ struct SolverT : SolverBase {
             make_plan(FL[d], OP_FILTER, per, dims[d], dxyz[d], c.alpha_f);
             if (D1[d].host.P != D2[d].host.P || D1[d].host.uniform != D2[d].host.uniform)
                 throw std::invalid_argument("inconsistent derivative plans");
-            int P = D1[d].host.P;
-            int LXs = (sizeof(real) == 4 ? 1024 : 512) / P;
-            if (LXs > 64) LXs = 64;
-            if (LXs < 1) throw std::invalid_argument("line too long for the fused sweeps");
-            sweepLX[d] = LXs;
-            sweepThreads[d] = LXs * P;
+            const int P = D1[d].host.P;
+            const int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
+            sweepG[d] = line_geom(P, kr, sweep_max_threads(), SWEEP_PLANES, sizeof(real));
+            if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
+            filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
+                                   filter_max_threads(), 5, sizeof(real));
+            if (filterG[d].LX < 1) throw std::invalid_argument("line too long for the filter");
             lm[d] = xyz_lines(d, Nx, Ny, Nz);
         }
         // IdealGas.hpp:35-47: p_ref = 1/gamma, rho_ref = T_ref = 1
@@ -309,7 +329,7 @@ struct SolverT : SolverBase {
         for (int d = 0; d < 3; d++)
             timed(d, [&] {
                 hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, D1[d].host.uniform, D1[d].host.M,
-                                      sweepThreads[d], sweepLX[d], lm[d], F, ph, sa, dsc, st),
+                                      sweepG[d], lm[d], F, ph, sa, dsc, st),
                    "sweep");
             });
     }
@@ -323,7 +343,7 @@ struct SolverT : SolverBase {
         }
         const PlanDev<real> &pl = FL[dir];
         timed(3, [&] {
-            hc(launch_filter5<real>(pl.dev, pl.host.uniform, pl.host.M, pl.threads, pl.LX,
+            hc(launch_filter5<real>(pl.dev, pl.host.uniform, pl.host.M, filterG[dir],
                                     dir == 0 ? MAP_ROW : MAP_COL, lm[dir], io, st),
                "filter");
         });

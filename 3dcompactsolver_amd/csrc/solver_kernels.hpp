@@ -75,20 +75,23 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
     return (b % nx) * (nb / nx) + b / nx;
 }
 
+// most solves batched through one exchange in a sweep kernel
+constexpr int SWEEP_PLANES = 13;
+
 template <typename real>
 struct Ptr5 {
     const real *in[5];
     real *out[5];
 };
 
-// launchers (stage_kernels.hip)
+// launchers (stage_*.hip)
 template <typename real>
 hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
-                        int threads, int LX, const LineMap &lm, const Fields<real> &F,
+                        const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
                         const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                         hipStream_t st);
 template <typename real>
-hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, int threads, int LX, int map,
+hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
                           const LineMap &lm, const Ptr5<real> &io, hipStream_t st);
 constexpr int RED_SLOTS_HOST = 8;
 template <typename real>

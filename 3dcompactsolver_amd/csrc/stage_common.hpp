@@ -8,12 +8,28 @@
 
 namespace cs3d {
 
+// Working-precision reciprocal and the viscosity power law.  FP32 uses the
+// hardware reciprocal and log2/exp2 (about 1 ulp each, far inside the FP32
+// error budget of the scheme); FP64 keeps correctly rounded division and pow.
+template <typename real>
+__device__ __forceinline__ real rcp(real x);
+template <>
+__device__ __forceinline__ float rcp<float>(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+template <>
+__device__ __forceinline__ double rcp<double>(double x)
+{
+    return 1.0 / x;
+}
+
 template <typename real>
 __device__ __forceinline__ real pow076(real x);
 template <>
 __device__ __forceinline__ float pow076<float>(float x)
 {
-    return powf(x, 0.76f);
+    return __builtin_amdgcn_exp2f(0.76f * __builtin_amdgcn_logf(x));
 }
 template <>
 __device__ __forceinline__ double pow076<double>(double x)
@@ -26,7 +42,7 @@ template <typename real>
 __device__ __forceinline__ void visc(const Phys<real> &ph, real T, real &mu, real &amu)
 {
     mu = ph.mu_ref * pow076(T * ph.iTref_ig);
-    amu = real(0.76) * mu / T;
+    amu = real(0.76) * mu * rcp(T);
 }
 
 // conserved (perturbation form) -> 1/rho, velocity, p' = p - p_ref,
@@ -37,7 +53,7 @@ __device__ __forceinline__ void prim(const Phys<real> &ph, real rp, real ru, rea
                                      real &Tp)
 {
     real rho = ph.rho_ref + rp;
-    irho = real(1) / rho;
+    irho = rcp(rho);
     U = ru * irho;
     V = rv * irho;
     W = rw * irho;
@@ -46,18 +62,30 @@ __device__ __forceinline__ void prim(const Phys<real> &ph, real rp, real ru, rea
     Tp = (pp - rp * ph.RT0s) * irho * ph.iR;
 }
 
-#define CS3D_LANE_SETUP(MAPV, OPP)                                                  \
-    extern __shared__ char smem_raw[];                                              \
-    real *xb = (real *)smem_raw;                                                    \
-    const int P = (OPP)->P;                                                         \
-    const int LX = blockDim.x / P;                                                  \
-    Lane<real, MAPV> c = make_lane<real, MAPV>((OPP), P, LX, xb);                   \
-    const int L = xcd_block(blockIdx.x, gridDim.x) * LX + c.line;                   \
-    const bool active = L < lm.nlines;                                              \
-    const long base = lm.base(active ? L : lm.nlines - 1);                          \
-    const long sp = lm.sp;                                                          \
-    const int n = (OPP)->n, per = (OPP)->cyclic;
+// Per-lane set-up of a sweep kernel: lane geometry, exchange ring (margin KRV,
+// PLANES planes per parity), the line this lane works on (inactive lanes of a
+// partial last workgroup take the last line and only skip their stores) and
+// the byte offsets `off` of the lane's window.  Needs M, HW and W in scope.
+#define CS3D_LANE_SETUP(MAPV, OPP, KRV, PLANES)                                         \
+    extern __shared__ char smem_raw[];                                                  \
+    real *xb = (real *)smem_raw;                                                        \
+    const int P = (OPP)->P;                                                             \
+    const int LX = blockDim.x / P;                                                      \
+    Lane<real, MAPV> c = make_lane<real, MAPV>((OPP), P, LX, (KRV), (PLANES), xb);      \
+    const int L = xcd_block(blockIdx.x, gridDim.x) * LX + c.line;                       \
+    const bool active = L < lm.nlines;                                                  \
+    const int Lc = active ? L : lm.nlines - 1;                                          \
+    const int n = (OPP)->n, per = (OPP)->cyclic;                                        \
+    unsigned off[W];                                                                    \
+    window_offsets<real, M, HW>((unsigned)lm.base(Lc), (unsigned)lm.sp, c.s, n, per, off);
 
+// ring margin of a sweep that uses both derivative operators
+template <typename real>
+__device__ __forceinline__ int sweep_kr(const OpDev<real> *d1, const OpDev<real> *d2)
+{
+    int k = d1->K > d2->K ? d1->K : d2->K;
+    return k > kc_deriv<real>() ? k : kc_deriv<real>();
+}
 
 // q windows a0..a4 (perturbation form) and ir = 1/rho at window index t
 #define CS3D_U(t) (a1[t] * ir[t])
@@ -79,15 +107,15 @@ template <typename real>
 constexpr int sweep_threads() { return sizeof(real) == 4 ? 1024 : 512; }
 
 template <typename real, bool UNI>
-hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, const StageArgs &sa, hipStream_t st);
 template <typename real, bool UNI>
-hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, hipStream_t st);
 template <typename real, bool UNI>
-hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         const StageArgs &sa, const StepScalars *sc, hipStream_t st);
 

@@ -9,23 +9,31 @@ namespace cs3d {
 // One filter sweep over the five conserved fields (Filter.cpp:129-171 as
 // called by filterConservedData, CSolver_AWS.cpp:1682-1893).
 
+// workgroups of at most 512 lanes: the five fields of a segment stay in
+// registers through the batched solve
+constexpr int FILTER_THREADS = 512;
+
 template <typename real, int M, bool UNI, int MAP>
-__global__ __launch_bounds__(1024) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
+__global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
                                                   Ptr5<real> io)
 {
-    CS3D_LANE_SETUP(MAP, fo)
-    constexpr int HW = 4;
-    for (int v = 0; v < 5; v++) {
-        const real *src = io.in[v];
-        real w[M + 2 * HW], r[M];
-        load_window<real, M, HW>([&](int p) { return src[base + p * sp]; }, c.s, n, per, w);
-        compact_op<OP_FILTER, real, M, HW, UNI, MAP>(fo, c, w, r);
-        if (active) {
-            real *dst = io.out[v];
+    constexpr int HW = 4, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP, fo, fo->K > kc_filter<real>() ? fo->K : kc_filter<real>(), 5)
+    real r[5][M];
 #pragma unroll
-            for (int t = 0; t < M; t++)
-                if (t < c.len) dst[base + (long)(c.s + t) * sp] = r[t];
-        }
+    for (int v = 0; v < 5; v++) {
+        real w[W];
+        load_w(io.in[v], off, w);
+        rhs<OP_FILTER, real, M, HW, UNI>(fo, c, w, r[v]);
+    }
+    solve_batch<5, kc_filter<real>(), real, M, UNI>(fo, c, r);
+    if (active) {
+#pragma unroll
+        for (int t = 0; t < M; t++)
+            if (UNI || t < c.len) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) sto(io.out[v], off[HW + t], r[v][t]);
+            }
     }
 }
 
@@ -172,28 +180,32 @@ __global__ void k_sponge(Fields<real> F, Phys<real> ph, StageArgs sa, const Step
 
 template <typename real>
 hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
-                        int threads, int LX, const LineMap &lm, const Fields<real> &F,
+                        const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
                         const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                         hipStream_t st)
 {
     if (M != SEG_DERIV) return hipErrorInvalidValue;
+    if (lg.LX < 1 || lg.threads < 1 || lg.threads > sweep_threads<real>())
+        return hipErrorInvalidConfiguration;
     int use_k = sa.rkStep > 1;
     if (which == 0)
-        return uni ? launch_xpass<real, true>(d1, d2, threads, LX, lm, F, ph, use_k, sa, st)
-                   : launch_xpass<real, false>(d1, d2, threads, LX, lm, F, ph, use_k, sa, st);
+        return uni ? launch_xpass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
+                   : launch_xpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
     if (which == 1)
-        return uni ? launch_ypass<real, true>(d1, d2, threads, LX, lm, F, ph, use_k, st)
-                   : launch_ypass<real, false>(d1, d2, threads, LX, lm, F, ph, use_k, st);
-    return uni ? launch_zpass<real, true>(d1, d2, threads, LX, lm, F, ph, sa, sc, st)
-               : launch_zpass<real, false>(d1, d2, threads, LX, lm, F, ph, sa, sc, st);
+        return uni ? launch_ypass<real, true>(d1, d2, lg, lm, F, ph, use_k, st)
+                   : launch_ypass<real, false>(d1, d2, lg, lm, F, ph, use_k, st);
+    return uni ? launch_zpass<real, true>(d1, d2, lg, lm, F, ph, sa, sc, st)
+               : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, sa, sc, st);
 }
 
 template <typename real>
-hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, int threads, int LX, int map,
+hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
                           const LineMap &lm, const Ptr5<real> &io, hipStream_t st)
 {
-    dim3 grid((lm.nlines + LX - 1) / LX), block(threads);
-    size_t shm = 2 * (size_t)threads * sizeof(real);
+    if (lg.LX < 1 || lg.threads < 1 || lg.threads > FILTER_THREADS)
+        return hipErrorInvalidConfiguration;
+    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    const size_t shm = lg.shm;
     if (uni) {
         if (M != SEG_FILTER) return hipErrorInvalidValue;
         if (map == MAP_ROW)
@@ -241,11 +253,12 @@ hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const Stag
 
 #define CS3D_INST(real)                                                                          \
     template hipError_t launch_sweep<real>(int, const OpDev<real> *, const OpDev<real> *, int, int, \
-                                           int, int, const LineMap &, const Fields<real> &,       \
-                                           const Phys<real> &, const StageArgs &,               \
-                                           const StepScalars *, hipStream_t);                   \
-    template hipError_t launch_filter5<real>(const OpDev<real> *, int, int, int, int, int,       \
-                                             const LineMap &, const Ptr5<real> &, hipStream_t);  \
+                                           const LaunchGeom &, const LineMap &,                 \
+                                           const Fields<real> &, const Phys<real> &,            \
+                                           const StageArgs &, const StepScalars *, hipStream_t); \
+    template hipError_t launch_filter5<real>(const OpDev<real> *, int, int, const LaunchGeom &,  \
+                                             int, const LineMap &, const Ptr5<real> &,         \
+                                             hipStream_t);                                     \
     template hipError_t launch_reduce<real>(const Fields<real> &, const Phys<real> &, long, int, \
                                             double *, int, double *, StepScalars *, hipStream_t); \
     template hipError_t launch_sponge<real>(const Fields<real> &, const Phys<real> &,            \

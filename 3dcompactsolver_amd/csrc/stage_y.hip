@@ -4,7 +4,9 @@
 // y Euler fluxes (CSolver_AWS.cpp:949-963) and their derivatives
 // (:1051-1073), first and second y-derivatives of U, V, W, T (:969-1049),
 // the mixed Uxy, Vxy from the stored Ux, Vx (:991-1003); stores Uy Vy Wy Ty
-// and adds every term linear in a y-derivative to the right-hand side.
+// and adds every term linear in a y-derivative to the right-hand side.  The
+// 15 line solves run as two batches (5 flux + 2 mixed first derivatives;
+// 4 first + 4 second derivatives).
 #include "stage_common.hpp"
 
 namespace cs3d {
@@ -14,153 +16,124 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     Fields<real> F, Phys<real> ph, int use_k)
 {
-    CS3D_LANE_SETUP(MAP_COL, d1)
     constexpr int HW = 2, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), SWEEP_PLANES)
     const real *q0 = use_k ? F.qk[0] : F.q[0];
     const real *q1 = use_k ? F.qk[1] : F.q[1];
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
-    real a0[W], a1[W], a2[W], a3[W], a4[W], ir[W];
-    load_window<real, M, HW>([&](int p) { return q0[base + p * sp]; }, c.s, n, per, a0);
-    load_window<real, M, HW>([&](int p) { return q1[base + p * sp]; }, c.s, n, per, a1);
-    load_window<real, M, HW>([&](int p) { return q2[base + p * sp]; }, c.s, n, per, a2);
-    load_window<real, M, HW>([&](int p) { return q3[base + p * sp]; }, c.s, n, per, a3);
-    load_window<real, M, HW>([&](int p) { return q4[base + p * sp]; }, c.s, n, per, a4);
+    real a1[W], a2[W], a3[W], a4[W];
+    real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
 #pragma unroll
-    for (int t = 0; t < W; t++) ir[t] = real(1) / (ph.rho_ref + a0[t]);
-    real pp[W], w[W], r[M];
-#pragma unroll
-    for (int t = 0; t < W; t++) pp[t] = pprime(ph, a1, a2, a3, a4, ir, t);
-    real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = (a4[t] + pp[t]) * CS3D_V(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) RE[t] = -r[t];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = a2[t] * CS3D_V(t) + pp[t];
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) Rv[t] = -r[t];
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, a2, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) R0[t] = -r[t];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = a1[t] * CS3D_V(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) Ru[t] = -r[t];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = a3[t] * CS3D_V(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) Rw[t] = -r[t];
-
-    auto store = [&](real *dst, const real(&v)[M]) {
-        if (active) {
-#pragma unroll
-            for (int t = 0; t < M; t++)
-                if (t < c.len) dst[base + (long)(c.s + t) * sp] = v[t];
-        }
-    };
-    real mu[M], Uc[M], Vc[M];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = (pp[t] - a0[t] * ph.RT0s) * ir[t] * ph.iR;
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        real amu;
-        visc(ph, ph.T0s + w[HW + t], mu[t], amu);
-        Uc[t] = CS3D_U(HW + t);
-        Vc[t] = CS3D_V(HW + t);
+    for (int t = 0; t < W; t++) {
+        const real a0 = ldo(q0, off[t]);
+        a1[t] = ldo(q1, off[t]);
+        a2[t] = ldo(q2, off[t]);
+        a3[t] = ldo(q3, off[t]);
+        a4[t] = ldo(q4, off[t]);
+        prim(ph, a0, a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
     }
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-    store(F.dY[3], r);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) RE[t] += ph.cpPr * (mu[t] * r[t]);
     const real c43 = real(4) / real(3), c13 = real(1) / real(3);
-    // U: Uy, Uyy
+    real R0[M], Ru[M], Rv[M], Rw[M], RE[M], Uxy[M], Vxy[M];
+    {
+        // y fluxes (the h_ref V part of the energy flux is added as h_ref Vy)
+        // and the mixed derivatives of the stored Ux, Vx
+        real r[7][M], f[W];
 #pragma unroll
-    for (int t = 0; t < W; t++) w[t] = CS3D_U(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-    store(F.dY[0], r);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
+        for (int t = 0; t < W; t++) f[t] = (a4[t] + pp[t]) * V[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[0]);
 #pragma unroll
-    for (int t = 0; t < M; t++) {
-        Ru[t] += mu[t] * r[t];
-        RE[t] += mu[t] * (Uc[t] * r[t]);
+        for (int t = 0; t < W; t++) f[t] = a2[t] * V[t] + pp[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, a2, r[2]);
+#pragma unroll
+        for (int t = 0; t < W; t++) f[t] = a1[t] * V[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[3]);
+#pragma unroll
+        for (int t = 0; t < W; t++) f[t] = a3[t] * V[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
+        load_w(F.dX[0], off, f);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
+        load_w(F.dX[1], off, f);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
+        solve_batch<7, kc_deriv<real>(), real, M, UNI>(d1, c, r);
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            RE[t] = -r[0][t];
+            Rv[t] = -r[1][t];
+            R0[t] = -r[2][t];
+            Ru[t] = -r[3][t];
+            Rw[t] = -r[4][t];
+            Uxy[t] = r[5][t];
+            Vxy[t] = r[6][t];
+        }
     }
-    // V: Vy, Vyy
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = CS3D_V(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-    store(F.dY[1], r);
-#pragma unroll
-    for (int t = 0; t < M; t++) RE[t] -= ph.h_ref * r[t];
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rv[t] += mu[t] * (c43 * r[t]);
-        RE[t] += mu[t] * (Vc[t] * (c43 * r[t]));
-    }
-    // W: Wy, Wyy
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = CS3D_W(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-    store(F.dY[2], r);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
+    // first and second derivatives of T', U, V, W
+    real r[8][M];
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, Tp, r[0]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, U, r[1]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, V, r[2]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, Wv, r[3]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Tp, r[4]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[5]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[6]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[7]);
+    solve_batch2<4, 4, kc_deriv<real>(), real, M, UNI>(d1, d2, c, r);
+
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        Rw[t] += mu[t] * r[t];
-        RE[t] += mu[t] * (w[HW + t] * r[t]);
-    }
-    // mixed Uxy = d/dy(Ux), Vxy = d/dy(Vx)
-    load_window<real, M, HW>([&](int p) { return F.dX[0][base + p * sp]; }, c.s, n, per, w);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rv[t] += mu[t] * (c13 * r[t]);
-        RE[t] += mu[t] * (c13 * (Vc[t] * r[t]));
-    }
-    load_window<real, M, HW>([&](int p) { return F.dX[1][base + p * sp]; }, c.s, n, per, w);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Ru[t] += mu[t] * (c13 * r[t]);
-        RE[t] += mu[t] * (c13 * (Uc[t] * r[t]));
+        real mu, amu;
+        visc(ph, ph.T0s + Tp[HW + t], mu, amu);
+        const real Uc = U[HW + t], Vc = V[HW + t], Wc = Wv[HW + t];
+        RE[t] += ph.cpPr * (mu * r[4][t]);
+        Ru[t] += mu * r[5][t];
+        RE[t] += mu * (Uc * r[5][t]);
+        RE[t] -= ph.h_ref * r[2][t];
+        Rv[t] += mu * (c43 * r[6][t]);
+        RE[t] += mu * (Vc * (c43 * r[6][t]));
+        Rw[t] += mu * r[7][t];
+        RE[t] += mu * (Wc * r[7][t]);
+        Rv[t] += mu * (c13 * Uxy[t]);
+        RE[t] += mu * (c13 * (Vc * Uxy[t]));
+        Ru[t] += mu * (c13 * Vxy[t]);
+        RE[t] += mu * (c13 * (Uc * Vxy[t]));
     }
     if (active) {
 #pragma unroll
         for (int t = 0; t < M; t++) {
-            if (t < c.len) {
-                long i = base + (long)(c.s + t) * sp;
-                F.R[0][i] = F.R[0][i] + R0[t];
-                F.R[1][i] = F.R[1][i] + Ru[t];
-                F.R[2][i] = F.R[2][i] + Rv[t];
-                F.R[3][i] = F.R[3][i] + Rw[t];
-                F.R[4][i] = F.R[4][i] + RE[t];
+            if (UNI || t < c.len) {
+                const unsigned o = off[HW + t];
+                sto(F.dY[0], o, r[1][t]);
+                sto(F.dY[1], o, r[2][t]);
+                sto(F.dY[2], o, r[3][t]);
+                sto(F.dY[3], o, r[0][t]);
+                sto(F.R[0], o, ldo(F.R[0], o) + R0[t]);
+                sto(F.R[1], o, ldo(F.R[1], o) + Ru[t]);
+                sto(F.R[2], o, ldo(F.R[2], o) + Rv[t]);
+                sto(F.R[3], o, ldo(F.R[3], o) + Rw[t]);
+                sto(F.R[4], o, ldo(F.R[4], o) + RE[t]);
             }
         }
     }
 }
 
 template <typename real, bool UNI>
-hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, hipStream_t st)
 {
-    dim3 grid((lm.nlines + LX - 1) / LX), block(threads);
-    size_t shm = 2 * (size_t)threads * sizeof(real);
-    hipLaunchKernelGGL((k_ypass<real, SEG_DERIV, UNI>), grid, block, shm, st, d1, d2, lm, F, ph,
-                       use_k);
+    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    hipLaunchKernelGGL((k_ypass<real, SEG_DERIV, UNI>), grid, block, lg.shm, st, d1, d2, lm, F,
+                       ph, use_k);
     return hipGetLastError();
 }
 
-#define CS3D_INST_Y(real, UNI)                                                              \
-    template hipError_t launch_ypass<real, UNI>(const OpDev<real> *, const OpDev<real> *, int, \
-                                                int, const LineMap &, const Fields<real> &,    \
-                                                const Phys<real> &, int, hipStream_t);
+#define CS3D_INST_Y(real, UNI)                                                                  \
+    template hipError_t launch_ypass<real, UNI>(const OpDev<real> *, const OpDev<real> *,          \
+                                                const LaunchGeom &, const LineMap &,               \
+                                                const Fields<real> &, const Phys<real> &, int,     \
+                                                hipStream_t);
 CS3D_INST_Y(float, true)
 CS3D_INST_Y(float, false)
 CS3D_INST_Y(double, true)

@@ -13,138 +13,139 @@
 
 namespace cs3d {
 
-template <typename real, int M, bool UNI>
+// QS: the update needs the stage state at the point (sponge source, or the
+// TVDRK3 combinations of stages 2 and 3); the RK4 path without sponge never
+// carries it.
+template <typename real, int M, bool UNI, bool QS>
 __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     Fields<real> F, Phys<real> ph, StageArgs sa, const StepScalars *__restrict__ sc)
 {
-    CS3D_LANE_SETUP(MAP_COL, d1)
     constexpr int HW = 2, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), SWEEP_PLANES)
     const bool use_k = sa.rkStep > 1;
     const real *q0 = use_k ? F.qk[0] : F.q[0];
     const real *q1 = use_k ? F.qk[1] : F.q[1];
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
-    real a0[W], a1[W], a2[W], a3[W], a4[W], ir[W];
-    load_window<real, M, HW>([&](int p) { return q0[base + p * sp]; }, c.s, n, per, a0);
-    load_window<real, M, HW>([&](int p) { return q1[base + p * sp]; }, c.s, n, per, a1);
-    load_window<real, M, HW>([&](int p) { return q2[base + p * sp]; }, c.s, n, per, a2);
-    load_window<real, M, HW>([&](int p) { return q3[base + p * sp]; }, c.s, n, per, a3);
-    load_window<real, M, HW>([&](int p) { return q4[base + p * sp]; }, c.s, n, per, a4);
+    real a1[W], a2[W], a3[W], a4[W];
+    real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
 #pragma unroll
-    for (int t = 0; t < W; t++) ir[t] = real(1) / (ph.rho_ref + a0[t]);
-    real pp[W], w[W], r[M];
+    for (int t = 0; t < W; t++) {
+        const real a0 = ldo(q0, off[t]);
+        a1[t] = ldo(q1, off[t]);
+        a2[t] = ldo(q2, off[t]);
+        a3[t] = ldo(q3, off[t]);
+        a4[t] = ldo(q4, off[t]);
+        prim(ph, a0, a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
+    }
+    const real c43 = real(4) / real(3), c13 = real(1) / real(3), c23 = real(2) / real(3);
+    // One batch of 13 solves from the stage state: the z fluxes (the h_ref W
+    // part of the energy flux is added as h_ref Wz in the assembly), then
+    // the first and second z-derivatives of T', U, V, W.  Every window dies
+    // here, before the mixed derivatives are loaded.
+    real r[13][M];
+    {
+        real f[W];
 #pragma unroll
-    for (int t = 0; t < W; t++) pp[t] = pprime(ph, a1, a2, a3, a4, ir, t);
+        for (int t = 0; t < W; t++) f[t] = (a4[t] + pp[t]) * Wv[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[0]);
+#pragma unroll
+        for (int t = 0; t < W; t++) f[t] = a3[t] * Wv[t] + pp[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, a3, r[2]);
+#pragma unroll
+        for (int t = 0; t < W; t++) f[t] = a1[t] * Wv[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[3]);
+#pragma unroll
+        for (int t = 0; t < W; t++) f[t] = a2[t] * Wv[t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
+    }
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, Tp, r[5]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, U, r[6]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, V, r[7]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, Wv, r[8]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Tp, r[9]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[10]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[11]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[12]);
+    real mu[M], amu[M], Uc[M], Vc[M], Wc[M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        visc(ph, ph.T0s + Tp[HW + t], mu[t], amu[t]);
+        Uc[t] = U[HW + t];
+        Vc[t] = V[HW + t];
+        Wc[t] = Wv[HW + t];
+    }
+    solve_batch2<9, 4, kc_deriv<real>(), real, M, UNI>(d1, d2, c, r);
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
 #pragma unroll
-    for (int t = 0; t < W; t++) w[t] = (a4[t] + pp[t]) * CS3D_W(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) RE[t] = -r[t];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = a3[t] * CS3D_W(t) + pp[t];
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) Rw[t] = -r[t];
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, a3, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) R0[t] = -r[t];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = a1[t] * CS3D_W(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) Ru[t] = -r[t];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = a2[t] * CS3D_W(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) Rv[t] = -r[t];
-
-    real mu[M], amu[M], Uc[M], Vc[M], Wc[M];
-    real Uz[M], Vz[M], Wz[M], Tz[M];
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = (pp[t] - a0[t] * ph.RT0s) * ir[t] * ph.iR;
-#pragma unroll
     for (int t = 0; t < M; t++) {
-        visc(ph, ph.T0s + w[HW + t], mu[t], amu[t]);
-        Uc[t] = CS3D_U(HW + t);
-        Vc[t] = CS3D_V(HW + t);
-        Wc[t] = CS3D_W(HW + t);
+        RE[t] = -r[0][t];
+        Rw[t] = -r[1][t];
+        R0[t] = -r[2][t];
+        Ru[t] = -r[3][t];
+        Rv[t] = -r[4][t];
+        RE[t] += ph.cpPr * (mu[t] * r[9][t]);
+        Ru[t] += mu[t] * r[10][t];
+        RE[t] += mu[t] * (Uc[t] * r[10][t]);
+        Rv[t] += mu[t] * r[11][t];
+        RE[t] += mu[t] * (Vc[t] * r[11][t]);
+        Rw[t] += mu[t] * (c43 * r[12][t]);
+        RE[t] += mu[t] * (Wc[t] * (c43 * r[12][t]));
     }
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, Tz);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
+    {
+        // mixed derivatives of the stored x- and y-derivatives.  Uxz and Vyz
+        // enter the right-hand side with the same coefficients
+        // (CSolver_AWS.cpp:1161-1192), so their sum is taken as one
+        // derivative of Ux + Vy.
+        real m3[3][M], f[W];
 #pragma unroll
-    for (int t = 0; t < M; t++) RE[t] += ph.cpPr * (mu[t] * r[t]);
-    const real c43 = real(4) / real(3), c13 = real(1) / real(3), c23 = real(2) / real(3);
+        for (int t = 0; t < W; t++) f[t] = ldo(F.dX[0], off[t]) + ldo(F.dY[1], off[t]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[0]);
+        load_w(F.dX[2], off, f);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[1]);
+        load_w(F.dY[2], off, f);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[2]);
+        solve_batch<3, kc_deriv<real>(), real, M, UNI>(d1, c, m3);
 #pragma unroll
-    for (int t = 0; t < W; t++) w[t] = CS3D_U(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, Uz);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Ru[t] += mu[t] * r[t];
-        RE[t] += mu[t] * (Uc[t] * r[t]);
+        for (int t = 0; t < M; t++) {
+            Rw[t] += mu[t] * (c13 * m3[0][t]);
+            RE[t] += mu[t] * (c13 * (Wc[t] * m3[0][t]));
+            Ru[t] += mu[t] * (c13 * m3[1][t]);
+            RE[t] += mu[t] * (c13 * (Uc[t] * m3[1][t]));
+            Rv[t] += mu[t] * (c13 * m3[2][t]);
+            RE[t] += mu[t] * (c13 * (Vc[t] * m3[2][t]));
+        }
     }
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = CS3D_V(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, Vz);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rv[t] += mu[t] * r[t];
-        RE[t] += mu[t] * (Vc[t] * r[t]);
-    }
-#pragma unroll
-    for (int t = 0; t < W; t++) w[t] = CS3D_W(t);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, Wz);
-    compact_op<OP_D2, real, M, HW, UNI, MAP_COL>(d2, c, w, r);
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rw[t] += mu[t] * (c43 * r[t]);
-        RE[t] += mu[t] * (Wc[t] * (c43 * r[t]));
-    }
-    // mixed derivatives from the stored x- and y-derivatives
-    load_window<real, M, HW>([&](int p) { return F.dX[0][base + p * sp]; }, c.s, n, per, w);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);  // Uxz
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rw[t] += mu[t] * (c13 * r[t]);
-        RE[t] += mu[t] * (c13 * (Wc[t] * r[t]));
-    }
-    load_window<real, M, HW>([&](int p) { return F.dX[2][base + p * sp]; }, c.s, n, per, w);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);  // Wxz
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Ru[t] += mu[t] * (c13 * r[t]);
-        RE[t] += mu[t] * (c13 * (Uc[t] * r[t]));
-    }
-    load_window<real, M, HW>([&](int p) { return F.dY[1][base + p * sp]; }, c.s, n, per, w);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);  // Vyz
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rw[t] += mu[t] * (c13 * r[t]);
-        RE[t] += mu[t] * (c13 * (Wc[t] * r[t]));
-    }
-    load_window<real, M, HW>([&](int p) { return F.dY[2][base + p * sp]; }, c.s, n, per, w);
-    compact_op<OP_D1, real, M, HW, UNI, MAP_COL>(d1, c, w, r);  // Wyz
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        Rv[t] += mu[t] * (c13 * r[t]);
-        RE[t] += mu[t] * (c13 * (Vc[t] * r[t]));
-    }
+    const real *Tz = r[5], *Uz = r[6], *Vz = r[7], *Wz = r[8];
     if (!active) return;
 
     const real dt = (real)sc->dt;
-    const int Nx = sa.Nx, Ny = sa.Ny;
-    const long Nxy = (long)Nx * Ny;
+    // stage-update coefficients, uniform over the launch
+    const bool first = sa.rkStep == 1, has_next = sa.rkStep < 4;
+    const real ra = (sa.rkStep == 1 || sa.rkStep == 4) ? real(1) / real(6) : real(1) / real(3);
+    const real rb = sa.rkStep == 3 ? real(1) : real(0.5);
+    const bool tvd_to_k = sa.rkStep < 3;
+    const real ta = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.75) : real(1) / real(3);
+    const real tb = sa.rkStep == 1 ? real(0) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
+    const real tc = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
+    // lines along z: L = i + Nx j; the point index along the line is k
+    const int li = Lc % sa.Nx, lj = Lc / sa.Nx;
+    const bool line_plane = (sa.sponge_face[0] && li == 0) ||
+                            (sa.sponge_face[1] && li == sa.Nx - 1) ||
+                            (sa.sponge_face[2] && lj == 0) ||
+                            (sa.sponge_face[3] && lj == sa.Ny - 1);
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        if (t >= c.len) continue;
-        const long i = base + (long)(c.s + t) * sp;
-        const real Ux = F.dX[0][i], Vx = F.dX[1][i], Wx = F.dX[2][i], Tx = F.dX[3][i];
-        const real Uy = F.dY[0][i], Vy = F.dY[1][i], Wy = F.dY[2][i], Ty = F.dY[3][i];
+        if (!UNI && t >= c.len) continue;
+        const unsigned i = off[HW + t];
+        const int kk = c.s + t;
+        const real Ux = ldo(F.dX[0], i), Vx = ldo(F.dX[1], i), Wx = ldo(F.dX[2], i),
+                   Tx = ldo(F.dX[3], i);
+        const real Uy = ldo(F.dY[0], i), Vy = ldo(F.dY[1], i), Wy = ldo(F.dY[2], i),
+                   Ty = ldo(F.dY[3], i);
         const real uz = Uz[t], vz = Vz[t], wz = Wz[t];
         const real MuX = amu[t] * Tx, MuY = amu[t] * Ty, MuZ = amu[t] * Tz[t];
         const real Uu = Uc[t], Vv = Vc[t], Ww = Wc[t], m_ = mu[t];
@@ -164,33 +165,31 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
                    Vv * MuX * (Uy + Vx) + Vv * MuZ * (vz + Wy) + Ww * MuX * (uz + Wx) +
                    Ww * MuY * (vz + Wy);
         real rhs[5];
-        rhs[0] = F.R[0][i] + R0[t];
-        rhs[1] = F.R[1][i] + Ru[t] + su;
-        rhs[2] = F.R[2][i] + Rv[t] + sv;
-        rhs[3] = F.R[3][i] + Rw[t] + sw;
-        rhs[4] = F.R[4][i] + RE[t] - ph.h_ref * wz + qn + m_ * quad + vt2;
-        real qs_[5];
-        if (F.sigma || (sa.rk == 3 && sa.rkStep > 1)) {
-            qs_[0] = q0[i];
-            qs_[1] = q1[i];
-            qs_[2] = q2[i];
-            qs_[3] = q3[i];
-            qs_[4] = q4[i];
+        rhs[0] = ldo(F.R[0], i) + R0[t];
+        rhs[1] = ldo(F.R[1], i) + Ru[t] + su;
+        rhs[2] = ldo(F.R[2], i) + Rv[t] + sv;
+        rhs[3] = ldo(F.R[3], i) + Rw[t] + sw;
+        rhs[4] = ldo(F.R[4], i) + RE[t] - ph.h_ref * wz + qn + m_ * quad + vt2;
+        real qs_[5] = {0, 0, 0, 0, 0};
+        if (QS) {
+            qs_[0] = ldo(q0, i);
+            qs_[1] = ldo(q1, i);
+            qs_[2] = ldo(q2, i);
+            qs_[3] = ldo(q3, i);
+            qs_[4] = ldo(q4, i);
         }
-        if (F.sigma) {
-            const real sg = F.sigma[i];
+        if (QS && F.sigma) {
+            const real sg = ldo(F.sigma, i);
 #pragma unroll
-            for (int v = 0; v < 5; v++) rhs[v] += sg * (F.avg[v][i] - qs_[v]);
-            rhs[4] += sg * (F.avg[4][i] - qs_[4]);
+            for (int v = 0; v < 5; v++) rhs[v] += sg * (ldo(F.avg[v], i) - qs_[v]);
+            rhs[4] += sg * (ldo(F.avg[4], i) - qs_[4]);
         }
         real k[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) k[v] = dt * rhs[v];
         {
-            const int ii = (int)(i % Nx), jj = (int)((i / Nx) % Ny), kk = (int)(i / Nxy);
-            const bool plane = (sa.sponge_face[0] && ii == 0) || (sa.sponge_face[1] && ii == Nx - 1) ||
-                               (sa.sponge_face[2] && jj == 0) || (sa.sponge_face[3] && jj == Ny - 1) ||
-                               (sa.sponge_face[4] && kk == 0) || (sa.sponge_face[5] && kk == sa.Nz - 1);
+            const bool plane = line_plane || (sa.sponge_face[4] && kk == 0) ||
+                               (sa.sponge_face[5] && kk == sa.Nz - 1);
             if (plane) {
 #pragma unroll
                 for (int v = 0; v < 5; v++) k[v] = 0;
@@ -215,60 +214,58 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
             td += S12 * Wy;
             td += S22 * wz;
             const real wx_ = Wy - vz, wy_ = uz - Wx, wz_ = Vx - Uy;
-            F.diag[0][i] = td;
-            F.diag[1][i] = wx_ * wx_ + wy_ * wy_ + wz_ * wz_;
-            F.diag[2][i] = dil;
-            F.diag[3][i] = (Ux * Ux + Vy * Vy + wz * wz) / real(3);
+            sto(F.diag[0], i, td);
+            sto(F.diag[1], i, wx_ * wx_ + wy_ * wy_ + wz_ * wz_);
+            sto(F.diag[2], i, dil);
+            sto(F.diag[3], i, (Ux * Ux + Vy * Vy + wz * wz) * c13);
         }
         if (sa.mode == 1) {
 #pragma unroll
-            for (int v = 0; v < 5; v++) F.k2[v][i] = k[v];
+            for (int v = 0; v < 5; v++) sto(F.k2[v], i, k[v]);
             continue;
         }
         if (sa.rk == 4) {
-            const real aa = (sa.rkStep == 1 || sa.rkStep == 4) ? real(1) / real(6) : real(1) / real(3);
-            const real bb = sa.rkStep == 3 ? real(1) : real(0.5);
+            // RK4.hpp:45-111: *2 accumulates the stages, *k is the next
+            // stage state
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                const real qn1 = F.q[v][i];
-                if (sa.rkStep == 1)
-                    F.acc[v][i] = qn1 + aa * k[v];
-                else
-                    F.acc[v][i] = F.acc[v][i] + aa * k[v];
-                if (sa.rkStep < 4) F.qk[v][i] = qn1 + bb * k[v];
+                const real qn1 = ldo(F.q[v], i);
+                const real a0 = first ? qn1 : ldo(F.acc[v], i);
+                sto(F.acc[v], i, a0 + ra * k[v]);
+                if (has_next) sto(F.qk[v], i, qn1 + rb * k[v]);
             }
         } else {
+            // TVDRK3.hpp:225-260: stages 1-2 write *k, stage 3 writes *2
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                const real qn1 = F.q[v][i];
-                if (sa.rkStep == 1)
-                    F.qk[v][i] = qn1 + k[v];
-                else if (sa.rkStep == 2)
-                    F.qk[v][i] = real(0.75) * qn1 + real(0.25) * qs_[v] + real(0.25) * k[v];
-                else
-                    F.acc[v][i] = (real(1) / real(3)) * qn1 + c23 * qs_[v] + c23 * k[v];
+                const real qn1 = ldo(F.q[v], i);
+                sto(tvd_to_k ? F.qk[v] : F.acc[v], i, ta * qn1 + tb * qs_[v] + tc * k[v]);
             }
         }
     }
 }
 
 template <typename real, bool UNI>
-hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, int threads, int LX,
+hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         const StageArgs &sa, const StepScalars *sc, hipStream_t st)
 {
-    dim3 grid((lm.nlines + LX - 1) / LX), block(threads);
-    size_t shm = 2 * (size_t)threads * sizeof(real);
-    hipLaunchKernelGGL((k_zpass<real, SEG_DERIV, UNI>), grid, block, shm, st, d1, d2, lm, F, ph,
-                       sa, sc);
+    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    if (F.sigma || sa.rk == 3)
+        hipLaunchKernelGGL((k_zpass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,
+                           lm, F, ph, sa, sc);
+    else
+        hipLaunchKernelGGL((k_zpass<real, SEG_DERIV, UNI, false>), grid, block, lg.shm, st, d1,
+                           d2, lm, F, ph, sa, sc);
     return hipGetLastError();
 }
 
-#define CS3D_INST_Z(real, UNI)                                                               \
-    template hipError_t launch_zpass<real, UNI>(const OpDev<real> *, const OpDev<real> *, int,  \
-                                                int, const LineMap &, const Fields<real> &,     \
-                                                const Phys<real> &, const StageArgs &,          \
-                                                const StepScalars *, hipStream_t);
+#define CS3D_INST_Z(real, UNI)                                                                  \
+    template hipError_t launch_zpass<real, UNI>(const OpDev<real> *, const OpDev<real> *,          \
+                                                const LaunchGeom &, const LineMap &,               \
+                                                const Fields<real> &, const Phys<real> &,          \
+                                                const StageArgs &, const StepScalars *,            \
+                                                hipStream_t);
 CS3D_INST_Z(float, true)
 CS3D_INST_Z(float, false)
 CS3D_INST_Z(double, true)
