@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <stdexcept>
+#include <cstdint>
 #include <string>
 
 #include "../../include/cs3d.h"
@@ -97,7 +98,11 @@ int cs3d_ops_apply(cs3d_ops *o, int kind, int layout, const void *in, void *out,
         if (e != hipSuccess) return set_error(CS3D_ERR_HIP, hip_msg(e, "cs3d_ops_apply: H2D"));
         din = o->dbuf_in;
         dout = o->dbuf_out;
-    } else if (ptr_kind != CS3D_DEVICE) {
+    } else if (ptr_kind == CS3D_DEVICE) {
+        // contiguous lines are read and written as 16-byte chunks
+        if (((uintptr_t)in | (uintptr_t)out) % 16 != 0)
+            return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: device pointers must be 16-byte aligned");
+    } else {
         return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: bad ptr_kind");
     }
     // an earlier asynchronous failure is reported as such, not as this launch's

@@ -161,27 +161,6 @@ __device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restri
 
 // ------------------------------------------------------------ addressing
 
-// Byte offsets of the window f[s-HW .. s+M+HW) of the line at element
-// `base` with point stride `sp`; periodic lines wrap, non-periodic ones clamp
-// (clamped entries are never used by a stencil).
-template <typename real, int M, int HW>
-__device__ __forceinline__ void window_offsets(unsigned base, unsigned sp, int s, int n,
-                                               int periodic, unsigned (&off)[M + 2 * HW])
-{
-#pragma unroll
-    for (int t = 0; t < M + 2 * HW; t++) {
-        int p = s - HW + t;
-        if (periodic) {
-            p += p < 0 ? n : 0;
-            p -= p >= n ? n : 0;
-        } else {
-            p = p < 0 ? 0 : p;
-            p = p >= n ? n - 1 : p;
-        }
-        off[t] = (base + (unsigned)p * sp) * (unsigned)sizeof(real);
-    }
-}
-
 template <typename real>
 __device__ __forceinline__ real ldo(const real *__restrict__ f, unsigned boff)
 {
@@ -194,12 +173,103 @@ __device__ __forceinline__ void sto(real *__restrict__ f, unsigned boff, real v)
     *(real *)((char *)f + boff) = v;
 }
 
-template <typename real, int W>
-__device__ __forceinline__ void load_w(const real *__restrict__ f, const unsigned (&off)[W],
-                                       real (&w)[W])
-{
+// A lane's window f[s-HW .. s+M+HW) of one line.
+//
+// Scalar form: the byte offset of every window point (periodic lines wrap,
+// non-periodic ones clamp; clamped entries are never used by a stencil).
+//
+// VEC form (contiguous lines, uniform periodic plans, M a multiple of the
+// 16-byte vector width VW): the window, widened to whole VW-aligned chunks,
+// is read with one 16-byte load per chunk, and the segment is stored the same
+// way.  With n a multiple of VW no chunk straddles the periodic wrap.
+template <typename real, int M, int HW, bool VEC>
+struct Win {
+    static constexpr int W = M + 2 * HW;
+    static constexpr int VW = 16 / (int)sizeof(real);
+    static constexpr int LO = VEC ? ((HW + VW - 1) / VW) * VW : HW;  // left extent
+    static constexpr int D = LO - HW;                                // chunk -> window shift
+    static constexpr int NC = VEC ? (M + HW + LO + VW - 1) / VW : W;
+    static_assert(!VEC || M % VW == 0, "vector windows need whole chunks per segment");
+    typedef real vec __attribute__((ext_vector_type(16 / sizeof(real))));
+    unsigned off[NC];
+
+    __device__ __forceinline__ void init(unsigned base, unsigned sp, int s, int n, int periodic)
+    {
 #pragma unroll
-    for (int t = 0; t < W; t++) w[t] = ldo(f, off[t]);
+        for (int c = 0; c < NC; c++) {
+            int p = VEC ? s - LO + c * VW : s - HW + c;
+            if (VEC || periodic) {
+                p += p < 0 ? n : 0;
+                p -= p >= n ? n : 0;
+            } else {
+                p = p < 0 ? 0 : p;
+                p = p >= n ? n - 1 : p;
+            }
+            off[c] = (base + (unsigned)p * sp) * (unsigned)sizeof(real);
+        }
+    }
+    // byte offset of window point t
+    __device__ __forceinline__ unsigned at(int t) const
+    {
+        if (VEC) return off[(t + D) / VW] + (unsigned)(((t + D) % VW) * sizeof(real));
+        return off[t];
+    }
+    __device__ __forceinline__ void load(const real *__restrict__ f, real (&w)[W]) const
+    {
+        if (VEC) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const vec v = *(const vec *)((const char *)f + off[c]);
+#pragma unroll
+                for (int e = 0; e < VW; e++) {
+                    const int t = c * VW + e - D;
+                    if (t >= 0 && t < W) w[t] = v[e];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < W; t++) w[t] = ldo(f, off[t]);
+        }
+    }
+    // the segment's own M values
+    __device__ __forceinline__ void load_seg(const real *__restrict__ f, real (&r)[M]) const
+    {
+        if (VEC) {
+#pragma unroll
+            for (int c = 0; c < M / VW; c++) {
+                const vec v = *(const vec *)((const char *)f + off[LO / VW + c]);
+#pragma unroll
+                for (int e = 0; e < VW; e++) r[c * VW + e] = v[e];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; t++) r[t] = ldo(f, off[HW + t]);
+        }
+    }
+    // store the segment's M values (the first len of them on table plans)
+    __device__ __forceinline__ void store(real *__restrict__ f, const real (&r)[M], int len) const
+    {
+        if (VEC) {
+#pragma unroll
+            for (int c = 0; c < M / VW; c++) {
+                vec v;
+#pragma unroll
+                for (int e = 0; e < VW; e++) v[e] = r[c * VW + e];
+                *(vec *)((char *)f + off[LO / VW + c]) = v;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; t++)
+                if (t < len) sto(f, off[HW + t], r[t]);
+        }
+    }
+};
+
+// whether a kernel instantiation reads its windows as 16-byte chunks
+template <typename real, int M, int MAP, bool UNI>
+constexpr bool vec_windows()
+{
+    return MAP == MAP_ROW && UNI && M % (16 / (int)sizeof(real)) == 0;
 }
 
 // ------------------------------------------------------------ stencils
@@ -220,14 +290,23 @@ __device__ __forceinline__ void stencil(const OpDev<real> *__restrict__ op,
         } else if (KIND == OP_D2) {
             static_assert(HW >= 2, "window too narrow");
             r[t] = t0 * w[o] + t1 * (w[o - 1] + w[o + 1]) + t2 * (w[o - 2] + w[o + 2]);
-        } else {
-            static_assert(KIND != OP_FILTER || HW >= 4, "window too narrow");
-            if constexpr (HW >= 4) {
-                const real t3 = op->tap[3], t4 = op->tap[4];
-                r[t] = t0 * w[o] + t1 * (w[o - 1] + w[o + 1]) + t2 * (w[o - 2] + w[o + 2]) +
-                       t3 * (w[o - 3] + w[o + 3]) + t4 * (w[o - 4] + w[o + 4]);
-            }
         }
+    }
+    if (KIND == OP_FILTER) {
+        // (A - B) f = kappa * D8 f, D8 as four nested second differences
+        static_assert(KIND != OP_FILTER || HW >= 4, "window too narrow");
+        constexpr int W = M + 2 * HW;
+        real e[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) e[j] = w[j];
+#pragma unroll
+        for (int lev = 1; lev <= 4; lev++) {
+#pragma unroll
+            for (int j = 0; j + 2 * lev < W; j++) e[j] = (e[j] + e[j + 2]) - real(2) * e[j + 1];
+        }
+        // e[j] is now D8 f at window index j + 4
+#pragma unroll
+        for (int t = 0; t < M; t++) r[t] = t0 * e[HW - 4 + t];
     }
 }
 

@@ -161,18 +161,19 @@ __global__ __launch_bounds__(1024) void k_field_op(const OpDev<real> *__restrict
     Lane<real, MAP> c = make_lane<real, MAP>(op, P, LX, op->K > KC ? op->K : KC, 1, xb);
     const int L = blockIdx.x * LX + c.line;
     const bool active = L < lm.nlines;
-    unsigned off[W];
-    window_offsets<real, M, HW>((unsigned)lm.base(active ? L : lm.nlines - 1), (unsigned)lm.sp,
-                                c.s, op->n, op->cyclic, off);
+    Win<real, M, HW, vec_windows<real, M, MAP, UNI>()> win;
+    win.init((unsigned)lm.base(active ? L : lm.nlines - 1), (unsigned)lm.sp, c.s, op->n,
+             op->cyclic);
     real w[W], r[1][M];
-    load_w(in, off, w);
+    win.load(in, w);
     rhs<KIND, real, M, HW, UNI>(op, c, w, r[0]);
     solve_batch<1, KC, real, M, UNI>(op, c, r);
-    if (active) {
+    if (KIND == OP_FILTER) {
+        // the filter output is f - A^-1 (A - B) f
 #pragma unroll
-        for (int t = 0; t < M; t++)
-            if (UNI || t < c.len) sto(out, off[HW + t], r[0][t]);
+        for (int t = 0; t < M; t++) r[0][t] = w[HW + t] - r[0][t];
     }
+    if (active) win.store(out, r[0], c.len);
 }
 
 template <int KIND, typename real, int M, bool UNI>

@@ -178,6 +178,34 @@ OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f,
         }
     }
 
+    // ---- the filter in difference form.  The kernels apply the filter as
+    // out = f - A^-1 (A - B) f.  In the interior (A - B) is exactly
+    // kappa * D8 with kappa = (1 - 2 alpha_f)/256 and D8 the 8th central
+    // difference (1, -8, 28, -56, 70, -56, 28, -8, 1): the two operators
+    // have the symbol (1 - 2 alpha_f)(1 - cos theta)^4 / 16.  Evaluated as
+    // nested second differences this never rounds the low-wavenumber gain
+    // (a direct B f in FP32 moves it by ~1e-8 per application, thousands of
+    // times the filter's own effect on resolved scales).  Boundary rows keep
+    // explicit (A - B) rows.
+    if (kind == OP_FILTER) {
+        const double al = alpha_f;
+        h.tap[0] = (1.0 - 2.0 * al) / 256.0;
+        for (int i = 1; i < 5; i++) h.tap[i] = 0.0;
+        if (!periodic) {
+            // A rows inside the 7-point closure windows: identity on rows
+            // 0-2 and n-3..n-1, (alpha, 1, alpha) on rows 3 and n-4
+            const double Al[4][7] = {{1, 0, 0, 0, 0, 0, 0}, {0, 1, 0, 0, 0, 0, 0},
+                                     {0, 0, 1, 0, 0, 0, 0}, {0, 0, al, 1, al, 0, 0}};
+            const double Ar[4][7] = {{0, 0, al, 1, al, 0, 0}, {0, 0, 0, 0, 1, 0, 0},
+                                     {0, 0, 0, 0, 0, 1, 0}, {0, 0, 0, 0, 0, 0, 1}};
+            for (int r = 0; r < 4; r++)
+                for (int u = 0; u < 7; u++) {
+                    h.bcl[r][u] = Al[r][u] - h.bcl[r][u];
+                    h.bcr[r][u] = Ar[r][u] - h.bcr[r][u];
+                }
+        }
+    }
+
     // ---- factorisation into the two recurrences
     h.m.assign(n, 0.0);
     h.iw.assign(n, 1.0);

@@ -49,10 +49,13 @@ struct OpDev {
     int K;        // cross-segment terms kept (forward and backward)
     int pad_;
     real scale;   // output scale (1 + rho^2 for periodic, 1 otherwise)
-    real tap[5];  // interior stencil: D1 t1,t2 (odd), D2 t0,t1,t2,
-                  // filter c0..c4 (even), already divided by dh or dh^2
+    real tap[5];  // interior stencil: D1 t1,t2 (odd), D2 t0,t1,t2, already
+                  // divided by dh or dh^2; filter: tap[0] = (1 - 2 alpha_f)/256,
+                  // the weight of the 8th difference in (A - B) f
     real bcl[4][7];  // row r < NB:        sum_u bcl[r][u] f[u]
     real bcr[4][7];  // row n - NB + r:    sum_u bcr[r][u] f[n - 7 + u]
+                     // (for the filter these are rows of A - B: the filter
+                     // output is f - A^-1 (A - B) f)
     // uniform path
     real um, umu, uiw;
     real upf[MAX_SEG];   // forward fix-up, local index t

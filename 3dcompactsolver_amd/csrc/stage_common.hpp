@@ -65,7 +65,7 @@ __device__ __forceinline__ void prim(const Phys<real> &ph, real rp, real ru, rea
 // Per-lane set-up of a sweep kernel: lane geometry, exchange ring (margin KRV,
 // PLANES planes per parity), the line this lane works on (inactive lanes of a
 // partial last workgroup take the last line and only skip their stores) and
-// the byte offsets `off` of the lane's window.  Needs M, HW and W in scope.
+// the lane's window `win`.  Needs M, HW, W and UNI in scope.
 #define CS3D_LANE_SETUP(MAPV, OPP, KRV, PLANES)                                         \
     extern __shared__ char smem_raw[];                                                  \
     real *xb = (real *)smem_raw;                                                        \
@@ -76,8 +76,8 @@ __device__ __forceinline__ void prim(const Phys<real> &ph, real rp, real ru, rea
     const bool active = L < lm.nlines;                                                  \
     const int Lc = active ? L : lm.nlines - 1;                                          \
     const int n = (OPP)->n, per = (OPP)->cyclic;                                        \
-    unsigned off[W];                                                                    \
-    window_offsets<real, M, HW>((unsigned)lm.base(Lc), (unsigned)lm.sp, c.s, n, per, off);
+    Win<real, M, HW, vec_windows<real, M, MAPV, UNI>()> win;                            \
+    win.init((unsigned)lm.base(Lc), (unsigned)lm.sp, c.s, n, per);
 
 // ring margin of a sweep that uses both derivative operators
 template <typename real>

@@ -23,17 +23,21 @@ __global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *_
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         real w[W];
-        load_w(io.in[v], off, w);
+        win.load(io.in[v], w);
         rhs<OP_FILTER, real, M, HW, UNI>(fo, c, w, r[v]);
     }
     solve_batch<5, kc_filter<real>(), real, M, UNI>(fo, c, r);
     if (active) {
 #pragma unroll
-        for (int t = 0; t < M; t++)
-            if (UNI || t < c.len) {
+        for (int v = 0; v < 5; v++) {
+            // out = f - A^-1 (A - B) f, with the segment's f read again
+            // (an L2 hit) rather than held through the solve
+            real f[M];
+            win.load_seg(io.in[v], f);
 #pragma unroll
-                for (int v = 0; v < 5; v++) sto(io.out[v], off[HW + t], r[v][t]);
-            }
+            for (int t = 0; t < M; t++) f[t] -= r[v][t];
+            win.store(io.out[v], f, c.len);
+        }
     }
 }
 

@@ -32,17 +32,16 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
-    real a1[W], a2[W], a3[W], a4[W];
+    real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
+    win.load(q0, a0);
+    win.load(q1, a1);
+    win.load(q2, a2);
+    win.load(q3, a3);
+    win.load(q4, a4);
 #pragma unroll
-    for (int t = 0; t < W; t++) {
-        const real a0 = ldo(q0, off[t]);
-        a1[t] = ldo(q1, off[t]);
-        a2[t] = ldo(q2, off[t]);
-        a3[t] = ldo(q3, off[t]);
-        a4[t] = ldo(q4, off[t]);
-        prim(ph, a0, a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
-    }
+    for (int t = 0; t < W; t++)
+        prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
     if (STALE) {
         // primitives on sponge planes from the pre-reset copy
         const int jj = Lc % sa.Ny, kk = Lc / sa.Ny;
@@ -58,9 +57,9 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
             const bool plane = line_plane || (sa.sponge_face[0] && p == 0) ||
                                (sa.sponge_face[1] && p == n - 1);
             if (plane)
-                prim(ph, ldo(F.qstale[0], off[t]), ldo(F.qstale[1], off[t]),
-                     ldo(F.qstale[2], off[t]), ldo(F.qstale[3], off[t]),
-                     ldo(F.qstale[4], off[t]), ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
+                prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
+                     ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
+                     ldo(F.qstale[4], win.at(t)), ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
         }
     }
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
@@ -118,21 +117,23 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
         RE[t] += mu * (Wv[HW + t] * r[7][t]);
     }
     if (active) {
+        real Ux[M], Vx[M], Wx[M], Tx[M];
 #pragma unroll
         for (int t = 0; t < M; t++) {
-            if (UNI || t < c.len) {
-                const unsigned o = off[HW + t];
-                sto(F.dX[0], o, r[1][t]);
-                sto(F.dX[1], o, r[2][t]);
-                sto(F.dX[2], o, r[3][t]);
-                sto(F.dX[3], o, r[0][t]);
-                sto(F.R[0], o, R0[t]);
-                sto(F.R[1], o, Ru[t]);
-                sto(F.R[2], o, Rv[t]);
-                sto(F.R[3], o, Rw[t]);
-                sto(F.R[4], o, RE[t]);
-            }
+            Tx[t] = r[0][t];
+            Ux[t] = r[1][t];
+            Vx[t] = r[2][t];
+            Wx[t] = r[3][t];
         }
+        win.store(F.dX[0], Ux, c.len);
+        win.store(F.dX[1], Vx, c.len);
+        win.store(F.dX[2], Wx, c.len);
+        win.store(F.dX[3], Tx, c.len);
+        win.store(F.R[0], R0, c.len);
+        win.store(F.R[1], Ru, c.len);
+        win.store(F.R[2], Rv, c.len);
+        win.store(F.R[3], Rw, c.len);
+        win.store(F.R[4], RE, c.len);
     }
 }
 

@@ -23,17 +23,16 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
-    real a1[W], a2[W], a3[W], a4[W];
+    real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
+    win.load(q0, a0);
+    win.load(q1, a1);
+    win.load(q2, a2);
+    win.load(q3, a3);
+    win.load(q4, a4);
 #pragma unroll
-    for (int t = 0; t < W; t++) {
-        const real a0 = ldo(q0, off[t]);
-        a1[t] = ldo(q1, off[t]);
-        a2[t] = ldo(q2, off[t]);
-        a3[t] = ldo(q3, off[t]);
-        a4[t] = ldo(q4, off[t]);
-        prim(ph, a0, a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
-    }
+    for (int t = 0; t < W; t++)
+        prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
     const real c43 = real(4) / real(3), c13 = real(1) / real(3);
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M], Uxy[M], Vxy[M];
     {
@@ -53,9 +52,9 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = a3[t] * V[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
-        load_w(F.dX[0], off, f);
+        win.load(F.dX[0], f);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
-        load_w(F.dX[1], off, f);
+        win.load(F.dX[1], f);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
         solve_batch<7, kc_deriv<real>(), real, M, UNI>(d1, c, r);
 #pragma unroll
@@ -103,7 +102,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
 #pragma unroll
         for (int t = 0; t < M; t++) {
             if (UNI || t < c.len) {
-                const unsigned o = off[HW + t];
+                const unsigned o = win.at(HW + t);
                 sto(F.dY[0], o, r[1][t]);
                 sto(F.dY[1], o, r[2][t]);
                 sto(F.dY[2], o, r[3][t]);

@@ -29,17 +29,16 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
-    real a1[W], a2[W], a3[W], a4[W];
+    real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
+    win.load(q0, a0);
+    win.load(q1, a1);
+    win.load(q2, a2);
+    win.load(q3, a3);
+    win.load(q4, a4);
 #pragma unroll
-    for (int t = 0; t < W; t++) {
-        const real a0 = ldo(q0, off[t]);
-        a1[t] = ldo(q1, off[t]);
-        a2[t] = ldo(q2, off[t]);
-        a3[t] = ldo(q3, off[t]);
-        a4[t] = ldo(q4, off[t]);
-        prim(ph, a0, a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
-    }
+    for (int t = 0; t < W; t++)
+        prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
     const real c43 = real(4) / real(3), c13 = real(1) / real(3), c23 = real(2) / real(3);
     // One batch of 13 solves from the stage state: the z fluxes (the h_ref W
     // part of the energy flux is added as h_ref Wz in the assembly), then
@@ -102,11 +101,11 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         // derivative of Ux + Vy.
         real m3[3][M], f[W];
 #pragma unroll
-        for (int t = 0; t < W; t++) f[t] = ldo(F.dX[0], off[t]) + ldo(F.dY[1], off[t]);
+        for (int t = 0; t < W; t++) f[t] = ldo(F.dX[0], win.at(t)) + ldo(F.dY[1], win.at(t));
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[0]);
-        load_w(F.dX[2], off, f);
+        win.load(F.dX[2], f);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[1]);
-        load_w(F.dY[2], off, f);
+        win.load(F.dY[2], f);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[2]);
         solve_batch<3, kc_deriv<real>(), real, M, UNI>(d1, c, m3);
 #pragma unroll
@@ -140,7 +139,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
 #pragma unroll
     for (int t = 0; t < M; t++) {
         if (!UNI && t >= c.len) continue;
-        const unsigned i = off[HW + t];
+        const unsigned i = win.at(HW + t);
         const int kk = c.s + t;
         const real Ux = ldo(F.dX[0], i), Vx = ldo(F.dX[1], i), Wx = ldo(F.dX[2], i),
                    Tx = ldo(F.dX[3], i);

@@ -3,7 +3,8 @@ CPU: a numpy emulation of the kernels' segmented two-recurrence solve with the
 library's own tables must reproduce the oracle's Thomas / Sherman-Morrison
 operators (Utils.cpp:4-46, Derivatives.cpp:250-284, Filter.cpp:98-117).
 This pins the factorisation, the segmentation, the cross-segment carries and
-their truncation without a GPU."""
+their truncation without a GPU, and the filter's difference form
+f - A^-1 (A - B) f with (A - B) = (1 - 2 alpha_f)/256 D8 in the interior."""
 import ctypes
 
 import numpy as np
@@ -54,7 +55,9 @@ def emulate(p, kind, f):
     elif kind == 2:
         r = t[0] * c(0) + t[1] * (c(-1) + c(1)) + t[2] * (c(-2) + c(2))
     else:
-        r = t[0] * c(0) + sum(t[k] * (c(-k) + c(k)) for k in range(1, 5))
+        # filter plans hold (A - B): kappa times the 8th central difference
+        d8 = (1, -8, 28, -56, 70, -56, 28, -8, 1)
+        r = t[0] * sum(w * c(k - 4) for k, w in enumerate(d8))
     for i in range(p["NB"]):
         r[i] = p["bcl"][i] @ f[0:7]
         r[n - p["NB"] + i] = p["bcr"][i] @ f[n - 7:n]
@@ -97,7 +100,7 @@ def emulate(p, kind, f):
                 break
             D += p["psi"][gi, k - 1] * starts[gg]
         out[s:s + ln] = p["scale"] * (x[s:s + ln] + p["pb"][s:s + ln] * D)
-    return out
+    return f - out if kind == 0 else out
 
 
 @pytest.mark.parametrize("n", [8, 9, 12, 42, 64, 128, 256])
@@ -140,3 +143,35 @@ def test_plan_segment_length_matches_a_compiled_kernel(kind, bc):
         else:
             assert p["M"] == 4
             assert not (bc == 0 and n % mu == 0)
+
+
+@pytest.mark.parametrize("af", [0.49, 0.495, 0.3, -0.2])
+def test_filter_difference_form_symbol(af):
+    """(A - B) of the 8th-order filter is exactly (1 - 2 alpha_f)/256 times the
+    8th central difference: the symbol (1 + 2 alpha_f cos t) - B(t), with B's
+    taps from Filter.hpp:54-63, equals (1 - 2 alpha_f)(1 - cos t)^4 / 16."""
+    a0 = (93 + 70 * af) / 128
+    a = [(7 + 18 * af) / 16, (-7 + 14 * af) / 32, (1 - 2 * af) / 16, (-1 + 2 * af) / 128]
+    th = np.linspace(0.3, np.pi, 40)
+    sym = (1 + 2 * af * np.cos(th)) - (a0 + sum(a[n - 1] * np.cos(n * th) for n in range(1, 5)))
+    ref = (1 - 2 * af) * (1 - np.cos(th)) ** 4 / 16
+    assert np.max(np.abs(sym - ref)) < 1e-15
+    p = plan(0, 0, 64, 0.1, af, cs3d.FP64)
+    assert abs(p["tap"][0] - (1 - 2 * af) / 256) < 1e-18 and not np.any(p["tap"][1:])
+
+
+def test_filter_low_wavenumber_gain_in_fp32():
+    """In FP32 the difference form keeps the gain of a resolved wave at 1 to
+    rounding: applied to a k = 1 wave on 64 points (the Taylor-Green scale),
+    the FP32-rounded plan moves the gain by < 1e-12, where a direct B f with
+    FP32 taps moves it by ~1e-8 per application."""
+    n = 64
+    p = plan(0, 0, n, 0.1, 0.49, cs3d.FP32)
+    q = dict(p)
+    for key in ("tap", "m", "iw", "mu", "pf", "pb", "phi", "psi", "bcl", "bcr"):
+        q[key] = p[key].astype(np.float32).astype(np.float64)
+    q["scale"] = float(np.float32(p["scale"]))
+    i = np.arange(n)
+    f = np.sin(2 * np.pi * i / n) + 0.3 * np.cos(2 * np.pi * i / n)
+    gain = np.dot(emulate(q, 0, f), f) / np.dot(f, f)
+    assert abs(gain - 1) < 1e-12
