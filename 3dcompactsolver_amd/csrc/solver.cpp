@@ -40,7 +40,14 @@ static int env_cap(const char *name, int cap)
 
 template <typename real>
 struct SolverT : SolverBase {
-    static int sweep_max_threads() { return env_cap("CS3D_SWEEP_THREADS", sizeof(real) == 4 ? 1024 : 512); }
+    // Y/Z sweeps want 16+ lines per workgroup (64-byte rows of x); the X
+    // sweep reads whole lines and runs best with small workgroups that let
+    // several resident per CU overlap their load and solve phases
+    static int sweep_max_threads(int dir)
+    {
+        if (dir == 0) return env_cap("CS3D_SWEEP_X_THREADS", 256);
+        return env_cap("CS3D_SWEEP_THREADS", sizeof(real) == 4 ? 1024 : 512);
+    }
     static int filter_max_threads() { return env_cap("CS3D_FILTER_THREADS", 512); }
 
     cs3d_config cfg;
@@ -96,7 +103,7 @@ struct SolverT : SolverBase {
                 throw std::invalid_argument("inconsistent derivative plans");
             const int P = D1[d].host.P;
             const int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
-            sweepG[d] = line_geom(P, kr, sweep_max_threads(), SWEEP_PLANES, sizeof(real));
+            sweepG[d] = line_geom(P, kr, sweep_max_threads(d), SWEEP_PLANES, sizeof(real));
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
                                    filter_max_threads(), 5, sizeof(real));

@@ -1,0 +1,536 @@
+/*
+ * cs3d.hpp -- C++ drop-in for the solver classes of emathew1/3DCompactSolver,
+ * implemented over the C ABI of cs3d.h (header-only; link
+ * lib3dcompact_hip.so).
+ *
+ * Same class names, constructors, public members and call sequence as the
+ * reference:
+ *   Domain        Domain.hpp:6-52         TimeStepping  TimeStepping.hpp:6-45
+ *   BC            BC.hpp:6-27             IdealGas      IdealGas.hpp:8-55
+ *   Derivatives   Derivatives.hpp:13-183  Filter        Filter.hpp:11-115
+ *   AbstractCSolver AbstractCSolver.hpp:9-43
+ *   AbstractRK    AbstractRK.hpp:6-14     RK4 / TVDRK3  RK4.hpp:7-170, TVDRK3.hpp:7-139
+ *   CSolver_AWS   CSolver_AWS.hpp:119 (constructor), CSolver_AWS.cpp (phases)
+ * The classes live in namespace cs3d; define CS3D_DROPIN_GLOBAL before the
+ * include to also bring them into the global namespace, in place of the
+ * reference's headers.
+ *
+ * What differs from the reference, by design:
+ *  - CSolver_AWS keeps its state on the GPU.  The conserved quadruples
+ *    (rho1 ... rhoE2) are host mirrors, filled by syncHostState() and by the
+ *    diagnostics/dump calls; the IC arrays rho0 ... W0 are read once by
+ *    setInitialConditions() (the reference deletes them there; here they
+ *    stay valid until the solver is destroyed).
+ *  - The RK stage update (AbstractRK::updateConservedData) is fused into the
+ *    last sweep of each stage on the GPU; RK4/TVDRK3::updateConservedData
+ *    are kept for the call sequence and do nothing.
+ *  - Errors: the reference returns void and prints; here a failed library
+ *    call prints the library's message to std::cerr and aborts.
+ */
+#ifndef CS3D_HPP
+#define CS3D_HPP
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "cs3d.h"
+
+namespace cs3d {
+
+inline void check(int rc, const char *what)
+{
+    if (rc != CS3D_OK) {
+        std::cerr << " > cs3d error in " << what << ": " << cs3d_last_error() << std::endl;
+        std::abort();
+    }
+}
+
+// Domain.hpp:6-52: x_i = i L / (N - 1)
+class Domain {
+  public:
+    int Nx, Ny, Nz, N;
+    double *x, *y, *z;
+    double Lx, Ly, Lz;
+    double dx, dy, dz;
+
+    Domain(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz)
+        : Nx(Nx), Ny(Ny), Nz(Nz), N(Nx * Ny * Nz), Lx(Lx), Ly(Ly), Lz(Lz),
+          xs(Nx), ys(Ny), zs(Nz)
+    {
+        for (int i = 0; i < Nx; i++) xs[i] = ((double)i / ((double)Nx - 1.0)) * Lx;
+        for (int j = 0; j < Ny; j++) ys[j] = ((double)j / ((double)Ny - 1.0)) * Ly;
+        for (int k = 0; k < Nz; k++) zs[k] = ((double)k / ((double)Nz - 1.0)) * Lz;
+        x = xs.data();
+        y = ys.data();
+        z = zs.data();
+        dx = x[1] - x[0];
+        dy = y[1] - y[0];
+        dz = z[1] - z[0];
+    }
+
+  private:
+    std::vector<double> xs, ys, zs;
+};
+
+// BC.hpp:6-27
+class BC {
+  public:
+    enum BCType { PERIODIC_SOLVE = CS3D_PERIODIC_SOLVE, DIRICHLET_SOLVE = CS3D_DIRICHLET_SOLVE };
+    enum BCKind {
+        PERIODIC = CS3D_BC_PERIODIC,
+        SPONGE = CS3D_BC_SPONGE,
+        ADIABATIC_WALL = CS3D_BC_ADIABATIC_WALL,
+        MOVING_ADIABATIC_WALL = CS3D_BC_MOVING_ADIABATIC_WALL,
+        INLET = CS3D_BC_INLET
+    };
+    BCType bcXType, bcYType, bcZType;
+    BCKind bcX0, bcX1, bcY0, bcY1, bcZ0, bcZ1;
+
+    BC(BCType bcXType, BCKind bcX0, BCKind bcX1, BCType bcYType, BCKind bcY0, BCKind bcY1,
+       BCType bcZType, BCKind bcZ0, BCKind bcZ1)
+        : bcXType(bcXType), bcYType(bcYType), bcZType(bcZType), bcX0(bcX0), bcX1(bcX1),
+          bcY0(bcY0), bcY1(bcY1), bcZ0(bcZ0), bcZ1(bcZ1)
+    {
+    }
+};
+
+// TimeStepping.hpp:6-45.  The reference's calcDtFromCFL always takes the
+// CFL branch (CSolver_AWS.cpp:530-534); CONST_DT is kept for the signature.
+class TimeStepping {
+  public:
+    enum TimeSteppingType { CONST_DT, CONST_CFL };
+    TimeSteppingType timeSteppingType;
+    double CFL;
+    double dt = 0.0;
+    int maxTimeStep;
+    double maxTime;
+    int filterStep;
+    int checkStep;
+    int dumpStep;
+
+    TimeStepping(TimeSteppingType timeSteppingType, double CFL, int maxTimeStep, double maxTime,
+                 int filterStep, int checkStep, int dumpStep)
+        : timeSteppingType(timeSteppingType), CFL(CFL), maxTimeStep(maxTimeStep),
+          maxTime(maxTime), filterStep(filterStep), checkStep(checkStep), dumpStep(dumpStep)
+    {
+    }
+};
+
+// IdealGas.hpp:8-55: the constants the solver reads (gamma and Pr may be
+// set after construction, as the reference's drivers do with ig->Pr)
+class IdealGas {
+  public:
+    double gamma = 1.4, Pr = 0.7;
+    double p_ref, rho_ref = 1.0, T_ref = 1.0, mu_ref;
+    IdealGas(Domain *, double mu_ref) : p_ref(1.0 / 1.4), mu_ref(mu_ref) {}
+};
+
+// Derivatives.hpp:13-183 over cs3d_ops.  Fields are in the direction's own
+// layout (DIRX: XYZ, DIRY: YZX, DIRZ: ZXY; Derivatives.cpp:13,27,40), FP64,
+// on the host.  calc*FieldDevice take device pointers in the same layout.
+class Derivatives {
+  public:
+    enum Direct { DIRX = CS3D_DIRX, DIRY = CS3D_DIRY, DIRZ = CS3D_DIRZ };
+    int Nx, Ny, Nz, N;
+    double dx, dy, dz;
+    Direct currentDir;
+    BC::BCType bcType;
+
+    Derivatives(Domain *dom, BC::BCType bcType, Direct currentDir)
+        : Nx(dom->Nx), Ny(dom->Ny), Nz(dom->Nz), N(dom->N), dx(dom->dx), dy(dom->dy),
+          dz(dom->dz), currentDir(currentDir), bcType(bcType)
+    {
+        check(cs3d_ops_create(Nx, Ny, Nz, dom->Lx, dom->Ly, dom->Lz, (int)bcType,
+                              (int)currentDir, 0.0, CS3D_FP64, &ops),
+              "Derivatives");
+    }
+    ~Derivatives() { cs3d_ops_destroy(ops); }
+    Derivatives(const Derivatives &) = delete;
+    Derivatives &operator=(const Derivatives &) = delete;
+
+    // Derivatives.cpp:3-96
+    void calc1stDerivField(double *dataIn, double *dataOut)
+    {
+        check(cs3d_ops_apply(ops, CS3D_DERIV1, CS3D_LAYOUT_REFERENCE, dataIn, dataOut, CS3D_HOST),
+              "calc1stDerivField");
+    }
+    void calc2ndDerivField(double *dataIn, double *dataOut)
+    {
+        check(cs3d_ops_apply(ops, CS3D_DERIV2, CS3D_LAYOUT_REFERENCE, dataIn, dataOut, CS3D_HOST),
+              "calc2ndDerivField");
+    }
+    void calc1stDerivFieldDevice(const double *dataIn, double *dataOut)
+    {
+        check(cs3d_ops_apply(ops, CS3D_DERIV1, CS3D_LAYOUT_REFERENCE, dataIn, dataOut,
+                             CS3D_DEVICE),
+              "calc1stDerivFieldDevice");
+    }
+    void calc2ndDerivFieldDevice(const double *dataIn, double *dataOut)
+    {
+        check(cs3d_ops_apply(ops, CS3D_DERIV2, CS3D_LAYOUT_REFERENCE, dataIn, dataOut,
+                             CS3D_DEVICE),
+              "calc2ndDerivFieldDevice");
+    }
+    void synchronize() { check(cs3d_ops_synchronize(ops), "Derivatives::synchronize"); }
+
+  private:
+    cs3d_ops *ops = nullptr;
+};
+
+// Filter.hpp:11-115 over cs3d_ops (filterField, Filter.cpp:129-171)
+class Filter {
+  public:
+    int Nx, Ny, Nz, N;
+    double alphaF;
+    Derivatives::Direct currentDir;
+    BC::BCType bcType;
+
+    Filter(double alphaF, Domain *dom, BC::BCType bcType, Derivatives::Direct currentDir)
+        : Nx(dom->Nx), Ny(dom->Ny), Nz(dom->Nz), N(dom->N), alphaF(alphaF),
+          currentDir(currentDir), bcType(bcType)
+    {
+        check(cs3d_ops_create(Nx, Ny, Nz, dom->Lx, dom->Ly, dom->Lz, (int)bcType,
+                              (int)currentDir, alphaF, CS3D_FP64, &ops),
+              "Filter");
+    }
+    ~Filter() { cs3d_ops_destroy(ops); }
+    Filter(const Filter &) = delete;
+    Filter &operator=(const Filter &) = delete;
+
+    void filterField(double *dataIn, double *dataOut)
+    {
+        check(cs3d_ops_apply(ops, CS3D_FILTER, CS3D_LAYOUT_REFERENCE, dataIn, dataOut, CS3D_HOST),
+              "filterField");
+    }
+    void filterFieldDevice(const double *dataIn, double *dataOut)
+    {
+        check(cs3d_ops_apply(ops, CS3D_FILTER, CS3D_LAYOUT_REFERENCE, dataIn, dataOut,
+                             CS3D_DEVICE),
+              "filterFieldDevice");
+    }
+
+  private:
+    cs3d_ops *ops = nullptr;
+};
+
+// AbstractCSolver.hpp:9-43, with one extension: the RK driver tells the
+// solver its order before setInitialConditions (the GPU state is built
+// there and fuses the stage update).
+class AbstractCSolver {
+  public:
+    int rkStep = 0;
+    bool rkLast = false;
+    bool endFlag = false;
+    bool useTiming = false;
+
+    double *rho0 = nullptr, *p0 = nullptr, *U0 = nullptr, *V0 = nullptr, *W0 = nullptr;
+
+    double *rho1 = nullptr, *rhok = nullptr, *rhok2 = nullptr, *rho2 = nullptr;
+    double *rhoU1 = nullptr, *rhoUk = nullptr, *rhoUk2 = nullptr, *rhoU2 = nullptr;
+    double *rhoV1 = nullptr, *rhoVk = nullptr, *rhoVk2 = nullptr, *rhoV2 = nullptr;
+    double *rhoW1 = nullptr, *rhoWk = nullptr, *rhoWk2 = nullptr, *rhoW2 = nullptr;
+    double *rhoE1 = nullptr, *rhoEk = nullptr, *rhoEk2 = nullptr, *rhoE2 = nullptr;
+
+    Domain *dom = nullptr;
+    BC *bc = nullptr;
+    TimeStepping *ts = nullptr;
+    IdealGas *ig = nullptr;
+    Derivatives *derivX = nullptr, *derivY = nullptr, *derivZ = nullptr;
+    Filter *filtX = nullptr, *filtY = nullptr, *filtZ = nullptr;
+
+    virtual ~AbstractCSolver() {}
+    virtual void initializeSolverData() = 0;
+    virtual void setInitialConditions() = 0;
+    virtual void preStep() = 0;
+    virtual void preSubStep() = 0;
+    virtual void solveEqnSet() = 0;
+    virtual void postSubStep() = 0;
+    virtual void updateData() = 0;
+    virtual void postStep() = 0;
+    virtual void setRKOrder(int) {}
+};
+
+// AbstractRK.hpp:6-14
+class AbstractRK {
+  public:
+    AbstractCSolver *cs = nullptr;
+    virtual ~AbstractRK() {}
+    virtual void executeSolverLoop() = 0;
+    virtual void updateConservedData() = 0;
+};
+
+// RK4.hpp:7-170: the reference's loop, stage by stage
+class RK4 : public AbstractRK {
+  public:
+    double a1 = 1.0 / 6.0, a2 = 1.0 / 3.0, a3 = 1.0 / 3.0, a4 = 1.0 / 6.0;
+    double b1 = 1.0 / 2.0, b2 = 1.0 / 2.0, b3 = 1.0;
+    explicit RK4(AbstractCSolver *cs) { this->cs = cs; }
+    void updateConservedData() override {}
+    void executeSolverLoop() override
+    {
+        cs->setRKOrder(4);
+        cs->setInitialConditions();
+        while (!cs->endFlag) {
+            cs->rkLast = false;
+            cs->preStep();
+            for (int s = 1; s <= 4; s++) {
+                cs->rkStep = s;
+                cs->rkLast = s == 4;
+                cs->preSubStep();
+                cs->solveEqnSet();
+                cs->postSubStep();
+                updateConservedData();
+                cs->updateData();
+            }
+            cs->postStep();
+        }
+    }
+};
+
+// TVDRK3.hpp:7-139
+class TVDRK3 : public AbstractRK {
+  public:
+    explicit TVDRK3(AbstractCSolver *cs) { this->cs = cs; }
+    void updateConservedData() override {}
+    void executeSolverLoop() override
+    {
+        cs->setRKOrder(3);
+        cs->setInitialConditions();
+        while (!cs->endFlag) {
+            cs->rkLast = false;
+            cs->preStep();
+            for (int s = 1; s <= 3; s++) {
+                cs->rkStep = s;
+                cs->rkLast = s == 3;
+                cs->preSubStep();
+                cs->solveEqnSet();
+                cs->postSubStep();
+                updateConservedData();
+                cs->updateData();
+            }
+            cs->postStep();
+        }
+    }
+};
+
+// CSolver_AWS (CSolver_AWS.hpp:119, CSolver_AWS.cpp) on the GPU.  The
+// pre/sub/post phases map onto cs3d_pre_step / cs3d_sub_step /
+// cs3d_post_step; solveEqnSet runs the whole stage (derivatives, RHS,
+// boundary handling and the RK update).
+class CSolver_AWS : public AbstractCSolver {
+  public:
+    double alphaF, mu_ref;
+    int blocksize;
+    double time = 0.0, dt = 0.0;
+    int timeStep = 0;
+    // GPU-path options (no reference counterpart)
+    int precision = CS3D_FP64;       // CS3D_FP32: FP32 bulk arithmetic
+    bool tgvDiagnostics = false;     // append taylorgreen.out every step
+    bool turbDiagnostics = false;    // append turbdata.out every step
+    std::string outDir = ".";
+
+    CSolver_AWS(Domain *dom, BC *bc, TimeStepping *ts, double alphaF, double mu_ref,
+                int blocksize, bool useTiming)
+        : alphaF(alphaF), mu_ref(mu_ref), blocksize(blocksize)
+    {
+        this->dom = dom;
+        this->bc = bc;
+        this->ts = ts;
+        this->useTiming = useTiming;
+        ig = new IdealGas(dom, mu_ref);
+        initializeSolverData();
+    }
+    ~CSolver_AWS() override
+    {
+        if (h) cs3d_destroy(h);
+        delete ig;
+    }
+    CSolver_AWS(const CSolver_AWS &) = delete;
+    CSolver_AWS &operator=(const CSolver_AWS &) = delete;
+
+    // CSolver_AWS.cpp:3-196: host arrays (IC inputs and the state mirrors)
+    void initializeSolverData() override
+    {
+        const size_t N = (size_t)dom->N;
+        store.assign(25 * N, 0.0);
+        double *p = store.data();
+        double **slots[25] = {&rho0,  &p0,    &U0,    &V0,     &W0,    &rho1,   &rhok,
+                              &rhok2, &rho2,  &rhoU1, &rhoUk,  &rhoUk2, &rhoU2, &rhoV1,
+                              &rhoVk, &rhoVk2, &rhoV2, &rhoW1, &rhoWk, &rhoWk2, &rhoW2,
+                              &rhoE1, &rhoEk, &rhoEk2, &rhoE2};
+        for (int i = 0; i < 25; i++) *slots[i] = p + i * N;
+    }
+
+    void setRKOrder(int order) override { rkOrder = order; }
+
+    // CSolver_AWS.cpp:199-504
+    void setInitialConditions() override
+    {
+        cs3d_config c;
+        check(cs3d_config_default(&c, dom->Nx, dom->Ny, dom->Nz), "cs3d_config_default");
+        c.Lx = dom->Lx;
+        c.Ly = dom->Ly;
+        c.Lz = dom->Lz;
+        c.bc_type[0] = bc->bcXType;
+        c.bc_type[1] = bc->bcYType;
+        c.bc_type[2] = bc->bcZType;
+        const int kinds[6] = {bc->bcX0, bc->bcX1, bc->bcY0, bc->bcY1, bc->bcZ0, bc->bcZ1};
+        for (int f = 0; f < 6; f++) c.bc_kind[f] = kinds[f];
+        c.cfl = ts->CFL;
+        c.filter_step = ts->filterStep;
+        c.max_steps = ts->maxTimeStep;
+        c.max_time = ts->maxTime;
+        c.alpha_f = alphaF;
+        c.mu_ref = mu_ref;
+        c.gamma = ig->gamma;
+        c.Pr = ig->Pr;
+        c.rk = rkOrder == 3 ? CS3D_TVDRK3 : CS3D_RK4;
+        c.precision = precision;
+        if (h) cs3d_destroy(h);
+        h = nullptr;
+        check(cs3d_create(&c, &h), "setInitialConditions (create)");
+        check(cs3d_set_initial_conditions(h, rho0, U0, V0, W0, p0), "setInitialConditions");
+        check(cs3d_set_diagnostics(h, tgvDiagnostics || turbDiagnostics), "set_diagnostics");
+        if (useTiming) check(cs3d_profile_enable(h, 1), "profile_enable");
+        endFlag = false;
+    }
+
+    // CSolver_AWS.cpp:2897-2904: the step-0 dump, then calcDtFromCFL
+    // (:507-551)
+    void preStep() override
+    {
+        if (timeStep == 0) dumpSolution();
+        check(cs3d_pre_step(h), "preStep");
+    }
+    void preSubStep() override {}
+    void solveEqnSet() override { check(cs3d_sub_step(h, rkStep, rkLast ? 1 : 0), "solveEqnSet"); }
+    void postSubStep() override {}
+    void updateData() override {}
+
+    // CSolver_AWS.cpp:2930-2946: diagnostics, updateSponge, checkSolution,
+    // dumpSolution, checkEnd
+    void postStep() override
+    {
+        check(cs3d_post_step(h), "postStep");
+        check(cs3d_get_time(h, &time, &dt, &timeStep), "get_time");
+        ts->dt = dt;
+        if (tgvDiagnostics) calcTaylorGreenQuantities();
+        if (turbDiagnostics) calcTurbulenceQuantities();
+        checkSolution();
+        dumpSolution();
+        if (timeStep >= ts->maxTimeStep || time >= ts->maxTime) endFlag = true;
+    }
+
+    // calcTaylorGreenQuantities (CSolver_AWS.cpp:2175-2257): appends a line
+    // to taylorgreen.out in the reference's format (:2240-2248)
+    void calcTaylorGreenQuantities()
+    {
+        double o[7];
+        check(cs3d_diag_taylor_green(h, o), "calcTaylorGreenQuantities");
+        std::ofstream f(outDir + "/taylorgreen.out", std::ios::app);
+        f << std::setprecision(17);
+        for (int i = 0; i < 7; i++) f << o[i] << (i < 6 ? " " : "");
+        f << std::endl;
+    }
+    // calcTurbulenceQuantities (CSolver_AWS.cpp:2033-2173): turbdata.out in
+    // the reference's format (:2154-2164, a blank after every column)
+    void calcTurbulenceQuantities()
+    {
+        double o[11];
+        check(cs3d_diag_turbulence(h, o), "calcTurbulenceQuantities");
+        std::ofstream f(outDir + "/turbdata.out", std::ios::app);
+        f << std::setprecision(17);
+        for (int i = 0; i < 11; i++) f << o[i] << " ";
+        f << std::endl;
+    }
+
+    // checkSolution (CSolver_AWS.cpp:2459-2491): every checkStep steps,
+    // step, time, dt and the ranges of the state (getRange, Utils.cpp:334-349)
+    void checkSolution()
+    {
+        if (ts->checkStep <= 0 || timeStep % ts->checkStep != 0) return;
+        const size_t N = (size_t)dom->N;
+        std::vector<double> prim(8 * N);
+        double *P[8];
+        for (int i = 0; i < 8; i++) P[i] = prim.data() + i * N;
+        check(cs3d_get_conserved(h, 1, rho1, rhoU1, rhoV1, rhoW1, rhoE1), "checkSolution");
+        check(cs3d_get_primitives(h, P[0], P[1], P[2], P[3], P[4], P[5], P[6], P[7]),
+              "checkSolution");
+        std::cout << std::endl
+                  << "-------------------------------------------------" << std::endl
+                  << " Step = " << timeStep << ", time = " << time << ", dt = " << dt
+                  << std::endl
+                  << "-------------------------------------------------" << std::endl;
+        const char *names[9] = {"RHO", "U", "V", "W", "P", "T", "mu", "RHOE", "SOS"};
+        const double *data[9] = {rho1, P[0], P[1], P[2], P[3], P[4], P[5], rhoE1, P[7]};
+        for (int v = 0; v < 9; v++) {
+            double lo = 1e300, hi = -1e300;
+            for (size_t i = 0; i < N; i++) {
+                lo = std::fmin(lo, data[v][i]);
+                hi = std::fmax(hi, data[v][i]);
+            }
+            std::cout << "  Range of " << names[v] << ": " << lo << ":" << hi << std::endl;
+        }
+        std::cout << std::endl;
+    }
+
+    // fills the host mirrors rho1 ... rhoE2 from the device state
+    void syncHostState()
+    {
+        check(cs3d_get_conserved(h, 1, rho1, rhoU1, rhoV1, rhoW1, rhoE1), "sync *1");
+        check(cs3d_get_conserved(h, 2, rhok, rhoUk, rhoVk, rhoWk, rhoEk), "sync *k");
+        check(cs3d_get_conserved(h, 3, rhok2, rhoUk2, rhoVk2, rhoWk2, rhoEk2), "sync *k2");
+        check(cs3d_get_conserved(h, 4, rho2, rhoU2, rhoV2, rhoW2, rhoE2), "sync *2");
+    }
+
+    // dumpSolution (CSolver_AWS.cpp:2493-2535): every dumpStep steps, raw
+    // FP64 XYZ arrays rho.out.<step> ... rhoE.out.<step>
+    void dumpSolution()
+    {
+        if (ts->dumpStep <= 0 || timeStep % ts->dumpStep != 0) return;
+        check(cs3d_get_conserved(h, 1, rho1, rhoU1, rhoV1, rhoW1, rhoE1), "dumpSolution");
+        const char *names[5] = {"rho", "rhoU", "rhoV", "rhoW", "rhoE"};
+        double *data[5] = {rho1, rhoU1, rhoV1, rhoW1, rhoE1};
+        for (int v = 0; v < 5; v++) {
+            std::string fn = outDir + "/" + names[v] + ".out." + std::to_string(timeStep);
+            FILE *fp = std::fopen(fn.c_str(), "wb");
+            if (!fp) {
+                std::cerr << " > cannot open " << fn << std::endl;
+                std::abort();
+            }
+            std::fwrite(data[v], sizeof(double), (size_t)dom->N, fp);
+            std::fclose(fp);
+        }
+    }
+
+    cs3d_solver *handle() { return h; }
+
+  private:
+    cs3d_solver *h = nullptr;
+    int rkOrder = 4;
+    std::vector<double> store;
+};
+
+}  // namespace cs3d
+
+#ifdef CS3D_DROPIN_GLOBAL
+using cs3d::AbstractCSolver;
+using cs3d::AbstractRK;
+using cs3d::BC;
+using cs3d::CSolver_AWS;
+using cs3d::Derivatives;
+using cs3d::Domain;
+using cs3d::Filter;
+using cs3d::IdealGas;
+using cs3d::RK4;
+using cs3d::TimeStepping;
+using cs3d::TVDRK3;
+#endif
+
+#endif
