@@ -242,8 +242,15 @@ int cs3d_diag_taylor_green(cs3d_solver *s, double out[7])
 int cs3d_diag_turbulence(cs3d_solver *s, double out[11])
 {
     CS3D_NEED(s, "cs3d_diag_turbulence");
-    (void)out;
-    return set_error(CS3D_ERR_UNSUPPORTED, "cs3d_diag_turbulence: not built yet");
+    if (!out) return set_error(CS3D_ERR_ARG, "cs3d_diag_turbulence: NULL");
+    return guard("cs3d_diag_turbulence", [&]() -> int {
+        int rc = s->impl->diag_turb(out);
+        if (rc == CS3D_ERR_STATE)
+            return set_error(rc, "cs3d_diag_turbulence: enable diagnostics and complete a step first");
+        if (rc == CS3D_ERR_NONFINITE)
+            return set_error(rc, "cs3d_diag_turbulence: non-finite state");
+        return rc;
+    });
 }
 
 int cs3d_kinetic_energy(cs3d_solver *s, double *ke)

@@ -518,6 +518,44 @@ struct SolverT : SolverBase {
         return h.nonfinite ? CS3D_ERR_NONFINITE : CS3D_OK;
     }
 
+    // calcTurbulenceQuantities (CSolver_AWS.cpp:2033-2173) from the post-step
+    // sums; the density variance uses the stored perturbation rho' (no
+    // cancellation against rho_ref)
+    int diag_turb(double out[11]) override
+    {
+        if (!diag_valid) return CS3D_ERR_STATE;
+        double r[RED_SLOTS_HOST];
+        StepScalars h;
+        hc(hipMemcpyAsync(r, dout, sizeof(r), hipMemcpyDeviceToHost, st), "diag");
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        const double NN = (double)N;
+        const double mRp = r[8] / NN;
+        const double mRho = rho_ref + mRp;
+        const double rhoprime = std::sqrt(std::fmax(r[9] / NN - mRp * mRp, 0.0));
+        const double mMu = r[6] / NN;
+        const double mTD = (r[2] / NN) / mRho;
+        const double up = std::sqrt(r[10] / NN / 3.0);
+        const double uip = std::sqrt(r[11] / NN);
+        const double urms = std::sqrt(r[10] / NN);
+        const double mSOS = r[12] / NN;
+        const double mNu = mMu / mRho;
+        const double taylorMicro2 = up / uip;
+        const double k0 = 8.0;  // spectrum peak of the initial field (:2130)
+        out[0] = h.time;
+        out[1] = std::sqrt(2.0 * M_PI) / k0 / up;           // tau
+        out[2] = up * taylorMicro2 / mNu;                   // taylorReyn3
+        out[3] = urms / mSOS;                               // turbMach
+        out[4] = mTD;                                       // meanTurbDiss
+        out[5] = up;                                        // uprime
+        out[6] = std::pow(std::pow(mNu, 3.0) / mTD, 0.25);  // kolNu
+        out[7] = rhoprime;
+        out[8] = std::sqrt(r[5] / NN);                      // dilprime
+        out[9] = std::sqrt(r[13] / NN);                     // vortprime
+        out[10] = r[1] / NN;                                // meanKineticEng
+        return h.nonfinite ? CS3D_ERR_NONFINITE : CS3D_OK;
+    }
+
     int kinetic_energy(double *ke) override
     {
         need_ic();

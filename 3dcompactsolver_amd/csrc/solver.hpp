@@ -26,6 +26,7 @@ struct SolverBase {
     virtual void get_time(double *time, double *dt, int *ts) = 0;
     virtual void set_diag(int on) = 0;
     virtual int diag_tgv(double out[7]) = 0;
+    virtual int diag_turb(double out[11]) = 0;
     virtual int kinetic_energy(double *ke) = 0;
     virtual void synchronize() = 0;
     virtual void *stream() = 0;

@@ -93,7 +93,7 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
 template <typename real>
 hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
                           const LineMap &lm, const Ptr5<real> &io, hipStream_t st);
-constexpr int RED_SLOTS_HOST = 8;
+constexpr int RED_SLOTS_HOST = 16;
 template <typename real>
 hipError_t launch_reduce(const Fields<real> &F, const Phys<real> &ph, long N, int diag,
                          double *part, int nb, double *out, StepScalars *sc, hipStream_t st);

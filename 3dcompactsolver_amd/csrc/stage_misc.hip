@@ -46,9 +46,12 @@ __global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *_
 // [0] max of the CFL wave-speed sum (calcDtFromCFL, CSolver_AWS.cpp:513-525)
 // and, when diag is set, the sums of calcTaylorGreenQuantities
 // (CSolver_AWS.cpp:2188-2228): [1] sum rho|u|^2/2 [2] sum 2 mu S:grad u
-// [3] sum rho|w|^2/2 [4] sum mu rho |w|^2 [5] sum div^2 [6] sum mu.
+// [3] sum rho|w|^2/2 [4] sum mu rho |w|^2 [5] sum div^2 [6] sum mu; [7]
+// counts NaNs; and for calcTurbulenceQuantities (CSolver_AWS.cpp:2040-2095)
+// [8] sum rho' [9] sum rho'^2 [10] sum |u|^2 [11] sum (Ux^2+Vy^2+Wz^2)/3
+// [12] sum of the sound speed [13] sum |w|^2.
 constexpr int RED_THREADS = 256;
-constexpr int RED_SLOTS = 8;
+constexpr int RED_SLOTS = RED_SLOTS_HOST;
 
 template <typename real>
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Phys<real> ph,
@@ -56,7 +59,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Ph
                                                                double *__restrict__ part)
 {
     __shared__ double sh[RED_SLOTS][RED_THREADS];
-    double acc[RED_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[RED_SLOTS] = {};
     for (long i = (long)blockIdx.x * RED_THREADS + threadIdx.x; i < N;
          i += (long)gridDim.x * RED_THREADS) {
         real irho, U, V, W, pp, Tp;
@@ -85,6 +88,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Ph
                 acc[3] += 0.5 * rho * w2;
                 acc[4] += m * rho * w2;
                 acc[5] += dv * dv;
+                const double rp = F.q[0][i];
+                acc[8] += rp;
+                acc[9] += rp * rp;
+                acc[10] += uv;
+                acc[11] += (double)F.diag[3][i];
+                acc[12] += cs;
+                acc[13] += w2;
             }
         }
     }
@@ -106,7 +116,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__re
                                                                StepScalars *sc)
 {
     __shared__ double sh[RED_SLOTS][RED_THREADS];
-    double acc[RED_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[RED_SLOTS] = {};
     for (int b = threadIdx.x; b < nb; b += RED_THREADS) {
         acc[0] = fmax(acc[0], part[(long)b * RED_SLOTS]);
         for (int s = 1; s < RED_SLOTS; s++) acc[s] += part[(long)b * RED_SLOTS + s];

@@ -62,5 +62,7 @@ def test_dropin_driver_matches_c_abi(tmp_path):
     for s in range(steps):
         g.step(1)
         ref = g.diag_taylor_green()
-        assert np.allclose(rows[s], ref, rtol=1e-15, atol=0), (s, rows[s], ref)
-    assert abs(ke_cpp - g.kinetic_energy()) <= 1e-15 * abs(ke_cpp)
+        # the driver builds the initial field with libm, the Python side with
+        # numpy: last-bit differences, grown over the steps
+        assert np.allclose(rows[s], ref, rtol=1e-12, atol=0), (s, rows[s], ref)
+    assert abs(ke_cpp - g.kinetic_energy()) <= 1e-12 * abs(ke_cpp)

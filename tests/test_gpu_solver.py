@@ -98,6 +98,26 @@ def test_tgv_diagnostics():
         assert abs(d_g[6] - d_o[6]) <= 1e-8 * abs(d_o[6])
 
 
+def test_turbulence_diagnostics_fp64():
+    """calcTurbulenceQuantities (CSolver_AWS.cpp:2033-2173) on a random
+    solenoid-like field with TVDRK3 (the HIT configuration's integrator),
+    periodic L = 2 pi (Q9), against the oracle after each of two steps."""
+    N = 32
+    kw = dict(Lx=2 * np.pi, Ly=2 * np.pi, Lz=2 * np.pi, alpha_f=0.49, mu_ref=2e-3, Pr=0.71,
+              cfl=0.5, filter_step=1, rk=3)
+    o, g = pair((N, N, N), random_ic((N, N, N), 7), cs3d.FP64, **kw)
+    g.set_diagnostics(True)
+    for _ in range(2):
+        o.step()
+        g.step(1)
+        d_o = o.diag_turb()
+        d_g = g.diag_turbulence()
+        print("turb oracle", d_o, "\nturb gpu   ", d_g, flush=True)
+        assert abs(d_g[0] - d_o[0]) <= 1e-15
+        for c in range(1, 11):
+            assert abs(d_g[c] - d_o[c]) <= 1e-10 * abs(d_o[c]), (c, d_g[c], d_o[c])
+
+
 def test_tvdrk3_noncubic_fp64():
     dims = (32, 16, 24)
     kw = dict(Lx=2 * np.pi, Ly=np.pi, Lz=1.5 * np.pi, alpha_f=0.49, mu_ref=1e-3, Pr=0.71,

@@ -125,3 +125,37 @@ def test_tgv_initial_kinetic_energy_closed_form():
     rho, U, V, W, p = tgv_ic(N, L, isothermal=False)
     ke = 0.5 * (rho * (U * U + V * V + W * W)).sum()
     assert abs(ke - N ** 3 / 8) < 1e-9
+
+
+def test_turbulence_quantities_tgv_closed_forms():
+    """calcTurbulenceQuantities (CSolver_AWS.cpp:2033-2173) pinned on the
+    Taylor-Green field at t ~ 0 (one step with dt = 1e-10, so the stage
+    derivatives are those of the initial field).  With u = sin x cos y cos z,
+    v = -cos x sin y cos z, w = 0, p0 = 100/gamma and rho = p/p0:
+    <u.u> = 1/4, <Ux^2 + Vy^2 + Wz^2> = 1/4, div u = 0, <|w|^2> = 3/4,
+    2 S:grad u = 2 (Ux^2 + Vy^2 + Uz^2/2 + Vz^2/2) averages to 3/4, the sound
+    speed is sqrt(gamma p0) = 10 everywhere, T = 100 so mu = 1/1600, and
+    std(rho) = sqrt(4.5)/(16 p0).  The compact derivatives carry the 6th-order
+    modified-wavenumber error, ~1e-6 at 32 points per period."""
+    N = 32
+    L = 2 * np.pi - 2 * np.pi / N
+    p0 = 100.0 / 1.4
+    o = po.Solver(po.make_config(N, N, N, Lx=L, Ly=L, Lz=L, alpha_f=0.49,
+                                 mu_ref=(1 / 1600) / 100 ** 0.76, Pr=0.71, cfl=0.5,
+                                 filter_step=1000, rk=4))
+    o.set_ic(*tgv_ic(N, L, p0=p0))
+    o.set_dt(1e-10)
+    o.step()
+    t, tau, re3, mt, td, up, kol, rp, dp, vp, ke = o.diag_turb()
+    nu = (1 / 1600) / 1.0
+    assert t == 0.0
+    assert abs(up - np.sqrt(1 / 12)) < 1e-8
+    assert abs(tau - np.sqrt(2 * np.pi) / 8 / np.sqrt(1 / 12)) < 1e-7
+    assert abs(mt - 0.05) < 1e-8
+    assert abs(td - 2 * nu * 3 / 8) < 1e-5 * 2 * nu * 3 / 8
+    assert abs(re3 - np.sqrt(1 / 12) / nu) < 1e-5 * np.sqrt(1 / 12) / nu
+    assert abs(kol - (nu ** 3 / (2 * nu * 3 / 8)) ** 0.25) < 1e-5 * kol
+    assert abs(rp - np.sqrt(4.5) / (16 * p0)) < 1e-6 * rp
+    assert dp < 1e-10
+    assert abs(vp - np.sqrt(3 / 4)) < 1e-5
+    assert abs(ke - 1 / 8) < 1e-3
