@@ -192,9 +192,11 @@ struct Win {
     static_assert(!VEC || M % VW == 0, "vector windows need whole chunks per segment");
     typedef real vec __attribute__((ext_vector_type(16 / sizeof(real))));
     unsigned off[NC];
+    unsigned spb;  // byte stride between consecutive points of the line
 
     __device__ __forceinline__ void init(unsigned base, unsigned sp, int s, int n, int periodic)
     {
+        spb = sp * (unsigned)sizeof(real);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             int p = VEC ? s - LO + c * VW : s - HW + c;
@@ -207,6 +209,13 @@ struct Win {
             }
             off[c] = (base + (unsigned)p * sp) * (unsigned)sizeof(real);
         }
+    }
+    // byte offset of the segment's own point t (0 <= t < len): these never
+    // wrap, so only the first one needs keeping
+    __device__ __forceinline__ unsigned mid(int t) const
+    {
+        if (VEC) return at(HW + t);
+        return off[HW] + (unsigned)t * spb;
     }
     // byte offset of window point t
     __device__ __forceinline__ unsigned at(int t) const

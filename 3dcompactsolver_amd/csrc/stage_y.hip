@@ -102,7 +102,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
 #pragma unroll
         for (int t = 0; t < M; t++) {
             if (UNI || t < c.len) {
-                const unsigned o = win.at(HW + t);
+                const unsigned o = win.mid(t);
                 sto(F.dY[0], o, r[1][t]);
                 sto(F.dY[1], o, r[2][t]);
                 sto(F.dY[2], o, r[3][t]);

@@ -121,6 +121,12 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     const real *Tz = r[5], *Uz = r[6], *Vz = r[7], *Wz = r[8];
     if (!active) return;
 
+    // The point values of Ux, Vy, Wx, Wy are read again here rather than
+    // kept from the mixed-derivative windows: copies of the field pointers
+    // the compiler cannot identify with the originals stop it from holding
+    // 16 window centres in registers across the solves.
+    const real *pdX0 = F.dX[0], *pdX2 = F.dX[2], *pdY1 = F.dY[1], *pdY2 = F.dY[2];
+    asm volatile("" : "+s"(pdX0), "+s"(pdX2), "+s"(pdY1), "+s"(pdY2));
     const real dt = (real)sc->dt;
     // stage-update coefficients, uniform over the launch
     const bool first = sa.rkStep == 1, has_next = sa.rkStep < 4;
@@ -139,11 +145,11 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
 #pragma unroll
     for (int t = 0; t < M; t++) {
         if (!UNI && t >= c.len) continue;
-        const unsigned i = win.at(HW + t);
+        const unsigned i = win.mid(t);
         const int kk = c.s + t;
-        const real Ux = ldo(F.dX[0], i), Vx = ldo(F.dX[1], i), Wx = ldo(F.dX[2], i),
+        const real Ux = ldo(pdX0, i), Vx = ldo(F.dX[1], i), Wx = ldo(pdX2, i),
                    Tx = ldo(F.dX[3], i);
-        const real Uy = ldo(F.dY[0], i), Vy = ldo(F.dY[1], i), Wy = ldo(F.dY[2], i),
+        const real Uy = ldo(F.dY[0], i), Vy = ldo(pdY1, i), Wy = ldo(pdY2, i),
                    Ty = ldo(F.dY[3], i);
         const real uz = Uz[t], vz = Vz[t], wz = Wz[t];
         const real MuX = amu[t] * Tx, MuY = amu[t] * Ty, MuZ = amu[t] * Tz[t];
