@@ -108,15 +108,22 @@ constexpr size_t LDS_BYTES = 160 * 1024;
 
 // LX = max_threads / P lines (at most 64, fewer when the exchange ring would
 // not fit the LDS) per workgroup, ring margin kr, `planes` values per ring
-// position; LX = 0 when a line does not fit.
-inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esize)
+// position, and room for stage_per_lane window values per lane
+// (load_windows_lds) in the same LDS; LX = 0 when a line does not fit.
+inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esize,
+                            int stage_per_lane = 0)
 {
+    auto bytes = [&](int LX) {
+        const size_t ring = ring_bytes(P, kr, LX, planes, esize);
+        const size_t stage = (size_t)stage_per_lane * LX * P * esize;
+        return ring > stage ? ring : stage;
+    };
     LaunchGeom g;
     g.LX = P > 0 ? max_threads / P : 0;
     if (g.LX > 64) g.LX = 64;
-    while (g.LX > 0 && ring_bytes(P, kr, g.LX, planes, esize) > LDS_BYTES) g.LX--;
+    while (g.LX > 0 && bytes(g.LX) > LDS_BYTES) g.LX--;
     g.threads = g.LX * P;
-    g.shm = g.LX > 0 ? ring_bytes(P, kr, g.LX, planes, esize) : 0;
+    g.shm = g.LX > 0 ? bytes(g.LX) : 0;
     return g;
 }
 
@@ -273,6 +280,50 @@ struct Win {
         }
     }
 };
+
+// Windows of NF fields for lanes of uniform periodic MAP_COL plans, without
+// re-reading the neighbours' points: each lane loads its own M points of
+// every field (src(f, offset) returns field f at a byte offset), publishes
+// them in LDS, and takes its HW halo points on each side from the
+// neighbouring segments' slots (cyclic).  The staging area is the exchange
+// buffer, NF * M * lanes elements; the function starts and ends with a
+// barrier, so it may be called between solve batches.
+template <int NF, typename real, int M, int HW, bool VEC, typename Src>
+__device__ __forceinline__ void load_windows_lds(const Lane<real, MAP_COL> &c,
+                                                 const Win<real, M, HW, VEC> &win, Src src,
+                                                 real (&w)[NF][M + 2 * HW])
+{
+    static_assert(HW <= M, "halo wider than a segment");
+    real *st = c.xb;
+    const int T = c.P * c.LX;
+    const int me = c.g * c.LX + c.line;
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            const real v = src(f, win.mid(t));
+            w[f][HW + t] = v;
+            st[(f * M + t) * T + me] = v;
+        }
+    }
+    __syncthreads();
+    const int gl = c.g == 0 ? c.P - 1 : c.g - 1;
+    const int gr = c.g == c.P - 1 ? 0 : c.g + 1;
+    const int ml = gl * c.LX + c.line, mr = gr * c.LX + c.line;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int h = 0; h < HW; h++) {
+            w[f][h] = st[(f * M + M - HW + h) * T + ml];
+            w[f][HW + M + h] = st[(f * M + h) * T + mr];
+        }
+    }
+    __syncthreads();
+}
+
+// LDS elements per lane that load_windows_lds stages for NF fields
+constexpr int stage_per_lane(int NF, int M) { return NF * M; }
 
 // whether a kernel instantiation reads its windows as 16-byte chunks
 template <typename real, int M, int MAP, bool UNI>

@@ -103,7 +103,8 @@ struct SolverT : SolverBase {
                 throw std::invalid_argument("inconsistent derivative plans");
             const int P = D1[d].host.P;
             const int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
-            sweepG[d] = line_geom(P, kr, sweep_max_threads(d), SWEEP_PLANES, sizeof(real));
+            sweepG[d] = line_geom(P, kr, sweep_max_threads(d), SWEEP_PLANES, sizeof(real),
+                                  SWEEP_STAGE_FIELDS * D1[d].host.M);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
                                    filter_max_threads(), 5, sizeof(real));

@@ -77,6 +77,8 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 
 // most solves batched through one exchange in a sweep kernel
 constexpr int SWEEP_PLANES = 13;
+// most fields whose windows a sweep kernel stages through LDS at once
+constexpr int SWEEP_STAGE_FIELDS = 5;
 
 template <typename real>
 struct Ptr5 {

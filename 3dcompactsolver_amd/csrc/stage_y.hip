@@ -23,13 +23,26 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
-    real a0[W], a1[W], a2[W], a3[W], a4[W];
+    real a[5][W];
+    real(&a0)[W] = a[0];
+    real(&a1)[W] = a[1];
+    real(&a2)[W] = a[2];
+    real(&a3)[W] = a[3];
+    real(&a4)[W] = a[4];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
-    win.load(q0, a0);
-    win.load(q1, a1);
-    win.load(q2, a2);
-    win.load(q3, a3);
-    win.load(q4, a4);
+    if constexpr (UNI) {
+        // own points from memory, halos through LDS (measured: staging the
+        // two mixed windows below as well is slower for this kernel)
+        load_windows_lds<5>(c, win, [&](int f, unsigned o) {
+            return ldo(f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : f == 3 ? q3 : q4, o);
+        }, a);
+    } else {
+        win.load(q0, a0);
+        win.load(q1, a1);
+        win.load(q2, a2);
+        win.load(q3, a3);
+        win.load(q4, a4);
+    }
 #pragma unroll
     for (int t = 0; t < W; t++)
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);

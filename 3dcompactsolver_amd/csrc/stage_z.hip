@@ -29,6 +29,8 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
+    // the conserved windows are read directly: staging them through LDS as
+    // well (like the mixed windows below) measured slower for this kernel
     real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
     win.load(q0, a0);
@@ -99,14 +101,25 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         // enter the right-hand side with the same coefficients
         // (CSolver_AWS.cpp:1161-1192), so their sum is taken as one
         // derivative of Ux + Vy.
-        real m3[3][M], f[W];
+        real m3[3][M];
+        if constexpr (UNI) {
+            real mw[3][W];
+            load_windows_lds<3>(c, win, [&](int fi, unsigned o) {
+                return fi == 0 ? ldo(F.dX[0], o) + ldo(F.dY[1], o)
+                               : ldo(fi == 1 ? F.dX[2] : F.dY[2], o);
+            }, mw);
 #pragma unroll
-        for (int t = 0; t < W; t++) f[t] = ldo(F.dX[0], win.at(t)) + ldo(F.dY[1], win.at(t));
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[0]);
-        win.load(F.dX[2], f);
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[1]);
-        win.load(F.dY[2], f);
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[2]);
+            for (int i = 0; i < 3; i++) rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[i], m3[i]);
+        } else {
+            real f[W];
+#pragma unroll
+            for (int t = 0; t < W; t++) f[t] = ldo(F.dX[0], win.at(t)) + ldo(F.dY[1], win.at(t));
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[0]);
+            win.load(F.dX[2], f);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[1]);
+            win.load(F.dY[2], f);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m3[2]);
+        }
         solve_batch<3, kc_deriv<real>(), real, M, UNI>(d1, c, m3);
 #pragma unroll
         for (int t = 0; t < M; t++) {
