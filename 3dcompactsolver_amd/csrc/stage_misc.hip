@@ -19,24 +19,24 @@ __global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *_
 {
     constexpr int HW = 4, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP, fo, fo->K > kc_filter<real>() ? fo->K : kc_filter<real>(), 5)
-    real r[5][M];
+    real r[5][M], fc[5][M];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         real w[W];
         win.load(io.in[v], w);
         rhs<OP_FILTER, real, M, HW, UNI>(fo, c, w, r[v]);
+        // the segment's own values, for out = f - A^-1 (A - B) f (held in
+        // registers: read again after the solve they come from HBM)
+#pragma unroll
+        for (int t = 0; t < M; t++) fc[v][t] = w[HW + t];
     }
     solve_batch<5, kc_filter<real>(), real, M, UNI>(fo, c, r);
     if (active) {
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-            // out = f - A^-1 (A - B) f, with the segment's f read again
-            // (an L2 hit) rather than held through the solve
-            real f[M];
-            win.load_seg(io.in[v], f);
 #pragma unroll
-            for (int t = 0; t < M; t++) f[t] -= r[v][t];
-            win.store(io.out[v], f, c.len);
+            for (int t = 0; t < M; t++) fc[v][t] -= r[v][t];
+            win.store(io.out[v], fc[v], c.len);
         }
     }
 }

@@ -37,6 +37,10 @@ HBM_PEAK = 8.0e12
 # (X sweep 5 reads + 9 writes, Y 12 + 9, Z 28 + 10, filter sweep 5 + 5,
 # post-step reduction 5 reads)
 ACCESSES = {"x_sweep": 14, "y_sweep": 21, "z_sweep": 38, "filter": 10, "reduce": 5}
+# HBM accesses per point actually moved per launch (reads, writes), FP32 at
+# 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
+# 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
+MEASURED_TRAFFIC = {"x_sweep": (5.00, 9.00), "y_sweep": (12.00, 9.00), "z_sweep": (30.59, 9.75)}
 BYTES_PER_POINT_STAGE = {cs3d.FP32: 322, cs3d.FP64: 644}
 
 
@@ -86,9 +90,12 @@ def roofline(prof, N, prec):
     avg_s = tot_ms / cnt * 1e-3
     alg_bytes = ACCESSES[best] * s * N ** 3
     achieved = alg_bytes / avg_s / 1e9
+    traffic = None
+    if prec == cs3d.FP32 and N == 256 and best in MEASURED_TRAFFIC:
+        traffic = int(sum(MEASURED_TRAFFIC[best]) * s * N ** 3)
     return best, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
                   "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
-                  "traffic": None, "kernel": best,
+                  "traffic": traffic, "kernel": best,
                   "algorithmic_bytes_per_launch": alg_bytes,
                   "avg_launch_ms": round(tot_ms / cnt, 4)}
 
@@ -124,7 +131,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-grid", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
