@@ -83,6 +83,8 @@ def lib():
     L.cs3d_diag_taylor_green.argtypes = [_vp, _dp]
     L.cs3d_diag_turbulence.argtypes = [_vp, _dp]
     L.cs3d_kinetic_energy.argtypes = [_vp, _dp]
+    L.cs3d_rhs_vjp.argtypes = [_vp, _int] + [_dp] * 10
+    L.cs3d_adjoint_ke_gradient.argtypes = [_vp, _int, _int] + [_dp] * 4
     L.cs3d_stream.argtypes = [_vp]
     L.cs3d_stream.restype = _vp
     L.cs3d_ops_create.argtypes = [_int, _int, _int, ctypes.c_double, ctypes.c_double,
@@ -271,6 +273,23 @@ class Solver:
         cnt = (_int * 6)()
         check(lib().cs3d_profile_report(self.h, _dptr(ms), cnt))
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PROFILE_CLASSES)}
+
+    def rhs_vjp(self, lam, set_=1):
+        """(d(dt*RHS)/dQ)^T lam at conserved set `set_`: five fields in, five out"""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in lam]
+        out = [np.empty(self.shape) for _ in range(5)]
+        check(lib().cs3d_rhs_vjp(self.h, set_, *[_dptr(a) for a in arrs],
+                                 *[_dptr(a) for a in out]))
+        return out
+
+    def ke_gradient(self, nsteps, checkpoint=8):
+        """advance nsteps; return (J, dJ/dU0, dJ/dV0, dJ/dW0) for the final
+        volume-averaged kinetic energy J"""
+        J = ctypes.c_double()
+        g = [np.empty(self.shape) for _ in range(3)]
+        check(lib().cs3d_adjoint_ke_gradient(self.h, nsteps, checkpoint, ctypes.byref(J),
+                                             *[_dptr(a) for a in g]))
+        return J.value, g[0], g[1], g[2]
 
     def kinetic_energy(self):
         v = ctypes.c_double()

@@ -264,6 +264,37 @@ int cs3d_kinetic_energy(cs3d_solver *s, double *ke)
     });
 }
 
+int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *lam_rhoU,
+                 const double *lam_rhoV, const double *lam_rhoW, const double *lam_rhoE,
+                 double *out_rho, double *out_rhoU, double *out_rhoV, double *out_rhoW,
+                 double *out_rhoE)
+{
+    CS3D_NEED(s, "cs3d_rhs_vjp");
+    if (set != 1 && set != 2) return set_error(CS3D_ERR_ARG, "cs3d_rhs_vjp: set must be 1 or 2");
+    const double *lam[5] = {lam_rho, lam_rhoU, lam_rhoV, lam_rhoW, lam_rhoE};
+    double *out[5] = {out_rho, out_rhoU, out_rhoV, out_rhoW, out_rhoE};
+    for (int v = 0; v < 5; v++)
+        if (!lam[v]) return set_error(CS3D_ERR_ARG, "cs3d_rhs_vjp: NULL cotangent");
+    return guard("cs3d_rhs_vjp", [&]() -> int {
+        s->impl->rhs_vjp(set, lam, out);
+        return CS3D_OK;
+    });
+}
+
+int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double *J, double *gU,
+                             double *gV, double *gW)
+{
+    CS3D_NEED(s, "cs3d_adjoint_ke_gradient");
+    if (nsteps < 1 || checkpoint < 1)
+        return set_error(CS3D_ERR_ARG, "cs3d_adjoint_ke_gradient: nsteps and checkpoint must be >= 1");
+    return guard("cs3d_adjoint_ke_gradient", [&]() -> int {
+        s->impl->ke_gradient(nsteps, checkpoint, J, gU, gV, gW);
+        int rc = s->impl->check_finite();
+        if (rc) return set_error(rc, "cs3d_adjoint_ke_gradient: non-finite value in the forward pass");
+        return CS3D_OK;
+    });
+}
+
 int cs3d_synchronize(cs3d_solver *s)
 {
     CS3D_NEED(s, "cs3d_synchronize");

@@ -18,6 +18,7 @@
 #include "capi_common.hpp"
 #include "plan_dev.hpp"
 #include "solver.hpp"
+#include "adjoint.hpp"
 #include "solver_kernels.hpp"
 
 namespace cs3d {
@@ -169,6 +170,7 @@ struct SolverT : SolverBase {
             D2[d].release();
             FL[d].release();
         }
+        for (void *p : adjalloc) (void)hipFree(p);
         if (mem) (void)hipFree(mem);
         if (dsc) (void)hipFree(dsc);
         if (dpart) (void)hipFree(dpart);
@@ -362,11 +364,8 @@ struct SolverT : SolverBase {
     {
         if (timeStep % cfg.filter_step == 0) {
             filterTimeStep++;
-            int m = filterTimeStep % 3;
             int o[3];
-            if (m == 1) { o[0] = 0; o[1] = 1; o[2] = 2; }
-            else if (m == 2) { o[0] = 1; o[1] = 2; o[2] = 0; }
-            else { o[0] = 2; o[1] = 0; o[2] = 1; }
+            filter_order(filterTimeStep % 3, o);
             filter_sweep(o[0], F.acc, F.q);
             filter_sweep(o[1], F.q, F.acc);
             filter_sweep(o[2], F.acc, F.q);
@@ -566,6 +565,378 @@ struct SolverT : SolverBase {
         hc(hipStreamSynchronize(st), "sync");
         *ke = r[1] / (double)N;
         return std::isfinite(*ke) ? CS3D_OK : CS3D_ERR_NONFINITE;
+    }
+
+    // ------------------------------------------------------------------
+    // Discrete adjoint.  The transpose of the discrete stage map on periodic
+    // lines without sponges: the forward derivative fields are recomputed
+    // with the standalone compact operators, the cotangent seeds formed
+    // pointwise (adjoint.hip), the transposed operators applied (periodic D1
+    // is skew-adjoint, D2 and the filter self-adjoint), and the result
+    // chained back to the conserved state.  RK4 is reversed stage by stage
+    // with the forward dt sequence frozen; states are recomputed from
+    // checkpoints.
+
+    std::vector<void *> adjalloc;
+    real *A[70] = {};  // vjp work fields
+
+    real *adj_alloc_field()
+    {
+        void *p = nullptr;
+        hc(hipMalloc(&p, (size_t)N * sizeof(real)), "hipMalloc(adjoint)");
+        adjalloc.push_back(p);
+        return (real *)p;
+    }
+
+    void adj_require()
+    {
+        need_ic();
+        for (int d = 0; d < 3; d++)
+            if (cfg.bc_type[d] != CS3D_PERIODIC_SOLVE)
+                throw std::invalid_argument("the adjoint needs periodic lines in every direction");
+        if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
+        if (!A[0])
+            for (int i = 0; i < 70; i++) A[i] = adj_alloc_field();
+    }
+
+    void fop(int kind, int dir, const real *in, real *out)
+    {
+        const PlanDev<real> &pl = kind == OP_D1 ? D1[dir] : D2[dir];
+        hc(launch_field_op<real>(pl, dir == 0 ? MAP_ROW : MAP_COL, lm[dir], in, out, st),
+           "adjoint operator");
+    }
+
+    void comb(real *out, const real *base, std::initializer_list<std::pair<real, const real *>> t)
+    {
+        AdjComb<real> a;
+        a.out = out;
+        a.base = base;
+        a.n = 0;
+        for (auto &x : t) {
+            a.c[a.n] = x.first;
+            a.in[a.n] = x.second;
+            a.n++;
+        }
+        hc(launch_adj_lincomb<real>(a, N, st), "adjoint combination");
+    }
+
+    // out = (d(dt R)/dQ)^T lam with v = dt * lam precomputed by the caller
+    void vjp(real *const *Q, real *const *v, real *const *out)
+    {
+        Ptr5<real> q;
+        for (int c = 0; c < 5; c++) q.in[c] = Q[c];
+        real *U = A[0], *V = A[1], *W = A[2], *Tp = A[3];
+        AdjPrim<real> pr{U, V, W, Tp, A[4], A[5]};
+        hc(launch_adj_prim<real>(q, ph, pr, N, st), "adjoint primitives");
+        real **d1 = A + 6, **d2 = A + 18, **mx = A + 30;
+        const real *prim_f[4] = {U, V, W, Tp};
+        for (int f = 0; f < 4; f++)
+            for (int j = 0; j < 3; j++) {
+                fop(OP_D1, j, prim_f[f], d1[3 * f + j]);
+                fop(OP_D2, j, prim_f[f], d2[3 * f + j]);
+            }
+        fop(OP_D1, 1, d1[0], mx[0]);  // Uxy
+        fop(OP_D1, 2, d1[0], mx[1]);  // Uxz
+        fop(OP_D1, 1, d1[3], mx[2]);  // Vxy
+        fop(OP_D1, 2, d1[4], mx[3]);  // Vyz
+        fop(OP_D1, 2, d1[6], mx[4]);  // Wxz
+        fop(OP_D1, 2, d1[7], mx[5]);  // Wyz
+        AdjSeedIn<real> si;
+        si.U = U; si.V = V; si.W = W; si.Tp = Tp; si.mu = A[4]; si.amu = A[5];
+        for (int i = 0; i < 12; i++) {
+            si.d1[i] = d1[i];
+            si.d2[i] = d2[i];
+        }
+        for (int i = 0; i < 6; i++) si.mx[i] = mx[i];
+        for (int c = 0; c < 5; c++) si.v[c] = v[c];
+        AdjSeedOut<real> so;
+        real **g1 = A + 36, **g2 = A + 48, **gm = A + 60;
+        for (int i = 0; i < 12; i++) {
+            so.g1[i] = g1[i];
+            so.g2[i] = g2[i];
+        }
+        for (int i = 0; i < 6; i++) so.gm[i] = gm[i];
+        so.ub = A[66]; so.vb = A[67]; so.wb = A[68]; so.Tb = A[69];
+        hc(launch_adj_seed<real>(si, so, ph, N, st), "adjoint seeds");
+        // the derivative fields are dead: their storage holds temporaries,
+        // the primitive cotangents and the flux cotangents
+        real **t = A + 6, **tot = A + 17, **Fb = A + 21;
+        const real one = 1, m1 = -1;
+        // U: D_x(D_y gUxy + D_z gUxz - gUx) - D_y gUy - D_z gUz + sum_j D2_j g2U_j
+        fop(OP_D1, 1, gm[0], t[0]);
+        fop(OP_D1, 2, gm[1], t[1]);
+        comb(t[2], nullptr, {{one, t[0]}, {one, t[1]}, {m1, g1[0]}});
+        fop(OP_D1, 0, t[2], t[3]);
+        fop(OP_D1, 1, g1[1], t[4]);
+        fop(OP_D1, 2, g1[2], t[5]);
+        fop(OP_D2, 0, g2[0], t[6]);
+        fop(OP_D2, 1, g2[1], t[7]);
+        fop(OP_D2, 2, g2[2], t[8]);
+        comb(tot[0], A[66], {{one, t[3]}, {m1, t[4]}, {m1, t[5]}, {one, t[6]}, {one, t[7]}, {one, t[8]}});
+        // V: D_x(D_y gVxy - gVx) + D_y(D_z gVyz - gVy) - D_z gVz + D2 terms
+        fop(OP_D1, 1, gm[2], t[0]);
+        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[3]}});
+        fop(OP_D1, 0, t[1], t[2]);
+        fop(OP_D1, 2, gm[3], t[0]);
+        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[4]}});
+        fop(OP_D1, 1, t[1], t[3]);
+        fop(OP_D1, 2, g1[5], t[4]);
+        fop(OP_D2, 0, g2[3], t[5]);
+        fop(OP_D2, 1, g2[4], t[6]);
+        fop(OP_D2, 2, g2[5], t[7]);
+        comb(tot[1], A[67], {{one, t[2]}, {one, t[3]}, {m1, t[4]}, {one, t[5]}, {one, t[6]}, {one, t[7]}});
+        // W: D_x(D_z gWxz - gWx) + D_y(D_z gWyz - gWy) - D_z gWz + D2 terms
+        fop(OP_D1, 2, gm[4], t[0]);
+        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[6]}});
+        fop(OP_D1, 0, t[1], t[2]);
+        fop(OP_D1, 2, gm[5], t[0]);
+        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[7]}});
+        fop(OP_D1, 1, t[1], t[3]);
+        fop(OP_D1, 2, g1[8], t[4]);
+        fop(OP_D2, 0, g2[6], t[5]);
+        fop(OP_D2, 1, g2[7], t[6]);
+        fop(OP_D2, 2, g2[8], t[7]);
+        comb(tot[2], A[68], {{one, t[2]}, {one, t[3]}, {m1, t[4]}, {one, t[5]}, {one, t[6]}, {one, t[7]}});
+        // T: -sum_j D_j gT_j + sum_j D2_j g2T_j
+        for (int j = 0; j < 3; j++) {
+            fop(OP_D1, j, g1[9 + j], t[j]);
+            fop(OP_D2, j, g2[9 + j], t[3 + j]);
+        }
+        comb(tot[3], A[69], {{m1, t[0]}, {m1, t[1]}, {m1, t[2]}, {one, t[3]}, {one, t[4]}, {one, t[5]}});
+        // flux cotangents D_j v_c
+        for (int c = 0; c < 5; c++)
+            for (int j = 0; j < 3; j++) fop(OP_D1, j, v[c], Fb[3 * c + j]);
+        AdjFinal<real> fin;
+        for (int i = 0; i < 15; i++) fin.Fb[i] = Fb[i];
+        for (int i = 0; i < 3; i++) fin.ub[i] = tot[i];
+        fin.Tb = tot[3];
+        for (int c = 0; c < 5; c++) fin.out[c] = out[c];
+        hc(launch_adj_final<real>(q, fin, ph, N, st), "adjoint chain");
+    }
+
+    // dt * RHS of the conserved state Q into k (the stage sweeps in RHS mode)
+    void rhs_into(real *const *Q, real *const *k, double dt)
+    {
+        hc(launch_dt(dsc, cfg.cfl, dt, st), "dt");
+        Fields<real> keep = F;
+        for (int v = 0; v < 5; v++) {
+            F.q[v] = Q[v];
+            F.k2[v] = k[v];
+        }
+        const bool stale = stale_pending;
+        stale_pending = false;
+        sweeps(1, 1, 0);
+        stale_pending = stale;
+        F = keep;
+    }
+
+    // the direction order filter_conserved uses for filter count m
+    static void filter_order(int m, int o[3])
+    {
+        if (m == 1) { o[0] = 0; o[1] = 1; o[2] = 2; }
+        else if (m == 2) { o[0] = 1; o[1] = 2; o[2] = 0; }
+        else { o[0] = 2; o[1] = 0; o[2] = 1; }
+    }
+
+    // the three filter sweeps in the order of filter count m; transposed,
+    // the order reverses (each periodic line filter is symmetric)
+    void filter_into(real *const *in, real *const *out, real *const *scratch, int m, bool transpose)
+    {
+        int o[3];
+        filter_order(m, o);
+        if (transpose) std::swap(o[0], o[2]);
+        filter_sweep(o[0], in, out);
+        filter_sweep(o[1], out, scratch);
+        filter_sweep(o[2], scratch, out);
+    }
+
+    struct AdjStep {
+        real *Q2[5], *Q3[5], *Q4[5], *k[5], *acc[5], *la[5], *qb[5], *v[5], *Qb[5], *s5[5];
+    };
+    AdjStep W{};
+
+    void adj_step_buffers()
+    {
+        if (W.Q2[0]) return;
+        real **sets[10] = {W.Q2, W.Q3, W.Q4, W.k, W.acc, W.la, W.qb, W.v, W.Qb, W.s5};
+        for (auto set : sets)
+            for (int c = 0; c < 5; c++) set[c] = adj_alloc_field();
+    }
+
+    // one RK4 step q0 -> q1 with a fixed dt (and the filter when filt)
+    void replay_step(real *const *q0, real *const *q1, double dt, int filt)
+    {
+        const real h = (real)0.5;
+        rhs_into(q0, W.k, dt);
+        for (int c = 0; c < 5; c++) {
+            comb(W.acc[c], q0[c], {{(real)(1.0 / 6.0), W.k[c]}});
+            comb(W.Q2[c], q0[c], {{h, W.k[c]}});
+        }
+        rhs_into(W.Q2, W.k, dt);
+        for (int c = 0; c < 5; c++) {
+            comb(W.acc[c], W.acc[c], {{(real)(1.0 / 3.0), W.k[c]}});
+            comb(W.Q3[c], q0[c], {{h, W.k[c]}});
+        }
+        rhs_into(W.Q3, W.k, dt);
+        for (int c = 0; c < 5; c++) {
+            comb(W.acc[c], W.acc[c], {{(real)(1.0 / 3.0), W.k[c]}});
+            comb(W.Q4[c], q0[c], {{(real)1, W.k[c]}});
+        }
+        rhs_into(W.Q4, W.k, dt);
+        for (int c = 0; c < 5; c++) comb(W.acc[c], W.acc[c], {{(real)(1.0 / 6.0), W.k[c]}});
+        if (filt >= 0)
+            filter_into(W.acc, q1, W.s5, filt, false);
+        else
+            for (int c = 0; c < 5; c++)
+                hc(hipMemcpyAsync(q1[c], W.acc[c], (size_t)N * sizeof(real),
+                                  hipMemcpyDeviceToDevice, st), "copy");
+    }
+
+    // lam (cotangent of q_{n+1}) -> cotangent of q_n, for the step from q0
+    void adjoint_step(real *const *q0, double dt, int filt, real *const *lam)
+    {
+        const real h = (real)0.5, r6 = (real)(1.0 / 6.0), r3 = (real)(1.0 / 3.0), rdt = (real)dt;
+        // stage states of the step
+        rhs_into(q0, W.k, dt);
+        for (int c = 0; c < 5; c++) comb(W.Q2[c], q0[c], {{h, W.k[c]}});
+        rhs_into(W.Q2, W.k, dt);
+        for (int c = 0; c < 5; c++) comb(W.Q3[c], q0[c], {{h, W.k[c]}});
+        rhs_into(W.Q3, W.k, dt);
+        for (int c = 0; c < 5; c++) comb(W.Q4[c], q0[c], {{(real)1, W.k[c]}});
+        // q_{n+1} = F(acc)
+        if (filt >= 0)
+            filter_into(lam, W.la, W.s5, filt, true);
+        else
+            for (int c = 0; c < 5; c++)
+                hc(hipMemcpyAsync(W.la[c], lam[c], (size_t)N * sizeof(real),
+                                  hipMemcpyDeviceToDevice, st), "copy");
+        // acc = q + k1/6 + k2/3 + k3/3 + k4/6; Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3
+        for (int c = 0; c < 5; c++) {
+            comb(W.qb[c], W.la[c], {});
+            comb(W.v[c], nullptr, {{rdt * r6, W.la[c]}});
+        }
+        vjp(W.Q4, W.v, W.Qb);
+        for (int c = 0; c < 5; c++) {
+            comb(W.qb[c], W.qb[c], {{(real)1, W.Qb[c]}});
+            comb(W.v[c], nullptr, {{rdt * r3, W.la[c]}, {rdt, W.Qb[c]}});
+        }
+        vjp(W.Q3, W.v, W.Qb);
+        for (int c = 0; c < 5; c++) {
+            comb(W.qb[c], W.qb[c], {{(real)1, W.Qb[c]}});
+            comb(W.v[c], nullptr, {{rdt * r3, W.la[c]}, {rdt * h, W.Qb[c]}});
+        }
+        vjp(W.Q2, W.v, W.Qb);
+        for (int c = 0; c < 5; c++) {
+            comb(W.qb[c], W.qb[c], {{(real)1, W.Qb[c]}});
+            comb(W.v[c], nullptr, {{rdt * r6, W.la[c]}, {rdt * h, W.Qb[c]}});
+        }
+        vjp(const_cast<real *const *>(q0), W.v, W.Qb);
+        for (int c = 0; c < 5; c++) comb(lam[c], W.qb[c], {{(real)1, W.Qb[c]}});
+    }
+
+    void upload5(const double *const *in, real *const *dst, bool conserved)
+    {
+        std::vector<real> buf((size_t)N);
+        for (int v = 0; v < 5; v++) {
+            double off = conserved ? (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0) : 0.0;
+            for (long i = 0; i < N; i++) buf[i] = (real)(in[v][i] - off);
+            hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
+        }
+    }
+
+    void download(const real *src, double *dst)
+    {
+        std::vector<real> buf((size_t)N);
+        hc(hipStreamSynchronize(st), "sync");
+        hc(hipMemcpy(buf.data(), src, (size_t)N * sizeof(real), hipMemcpyDeviceToHost), "download");
+        for (long i = 0; i < N; i++) dst[i] = (double)buf[i];
+    }
+
+    void rhs_vjp(int set, const double *const *lam, double *const *out) override
+    {
+        adj_require();
+        adj_step_buffers();
+        StepScalars h;
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        upload5(lam, W.la, false);
+        for (int c = 0; c < 5; c++) comb(W.v[c], nullptr, {{(real)h.dt, W.la[c]}});
+        real *const *Q = set == 1 ? F.q : F.qk;
+        vjp(Q, W.v, W.Qb);
+        for (int c = 0; c < 5; c++)
+            if (out[c]) download(W.Qb[c], out[c]);
+    }
+
+    void ke_gradient(int nsteps, int checkpoint, double *J, double *gU, double *gV,
+                     double *gW) override
+    {
+        if (nsteps < 1 || checkpoint < 1) throw std::invalid_argument("nsteps and checkpoint must be >= 1");
+        if (cfg.rk != CS3D_RK4) throw std::invalid_argument("the adjoint reverses RK4 only");
+        adj_require();
+        adj_step_buffers();
+        const size_t keep = adjalloc.size();  // per-call buffers follow
+        const int nseg = (nsteps + checkpoint - 1) / checkpoint;
+        std::vector<std::vector<real *>> ck(nseg), seg(checkpoint + 1);
+        for (auto &c : ck)
+            for (int v = 0; v < 5; v++) c.push_back(adj_alloc_field());
+        for (auto &c : seg)
+            for (int v = 0; v < 5; v++) c.push_back(adj_alloc_field());
+        std::vector<double> dts(nsteps);
+        std::vector<int> filt(nsteps);  // filter count mod 3, -1 without filter
+        // forward pass on the solver's own path, keeping checkpoints
+        for (int n = 0; n < nsteps; n++) {
+            if (n % checkpoint == 0)
+                for (int v = 0; v < 5; v++)
+                    hc(hipMemcpyAsync(ck[n / checkpoint][v], F.q[v], (size_t)N * sizeof(real),
+                                      hipMemcpyDeviceToDevice, st), "checkpoint");
+            pre_step();
+            StepScalars hs;
+            hc(hipMemcpyAsync(&hs, dsc, sizeof(hs), hipMemcpyDeviceToHost, st), "scalars");
+            hc(hipStreamSynchronize(st), "sync");
+            dts[n] = hs.dt;
+            filt[n] = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
+            for (int s = 1; s <= 4; s++) sub_step(s, s == 4);
+            post_step();
+        }
+        StepScalars after;
+        hc(hipMemcpyAsync(&after, dsc, sizeof(after), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        if (J) hc(kinetic_energy(J) == CS3D_OK ? hipSuccess : hipErrorUnknown, "kinetic energy");
+        // cotangent of the final state
+        real *lam[5];
+        for (int v = 0; v < 5; v++) lam[v] = adj_alloc_field();
+        Ptr5<real> qf, lo;
+        for (int v = 0; v < 5; v++) {
+            qf.in[v] = F.q[v];
+            lo.out[v] = lam[v];
+        }
+        hc(launch_adj_ke_seed<real>(qf, ph, lo, N, st), "adjoint seed");
+        // reverse, segment by segment
+        for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+            const int n0 = sgi * checkpoint, n1 = std::min(nsteps, n0 + checkpoint);
+            for (int v = 0; v < 5; v++)
+                hc(hipMemcpyAsync(seg[0][v], ck[sgi][v], (size_t)N * sizeof(real),
+                                  hipMemcpyDeviceToDevice, st), "restore");
+            for (int n = n0; n < n1 - 1; n++)
+                replay_step(seg[n - n0].data(), seg[n - n0 + 1].data(), dts[n], filt[n]);
+            for (int n = n1 - 1; n >= n0; n--) adjoint_step(seg[n - n0].data(), dts[n], filt[n], lam);
+        }
+        // to the initial primitive velocities at fixed rho and p
+        Ptr5<real> q0, lb;
+        for (int v = 0; v < 5; v++) {
+            q0.in[v] = ck[0][v];
+            lb.in[v] = lam[v];
+        }
+        real *g[3] = {W.Qb[0], W.Qb[1], W.Qb[2]};
+        hc(launch_adj_to_velocity<real>(q0, lb, ph, g[0], g[1], g[2], N, st), "velocity gradient");
+        if (gU) download(g[0], gU);
+        if (gV) download(g[1], gV);
+        if (gW) download(g[2], gW);
+        hc(hipMemcpyAsync(dsc, &after, sizeof(after), hipMemcpyHostToDevice, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        // the per-call buffers (checkpoints, segment states, lam) go back
+        for (size_t i = keep; i < adjalloc.size(); i++) (void)hipFree(adjalloc[i]);
+        adjalloc.resize(keep);
     }
 
     void synchronize() override { hc(hipStreamSynchronize(st), "sync"); }

@@ -32,6 +32,12 @@ struct SolverBase {
     virtual void *stream() = 0;
     virtual void profile_enable(int on) = 0;
     virtual void profile_report(double *ms, int *counts) = 0;
+    // discrete adjoint (periodic lines, no sponge): lam^T d(dt*RHS)/dQ for
+    // the conserved set `set`, and the gradient of the final volume-averaged
+    // kinetic energy after nsteps with respect to the initial velocities
+    virtual void rhs_vjp(int set, const double *const *lam, double *const *out) = 0;
+    virtual void ke_gradient(int nsteps, int checkpoint, double *J, double *gU, double *gV,
+                             double *gW) = 0;
 };
 
 SolverBase *make_solver(const cs3d_config &c);

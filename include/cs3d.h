@@ -145,6 +145,29 @@ int cs3d_diag_turbulence(cs3d_solver *s, double out[11]);
 /* volume-averaged kinetic energy of the current state, FP64 */
 int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
 
+/* Discrete adjoint (no reference counterpart: the reference has no
+ * adjoint; these serve gradient-based callers such as optimisation and
+ * sensitivity studies).  Periodic lines in every direction, no sponge.
+ *
+ * cs3d_rhs_vjp: the transpose of the linearised stage map Q -> dt*RHS(Q)
+ * (the map cs3d_rhs evaluates, with the dt of the last pre-step) applied to
+ * the cotangents lam_*, at the conserved set `set` (1 = *1, 2 = *k).  Host
+ * fields in XYZ order; a NULL out_* skips that field. */
+int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *lam_rhoU,
+                 const double *lam_rhoV, const double *lam_rhoW, const double *lam_rhoE,
+                 double *out_rho, double *out_rhoU, double *out_rhoV, double *out_rhoW,
+                 double *out_rhoE);
+/* Advance nsteps RK4 steps from the current state (as cs3d_step does) and
+ * return J, the final volume-averaged kinetic energy, and dJ/dU0, dJ/dV0,
+ * dJ/dW0: its gradient with respect to the initial velocity fields at fixed
+ * initial density and pressure, by the discrete adjoint of the steps taken
+ * (the dt sequence of the forward pass held fixed, the filter included).
+ * States are recomputed from a checkpoint every `checkpoint` steps; the
+ * reverse pass holds checkpoint + nsteps/checkpoint extra states.  The
+ * solver is left at the final state.  NULL outputs are skipped. */
+int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double *J, double *gU,
+                             double *gV, double *gW);
+
 int cs3d_synchronize(cs3d_solver *s);
 /* HIP stream the solver launches on (hipStream_t) */
 void *cs3d_stream(cs3d_solver *s);
