@@ -282,14 +282,14 @@ class Solver:
                                  *[_dptr(a) for a in out]))
         return out
 
-    def ke_gradient(self, nsteps, checkpoint=8):
+    def ke_gradient(self, nsteps, checkpoint=8, outputs=True):
         """advance nsteps; return (J, dJ/dU0, dJ/dV0, dJ/dW0) for the final
-        volume-averaged kinetic energy J"""
+        volume-averaged kinetic energy J (only J when outputs is False)"""
         J = ctypes.c_double()
-        g = [np.empty(self.shape) for _ in range(3)]
+        g = [np.empty(self.shape) if outputs else None for _ in range(3)]
         check(lib().cs3d_adjoint_ke_gradient(self.h, nsteps, checkpoint, ctypes.byref(J),
                                              *[_dptr(a) for a in g]))
-        return J.value, g[0], g[1], g[2]
+        return (J.value, g[0], g[1], g[2]) if outputs else J.value
 
     def kinetic_energy(self):
         v = ctypes.c_double()

@@ -39,9 +39,9 @@ __global__ void k_adj_prim(Ptr5<real> q, Phys<real> ph, AdjPrim<real> o, long N)
 }
 
 // Cotangent seeds of the derivative fields and the pointwise cotangents of
-// U, V, W, T from the viscous and heat-flux terms.  Index of first and second
-// derivative fields: 3 * f + j with f = U, V, W, T and j = x, y, z; mixed:
-// Uxy, Uxz, Vxy, Vyz, Wxz, Wyz.
+// U, V, W, T from the viscous and heat-flux terms.  Index of the input first
+// and second derivative fields: 3 * f + j with f = U, V, W, T and
+// j = x, y, z; mixed: Uxy, Uxz, Vxy, Vyz, Wxz, Wyz.
 template <typename real>
 __global__ void k_adj_seed(AdjSeedIn<real> in, AdjSeedOut<real> o, Phys<real> ph, long N)
 {
@@ -63,26 +63,12 @@ __global__ void k_adj_seed(AdjSeedIn<real> in, AdjSeedOut<real> o, Phys<real> ph
         const real Uxy = in.mx[0][i], Uxz = in.mx[1][i], Vxy = in.mx[2][i];
         const real Vyz = in.mx[3][i], Wxz = in.mx[4][i], Wyz = in.mx[5][i];
 
-        // second and mixed derivatives
-        o.g2[0][i] = mu * c43 * w1;
-        o.g2[1][i] = mu * w1;
-        o.g2[2][i] = mu * w1;
-        o.g2[3][i] = mu * w2;
-        o.g2[4][i] = mu * c43 * w2;
-        o.g2[5][i] = mu * w2;
-        o.g2[6][i] = mu * w3;
-        o.g2[7][i] = mu * w3;
-        o.g2[8][i] = mu * c43 * w3;
-        const real gq = ph.cpPr * mu * vE;
-        o.g2[9][i] = gq;
-        o.g2[10][i] = gq;
-        o.g2[11][i] = gq;
-        o.gm[0][i] = mu * c13 * w2;  // Uxy
-        o.gm[1][i] = mu * c13 * w3;  // Uxz
-        o.gm[2][i] = mu * c13 * w1;  // Vxy
-        o.gm[3][i] = mu * c13 * w3;  // Vyz
-        o.gm[4][i] = mu * c13 * w1;  // Wxz
-        o.gm[5][i] = mu * c13 * w2;  // Wyz
+        // second and mixed derivatives: multiples of s_i = mu w_i and of
+        // gq = cp/Pr mu vE
+        o.s[0][i] = mu * w1;
+        o.s[1][i] = mu * w2;
+        o.s[2][i] = mu * w3;
+        o.gq[i] = ph.cpPr * mu * vE;
 
         // first derivatives: the a-terms of the momenta (weights w), the
         // quadratic form (weight mu vE) and the heat-flux square (cp/Pr a vE)
@@ -91,25 +77,19 @@ __global__ void k_adj_seed(AdjSeedIn<real> in, AdjSeedOut<real> o, Phys<real> ph
         const real P3 = c43 * (Wz - real(0.5) * Ux - real(0.5) * Vy);
         const real Sxy = Uy + Vx, Sxz = Wx + Uz, Syz = Wy + Vz;
         const real mq = mu * vE;
-        o.g1[0][i] = a * (c43 * Tx * w1 - c23 * Ty * w2 - c23 * Tz * w3) +
-                     mq * (real(8) / real(3) * Ux - c43 * (Vy + Wz));
-        o.g1[4][i] = a * (-c23 * Tx * w1 + c43 * Ty * w2 - c23 * Tz * w3) +
-                     mq * (real(8) / real(3) * Vy - c43 * (Ux + Wz));
-        o.g1[8][i] = a * (-c23 * Tx * w1 - c23 * Ty * w2 + c43 * Tz * w3) +
-                     mq * (real(8) / real(3) * Wz - c43 * (Ux + Vy));
-        const real gxy = a * (Ty * w1 + Tx * w2) + real(2) * mq * Sxy;
-        const real gxz = a * (Tz * w1 + Tx * w3) + real(2) * mq * Sxz;
-        const real gyz = a * (Tz * w2 + Ty * w3) + real(2) * mq * Syz;
-        o.g1[1][i] = gxy;  // Uy
-        o.g1[3][i] = gxy;  // Vx
-        o.g1[2][i] = gxz;  // Uz
-        o.g1[6][i] = gxz;  // Wx
-        o.g1[5][i] = gyz;  // Vz
-        o.g1[7][i] = gyz;  // Wy
+        o.g1[G1_UX][i] = a * (c43 * Tx * w1 - c23 * Ty * w2 - c23 * Tz * w3) +
+                         mq * (real(8) / real(3) * Ux - c43 * (Vy + Wz));
+        o.g1[G1_VY][i] = a * (-c23 * Tx * w1 + c43 * Ty * w2 - c23 * Tz * w3) +
+                         mq * (real(8) / real(3) * Vy - c43 * (Ux + Wz));
+        o.g1[G1_WZ][i] = a * (-c23 * Tx * w1 - c23 * Ty * w2 + c43 * Tz * w3) +
+                         mq * (real(8) / real(3) * Wz - c43 * (Ux + Vy));
+        o.g1[G1_XY][i] = a * (Ty * w1 + Tx * w2) + real(2) * mq * Sxy;  // Uy and Vx
+        o.g1[G1_XZ][i] = a * (Tz * w1 + Tx * w3) + real(2) * mq * Sxz;  // Uz and Wx
+        o.g1[G1_YZ][i] = a * (Tz * w2 + Ty * w3) + real(2) * mq * Syz;  // Vz and Wy
         const real ht = real(2) * ph.cpPr * a * vE;
-        o.g1[9][i] = a * (P1 * w1 + Sxy * w2 + Sxz * w3) + ht * Tx;
-        o.g1[10][i] = a * (Sxy * w1 + P2 * w2 + Syz * w3) + ht * Ty;
-        o.g1[11][i] = a * (Sxz * w1 + Syz * w2 + P3 * w3) + ht * Tz;
+        o.g1[G1_TX][i] = a * (P1 * w1 + Sxy * w2 + Sxz * w3) + ht * Tx;
+        o.g1[G1_TY][i] = a * (Sxy * w1 + P2 * w2 + Syz * w3) + ht * Ty;
+        o.g1[G1_TZ][i] = a * (Sxz * w1 + Syz * w2 + P3 * w3) + ht * Tz;
 
         // cotangents of mu and a, then of T through mu(T) and a(T)
         const real quad = c43 * (Ux * Ux + Vy * Vy + Wz * Wz) + Uy * Uy + Uz * Uz + Vx * Vx +
@@ -190,12 +170,110 @@ __global__ void k_adj_final(Ptr5<real> q, AdjFinal<real> f, Phys<real> ph, long 
             mb[k] += ub[k] * ir;
             rb -= ub[k] * u[k] * ir;
         }
-        f.out[0][i] = rb;
-        f.out[1][i] = mb[0];
-        f.out[2][i] = mb[1];
-        f.out[3][i] = mb[2];
-        f.out[4][i] = Eb;
+        const real res[5] = {rb, mb[0], mb[1], mb[2], Eb};
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            if (f.out[c]) f.out[c][i] = res[c];
+            if (f.acc[c]) f.acc[c][i] = (f.acc_base[c] ? f.acc_base[c][i] : real(0)) + res[c];
+            if (f.vn[c]) f.vn[c][i] = f.ca * f.la[c][i] + f.cb * res[c];
+        }
     }
+}
+
+// N1 first-derivative and N2 second-derivative line solves of one direction
+// through one exchange (see LineTerms).
+template <int N1, int N2, bool ACC, typename real, int M, bool UNI, int MAP>
+__global__ __launch_bounds__(line_terms_threads<real>()) void k_line_terms(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
+    LineTerms<real> a)
+{
+    constexpr int HW = 2, W = M + 2 * HW, N = N1 + N2;
+    constexpr int KC = kc_deriv<real>();
+    CS3D_LANE_SETUP(MAP, d1, sweep_kr(d1, d2), N)
+    (void)n;
+    (void)per;
+    real r[N][M];
+#pragma unroll
+    for (int t = 0; t < N; t++) {
+        real w[W];
+        win.load(a.in[t], w);
+        if (t < N1)
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[t]);
+        else
+            rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[t]);
+    }
+    solve_batch2<N1, N2, KC, real, M, UNI>(d1, d2, c, r);
+    if (ACC) {
+        real o[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) o[t] = 0;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            if (a.base[b]) {
+                real bv[M];
+                win.load_seg(a.base[b], bv);
+#pragma unroll
+                for (int t = 0; t < M; t++) o[t] += a.bc[b] * bv[t];
+            }
+#pragma unroll
+        for (int k = 0; k < N; k++)
+#pragma unroll
+            for (int t = 0; t < M; t++) o[t] += a.c[k] * r[k][t];
+        if (active) win.store(a.out[0], o, c.len);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+#pragma unroll
+            for (int t = 0; t < M; t++) r[k][t] *= a.c[k];
+            if (active) win.store(a.out[k], r[k], c.len);
+        }
+    }
+}
+
+template <int N1, int N2, bool ACC, typename real, bool UNI>
+static hipError_t launch_lt(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &g,
+                            int map, const LineMap &lm, const LineTerms<real> &a, hipStream_t st)
+{
+    constexpr int M = SEG_DERIV;
+    dim3 grid((lm.nlines + g.LX - 1) / g.LX), block(g.threads);
+    if (map == MAP_ROW)
+        hipLaunchKernelGGL((k_line_terms<N1, N2, ACC, real, M, UNI, MAP_ROW>), grid, block, g.shm,
+                           st, d1, d2, lm, a);
+    else
+        hipLaunchKernelGGL((k_line_terms<N1, N2, ACC, real, M, UNI, MAP_COL>), grid, block, g.shm,
+                           st, d1, d2, lm, a);
+    return hipGetLastError();
+}
+
+template <int N1, int N2, bool ACC, typename real>
+static hipError_t launch_lt2(const OpDev<real> *d1, const OpDev<real> *d2, int uni,
+                             const LaunchGeom &g, int map, const LineMap &lm,
+                             const LineTerms<real> &a, hipStream_t st)
+{
+    return uni ? launch_lt<N1, N2, ACC, real, true>(d1, d2, g, map, lm, a, st)
+               : launch_lt<N1, N2, ACC, real, false>(d1, d2, g, map, lm, a, st);
+}
+
+template <typename real>
+hipError_t launch_line_terms(int n1, int n2, bool acc, const OpDev<real> *d1,
+                             const OpDev<real> *d2, int uni, int M, const LaunchGeom &g, int map,
+                             const LineMap &lm, const LineTerms<real> &a, hipStream_t st)
+{
+    if (M != SEG_DERIV) return hipErrorInvalidValue;
+    if (g.LX < 1 || g.threads < 1 || g.threads > line_terms_threads<real>())
+        return hipErrorInvalidConfiguration;
+#define CS3D_LT(A, B, C)                                                                   \
+    if (n1 == A && n2 == B && acc == C) return launch_lt2<A, B, C, real>(d1, d2, uni, g, map, lm, a, st);
+    CS3D_LT(4, 4, false)
+    CS3D_LT(2, 0, false)
+    CS3D_LT(4, 0, false)
+    CS3D_LT(3, 0, false)
+    CS3D_LT(5, 0, false)
+    CS3D_LT(1, 1, true)
+    CS3D_LT(2, 1, true)
+    CS3D_LT(1, 0, true)
+#undef CS3D_LT
+    return hipErrorInvalidValue;
 }
 
 // Gradient of J = (1/N) sum rho |u|^2 / 2 with respect to the conserved
@@ -287,6 +365,10 @@ hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, con
     template hipError_t launch_adj_seed<real>(const AdjSeedIn<real> &, const AdjSeedOut<real> &,   \
                                               const Phys<real> &, long, hipStream_t);            \
     template hipError_t launch_adj_lincomb<real>(const AdjComb<real> &, long, hipStream_t);        \
+    template hipError_t launch_line_terms<real>(int, int, bool, const OpDev<real> *,               \
+                                                const OpDev<real> *, int, int, const LaunchGeom &, \
+                                                int, const LineMap &, const LineTerms<real> &,     \
+                                                hipStream_t);                                      \
     template hipError_t launch_adj_final<real>(const Ptr5<real> &, const AdjFinal<real> &,         \
                                                const Phys<real> &, long, hipStream_t);           \
     template hipError_t launch_adj_ke_seed<real>(const Ptr5<real> &, const Phys<real> &,           \

@@ -22,11 +22,17 @@ struct AdjSeedIn {
     const real *v[5];  // cotangent of the stage increment dt*R, times dt
 };
 
+// The cotangent seeds, deduplicated: every second-derivative and mixed
+// seed is a multiple of s_i = mu w_i (w_i = v_i + u_i v_E) or of
+// gq = cp/Pr mu v_E, and the first-derivative seeds of the symmetric pairs
+// (Uy, Vx), (Uz, Wx), (Vz, Wy) coincide.
+enum { G1_UX, G1_VY, G1_WZ, G1_XY, G1_XZ, G1_YZ, G1_TX, G1_TY, G1_TZ, G1_COUNT };
+
 template <typename real>
 struct AdjSeedOut {
-    real *g1[12];
-    real *g2[12];
-    real *gm[6];
+    real *s[3];
+    real *gq;
+    real *g1[G1_COUNT];
     real *ub, *vb, *wb, *Tb;
 };
 
@@ -41,12 +47,43 @@ struct AdjComb {
     const real *in[ADJ_COMB_MAX];
 };
 
+// The conserved cotangent res of the vector-Jacobian product, and what is
+// made of it: out = res, acc = acc_base + res, vn = ca la + cb res (each
+// skipped when its pointer is null; acc may alias acc_base, vn the v the
+// product was formed from).
 template <typename real>
 struct AdjFinal {
     const real *Fb[15];
     const real *ub[3];
     const real *Tb;
     real *out[5];
+    real *acc[5];
+    const real *acc_base[5];
+    real *vn[5];
+    const real *la[5];
+    real ca, cb;
+};
+
+// Batched compact line operators along one direction, N1 first-derivative
+// terms then N2 second-derivative terms, term t acting on in[t] with weight
+// c[t].  Separate form: out[t] = c[t] OP_t(in[t]).  Accumulated form:
+// out[0] = sum_b bc[b] base[b] + sum_t c[t] OP_t(in[t]) (null bases skipped).
+constexpr int LINE_TERMS_MAX = 8;
+// most lanes per block of the batched line kernels (FP64 needs the larger
+// register budget of half-size blocks)
+template <typename real>
+constexpr int line_terms_threads()
+{
+    return sizeof(real) == 4 ? 1024 : 512;
+}
+
+template <typename real>
+struct LineTerms {
+    const real *in[LINE_TERMS_MAX];
+    real *out[LINE_TERMS_MAX];
+    real c[LINE_TERMS_MAX];
+    const real *base[2];
+    real bc[2];
 };
 
 template <typename real>
@@ -63,6 +100,10 @@ hipError_t launch_adj_final(const Ptr5<real> &q, const AdjFinal<real> &f, const 
 template <typename real>
 hipError_t launch_adj_ke_seed(const Ptr5<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
                               long N, hipStream_t st);
+template <typename real>
+hipError_t launch_line_terms(int n1, int n2, bool acc, const OpDev<real> *d1,
+                             const OpDev<real> *d2, int uni, int M, const LaunchGeom &g, int map,
+                             const LineMap &lm, const LineTerms<real> &a, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
                                   real *gU, real *gV, real *gW, long N, hipStream_t st);

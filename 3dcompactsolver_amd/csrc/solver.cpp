@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -50,6 +51,11 @@ struct SolverT : SolverBase {
         return env_cap("CS3D_SWEEP_THREADS", sizeof(real) == 4 ? 1024 : 512);
     }
     static int filter_max_threads() { return env_cap("CS3D_FILTER_THREADS", 512); }
+    static int line_terms_max_threads(int dir)
+    {
+        if (dir == 0) return env_cap("CS3D_ADJ_X_THREADS", 256);
+        return env_cap("CS3D_ADJ_THREADS", line_terms_threads<real>());
+    }
 
     cs3d_config cfg;
     int Nx, Ny, Nz;
@@ -570,15 +576,18 @@ struct SolverT : SolverBase {
     // ------------------------------------------------------------------
     // Discrete adjoint.  The transpose of the discrete stage map on periodic
     // lines without sponges: the forward derivative fields are recomputed
-    // with the standalone compact operators, the cotangent seeds formed
-    // pointwise (adjoint.hip), the transposed operators applied (periodic D1
-    // is skew-adjoint, D2 and the filter self-adjoint), and the result
-    // chained back to the conserved state.  RK4 is reversed stage by stage
-    // with the forward dt sequence frozen; states are recomputed from
-    // checkpoints.
+    // with batched compact line solves, the cotangent seeds formed pointwise
+    // (adjoint.hip), the transposed operators applied (periodic D1 is
+    // skew-adjoint, D2 and the filter self-adjoint) with the terms of each
+    // direction summed inside one batched solve, and the result chained back
+    // to the conserved state.  RK4 is reversed stage by stage with the
+    // forward dt sequence frozen; the stage states of each checkpoint
+    // segment are replayed with the fused forward stages and kept.
 
+    static constexpr int ADJ_FIELDS = 53;
     std::vector<void *> adjalloc;
-    real *A[70] = {};  // vjp work fields
+    real *A[ADJ_FIELDS] = {};  // vjp work fields
+    real *lamb[5] = {}, *vb5[5] = {}, *qb5[5] = {}, *la5[5] = {}, *sc5[5] = {};
 
     real *adj_alloc_field()
     {
@@ -595,18 +604,67 @@ struct SolverT : SolverBase {
             if (cfg.bc_type[d] != CS3D_PERIODIC_SOLVE)
                 throw std::invalid_argument("the adjoint needs periodic lines in every direction");
         if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
-        if (!A[0])
-            for (int i = 0; i < 70; i++) A[i] = adj_alloc_field();
+        if (!A[0]) {
+            for (int i = 0; i < ADJ_FIELDS; i++) A[i] = adj_alloc_field();
+            real **sets[5] = {lamb, vb5, qb5, la5, sc5};
+            for (auto set : sets)
+                for (int c = 0; c < 5; c++) set[c] = adj_alloc_field();
+        }
     }
 
-    void fop(int kind, int dir, const real *in, real *out)
+    typedef std::pair<real, const real *> Term;
+
+    void line_terms(int dir, int n1, int n2, bool acc, const LineTerms<real> &a)
     {
-        const PlanDev<real> &pl = kind == OP_D1 ? D1[dir] : D2[dir];
-        hc(launch_field_op<real>(pl, dir == 0 ? MAP_ROW : MAP_COL, lm[dir], in, out, st),
-           "adjoint operator");
+        const OpHost &h1 = D1[dir].host, &h2 = D2[dir].host;
+        if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
+            throw std::logic_error("derivative plans of one direction differ in segmentation");
+        const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
+        LaunchGeom g = line_geom(h1.P, kr, line_terms_max_threads(dir), n1 + n2, sizeof(real));
+        if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
+        hc(launch_line_terms<real>(n1, n2, acc, D1[dir].dev, D2[dir].dev, h1.uniform, h1.M, g,
+                                   dir == 0 ? MAP_ROW : MAP_COL, lm[dir], a, st),
+           "adjoint line solves");
     }
 
-    void comb(real *out, const real *base, std::initializer_list<std::pair<real, const real *>> t)
+    // out[t] = OP_t(in[t]): n1 first derivatives, then n2 second derivatives
+    void sep(int dir, int n1, int n2, std::initializer_list<const real *> in,
+             std::initializer_list<real *> out)
+    {
+        LineTerms<real> a{};
+        int t = 0;
+        for (auto p : in) a.in[t++] = p;
+        t = 0;
+        for (auto p : out) {
+            a.out[t] = p;
+            a.c[t++] = (real)1;
+        }
+        line_terms(dir, n1, n2, false, a);
+    }
+
+    // out = b0 + b1 + sum c D1(in) + sum c D2(in)
+    void acc(int dir, real *out, Term b0, Term b1, std::initializer_list<Term> t1,
+             std::initializer_list<Term> t2)
+    {
+        LineTerms<real> a{};
+        a.out[0] = out;
+        a.base[0] = b0.second;
+        a.bc[0] = b0.first;
+        a.base[1] = b1.second;
+        a.bc[1] = b1.first;
+        int t = 0;
+        for (auto &x : t1) {
+            a.c[t] = x.first;
+            a.in[t++] = x.second;
+        }
+        for (auto &x : t2) {
+            a.c[t] = x.first;
+            a.in[t++] = x.second;
+        }
+        line_terms(dir, (int)t1.size(), (int)t2.size(), true, a);
+    }
+
+    void comb(real *out, const real *base, std::initializer_list<Term> t)
     {
         AdjComb<real> a;
         a.out = out;
@@ -620,27 +678,22 @@ struct SolverT : SolverBase {
         hc(launch_adj_lincomb<real>(a, N, st), "adjoint combination");
     }
 
-    // out = (d(dt R)/dQ)^T lam with v = dt * lam precomputed by the caller
-    void vjp(real *const *Q, real *const *v, real *const *out)
+    // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
+    // with v = dt * lam given; fin says where res goes (AdjFinal).
+    void vjp(real *const *Q, real *const *v, AdjFinal<real> fin)
     {
         Ptr5<real> q;
         for (int c = 0; c < 5; c++) q.in[c] = Q[c];
         real *U = A[0], *V = A[1], *W = A[2], *Tp = A[3];
         AdjPrim<real> pr{U, V, W, Tp, A[4], A[5]};
         hc(launch_adj_prim<real>(q, ph, pr, N, st), "adjoint primitives");
+        // forward derivative fields
         real **d1 = A + 6, **d2 = A + 18, **mx = A + 30;
-        const real *prim_f[4] = {U, V, W, Tp};
-        for (int f = 0; f < 4; f++)
-            for (int j = 0; j < 3; j++) {
-                fop(OP_D1, j, prim_f[f], d1[3 * f + j]);
-                fop(OP_D2, j, prim_f[f], d2[3 * f + j]);
-            }
-        fop(OP_D1, 1, d1[0], mx[0]);  // Uxy
-        fop(OP_D1, 2, d1[0], mx[1]);  // Uxz
-        fop(OP_D1, 1, d1[3], mx[2]);  // Vxy
-        fop(OP_D1, 2, d1[4], mx[3]);  // Vyz
-        fop(OP_D1, 2, d1[6], mx[4]);  // Wxz
-        fop(OP_D1, 2, d1[7], mx[5]);  // Wyz
+        for (int j = 0; j < 3; j++)
+            sep(j, 4, 4, {U, V, W, Tp, U, V, W, Tp},
+                {d1[j], d1[3 + j], d1[6 + j], d1[9 + j], d2[j], d2[3 + j], d2[6 + j], d2[9 + j]});
+        sep(1, 2, 0, {d1[0], d1[3]}, {mx[0], mx[2]});                        // Uxy Vxy
+        sep(2, 4, 0, {d1[0], d1[6], d1[4], d1[7]}, {mx[1], mx[4], mx[3], mx[5]});  // Uxz Wxz Vyz Wyz
         AdjSeedIn<real> si;
         si.U = U; si.V = V; si.W = W; si.Tp = Tp; si.mu = A[4]; si.amu = A[5];
         for (int i = 0; i < 12; i++) {
@@ -649,85 +702,46 @@ struct SolverT : SolverBase {
         }
         for (int i = 0; i < 6; i++) si.mx[i] = mx[i];
         for (int c = 0; c < 5; c++) si.v[c] = v[c];
+        real **s = A + 36, *gq = A[39], **g1 = A + 40;
+        real *ub = A[49], *vb = A[50], *wb = A[51], *Tb = A[52];
         AdjSeedOut<real> so;
-        real **g1 = A + 36, **g2 = A + 48, **gm = A + 60;
-        for (int i = 0; i < 12; i++) {
-            so.g1[i] = g1[i];
-            so.g2[i] = g2[i];
-        }
-        for (int i = 0; i < 6; i++) so.gm[i] = gm[i];
-        so.ub = A[66]; so.vb = A[67]; so.wb = A[68]; so.Tb = A[69];
+        for (int i = 0; i < 3; i++) so.s[i] = s[i];
+        so.gq = gq;
+        for (int i = 0; i < G1_COUNT; i++) so.g1[i] = g1[i];
+        so.ub = ub; so.vb = vb; so.wb = wb; so.Tb = Tb;
         hc(launch_adj_seed<real>(si, so, ph, N, st), "adjoint seeds");
-        // the derivative fields are dead: their storage holds temporaries,
-        // the primitive cotangents and the flux cotangents
-        real **t = A + 6, **tot = A + 17, **Fb = A + 21;
-        const real one = 1, m1 = -1;
-        // U: D_x(D_y gUxy + D_z gUxz - gUx) - D_y gUy - D_z gUz + sum_j D2_j g2U_j
-        fop(OP_D1, 1, gm[0], t[0]);
-        fop(OP_D1, 2, gm[1], t[1]);
-        comb(t[2], nullptr, {{one, t[0]}, {one, t[1]}, {m1, g1[0]}});
-        fop(OP_D1, 0, t[2], t[3]);
-        fop(OP_D1, 1, g1[1], t[4]);
-        fop(OP_D1, 2, g1[2], t[5]);
-        fop(OP_D2, 0, g2[0], t[6]);
-        fop(OP_D2, 1, g2[1], t[7]);
-        fop(OP_D2, 2, g2[2], t[8]);
-        comb(tot[0], A[66], {{one, t[3]}, {m1, t[4]}, {m1, t[5]}, {one, t[6]}, {one, t[7]}, {one, t[8]}});
-        // V: D_x(D_y gVxy - gVx) + D_y(D_z gVyz - gVy) - D_z gVz + D2 terms
-        fop(OP_D1, 1, gm[2], t[0]);
-        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[3]}});
-        fop(OP_D1, 0, t[1], t[2]);
-        fop(OP_D1, 2, gm[3], t[0]);
-        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[4]}});
-        fop(OP_D1, 1, t[1], t[3]);
-        fop(OP_D1, 2, g1[5], t[4]);
-        fop(OP_D2, 0, g2[3], t[5]);
-        fop(OP_D2, 1, g2[4], t[6]);
-        fop(OP_D2, 2, g2[5], t[7]);
-        comb(tot[1], A[67], {{one, t[2]}, {one, t[3]}, {m1, t[4]}, {one, t[5]}, {one, t[6]}, {one, t[7]}});
-        // W: D_x(D_z gWxz - gWx) + D_y(D_z gWyz - gWy) - D_z gWz + D2 terms
-        fop(OP_D1, 2, gm[4], t[0]);
-        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[6]}});
-        fop(OP_D1, 0, t[1], t[2]);
-        fop(OP_D1, 2, gm[5], t[0]);
-        comb(t[1], nullptr, {{one, t[0]}, {m1, g1[7]}});
-        fop(OP_D1, 1, t[1], t[3]);
-        fop(OP_D1, 2, g1[8], t[4]);
-        fop(OP_D2, 0, g2[6], t[5]);
-        fop(OP_D2, 1, g2[7], t[6]);
-        fop(OP_D2, 2, g2[8], t[7]);
-        comb(tot[2], A[68], {{one, t[2]}, {one, t[3]}, {m1, t[4]}, {one, t[5]}, {one, t[6]}, {one, t[7]}});
-        // T: -sum_j D_j gT_j + sum_j D2_j g2T_j
-        for (int j = 0; j < 3; j++) {
-            fop(OP_D1, j, g1[9 + j], t[j]);
-            fop(OP_D2, j, g2[9 + j], t[3 + j]);
-        }
-        comb(tot[3], A[69], {{m1, t[0]}, {m1, t[1]}, {m1, t[2]}, {one, t[3]}, {one, t[4]}, {one, t[5]}});
+        // transposed operators; the derivative fields are dead and their
+        // storage holds the partial sums
+        real *m1 = A[6], *m2 = A[7], *m3 = A[8], *cU = A[9], *cV = A[10];
+        real **tot = A + 11, **Fb = A + 15;
+        const real one = 1, m = -1, c13 = (real)(1.0 / 3.0), c43 = (real)(4.0 / 3.0);
+        const Term none{0, nullptr};
+        // z: D_z s_i for the mixed terms; the z parts of the totals
+        sep(2, 3, 0, {s[0], s[1], s[2]}, {m1, m2, m3});
+        acc(2, tot[0], {one, ub}, none, {{m, g1[G1_XZ]}}, {{one, s[0]}});
+        acc(2, tot[1], {one, vb}, none, {{m, g1[G1_YZ]}}, {{one, s[1]}});
+        acc(2, tot[2], {one, wb}, none, {{m, g1[G1_WZ]}}, {{c43, s[2]}});
+        acc(2, tot[3], {one, Tb}, none, {{m, g1[G1_TZ]}}, {{one, gq}});
+        // y
+        acc(1, tot[0], {one, tot[0]}, none, {{m, g1[G1_XY]}}, {{one, s[0]}});
+        acc(1, tot[1], {one, tot[1]}, none, {{c13, m3}, {m, g1[G1_VY]}}, {{c43, s[1]}});
+        acc(1, tot[2], {one, tot[2]}, none, {{c13, m2}, {m, g1[G1_YZ]}}, {{one, s[2]}});
+        acc(1, tot[3], {one, tot[3]}, none, {{m, g1[G1_TY]}}, {{one, gq}});
+        acc(1, cU, {c13, m3}, {m, g1[G1_UX]}, {{c13, s[1]}}, {});
+        acc(1, cV, {m, g1[G1_XY]}, none, {{c13, s[0]}}, {});
+        // x
+        acc(0, tot[0], {one, tot[0]}, none, {{one, cU}}, {{c43, s[0]}});
+        acc(0, tot[1], {one, tot[1]}, none, {{one, cV}}, {{one, s[1]}});
+        acc(0, tot[2], {one, tot[2]}, none, {{c13, m1}, {m, g1[G1_XZ]}}, {{one, s[2]}});
+        acc(0, tot[3], {one, tot[3]}, none, {{m, g1[G1_TX]}}, {{one, gq}});
         // flux cotangents D_j v_c
-        for (int c = 0; c < 5; c++)
-            for (int j = 0; j < 3; j++) fop(OP_D1, j, v[c], Fb[3 * c + j]);
-        AdjFinal<real> fin;
+        for (int j = 0; j < 3; j++)
+            sep(j, 5, 0, {v[0], v[1], v[2], v[3], v[4]},
+                {Fb[j], Fb[3 + j], Fb[6 + j], Fb[9 + j], Fb[12 + j]});
         for (int i = 0; i < 15; i++) fin.Fb[i] = Fb[i];
         for (int i = 0; i < 3; i++) fin.ub[i] = tot[i];
         fin.Tb = tot[3];
-        for (int c = 0; c < 5; c++) fin.out[c] = out[c];
         hc(launch_adj_final<real>(q, fin, ph, N, st), "adjoint chain");
-    }
-
-    // dt * RHS of the conserved state Q into k (the stage sweeps in RHS mode)
-    void rhs_into(real *const *Q, real *const *k, double dt)
-    {
-        hc(launch_dt(dsc, cfg.cfl, dt, st), "dt");
-        Fields<real> keep = F;
-        for (int v = 0; v < 5; v++) {
-            F.q[v] = Q[v];
-            F.k2[v] = k[v];
-        }
-        const bool stale = stale_pending;
-        stale_pending = false;
-        sweeps(1, 1, 0);
-        stale_pending = stale;
-        F = keep;
     }
 
     // the direction order filter_conserved uses for filter count m
@@ -738,108 +752,103 @@ struct SolverT : SolverBase {
         else { o[0] = 2; o[1] = 0; o[2] = 1; }
     }
 
-    // the three filter sweeps in the order of filter count m; transposed,
-    // the order reverses (each periodic line filter is symmetric)
-    void filter_into(real *const *in, real *const *out, real *const *scratch, int m, bool transpose)
+    // the transposed filter of filter count m: the three sweeps in reverse
+    // order (each periodic line filter is symmetric)
+    void filter_transposed(real *const *in, real *const *out, real *const *scratch, int m)
     {
         int o[3];
         filter_order(m, o);
-        if (transpose) std::swap(o[0], o[2]);
-        filter_sweep(o[0], in, out);
+        filter_sweep(o[2], in, out);
         filter_sweep(o[1], out, scratch);
-        filter_sweep(o[2], scratch, out);
+        filter_sweep(o[0], scratch, out);
     }
 
-    struct AdjStep {
-        real *Q2[5], *Q3[5], *Q4[5], *k[5], *acc[5], *la[5], *qb[5], *v[5], *Qb[5], *s5[5];
+    void copy5(real *const *dst, real *const *src)
+    {
+        for (int c = 0; c < 5; c++)
+            hc(hipMemcpyAsync(dst[c], src[c], (size_t)N * sizeof(real), hipMemcpyDeviceToDevice,
+                              st),
+               "copy");
+    }
+
+    // one step on the fused forward path from F.q; stg (when given) receives
+    // the stage states q, Q2, Q3, Q4.  Replays (dt > 0) restore the step
+    // counters the filter round robin reads.
+    struct StepRec {
+        double dt;
+        int timeStep, filterTimeStep, order;  // counters before the step; order -1: no filter
     };
-    AdjStep W{};
 
-    void adj_step_buffers()
+    StepRec fused_step(const StepRec *replay, real *const (*stg)[5])
     {
-        if (W.Q2[0]) return;
-        real **sets[10] = {W.Q2, W.Q3, W.Q4, W.k, W.acc, W.la, W.qb, W.v, W.Qb, W.s5};
-        for (auto set : sets)
-            for (int c = 0; c < 5; c++) set[c] = adj_alloc_field();
+        StepRec r;
+        if (replay) {
+            r = *replay;
+            timeStep = r.timeStep + 1;
+            filterTimeStep = r.filterTimeStep;
+            hc(launch_dt(dsc, cfg.cfl, r.dt, st), "dt");
+        } else {
+            r.timeStep = timeStep;
+            r.filterTimeStep = filterTimeStep;
+            pre_step();
+            StepScalars hs;
+            hc(hipMemcpyAsync(&hs, dsc, sizeof(hs), hipMemcpyDeviceToHost, st), "scalars");
+            hc(hipStreamSynchronize(st), "sync");
+            r.dt = hs.dt;
+            r.order = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
+        }
+        if (stg) copy5(stg[0], F.q);
+        for (int s = 1; s <= 4; s++) {
+            sub_step(s, s == 4);
+            if (stg && s < 4) copy5(stg[s], F.qk);
+        }
+        post_step();
+        return r;
     }
 
-    // one RK4 step q0 -> q1 with a fixed dt (and the filter when filt)
-    void replay_step(real *const *q0, real *const *q1, double dt, int filt)
+    // lam (cotangent of q_{n+1}) -> cotangent of q_n for the step whose
+    // stage states are stg
+    void adjoint_step(real *const (*stg)[5], const StepRec &r, real *const *lam)
     {
-        const real h = (real)0.5;
-        rhs_into(q0, W.k, dt);
-        for (int c = 0; c < 5; c++) {
-            comb(W.acc[c], q0[c], {{(real)(1.0 / 6.0), W.k[c]}});
-            comb(W.Q2[c], q0[c], {{h, W.k[c]}});
+        const real rdt = (real)r.dt;
+        real *const *la = lam;
+        if (r.order >= 0) {
+            filter_transposed(lam, la5, sc5, r.order);
+            la = la5;
         }
-        rhs_into(W.Q2, W.k, dt);
+        // acc = q + k1/6 + k2/3 + k3/3 + k4/6 (filtered into q_{n+1});
+        // Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3; k_s = dt R(Q_s)
+        for (int c = 0; c < 5; c++) comb(vb5[c], nullptr, {{rdt / 6, la[c]}});
+        AdjFinal<real> f{};
         for (int c = 0; c < 5; c++) {
-            comb(W.acc[c], W.acc[c], {{(real)(1.0 / 3.0), W.k[c]}});
-            comb(W.Q3[c], q0[c], {{h, W.k[c]}});
+            f.acc[c] = qb5[c];
+            f.acc_base[c] = la[c];
+            f.vn[c] = vb5[c];
+            f.la[c] = la[c];
         }
-        rhs_into(W.Q3, W.k, dt);
+        f.ca = rdt / 3;
+        f.cb = rdt;
+        vjp(stg[3], vb5, f);
+        for (int c = 0; c < 5; c++) f.acc_base[c] = qb5[c];
+        f.ca = rdt / 3;
+        f.cb = rdt / 2;
+        vjp(stg[2], vb5, f);
+        f.ca = rdt / 6;
+        f.cb = rdt / 2;
+        vjp(stg[1], vb5, f);
+        AdjFinal<real> l{};
         for (int c = 0; c < 5; c++) {
-            comb(W.acc[c], W.acc[c], {{(real)(1.0 / 3.0), W.k[c]}});
-            comb(W.Q4[c], q0[c], {{(real)1, W.k[c]}});
+            l.acc[c] = lam[c];
+            l.acc_base[c] = qb5[c];
         }
-        rhs_into(W.Q4, W.k, dt);
-        for (int c = 0; c < 5; c++) comb(W.acc[c], W.acc[c], {{(real)(1.0 / 6.0), W.k[c]}});
-        if (filt >= 0)
-            filter_into(W.acc, q1, W.s5, filt, false);
-        else
-            for (int c = 0; c < 5; c++)
-                hc(hipMemcpyAsync(q1[c], W.acc[c], (size_t)N * sizeof(real),
-                                  hipMemcpyDeviceToDevice, st), "copy");
+        vjp(stg[0], vb5, l);
     }
 
-    // lam (cotangent of q_{n+1}) -> cotangent of q_n, for the step from q0
-    void adjoint_step(real *const *q0, double dt, int filt, real *const *lam)
-    {
-        const real h = (real)0.5, r6 = (real)(1.0 / 6.0), r3 = (real)(1.0 / 3.0), rdt = (real)dt;
-        // stage states of the step
-        rhs_into(q0, W.k, dt);
-        for (int c = 0; c < 5; c++) comb(W.Q2[c], q0[c], {{h, W.k[c]}});
-        rhs_into(W.Q2, W.k, dt);
-        for (int c = 0; c < 5; c++) comb(W.Q3[c], q0[c], {{h, W.k[c]}});
-        rhs_into(W.Q3, W.k, dt);
-        for (int c = 0; c < 5; c++) comb(W.Q4[c], q0[c], {{(real)1, W.k[c]}});
-        // q_{n+1} = F(acc)
-        if (filt >= 0)
-            filter_into(lam, W.la, W.s5, filt, true);
-        else
-            for (int c = 0; c < 5; c++)
-                hc(hipMemcpyAsync(W.la[c], lam[c], (size_t)N * sizeof(real),
-                                  hipMemcpyDeviceToDevice, st), "copy");
-        // acc = q + k1/6 + k2/3 + k3/3 + k4/6; Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3
-        for (int c = 0; c < 5; c++) {
-            comb(W.qb[c], W.la[c], {});
-            comb(W.v[c], nullptr, {{rdt * r6, W.la[c]}});
-        }
-        vjp(W.Q4, W.v, W.Qb);
-        for (int c = 0; c < 5; c++) {
-            comb(W.qb[c], W.qb[c], {{(real)1, W.Qb[c]}});
-            comb(W.v[c], nullptr, {{rdt * r3, W.la[c]}, {rdt, W.Qb[c]}});
-        }
-        vjp(W.Q3, W.v, W.Qb);
-        for (int c = 0; c < 5; c++) {
-            comb(W.qb[c], W.qb[c], {{(real)1, W.Qb[c]}});
-            comb(W.v[c], nullptr, {{rdt * r3, W.la[c]}, {rdt * h, W.Qb[c]}});
-        }
-        vjp(W.Q2, W.v, W.Qb);
-        for (int c = 0; c < 5; c++) {
-            comb(W.qb[c], W.qb[c], {{(real)1, W.Qb[c]}});
-            comb(W.v[c], nullptr, {{rdt * r6, W.la[c]}, {rdt * h, W.Qb[c]}});
-        }
-        vjp(const_cast<real *const *>(q0), W.v, W.Qb);
-        for (int c = 0; c < 5; c++) comb(lam[c], W.qb[c], {{(real)1, W.Qb[c]}});
-    }
-
-    void upload5(const double *const *in, real *const *dst, bool conserved)
+    void upload5(const double *const *in, real *const *dst)
     {
         std::vector<real> buf((size_t)N);
         for (int v = 0; v < 5; v++) {
-            double off = conserved ? (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0) : 0.0;
-            for (long i = 0; i < N; i++) buf[i] = (real)(in[v][i] - off);
+            for (long i = 0; i < N; i++) buf[i] = (real)in[v][i];
             hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
         }
     }
@@ -855,86 +864,83 @@ struct SolverT : SolverBase {
     void rhs_vjp(int set, const double *const *lam, double *const *out) override
     {
         adj_require();
-        adj_step_buffers();
         StepScalars h;
         hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
-        upload5(lam, W.la, false);
-        for (int c = 0; c < 5; c++) comb(W.v[c], nullptr, {{(real)h.dt, W.la[c]}});
-        real *const *Q = set == 1 ? F.q : F.qk;
-        vjp(Q, W.v, W.Qb);
+        upload5(lam, la5);
+        for (int c = 0; c < 5; c++) comb(vb5[c], nullptr, {{(real)h.dt, la5[c]}});
+        AdjFinal<real> f{};
+        for (int c = 0; c < 5; c++) f.out[c] = qb5[c];
+        vjp(set == 1 ? F.q : F.qk, vb5, f);
         for (int c = 0; c < 5; c++)
-            if (out[c]) download(W.Qb[c], out[c]);
+            if (out[c]) download(qb5[c], out[c]);
     }
 
-    void ke_gradient(int nsteps, int checkpoint, double *J, double *gU, double *gV,
-                     double *gW) override
+    void ke_gradient(int nsteps, int K, double *J, double *gU, double *gV, double *gW) override
     {
-        if (nsteps < 1 || checkpoint < 1) throw std::invalid_argument("nsteps and checkpoint must be >= 1");
+        if (nsteps < 1 || K < 1) throw std::invalid_argument("nsteps and checkpoint must be >= 1");
         if (cfg.rk != CS3D_RK4) throw std::invalid_argument("the adjoint reverses RK4 only");
         adj_require();
-        adj_step_buffers();
         const size_t keep = adjalloc.size();  // per-call buffers follow
-        const int nseg = (nsteps + checkpoint - 1) / checkpoint;
-        std::vector<std::vector<real *>> ck(nseg), seg(checkpoint + 1);
-        for (auto &c : ck)
-            for (int v = 0; v < 5; v++) c.push_back(adj_alloc_field());
-        for (auto &c : seg)
-            for (int v = 0; v < 5; v++) c.push_back(adj_alloc_field());
-        std::vector<double> dts(nsteps);
-        std::vector<int> filt(nsteps);  // filter count mod 3, -1 without filter
-        // forward pass on the solver's own path, keeping checkpoints
+        const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
+        auto state = [&]() {
+            std::array<real *, 5> a;
+            for (auto &p : a) p = adj_alloc_field();
+            return a;
+        };
+        std::vector<std::array<real *, 5>> ck(nseg), stg(4 * (size_t)K);
+        for (auto &c : ck) c = state();
+        for (auto &c : stg) c = state();
+        std::array<real *, 5> fin = state();
+        auto stages = [&](int i) { return (real *const(*)[5])stg[4 * (size_t)i].data(); };
+        static_assert(sizeof(std::array<real *, 5>) == 5 * sizeof(real *), "packed state");
+        std::vector<StepRec> rec(nsteps);
+        // forward pass on the fused path: checkpoints, and the stage states
+        // of the last segment
         for (int n = 0; n < nsteps; n++) {
-            if (n % checkpoint == 0)
-                for (int v = 0; v < 5; v++)
-                    hc(hipMemcpyAsync(ck[n / checkpoint][v], F.q[v], (size_t)N * sizeof(real),
-                                      hipMemcpyDeviceToDevice, st), "checkpoint");
-            pre_step();
-            StepScalars hs;
-            hc(hipMemcpyAsync(&hs, dsc, sizeof(hs), hipMemcpyDeviceToHost, st), "scalars");
-            hc(hipStreamSynchronize(st), "sync");
-            dts[n] = hs.dt;
-            filt[n] = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
-            for (int s = 1; s <= 4; s++) sub_step(s, s == 4);
-            post_step();
+            if (n % K == 0) copy5(ck[n / K].data(), F.q);
+            rec[n] = fused_step(nullptr, n >= last0 ? stages(n - last0) : nullptr);
         }
         StepScalars after;
         hc(hipMemcpyAsync(&after, dsc, sizeof(after), hipMemcpyDeviceToHost, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
+        const int ts_end = timeStep, fts_end = filterTimeStep;
         if (J) hc(kinetic_energy(J) == CS3D_OK ? hipSuccess : hipErrorUnknown, "kinetic energy");
+        copy5(fin.data(), F.q);
         // cotangent of the final state
-        real *lam[5];
-        for (int v = 0; v < 5; v++) lam[v] = adj_alloc_field();
         Ptr5<real> qf, lo;
         for (int v = 0; v < 5; v++) {
             qf.in[v] = F.q[v];
-            lo.out[v] = lam[v];
+            lo.out[v] = lamb[v];
         }
         hc(launch_adj_ke_seed<real>(qf, ph, lo, N, st), "adjoint seed");
         // reverse, segment by segment
         for (int sgi = nseg - 1; sgi >= 0; sgi--) {
-            const int n0 = sgi * checkpoint, n1 = std::min(nsteps, n0 + checkpoint);
-            for (int v = 0; v < 5; v++)
-                hc(hipMemcpyAsync(seg[0][v], ck[sgi][v], (size_t)N * sizeof(real),
-                                  hipMemcpyDeviceToDevice, st), "restore");
-            for (int n = n0; n < n1 - 1; n++)
-                replay_step(seg[n - n0].data(), seg[n - n0 + 1].data(), dts[n], filt[n]);
-            for (int n = n1 - 1; n >= n0; n--) adjoint_step(seg[n - n0].data(), dts[n], filt[n], lam);
+            const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
+            if (sgi != nseg - 1) {
+                copy5(F.q, ck[sgi].data());
+                for (int n = n0; n < n1; n++) fused_step(&rec[n], stages(n - n0));
+            }
+            for (int n = n1 - 1; n >= n0; n--) adjoint_step(stages(n - n0), rec[n], lamb);
         }
         // to the initial primitive velocities at fixed rho and p
         Ptr5<real> q0, lb;
         for (int v = 0; v < 5; v++) {
             q0.in[v] = ck[0][v];
-            lb.in[v] = lam[v];
+            lb.in[v] = lamb[v];
         }
-        real *g[3] = {W.Qb[0], W.Qb[1], W.Qb[2]};
-        hc(launch_adj_to_velocity<real>(q0, lb, ph, g[0], g[1], g[2], N, st), "velocity gradient");
-        if (gU) download(g[0], gU);
-        if (gV) download(g[1], gV);
-        if (gW) download(g[2], gW);
+        hc(launch_adj_to_velocity<real>(q0, lb, ph, qb5[0], qb5[1], qb5[2], N, st),
+           "velocity gradient");
+        if (gU) download(qb5[0], gU);
+        if (gV) download(qb5[1], gV);
+        if (gW) download(qb5[2], gW);
+        // back to the final state of the forward pass
+        copy5(F.q, fin.data());
+        timeStep = ts_end;
+        filterTimeStep = fts_end;
         hc(hipMemcpyAsync(dsc, &after, sizeof(after), hipMemcpyHostToDevice, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
-        // the per-call buffers (checkpoints, segment states, lam) go back
+        diag_valid = false;  // the replays overwrote the step diagnostics
         for (size_t i = keep; i < adjalloc.size(); i++) (void)hipFree(adjalloc[i]);
         adjalloc.resize(keep);
     }

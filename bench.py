@@ -12,6 +12,9 @@ value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
          of RK4 + filter in FP32, 644 B in FP64).
 roofline = the dominant kernel class, timed with HIP events recorded on the
          solver's own stream around every launch of the timed region.
+adjoint = cost of the kinetic-energy gradient through 8 RK4 steps of the
+         same workload (forward pass, checkpointed replay, reverse sweep)
+         against 8 forward steps.
 cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
          with the reference's flags) timed on this host's cores on a
          bounded sample.
@@ -83,6 +86,27 @@ def run_gpu(N, prec, steps, warmup, ic):
     return ms, mpts, prof, ke
 
 
+def run_adjoint(N, prec, ic, nsteps, checkpoint, fwd_ms):
+    """kinetic-energy gradient through nsteps RK4 steps (forward pass,
+    checkpointed replay, reverse sweep), gradients left on the device"""
+    kw, _ = tgv_kw(N)
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=prec, **kw))
+    g.set_initial_conditions(*ic)
+    g.ke_gradient(1, 1, outputs=False)  # buffer set-up outside the timed call
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.ke_gradient(nsteps, checkpoint, outputs=False)
+    g.synchronize()
+    t1 = time.perf_counter()
+    g.close()
+    total = (t1 - t0) * 1e3
+    fwd = nsteps * fwd_ms
+    return {"steps": nsteps, "checkpoint": checkpoint, "gradient_ms": round(total, 2),
+            "forward_ms": round(fwd, 2),
+            "reverse_over_forward": round((total - fwd) / fwd, 2),
+            "gradient_over_forward": round(total / fwd, 2)}
+
+
 def roofline(prof, N, prec):
     s = 4 if prec == cs3d.FP32 else 8
     best = max((k for k in ACCESSES), key=lambda k: prof[k][0])
@@ -132,6 +156,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-grid", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
+    ap.add_argument("--adjoint-steps", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -151,6 +177,7 @@ def main():
                 "roofline": roof64,
                 "kinetic_energy": ke64,
                 "fp32_ke_rel_err": abs(ke32 - ke64) / abs(ke64)}
+    adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, 4, ms32)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_grid, args.cpu_steps)
     line = {
         "metric": METRIC,
@@ -171,7 +198,7 @@ def main():
         "hbm_roof_frac": round(v32 * 1e6 * BYTES_PER_POINT_STAGE[cs3d.FP32] / HBM_PEAK, 4),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof32.items()},
         "fp64": fp64,
-        "adjoint_forward_ratio": None,
+        "adjoint": adj,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -7,12 +7,16 @@ Q -> dt*RHS(Q) must satisfy the dot-product identity
 forward stage (itself pinned to the oracle by test_gpu_solver), and the
 gradient of the final kinetic energy through RK4 steps with the filter
 must match central differences of whole forward runs along random
-directions (dt held fixed, so the forward map is the one differentiated)."""
+directions (dt held fixed, so the forward map is the one differentiated).
+Field by field, both agree with the FP64 numpy adjoint of the oracle
+(oracle/adjoint.py, dense transposed operators, pinned on the CPU)."""
 import numpy as np
 import pytest
 
+import adjoint as oa
 import cs3d
-from helpers import random_ic, smooth, tgv_ic
+import pyoracle as po
+from helpers import random_ic, relerr, smooth, tgv_ic
 
 pytestmark = pytest.mark.gpu
 
@@ -139,3 +143,87 @@ def test_adjoint_rejects_unsupported_configurations():
         g.ke_gradient(2, 1)
     with pytest.raises(cs3d.CS3DError):
         g.ke_gradient(0, 1)
+
+
+def oracle_cfg(dims, kw):
+    Nx, Ny, Nz = dims
+    keep = {k: v for k, v in kw.items() if k in ("Lx", "Ly", "Lz", "alpha_f", "mu_ref", "Pr",
+                                                  "filter_step", "rk")}
+    return po.make_config(Nx, Ny, Nz, **keep)
+
+
+def test_rhs_vjp_matches_oracle_adjoint_fp64():
+    dims = (20, 12, 14)
+    Nx, Ny, Nz = dims
+    shape = (Nz, Ny, Nx)
+    kw = adj_kw(dims, dt_fixed=0.013)
+    g = cs3d.Solver(cs3d.config(Nx, Ny, Nz, precision=cs3d.FP64, **kw))
+    g.set_initial_conditions(*random_ic(shape, seed=21, amp=0.4))
+    g.pre_step()
+    q = g.conserved(1)
+    rng = np.random.default_rng(8)
+    lam = [rng.standard_normal(shape) for _ in range(5)]
+    got = g.rhs_vjp(lam, 1)
+    ref = oa.Adjoint(oracle_cfg(dims, kw)).rhs_vjp(q, lam, 0.013)
+    errs = [relerr(a, b) for a, b in zip(got, ref)]
+    print("vjp vs oracle adjoint:", " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) <= 1e-11, errs
+
+
+@pytest.mark.parametrize("filter_step", [1, 2])
+def test_ke_gradient_matches_oracle_adjoint_fp64(filter_step):
+    dims = (16, 12, 20)
+    Nx, Ny, Nz = dims
+    shape = (Nz, Ny, Nx)
+    kw = adj_kw(dims, dt_fixed=0.02, filter_step=filter_step)
+    ic = random_ic(shape, seed=23, amp=0.5)
+    g = cs3d.Solver(cs3d.config(Nx, Ny, Nz, precision=cs3d.FP64, **kw))
+    g.set_initial_conditions(*ic)
+    J, *grad = g.ke_gradient(5, 2)
+    Jo, *gref = oa.Adjoint(oracle_cfg(dims, kw)).ke_gradient(ic, 5, 0.02, filter_step)
+    assert abs(J - Jo) <= 1e-13 * abs(Jo)
+    errs = [relerr(a, b) for a, b in zip(grad, gref)]
+    print("gradient vs oracle adjoint:", " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) <= 1e-10, errs
+
+
+def test_ke_gradient_tgv128_fp32_200_steps():
+    """Taylor-Green 128^3, FP32 adjoint through 200 RK4 steps with the
+    filter every step, checkpoints every 20 steps, against FP64 central
+    differences (eps 1e-3) along 8 seeded Gaussian directions; the dt of
+    the run is frozen at the first step's CFL dt.  1e-4 relative."""
+    N = 128
+    dims = (N, N, N)
+    kw = adj_kw(dims, mu_ref=(1 / 1600) / 100 ** 0.76, alpha_f=0.49)
+    ic = tgv_ic(N, kw["Lx"])
+    probe = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
+    probe.set_initial_conditions(*ic)
+    probe.pre_step()
+    dt = probe.time()[1]
+    probe.close()
+    kw["dt_fixed"] = dt
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP32, **kw))
+    g.set_initial_conditions(*ic)
+    J, gU, gV, gW = g.ke_gradient(200, 20)
+    g.close()
+    print(f"J = {J:.10e}, dt = {dt:.6e}", flush=True)
+    rho, U, V, W, p = ic
+    eps = 1e-3
+    worst = 0.0
+    for seed in range(8):
+        rng = np.random.default_rng(seed)
+        d = [rng.standard_normal(U.shape) for _ in range(3)]
+        Js = []
+        for sgn in (1, -1):
+            f = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
+            f.set_initial_conditions(rho, U + sgn * eps * d[0], V + sgn * eps * d[1],
+                                     W + sgn * eps * d[2], p)
+            f.step(200)
+            Js.append(f.kinetic_energy())
+            f.close()
+        fd = (Js[0] - Js[1]) / (2 * eps)
+        ad = float(np.vdot(gU, d[0]) + np.vdot(gV, d[1]) + np.vdot(gW, d[2]))
+        rel = abs(ad - fd) / abs(fd)
+        worst = max(worst, rel)
+        print(f"direction {seed}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}", flush=True)
+    assert worst <= 1e-4
