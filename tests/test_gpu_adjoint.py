@@ -227,3 +227,26 @@ def test_ke_gradient_tgv128_fp32_200_steps():
         worst = max(worst, rel)
         print(f"direction {seed}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}", flush=True)
     assert worst <= 1e-4
+
+
+def test_autograd_binding_fp64():
+    """torch.autograd through the adjoint: backward of J gives the adjoint
+    gradient, and a torch-side chain rule on top of it matches finite
+    differences of the composite"""
+    import torch
+    import cs3d_autograd as ag
+    N = 16
+    dims = (N, N, N)
+    kw = adj_kw(dims, dt_fixed=0.02)
+    rho, U, V, W, p = tgv_ic(N, kw["Lx"], p0=10.0 / 1.4)
+    cfg = cs3d.config(N, N, N, precision=cs3d.FP64, **kw)
+    a = torch.tensor(0.7, dtype=torch.float64, requires_grad=True)
+    Ut, Vt, Wt = (torch.from_numpy(x.copy()) for x in (U, V, W))
+    J = ag.kinetic_energy(a * Ut, a * Vt, Wt, cfg, rho, p, 3, 2)
+    J.backward()
+    eps = 1e-6
+    Jp = ag.kinetic_energy((a.detach() + eps) * Ut, (a.detach() + eps) * Vt, Wt, cfg, rho, p, 3, 2)
+    Jm = ag.kinetic_energy((a.detach() - eps) * Ut, (a.detach() - eps) * Vt, Wt, cfg, rho, p, 3, 2)
+    fd = float(Jp - Jm) / (2 * eps)
+    print(f"autograd dJ/da {a.grad.item():.12e} fd {fd:.12e}", flush=True)
+    assert abs(a.grad.item() - fd) <= 1e-6 * abs(fd)
