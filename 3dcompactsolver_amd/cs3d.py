@@ -85,6 +85,7 @@ def lib():
     L.cs3d_kinetic_energy.argtypes = [_vp, _dp]
     L.cs3d_rhs_vjp.argtypes = [_vp, _int] + [_dp] * 10
     L.cs3d_adjoint_ke_gradient.argtypes = [_vp, _int, _int] + [_dp] * 4
+    L.cs3d_adjoint_release.argtypes = [_vp]
     L.cs3d_stream.argtypes = [_vp]
     L.cs3d_stream.restype = _vp
     L.cs3d_ops_create.argtypes = [_int, _int, _int, ctypes.c_double, ctypes.c_double,
@@ -290,6 +291,10 @@ class Solver:
         check(lib().cs3d_adjoint_ke_gradient(self.h, nsteps, checkpoint, ctypes.byref(J),
                                              *[_dptr(a) for a in g]))
         return (J.value, g[0], g[1], g[2]) if outputs else J.value
+
+    def adjoint_release(self):
+        """free the adjoint's device memory"""
+        check(lib().cs3d_adjoint_release(self.h))
 
     def kinetic_energy(self):
         v = ctypes.c_double()

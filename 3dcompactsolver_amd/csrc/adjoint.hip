@@ -38,78 +38,83 @@ __global__ void k_adj_prim(Ptr5<real> q, Phys<real> ph, AdjPrim<real> o, long N)
     }
 }
 
-// Cotangent seeds of the derivative fields and the pointwise cotangents of
-// U, V, W, T from the viscous and heat-flux terms.  Index of the input first
-// and second derivative fields: 3 * f + j with f = U, V, W, T and
+// One point of the cotangent seeds of the derivative fields, and of the
+// pointwise cotangents of U, V, W, T from the viscous and heat-flux terms.
+// First and second derivatives: index 3 * f + j with f = U, V, W, T and
 // j = x, y, z; mixed: Uxy, Uxz, Vxy, Vyz, Wxz, Wyz.
 template <typename real>
-__global__ void k_adj_seed(AdjSeedIn<real> in, AdjSeedOut<real> o, Phys<real> ph, long N)
+struct SeedPt {
+    real U, V, W, Tp, mu, a, v1, v2, v3, vE;
+    real d1[12], d2[12], mx[6];
+};
+enum { SD_S = 0, SD_GQ = 3, SD_G1 = 4, SD_UB = SD_G1 + G1_COUNT, SD_VB, SD_WB, SD_TB, SD_COUNT };
+
+template <typename real>
+__device__ __forceinline__ void seed_point(const Phys<real> &ph, const SeedPt<real> &p,
+                                           real (&o)[SD_COUNT])
 {
     const real c43 = real(4) / real(3), c23 = real(2) / real(3), c13 = real(1) / real(3);
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (long)gridDim.x * blockDim.x) {
-        const real U = in.U[i], V = in.V[i], W = in.W[i];
-        const real T = ph.T0s + in.Tp[i], mu = in.mu[i], a = in.amu[i];
-        const real v1 = in.v[1][i], v2 = in.v[2][i], v3 = in.v[3][i], vE = in.v[4][i];
-        const real w1 = v1 + U * vE, w2 = v2 + V * vE, w3 = v3 + W * vE;
-        const real Ux = in.d1[0][i], Uy = in.d1[1][i], Uz = in.d1[2][i];
-        const real Vx = in.d1[3][i], Vy = in.d1[4][i], Vz = in.d1[5][i];
-        const real Wx = in.d1[6][i], Wy = in.d1[7][i], Wz = in.d1[8][i];
-        const real Tx = in.d1[9][i], Ty = in.d1[10][i], Tz = in.d1[11][i];
-        const real Uxx = in.d2[0][i], Uyy = in.d2[1][i], Uzz = in.d2[2][i];
-        const real Vxx = in.d2[3][i], Vyy = in.d2[4][i], Vzz = in.d2[5][i];
-        const real Wxx = in.d2[6][i], Wyy = in.d2[7][i], Wzz = in.d2[8][i];
-        const real Txx = in.d2[9][i], Tyy = in.d2[10][i], Tzz = in.d2[11][i];
-        const real Uxy = in.mx[0][i], Uxz = in.mx[1][i], Vxy = in.mx[2][i];
-        const real Vyz = in.mx[3][i], Wxz = in.mx[4][i], Wyz = in.mx[5][i];
+    const real U = p.U, V = p.V, W = p.W;
+    const real T = ph.T0s + p.Tp, mu = p.mu, a = p.a;
+    const real v1 = p.v1, v2 = p.v2, v3 = p.v3, vE = p.vE;
+    const real w1 = v1 + U * vE, w2 = v2 + V * vE, w3 = v3 + W * vE;
+    const real Ux = p.d1[0], Uy = p.d1[1], Uz = p.d1[2];
+    const real Vx = p.d1[3], Vy = p.d1[4], Vz = p.d1[5];
+    const real Wx = p.d1[6], Wy = p.d1[7], Wz = p.d1[8];
+    const real Tx = p.d1[9], Ty = p.d1[10], Tz = p.d1[11];
+    const real Uxx = p.d2[0], Uyy = p.d2[1], Uzz = p.d2[2];
+    const real Vxx = p.d2[3], Vyy = p.d2[4], Vzz = p.d2[5];
+    const real Wxx = p.d2[6], Wyy = p.d2[7], Wzz = p.d2[8];
+    const real Txx = p.d2[9], Tyy = p.d2[10], Tzz = p.d2[11];
+    const real Uxy = p.mx[0], Uxz = p.mx[1], Vxy = p.mx[2];
+    const real Vyz = p.mx[3], Wxz = p.mx[4], Wyz = p.mx[5];
 
-        // second and mixed derivatives: multiples of s_i = mu w_i and of
-        // gq = cp/Pr mu vE
-        o.s[0][i] = mu * w1;
-        o.s[1][i] = mu * w2;
-        o.s[2][i] = mu * w3;
-        o.gq[i] = ph.cpPr * mu * vE;
+    // second and mixed derivatives: multiples of s_i = mu w_i and of
+    // gq = cp/Pr mu vE
+    o[SD_S + 0] = mu * w1;
+    o[SD_S + 1] = mu * w2;
+    o[SD_S + 2] = mu * w3;
+    o[SD_GQ] = ph.cpPr * mu * vE;
 
-        // first derivatives: the a-terms of the momenta (weights w), the
-        // quadratic form (weight mu vE) and the heat-flux square (cp/Pr a vE)
-        const real P1 = c43 * (Ux - real(0.5) * Vy - real(0.5) * Wz);
-        const real P2 = c43 * (Vy - real(0.5) * Ux - real(0.5) * Wz);
-        const real P3 = c43 * (Wz - real(0.5) * Ux - real(0.5) * Vy);
-        const real Sxy = Uy + Vx, Sxz = Wx + Uz, Syz = Wy + Vz;
-        const real mq = mu * vE;
-        o.g1[G1_UX][i] = a * (c43 * Tx * w1 - c23 * Ty * w2 - c23 * Tz * w3) +
-                         mq * (real(8) / real(3) * Ux - c43 * (Vy + Wz));
-        o.g1[G1_VY][i] = a * (-c23 * Tx * w1 + c43 * Ty * w2 - c23 * Tz * w3) +
-                         mq * (real(8) / real(3) * Vy - c43 * (Ux + Wz));
-        o.g1[G1_WZ][i] = a * (-c23 * Tx * w1 - c23 * Ty * w2 + c43 * Tz * w3) +
-                         mq * (real(8) / real(3) * Wz - c43 * (Ux + Vy));
-        o.g1[G1_XY][i] = a * (Ty * w1 + Tx * w2) + real(2) * mq * Sxy;  // Uy and Vx
-        o.g1[G1_XZ][i] = a * (Tz * w1 + Tx * w3) + real(2) * mq * Sxz;  // Uz and Wx
-        o.g1[G1_YZ][i] = a * (Tz * w2 + Ty * w3) + real(2) * mq * Syz;  // Vz and Wy
-        const real ht = real(2) * ph.cpPr * a * vE;
-        o.g1[G1_TX][i] = a * (P1 * w1 + Sxy * w2 + Sxz * w3) + ht * Tx;
-        o.g1[G1_TY][i] = a * (Sxy * w1 + P2 * w2 + Syz * w3) + ht * Ty;
-        o.g1[G1_TZ][i] = a * (Sxz * w1 + Syz * w2 + P3 * w3) + ht * Tz;
+    // first derivatives: the a-terms of the momenta (weights w), the
+    // quadratic form (weight mu vE) and the heat-flux square (cp/Pr a vE)
+    const real P1 = c43 * (Ux - real(0.5) * Vy - real(0.5) * Wz);
+    const real P2 = c43 * (Vy - real(0.5) * Ux - real(0.5) * Wz);
+    const real P3 = c43 * (Wz - real(0.5) * Ux - real(0.5) * Vy);
+    const real Sxy = Uy + Vx, Sxz = Wx + Uz, Syz = Wy + Vz;
+    const real mq = mu * vE;
+    o[SD_G1 + G1_UX] = a * (c43 * Tx * w1 - c23 * Ty * w2 - c23 * Tz * w3) +
+                     mq * (real(8) / real(3) * Ux - c43 * (Vy + Wz));
+    o[SD_G1 + G1_VY] = a * (-c23 * Tx * w1 + c43 * Ty * w2 - c23 * Tz * w3) +
+                     mq * (real(8) / real(3) * Vy - c43 * (Ux + Wz));
+    o[SD_G1 + G1_WZ] = a * (-c23 * Tx * w1 - c23 * Ty * w2 + c43 * Tz * w3) +
+                     mq * (real(8) / real(3) * Wz - c43 * (Ux + Vy));
+    o[SD_G1 + G1_XY] = a * (Ty * w1 + Tx * w2) + real(2) * mq * Sxy;  // Uy and Vx
+    o[SD_G1 + G1_XZ] = a * (Tz * w1 + Tx * w3) + real(2) * mq * Sxz;  // Uz and Wx
+    o[SD_G1 + G1_YZ] = a * (Tz * w2 + Ty * w3) + real(2) * mq * Syz;  // Vz and Wy
+    const real ht = real(2) * ph.cpPr * a * vE;
+    o[SD_G1 + G1_TX] = a * (P1 * w1 + Sxy * w2 + Sxz * w3) + ht * Tx;
+    o[SD_G1 + G1_TY] = a * (Sxy * w1 + P2 * w2 + Syz * w3) + ht * Ty;
+    o[SD_G1 + G1_TZ] = a * (Sxz * w1 + Syz * w2 + P3 * w3) + ht * Tz;
 
-        // cotangents of mu and a, then of T through mu(T) and a(T)
-        const real quad = c43 * (Ux * Ux + Vy * Vy + Wz * Wz) + Uy * Uy + Uz * Uz + Vx * Vx +
-                          Vz * Vz + Wx * Wx + Wy * Wy - c43 * (Ux * Vy + Ux * Wz + Vy * Wz) +
-                          real(2) * (Uy * Vx + Uz * Wx + Vz * Wy);
-        const real L1 = c43 * Uxx + Uyy + Uzz + c13 * Vxy + c13 * Wxz;
-        const real L2 = Vxx + c43 * Vyy + Vzz + c13 * Uxy + c13 * Wyz;
-        const real L3 = Wxx + Wyy + c43 * Wzz + c13 * Uxz + c13 * Vyz;
-        const real A1 = Tx * P1 + Ty * Sxy + Tz * Sxz;
-        const real A2 = Ty * P2 + Tx * Sxy + Tz * Syz;
-        const real A3 = Tz * P3 + Tx * Sxz + Ty * Syz;
-        const real mub = w1 * L1 + w2 * L2 + w3 * L3 + vE * quad +
-                         ph.cpPr * (Txx + Tyy + Tzz) * vE;
-        const real ab = A1 * w1 + A2 * w2 + A3 * w3 + ph.cpPr * (Tx * Tx + Ty * Ty + Tz * Tz) * vE;
-        o.Tb[i] = mub * a + ab * (real(-0.24) * a / T);
-        // the energy equation's sum_i u_i Rv_i
-        o.ub[i] = vE * (mu * L1 + a * A1);
-        o.vb[i] = vE * (mu * L2 + a * A2);
-        o.wb[i] = vE * (mu * L3 + a * A3);
-    }
+    // cotangents of mu and a, then of T through mu(T) and a(T)
+    const real quad = c43 * (Ux * Ux + Vy * Vy + Wz * Wz) + Uy * Uy + Uz * Uz + Vx * Vx +
+                      Vz * Vz + Wx * Wx + Wy * Wy - c43 * (Ux * Vy + Ux * Wz + Vy * Wz) +
+                      real(2) * (Uy * Vx + Uz * Wx + Vz * Wy);
+    const real L1 = c43 * Uxx + Uyy + Uzz + c13 * Vxy + c13 * Wxz;
+    const real L2 = Vxx + c43 * Vyy + Vzz + c13 * Uxy + c13 * Wyz;
+    const real L3 = Wxx + Wyy + c43 * Wzz + c13 * Uxz + c13 * Vyz;
+    const real A1 = Tx * P1 + Ty * Sxy + Tz * Sxz;
+    const real A2 = Ty * P2 + Tx * Sxy + Tz * Syz;
+    const real A3 = Tz * P3 + Tx * Sxz + Ty * Syz;
+    const real mub = w1 * L1 + w2 * L2 + w3 * L3 + vE * quad +
+                     ph.cpPr * (Txx + Tyy + Tzz) * vE;
+    const real ab = A1 * w1 + A2 * w2 + A3 * w3 + ph.cpPr * (Tx * Tx + Ty * Ty + Tz * Tz) * vE;
+    o[SD_TB] = mub * a + ab * (real(-0.24) * a / T);
+    // the energy equation's sum_i u_i Rv_i
+    o[SD_UB] = vE * (mu * L1 + a * A1);
+    o[SD_VB] = vE * (mu * L2 + a * A2);
+    o[SD_WB] = vE * (mu * L3 + a * A3);
 }
 
 // out = sum_k c[k] * in[k] (n terms)
@@ -125,59 +130,61 @@ __global__ void k_adj_lincomb(AdjComb<real> a, long N)
 }
 
 // Chain the primitive cotangents (U, V, W, T) and the flux cotangents
-// Fb[3 * c + j] = D_j v_c back to the conserved state.
+// Fb[3 * c + j] = D_j v_c of one point back to the conserved state (q in
+// the stored perturbation form).
 template <typename real>
-__global__ void k_adj_final(Ptr5<real> q, AdjFinal<real> f, Phys<real> ph, long N)
+__device__ __forceinline__ void chain_point(const Phys<real> &ph, const real (&q)[5],
+                                            const real (&Fb)[15], const real (&ubi)[3], real Tbi,
+                                            real (&res)[5])
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (long)gridDim.x * blockDim.x) {
-        const real rho = ph.rho_ref + q.in[0][i];
-        const real ir = real(1) / rho;
-        const real m[3] = {q.in[1][i], q.in[2][i], q.in[3][i]};
-        const real u[3] = {m[0] * ir, m[1] * ir, m[2] * ir};
-        const real E = ph.rhoE_ref + q.in[4][i];
-        const real pp = ph.gm1 * (q.in[4][i] - real(0.5) * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
-        const real p = ph.p_ref + pp;
-        const real T = ph.T0s + (pp - q.in[0][i] * ph.RT0s) * ir * ph.iR;
-        real mb[3], ub[3] = {f.ub[0][i], f.ub[1][i], f.ub[2][i]};
-        real pb = 0, Eb = 0, rb = 0;
-        for (int k = 0; k < 3; k++) mb[k] = f.Fb[k][i];  // continuity fluxes m_j
-        for (int ii = 0; ii < 3; ii++) {                  // momentum fluxes m_i u_j + p d_ij
-            for (int j = 0; j < 3; j++) {
-                const real Fb = f.Fb[3 * (ii + 1) + j][i];
-                mb[ii] += Fb * u[j];
-                ub[j] += Fb * m[ii];
-                if (ii == j) pb += Fb;
-            }
-        }
-        for (int j = 0; j < 3; j++) {  // energy fluxes (E + p) u_j
-            const real Fb = f.Fb[12 + j][i];
-            Eb += Fb * u[j];
-            pb += Fb * u[j];
-            ub[j] += Fb * (E + p);
-        }
-        // T = p / (rho R)
-        const real Tb = f.Tb[i];
-        pb += Tb * ir * ph.iR;
-        rb -= Tb * T * ir;
-        // p = (gamma - 1)(E - |m|^2 / (2 rho))
-        Eb += ph.gm1 * pb;
-        const real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-        rb += ph.gm1 * real(0.5) * u2 * pb;
-        for (int k = 0; k < 3; k++) mb[k] -= ph.gm1 * u[k] * pb;
-        // u = m / rho
-        for (int k = 0; k < 3; k++) {
-            mb[k] += ub[k] * ir;
-            rb -= ub[k] * u[k] * ir;
-        }
-        const real res[5] = {rb, mb[0], mb[1], mb[2], Eb};
+    const real rho = ph.rho_ref + q[0];
+    const real ir = real(1) / rho;
+    const real m[3] = {q[1], q[2], q[3]};
+    const real u[3] = {m[0] * ir, m[1] * ir, m[2] * ir};
+    const real E = ph.rhoE_ref + q[4];
+    const real pp = ph.gm1 * (q[4] - real(0.5) * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
+    const real p = ph.p_ref + pp;
+    const real T = ph.T0s + (pp - q[0] * ph.RT0s) * ir * ph.iR;
+    real mb[3], ub[3] = {ubi[0], ubi[1], ubi[2]};
+    real pb = 0, Eb = 0, rb = 0;
 #pragma unroll
-        for (int c = 0; c < 5; c++) {
-            if (f.out[c]) f.out[c][i] = res[c];
-            if (f.acc[c]) f.acc[c][i] = (f.acc_base[c] ? f.acc_base[c][i] : real(0)) + res[c];
-            if (f.vn[c]) f.vn[c][i] = f.ca * f.la[c][i] + f.cb * res[c];
+    for (int k = 0; k < 3; k++) mb[k] = Fb[k];  // continuity fluxes m_j
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {  // momentum fluxes m_i u_j + p d_ij
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const real fb = Fb[3 * (ii + 1) + j];
+            mb[ii] += fb * u[j];
+            ub[j] += fb * m[ii];
+            if (ii == j) pb += fb;
         }
     }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {  // energy fluxes (E + p) u_j
+        const real fb = Fb[12 + j];
+        Eb += fb * u[j];
+        pb += fb * u[j];
+        ub[j] += fb * (E + p);
+    }
+    // T = p / (rho R)
+    const real Tb = Tbi;
+    pb += Tb * ir * ph.iR;
+    rb -= Tb * T * ir;
+    // p = (gamma - 1)(E - |m|^2 / (2 rho))
+    Eb += ph.gm1 * pb;
+    const real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    rb += ph.gm1 * real(0.5) * u2 * pb;
+    for (int k = 0; k < 3; k++) mb[k] -= ph.gm1 * u[k] * pb;
+    // u = m / rho
+    for (int k = 0; k < 3; k++) {
+        mb[k] += ub[k] * ir;
+        rb -= ub[k] * u[k] * ir;
+    }
+    res[0] = rb;
+    res[1] = mb[0];
+    res[2] = mb[1];
+    res[3] = mb[2];
+    res[4] = Eb;
 }
 
 // N1 first-derivative and N2 second-derivative line solves of one direction
@@ -279,6 +286,211 @@ hipError_t launch_line_terms(int n1, int n2, bool acc, const OpDev<real> *d1,
 // Gradient of J = (1/N) sum rho |u|^2 / 2 with respect to the conserved
 // state, and the map of a conserved cotangent to the initial primitive
 // velocities at fixed rho and p: m_i = rho u_i, E = p/(gamma-1) + rho|u|^2/2.
+// The x-direction derivatives of the stage (D1 of U, V, W, T and of Uy, Uz,
+// Vy, Wz for the mixed derivatives, D2 of U, V, W, T) through one batched
+// solve, then the cotangent seeds at every point from them and the stored
+// y and z derivatives.  The mixed derivatives are taken as D_x D_y U etc.:
+// periodic operators along different axes commute.
+template <typename real, int M, bool UNI>
+__global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xseed(const OpDev<real> *__restrict__ d1,
+                                                             const OpDev<real> *__restrict__ d2,
+                                                             LineMap lm, AdjXSeed<real> in,
+                                                             AdjSeedOut<real> o, Phys<real> ph)
+{
+    constexpr int HW = 2, W = M + 2 * HW;
+    constexpr int KC = kc_deriv<real>();
+    CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 12)
+    (void)n;
+    (void)per;
+    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx
+    real r[12][M];
+    {
+        const real *prim[4] = {in.U, in.V, in.W, in.Tp};
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            real w[W];
+            win.load(prim[f], w);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[f]);
+            rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[8 + f]);
+        }
+        const real *inner[4] = {in.dy1[0], in.dz1[0], in.dy1[1], in.dz1[2]};
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            real w[W];
+            win.load(inner[f], w);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[4 + f]);
+        }
+    }
+    solve_batch2<8, 4, KC, real, M, UNI>(d1, d2, c, r);
+    real out[SD_COUNT][M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        const unsigned at = win.mid(t);
+        SeedPt<real> p;
+        p.U = ldo(in.U, at);
+        p.V = ldo(in.V, at);
+        p.W = ldo(in.W, at);
+        p.Tp = ldo(in.Tp, at);
+        p.mu = ldo(in.mu, at);
+        p.a = ldo(in.amu, at);
+        p.v1 = ldo(in.v[1], at);
+        p.v2 = ldo(in.v[2], at);
+        p.v3 = ldo(in.v[3], at);
+        p.vE = ldo(in.v[4], at);
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            p.d1[3 * f] = r[f][t];
+            p.d1[3 * f + 1] = ldo(in.dy1[f], at);
+            p.d1[3 * f + 2] = ldo(in.dz1[f], at);
+            p.d2[3 * f] = r[8 + f][t];
+            p.d2[3 * f + 1] = ldo(in.dy2[f], at);
+            p.d2[3 * f + 2] = ldo(in.dz2[f], at);
+        }
+        p.mx[0] = r[4][t];  // Uxy
+        p.mx[1] = r[5][t];  // Uxz
+        p.mx[2] = r[6][t];  // Vxy
+        p.mx[3] = ldo(in.Vyz, at);
+        p.mx[4] = r[7][t];  // Wxz
+        p.mx[5] = ldo(in.Wyz, at);
+        real sd[SD_COUNT];
+        seed_point(ph, p, sd);
+#pragma unroll
+        for (int k = 0; k < SD_COUNT; k++) out[k][t] = sd[k];
+    }
+    if (!active) return;
+    real *dst[SD_COUNT];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dst[SD_S + k] = o.s[k];
+    dst[SD_GQ] = o.gq;
+#pragma unroll
+    for (int k = 0; k < G1_COUNT; k++) dst[SD_G1 + k] = o.g1[k];
+    dst[SD_UB] = o.ub;
+    dst[SD_VB] = o.vb;
+    dst[SD_WB] = o.wb;
+    dst[SD_TB] = o.Tb;
+#pragma unroll
+    for (int k = 0; k < SD_COUNT; k++) win.store(dst[k], out[k], c.len);
+}
+
+// The x-direction transposed terms of the primitive cotangents and the x
+// flux cotangents through one batched solve, then the chain back to the
+// conserved state at every point, with its epilogue (AdjFinal).
+template <typename real, int M, bool UNI>
+__global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> *__restrict__ d1,
+                                                              const OpDev<real> *__restrict__ d2,
+                                                              LineMap lm, AdjFinal<real> f,
+                                                              Phys<real> ph)
+{
+    constexpr int HW = 2, W = M + 2 * HW;
+    constexpr int KC = kc_deriv<real>();
+    CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 14)
+    (void)n;
+    (void)per;
+    // r: D_x of cU cV m1 gxz gTx v0..v4 | D2_x of s1 s2 s3 gq
+    real r[14][M];
+    {
+        const real *in1[10] = {f.cU, f.cV, f.m1, f.gxz, f.gTx, f.v[0], f.v[1], f.v[2], f.v[3], f.v[4]};
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            real w[W];
+            win.load(in1[k], w);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[k]);
+        }
+        const real *in2[4] = {f.s[0], f.s[1], f.s[2], f.gq};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            real w[W];
+            win.load(in2[k], w);
+            rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[10 + k]);
+        }
+    }
+    solve_batch2<10, 4, KC, real, M, UNI>(d1, d2, c, r);
+    const real c13 = real(1) / real(3), c43 = real(4) / real(3);
+    real res[5][M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        const unsigned at = win.mid(t);
+        real q[5], Fb[15], ub[3];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            q[k] = ldo(f.q[k], at);
+            Fb[3 * k] = r[5 + k][t];
+            Fb[3 * k + 1] = ldo(f.Fy[k], at);
+            Fb[3 * k + 2] = ldo(f.Fz[k], at);
+        }
+        ub[0] = ldo(f.tot[0], at) + r[0][t] + c43 * r[10][t];
+        ub[1] = ldo(f.tot[1], at) + r[1][t] + r[11][t];
+        ub[2] = ldo(f.tot[2], at) + c13 * r[2][t] - r[3][t] + r[12][t];
+        const real Tb = ldo(f.tot[3], at) - r[4][t] + r[13][t];
+        real rp[5];
+        chain_point(ph, q, Fb, ub, Tb, rp);
+#pragma unroll
+        for (int k = 0; k < 5; k++) res[k][t] = rp[k];
+    }
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        if (f.out[k]) win.store(f.out[k], res[k], c.len);
+        if (f.acc[k]) {
+            real b[M];
+            if (f.acc_base[k])
+                win.load_seg(f.acc_base[k], b);
+            else
+#pragma unroll
+                for (int t = 0; t < M; t++) b[t] = 0;
+#pragma unroll
+            for (int t = 0; t < M; t++) b[t] += res[k][t];
+            win.store(f.acc[k], b, c.len);
+        }
+        if (f.vn[k]) {
+            real l[M];
+            win.load_seg(f.la[k], l);
+#pragma unroll
+            for (int t = 0; t < M; t++) l[t] = f.ca * l[t] + f.cb * res[k][t];
+            win.store(f.vn[k], l, c.len);
+        }
+    }
+}
+
+template <typename real, bool UNI>
+static hipError_t launch_x(int which, const OpDev<real> *d1, const OpDev<real> *d2,
+                           const LaunchGeom &g, const LineMap &lm, const AdjXSeed<real> *in,
+                           const AdjSeedOut<real> *so, const AdjFinal<real> *fin,
+                           const Phys<real> &ph, hipStream_t st)
+{
+    constexpr int M = SEG_DERIV;
+    dim3 grid((lm.nlines + g.LX - 1) / g.LX), block(g.threads);
+    if (which == 0)
+        hipLaunchKernelGGL((k_adj_xseed<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, *in,
+                           *so, ph);
+    else
+        hipLaunchKernelGGL((k_adj_xfinal<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, *fin,
+                           ph);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_adj_xseed(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                            const LaunchGeom &g, const LineMap &lm, const AdjXSeed<real> &in,
+                            const AdjSeedOut<real> &o, const Phys<real> &ph, hipStream_t st)
+{
+    if (M != SEG_DERIV) return hipErrorInvalidValue;
+    if (g.LX < 1 || g.threads < 1 || g.threads > ADJ_X_THREADS) return hipErrorInvalidConfiguration;
+    return uni ? launch_x<real, true>(0, d1, d2, g, lm, &in, &o, nullptr, ph, st)
+               : launch_x<real, false>(0, d1, d2, g, lm, &in, &o, nullptr, ph, st);
+}
+
+template <typename real>
+hipError_t launch_adj_xfinal(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                             const LaunchGeom &g, const LineMap &lm, const AdjFinal<real> &f,
+                             const Phys<real> &ph, hipStream_t st)
+{
+    if (M != SEG_DERIV) return hipErrorInvalidValue;
+    if (g.LX < 1 || g.threads < 1 || g.threads > ADJ_X_THREADS) return hipErrorInvalidConfiguration;
+    return uni ? launch_x<real, true>(1, d1, d2, g, lm, nullptr, nullptr, &f, ph, st)
+               : launch_x<real, false>(1, d1, d2, g, lm, nullptr, nullptr, &f, ph, st);
+}
+
 template <typename real>
 __global__ void k_adj_ke_seed(Ptr5<real> q, Phys<real> ph, Ptr5<real> out, long N)
 {
@@ -324,23 +536,9 @@ hipError_t launch_adj_prim(const Ptr5<real> &q, const Phys<real> &ph, const AdjP
     return hipGetLastError();
 }
 template <typename real>
-hipError_t launch_adj_seed(const AdjSeedIn<real> &in, const AdjSeedOut<real> &o,
-                           const Phys<real> &ph, long N, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_adj_seed<real>), grid_for(N), dim3(256), 0, st, in, o, ph, N);
-    return hipGetLastError();
-}
-template <typename real>
 hipError_t launch_adj_lincomb(const AdjComb<real> &a, long N, hipStream_t st)
 {
     hipLaunchKernelGGL((k_adj_lincomb<real>), grid_for(N), dim3(256), 0, st, a, N);
-    return hipGetLastError();
-}
-template <typename real>
-hipError_t launch_adj_final(const Ptr5<real> &q, const AdjFinal<real> &f, const Phys<real> &ph,
-                            long N, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_adj_final<real>), grid_for(N), dim3(256), 0, st, q, f, ph, N);
     return hipGetLastError();
 }
 template <typename real>
@@ -362,15 +560,19 @@ hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, con
 #define CS3D_INST_ADJ(real)                                                                       \
     template hipError_t launch_adj_prim<real>(const Ptr5<real> &, const Phys<real> &,              \
                                               const AdjPrim<real> &, long, hipStream_t);         \
-    template hipError_t launch_adj_seed<real>(const AdjSeedIn<real> &, const AdjSeedOut<real> &,   \
-                                              const Phys<real> &, long, hipStream_t);            \
+    template hipError_t launch_adj_xseed<real>(const OpDev<real> *, const OpDev<real> *, int, int, \
+                                               const LaunchGeom &, const LineMap &,              \
+                                               const AdjXSeed<real> &, const AdjSeedOut<real> &, \
+                                               const Phys<real> &, hipStream_t);                 \
     template hipError_t launch_adj_lincomb<real>(const AdjComb<real> &, long, hipStream_t);        \
     template hipError_t launch_line_terms<real>(int, int, bool, const OpDev<real> *,               \
                                                 const OpDev<real> *, int, int, const LaunchGeom &, \
                                                 int, const LineMap &, const LineTerms<real> &,     \
                                                 hipStream_t);                                      \
-    template hipError_t launch_adj_final<real>(const Ptr5<real> &, const AdjFinal<real> &,         \
-                                               const Phys<real> &, long, hipStream_t);           \
+    template hipError_t launch_adj_xfinal<real>(const OpDev<real> *, const OpDev<real> *, int,     \
+                                                int, const LaunchGeom &, const LineMap &,        \
+                                                const AdjFinal<real> &, const Phys<real> &,      \
+                                                hipStream_t);                                    \
     template hipError_t launch_adj_ke_seed<real>(const Ptr5<real> &, const Phys<real> &,           \
                                                  const Ptr5<real> &, long, hipStream_t);         \
     template hipError_t launch_adj_to_velocity<real>(const Ptr5<real> &, const Ptr5<real> &,       \

@@ -13,13 +13,15 @@ struct AdjPrim {
     real *U, *V, *W, *Tp, *mu, *amu;
 };
 
+// Inputs of the fused x seed kernel: the primitives, the stored y and z
+// derivatives (U, V, W, T order), the two mixed derivatives that are not
+// taken along x, and the stage cotangent v = dt * lam.
 template <typename real>
-struct AdjSeedIn {
+struct AdjXSeed {
     const real *U, *V, *W, *Tp, *mu, *amu;
-    const real *d1[12];
-    const real *d2[12];
-    const real *mx[6];
-    const real *v[5];  // cotangent of the stage increment dt*R, times dt
+    const real *dy1[4], *dz1[4], *dy2[4], *dz2[4];
+    const real *Vyz, *Wyz;
+    const real *v[5];
 };
 
 // The cotangent seeds, deduplicated: every second-derivative and mixed
@@ -47,15 +49,17 @@ struct AdjComb {
     const real *in[ADJ_COMB_MAX];
 };
 
-// The conserved cotangent res of the vector-Jacobian product, and what is
-// made of it: out = res, acc = acc_base + res, vn = ca la + cb res (each
-// skipped when its pointer is null; acc may alias acc_base, vn the v the
-// product was formed from).
+// Inputs of the fused x chain kernel and what is made of its result res,
+// the conserved cotangent of the vector-Jacobian product: the x solves
+// (D_x of cU, cV, m1/3, -gxz, -gTx and of v_c; D2_x of 4/3 s1, s2, s3, gq),
+// the y and z parts of the primitive cotangents (tot), the y and z flux
+// cotangents D_y v_c, D_z v_c, and the state q.  Epilogue: out = res,
+// acc = acc_base + res, vn = ca la + cb res (each skipped when its pointer
+// is null; acc may alias acc_base; vn must not alias v).
 template <typename real>
 struct AdjFinal {
-    const real *Fb[15];
-    const real *ub[3];
-    const real *Tb;
+    const real *cU, *cV, *m1, *gxz, *gTx, *s[3], *gq, *v[5];
+    const real *tot[4], *Fy[5], *Fz[5], *q[5];
     real *out[5];
     real *acc[5];
     const real *acc_base[5];
@@ -63,6 +67,8 @@ struct AdjFinal {
     const real *la[5];
     real ca, cb;
 };
+
+constexpr int ADJ_X_THREADS = 256;
 
 // Batched compact line operators along one direction, N1 first-derivative
 // terms then N2 second-derivative terms, term t acting on in[t] with weight
@@ -90,13 +96,15 @@ template <typename real>
 hipError_t launch_adj_prim(const Ptr5<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
                            long N, hipStream_t st);
 template <typename real>
-hipError_t launch_adj_seed(const AdjSeedIn<real> &in, const AdjSeedOut<real> &o,
-                           const Phys<real> &ph, long N, hipStream_t st);
+hipError_t launch_adj_xseed(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                            const LaunchGeom &g, const LineMap &lm, const AdjXSeed<real> &in,
+                            const AdjSeedOut<real> &o, const Phys<real> &ph, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_lincomb(const AdjComb<real> &a, long N, hipStream_t st);
 template <typename real>
-hipError_t launch_adj_final(const Ptr5<real> &q, const AdjFinal<real> &f, const Phys<real> &ph,
-                            long N, hipStream_t st);
+hipError_t launch_adj_xfinal(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                             const LaunchGeom &g, const LineMap &lm, const AdjFinal<real> &f,
+                             const Phys<real> &ph, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_ke_seed(const Ptr5<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
                               long N, hipStream_t st);

@@ -295,6 +295,15 @@ int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double 
     });
 }
 
+int cs3d_adjoint_release(cs3d_solver *s)
+{
+    CS3D_NEED(s, "cs3d_adjoint_release");
+    return guard("cs3d_adjoint_release", [&]() -> int {
+        s->impl->adjoint_release();
+        return CS3D_OK;
+    });
+}
+
 int cs3d_synchronize(cs3d_solver *s)
 {
     CS3D_NEED(s, "cs3d_synchronize");

@@ -584,10 +584,11 @@ struct SolverT : SolverBase {
     // forward dt sequence frozen; the stage states of each checkpoint
     // segment are replayed with the fused forward stages and kept.
 
-    static constexpr int ADJ_FIELDS = 53;
+    static constexpr int ADJ_FIELDS = 41;
     std::vector<void *> adjalloc;
     real *A[ADJ_FIELDS] = {};  // vjp work fields
-    real *lamb[5] = {}, *vb5[5] = {}, *qb5[5] = {}, *la5[5] = {}, *sc5[5] = {};
+    real *lamb[5] = {}, *vb5[5] = {}, *vb5b[5] = {}, *qb5[5] = {}, *la5[5] = {}, *sc5[5] = {};
+    std::vector<std::array<real *, 5>> adjpool;  // gradient states kept across calls
 
     real *adj_alloc_field()
     {
@@ -606,7 +607,7 @@ struct SolverT : SolverBase {
         if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
         if (!A[0]) {
             for (int i = 0; i < ADJ_FIELDS; i++) A[i] = adj_alloc_field();
-            real **sets[5] = {lamb, vb5, qb5, la5, sc5};
+            real **sets[6] = {lamb, vb5, vb5b, qb5, la5, sc5};
             for (auto set : sets)
                 for (int c = 0; c < 5; c++) set[c] = adj_alloc_field();
         }
@@ -678,6 +679,16 @@ struct SolverT : SolverBase {
         hc(launch_adj_lincomb<real>(a, N, st), "adjoint combination");
     }
 
+    void x_geom(int planes, LaunchGeom &g)
+    {
+        const OpHost &h1 = D1[0].host, &h2 = D2[0].host;
+        if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
+            throw std::logic_error("derivative plans of one direction differ in segmentation");
+        g = line_geom(h1.P, std::max(std::max(h1.K, h2.K), kc_deriv<real>()), ADJ_X_THREADS,
+                      planes, sizeof(real));
+        if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
+    }
+
     // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
     // with v = dt * lam given; fin says where res goes (AdjFinal).
     void vjp(real *const *Q, real *const *v, AdjFinal<real> fin)
@@ -687,61 +698,82 @@ struct SolverT : SolverBase {
         real *U = A[0], *V = A[1], *W = A[2], *Tp = A[3];
         AdjPrim<real> pr{U, V, W, Tp, A[4], A[5]};
         hc(launch_adj_prim<real>(q, ph, pr, N, st), "adjoint primitives");
-        // forward derivative fields
-        real **d1 = A + 6, **d2 = A + 18, **mx = A + 30;
-        for (int j = 0; j < 3; j++)
-            sep(j, 4, 4, {U, V, W, Tp, U, V, W, Tp},
-                {d1[j], d1[3 + j], d1[6 + j], d1[9 + j], d2[j], d2[3 + j], d2[6 + j], d2[9 + j]});
-        sep(1, 2, 0, {d1[0], d1[3]}, {mx[0], mx[2]});                        // Uxy Vxy
-        sep(2, 4, 0, {d1[0], d1[6], d1[4], d1[7]}, {mx[1], mx[4], mx[3], mx[5]});  // Uxz Wxz Vyz Wyz
-        AdjSeedIn<real> si;
-        si.U = U; si.V = V; si.W = W; si.Tp = Tp; si.mu = A[4]; si.amu = A[5];
-        for (int i = 0; i < 12; i++) {
-            si.d1[i] = d1[i];
-            si.d2[i] = d2[i];
+        // y and z derivative fields (U, V, W, T order) and the mixed D_z Vy, D_z Wy
+        real **dy1 = A + 6, **dy2 = A + 10, **dz1 = A + 14, **dz2 = A + 18;
+        real *Vyz = A[22], *Wyz = A[23];
+        sep(1, 4, 4, {U, V, W, Tp, U, V, W, Tp},
+            {dy1[0], dy1[1], dy1[2], dy1[3], dy2[0], dy2[1], dy2[2], dy2[3]});
+        sep(2, 4, 4, {U, V, W, Tp, U, V, W, Tp},
+            {dz1[0], dz1[1], dz1[2], dz1[3], dz2[0], dz2[1], dz2[2], dz2[3]});
+        sep(2, 2, 0, {dy1[1], dy1[2]}, {Vyz, Wyz});
+        // the x derivatives and the seeds in one pass
+        real **s = A + 24, *gq = A[27], **g1 = A + 28;
+        real *ub = A[37], *vb = A[38], *wb = A[39], *Tb = A[40];
+        {
+            AdjXSeed<real> xi;
+            xi.U = U; xi.V = V; xi.W = W; xi.Tp = Tp; xi.mu = A[4]; xi.amu = A[5];
+            for (int f = 0; f < 4; f++) {
+                xi.dy1[f] = dy1[f];
+                xi.dy2[f] = dy2[f];
+                xi.dz1[f] = dz1[f];
+                xi.dz2[f] = dz2[f];
+            }
+            xi.Vyz = Vyz;
+            xi.Wyz = Wyz;
+            for (int c = 0; c < 5; c++) xi.v[c] = v[c];
+            AdjSeedOut<real> so;
+            for (int i = 0; i < 3; i++) so.s[i] = s[i];
+            so.gq = gq;
+            for (int i = 0; i < G1_COUNT; i++) so.g1[i] = g1[i];
+            so.ub = ub; so.vb = vb; so.wb = wb; so.Tb = Tb;
+            LaunchGeom g;
+            x_geom(12, g);
+            hc(launch_adj_xseed<real>(D1[0].dev, D2[0].dev, D1[0].host.uniform, D1[0].host.M, g,
+                                      lm[0], xi, so, ph, st),
+               "adjoint seeds");
         }
-        for (int i = 0; i < 6; i++) si.mx[i] = mx[i];
-        for (int c = 0; c < 5; c++) si.v[c] = v[c];
-        real **s = A + 36, *gq = A[39], **g1 = A + 40;
-        real *ub = A[49], *vb = A[50], *wb = A[51], *Tb = A[52];
-        AdjSeedOut<real> so;
-        for (int i = 0; i < 3; i++) so.s[i] = s[i];
-        so.gq = gq;
-        for (int i = 0; i < G1_COUNT; i++) so.g1[i] = g1[i];
-        so.ub = ub; so.vb = vb; so.wb = wb; so.Tb = Tb;
-        hc(launch_adj_seed<real>(si, so, ph, N, st), "adjoint seeds");
-        // transposed operators; the derivative fields are dead and their
-        // storage holds the partial sums
-        real *m1 = A[6], *m2 = A[7], *m3 = A[8], *cU = A[9], *cV = A[10];
-        real **tot = A + 11, **Fb = A + 15;
+        // transposed operators, z then y; the primitive and derivative
+        // fields are dead and their storage holds the partial sums
+        real *m1 = A[0], *m2 = A[1], *m3 = A[2], *cU = A[3], *cV = A[4];
+        real **tot = A + 6, **Fy = A + 10, **Fz = A + 15;
         const real one = 1, m = -1, c13 = (real)(1.0 / 3.0), c43 = (real)(4.0 / 3.0);
         const Term none{0, nullptr};
-        // z: D_z s_i for the mixed terms; the z parts of the totals
         sep(2, 3, 0, {s[0], s[1], s[2]}, {m1, m2, m3});
         acc(2, tot[0], {one, ub}, none, {{m, g1[G1_XZ]}}, {{one, s[0]}});
         acc(2, tot[1], {one, vb}, none, {{m, g1[G1_YZ]}}, {{one, s[1]}});
         acc(2, tot[2], {one, wb}, none, {{m, g1[G1_WZ]}}, {{c43, s[2]}});
         acc(2, tot[3], {one, Tb}, none, {{m, g1[G1_TZ]}}, {{one, gq}});
-        // y
         acc(1, tot[0], {one, tot[0]}, none, {{m, g1[G1_XY]}}, {{one, s[0]}});
         acc(1, tot[1], {one, tot[1]}, none, {{c13, m3}, {m, g1[G1_VY]}}, {{c43, s[1]}});
         acc(1, tot[2], {one, tot[2]}, none, {{c13, m2}, {m, g1[G1_YZ]}}, {{one, s[2]}});
         acc(1, tot[3], {one, tot[3]}, none, {{m, g1[G1_TY]}}, {{one, gq}});
         acc(1, cU, {c13, m3}, {m, g1[G1_UX]}, {{c13, s[1]}}, {});
         acc(1, cV, {m, g1[G1_XY]}, none, {{c13, s[0]}}, {});
-        // x
-        acc(0, tot[0], {one, tot[0]}, none, {{one, cU}}, {{c43, s[0]}});
-        acc(0, tot[1], {one, tot[1]}, none, {{one, cV}}, {{one, s[1]}});
-        acc(0, tot[2], {one, tot[2]}, none, {{c13, m1}, {m, g1[G1_XZ]}}, {{one, s[2]}});
-        acc(0, tot[3], {one, tot[3]}, none, {{m, g1[G1_TX]}}, {{one, gq}});
-        // flux cotangents D_j v_c
-        for (int j = 0; j < 3; j++)
-            sep(j, 5, 0, {v[0], v[1], v[2], v[3], v[4]},
-                {Fb[j], Fb[3 + j], Fb[6 + j], Fb[9 + j], Fb[12 + j]});
-        for (int i = 0; i < 15; i++) fin.Fb[i] = Fb[i];
-        for (int i = 0; i < 3; i++) fin.ub[i] = tot[i];
-        fin.Tb = tot[3];
-        hc(launch_adj_final<real>(q, fin, ph, N, st), "adjoint chain");
+        // flux cotangents D_y v_c, D_z v_c
+        sep(1, 5, 0, {v[0], v[1], v[2], v[3], v[4]}, {Fy[0], Fy[1], Fy[2], Fy[3], Fy[4]});
+        sep(2, 5, 0, {v[0], v[1], v[2], v[3], v[4]}, {Fz[0], Fz[1], Fz[2], Fz[3], Fz[4]});
+        // x: the remaining terms and the chain to the conserved state
+        fin.cU = cU;
+        fin.cV = cV;
+        fin.m1 = m1;
+        fin.gxz = g1[G1_XZ];
+        fin.gTx = g1[G1_TX];
+        for (int i = 0; i < 3; i++) fin.s[i] = s[i];
+        fin.gq = gq;
+        for (int c = 0; c < 5; c++) {
+            fin.v[c] = v[c];
+            fin.Fy[c] = Fy[c];
+            fin.Fz[c] = Fz[c];
+            fin.q[c] = Q[c];
+        }
+        for (int i = 0; i < 4; i++) fin.tot[i] = tot[i];
+        for (int c = 0; c < 5; c++)
+            if (fin.vn[c] == v[c]) throw std::logic_error("adjoint: next cotangent aliases v");
+        LaunchGeom g;
+        x_geom(14, g);
+        hc(launch_adj_xfinal<real>(D1[0].dev, D2[0].dev, D1[0].host.uniform, D1[0].host.M, g,
+                                   lm[0], fin, ph, st),
+           "adjoint chain");
     }
 
     // the direction order filter_conserved uses for filter count m
@@ -819,29 +851,39 @@ struct SolverT : SolverBase {
         // acc = q + k1/6 + k2/3 + k3/3 + k4/6 (filtered into q_{n+1});
         // Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3; k_s = dt R(Q_s)
         for (int c = 0; c < 5; c++) comb(vb5[c], nullptr, {{rdt / 6, la[c]}});
+        // stage cotangents alternate between two buffers: the x chain kernel
+        // reads v's windows while it writes the next one
+        real *const *vin = vb5, *const *vout = vb5b;
         AdjFinal<real> f{};
         for (int c = 0; c < 5; c++) {
             f.acc[c] = qb5[c];
             f.acc_base[c] = la[c];
-            f.vn[c] = vb5[c];
+            f.vn[c] = vout[c];
             f.la[c] = la[c];
         }
         f.ca = rdt / 3;
         f.cb = rdt;
-        vjp(stg[3], vb5, f);
-        for (int c = 0; c < 5; c++) f.acc_base[c] = qb5[c];
+        vjp(stg[3], vin, f);
+        std::swap(vin, vout);
+        for (int c = 0; c < 5; c++) {
+            f.acc_base[c] = qb5[c];
+            f.vn[c] = vout[c];
+        }
         f.ca = rdt / 3;
         f.cb = rdt / 2;
-        vjp(stg[2], vb5, f);
+        vjp(stg[2], vin, f);
+        std::swap(vin, vout);
+        for (int c = 0; c < 5; c++) f.vn[c] = vout[c];
         f.ca = rdt / 6;
         f.cb = rdt / 2;
-        vjp(stg[1], vb5, f);
+        vjp(stg[1], vin, f);
+        std::swap(vin, vout);
         AdjFinal<real> l{};
         for (int c = 0; c < 5; c++) {
             l.acc[c] = lam[c];
             l.acc_base[c] = qb5[c];
         }
-        vjp(stg[0], vb5, l);
+        vjp(stg[0], vin, l);
     }
 
     void upload5(const double *const *in, real *const *dst)
@@ -881,17 +923,22 @@ struct SolverT : SolverBase {
         if (nsteps < 1 || K < 1) throw std::invalid_argument("nsteps and checkpoint must be >= 1");
         if (cfg.rk != CS3D_RK4) throw std::invalid_argument("the adjoint reverses RK4 only");
         adj_require();
-        const size_t keep = adjalloc.size();  // per-call buffers follow
         const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
+        // checkpoints, the stage states of one segment and the final state
+        // come from a pool that persists across calls
+        size_t used = 0;
         auto state = [&]() {
-            std::array<real *, 5> a;
-            for (auto &p : a) p = adj_alloc_field();
-            return a;
+            if (used == adjpool.size()) {
+                std::array<real *, 5> a;
+                for (auto &p : a) p = adj_alloc_field();
+                adjpool.push_back(a);
+            }
+            return adjpool[used++];
         };
         std::vector<std::array<real *, 5>> ck(nseg), stg(4 * (size_t)K);
-        for (auto &c : ck) c = state();
         for (auto &c : stg) c = state();
         std::array<real *, 5> fin = state();
+        for (auto &c : ck) c = state();
         auto stages = [&](int i) { return (real *const(*)[5])stg[4 * (size_t)i].data(); };
         static_assert(sizeof(std::array<real *, 5>) == 5 * sizeof(real *), "packed state");
         std::vector<StepRec> rec(nsteps);
@@ -941,8 +988,15 @@ struct SolverT : SolverBase {
         hc(hipMemcpyAsync(dsc, &after, sizeof(after), hipMemcpyHostToDevice, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
         diag_valid = false;  // the replays overwrote the step diagnostics
-        for (size_t i = keep; i < adjalloc.size(); i++) (void)hipFree(adjalloc[i]);
-        adjalloc.resize(keep);
+    }
+
+    void adjoint_release() override
+    {
+        hc(hipStreamSynchronize(st), "sync");
+        for (void *p : adjalloc) (void)hipFree(p);
+        adjalloc.clear();
+        adjpool.clear();
+        for (auto &p : A) p = nullptr;
     }
 
     void synchronize() override { hc(hipStreamSynchronize(st), "sync"); }

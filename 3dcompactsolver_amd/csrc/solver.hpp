@@ -38,6 +38,8 @@ struct SolverBase {
     virtual void rhs_vjp(int set, const double *const *lam, double *const *out) = 0;
     virtual void ke_gradient(int nsteps, int checkpoint, double *J, double *gU, double *gV,
                              double *gW) = 0;
+    // frees the adjoint's work fields and state pool
+    virtual void adjoint_release() = 0;
 };
 
 SolverBase *make_solver(const cs3d_config &c);

@@ -92,7 +92,7 @@ def run_adjoint(N, prec, ic, nsteps, checkpoint, fwd_ms):
     kw, _ = tgv_kw(N)
     g = cs3d.Solver(cs3d.config(N, N, N, precision=prec, **kw))
     g.set_initial_conditions(*ic)
-    g.ke_gradient(1, 1, outputs=False)  # buffer set-up outside the timed call
+    g.ke_gradient(nsteps, checkpoint, outputs=False)  # buffer set-up outside the timed call
     g.synchronize()
     t0 = time.perf_counter()
     g.ke_gradient(nsteps, checkpoint, outputs=False)

@@ -163,10 +163,14 @@ int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *l
  * initial density and pressure, by the discrete adjoint of the steps taken
  * (the dt sequence of the forward pass held fixed, the filter included).
  * States are recomputed from a checkpoint every `checkpoint` steps; the
- * reverse pass holds checkpoint + nsteps/checkpoint extra states.  The
- * solver is left at the final state.  NULL outputs are skipped. */
+ * reverse pass holds ceil(nsteps/checkpoint) + 4*checkpoint + 1 states
+ * (5 fields each) and 71 work fields, kept from call to call until
+ * cs3d_adjoint_release or cs3d_destroy.  The solver is left at the final
+ * state.  NULL outputs are skipped. */
 int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double *J, double *gU,
                              double *gV, double *gW);
+/* frees the adjoint's device memory (the next adjoint call allocates again) */
+int cs3d_adjoint_release(cs3d_solver *s);
 
 int cs3d_synchronize(cs3d_solver *s);
 /* HIP stream the solver launches on (hipStream_t) */

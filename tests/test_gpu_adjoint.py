@@ -250,3 +250,23 @@ def test_autograd_binding_fp64():
     fd = float(Jp - Jm) / (2 * eps)
     print(f"autograd dJ/da {a.grad.item():.12e} fd {fd:.12e}", flush=True)
     assert abs(a.grad.item() - fd) <= 1e-6 * abs(fd)
+
+
+def test_gradient_repeatable_across_pool_reuse_and_release():
+    """the state pool is reused by a second call with other sizes, and
+    after cs3d_adjoint_release the next call allocates again; all give the
+    same gradient for the same initial state"""
+    N = 16
+    kw = adj_kw((N, N, N), dt_fixed=0.02)
+    ic = tgv_ic(N, kw["Lx"], p0=10.0 / 1.4)
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
+    out = []
+    for k, release in ((2, False), (3, False), (1, True), (2, False)):
+        g.set_initial_conditions(*ic)
+        out.append(g.ke_gradient(4, k))
+        if release:
+            g.adjoint_release()
+    for J, *gr in out[1:]:
+        assert abs(J - out[0][0]) <= 1e-15 * abs(J)
+        for a, b in zip(gr, out[0][1:]):
+            assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
