@@ -302,56 +302,68 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xseed(const OpDev<real> *
     CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 12)
     (void)n;
     (void)per;
-    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx
-    real r[12][M];
+    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx; own: the
+    // segment's own values of the eight window fields
+    real r[12][M], own[8][M];
     {
-        const real *prim[4] = {in.U, in.V, in.W, in.Tp};
+        const real *src[8] = {in.U, in.V, in.W, in.Tp, in.dy1[0], in.dz1[0], in.dy1[1], in.dz1[2]};
 #pragma unroll
-        for (int f = 0; f < 4; f++) {
+        for (int f = 0; f < 8; f++) {
             real w[W];
-            win.load(prim[f], w);
+            win.load(src[f], w);
+#pragma unroll
+            for (int t = 0; t < M; t++) own[f][t] = w[HW + t];
             rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[f]);
-            rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[8 + f]);
-        }
-        const real *inner[4] = {in.dy1[0], in.dz1[0], in.dy1[1], in.dz1[2]};
-#pragma unroll
-        for (int f = 0; f < 4; f++) {
-            real w[W];
-            win.load(inner[f], w);
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[4 + f]);
+            if (f < 4) rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[8 + f]);
         }
     }
     solve_batch2<8, 4, KC, real, M, UNI>(d1, d2, c, r);
+    // the remaining per-point inputs, whole segments at a time
+    enum { P_MU, P_A, P_V1, P_V2, P_V3, P_VE, P_WY, P_TY, P_VZ, P_TZ, P_VYZ, P_WYZ, P_D2 };
+    real pt[P_D2 + 8][M];
+    {
+        const real *src[P_D2 + 8] = {in.mu, in.amu, in.v[1], in.v[2], in.v[3], in.v[4],
+                                     in.dy1[2], in.dy1[3], in.dz1[1], in.dz1[3], in.Vyz, in.Wyz,
+                                     in.dy2[0], in.dz2[0], in.dy2[1], in.dz2[1],
+                                     in.dy2[2], in.dz2[2], in.dy2[3], in.dz2[3]};
+#pragma unroll
+        for (int k = 0; k < P_D2 + 8; k++) win.load_seg(src[k], pt[k]);
+    }
     real out[SD_COUNT][M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        const unsigned at = win.mid(t);
         SeedPt<real> p;
-        p.U = ldo(in.U, at);
-        p.V = ldo(in.V, at);
-        p.W = ldo(in.W, at);
-        p.Tp = ldo(in.Tp, at);
-        p.mu = ldo(in.mu, at);
-        p.a = ldo(in.amu, at);
-        p.v1 = ldo(in.v[1], at);
-        p.v2 = ldo(in.v[2], at);
-        p.v3 = ldo(in.v[3], at);
-        p.vE = ldo(in.v[4], at);
+        p.U = own[0][t];
+        p.V = own[1][t];
+        p.W = own[2][t];
+        p.Tp = own[3][t];
+        p.mu = pt[P_MU][t];
+        p.a = pt[P_A][t];
+        p.v1 = pt[P_V1][t];
+        p.v2 = pt[P_V2][t];
+        p.v3 = pt[P_V3][t];
+        p.vE = pt[P_VE][t];
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             p.d1[3 * f] = r[f][t];
-            p.d1[3 * f + 1] = ldo(in.dy1[f], at);
-            p.d1[3 * f + 2] = ldo(in.dz1[f], at);
             p.d2[3 * f] = r[8 + f][t];
-            p.d2[3 * f + 1] = ldo(in.dy2[f], at);
-            p.d2[3 * f + 2] = ldo(in.dz2[f], at);
+            p.d2[3 * f + 1] = pt[P_D2 + 2 * f][t];
+            p.d2[3 * f + 2] = pt[P_D2 + 2 * f + 1][t];
         }
+        p.d1[1] = own[4][t];  // Uy
+        p.d1[2] = own[5][t];  // Uz
+        p.d1[4] = own[6][t];  // Vy
+        p.d1[5] = pt[P_VZ][t];
+        p.d1[7] = pt[P_WY][t];
+        p.d1[8] = own[7][t];  // Wz
+        p.d1[10] = pt[P_TY][t];
+        p.d1[11] = pt[P_TZ][t];
         p.mx[0] = r[4][t];  // Uxy
         p.mx[1] = r[5][t];  // Uxz
         p.mx[2] = r[6][t];  // Vxy
-        p.mx[3] = ldo(in.Vyz, at);
+        p.mx[3] = pt[P_VYZ][t];
         p.mx[4] = r[7][t];  // Wxz
-        p.mx[5] = ldo(in.Wyz, at);
+        p.mx[5] = pt[P_WYZ][t];
         real sd[SD_COUNT];
         seed_point(ph, p, sd);
 #pragma unroll
@@ -406,22 +418,30 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> 
     }
     solve_batch2<10, 4, KC, real, M, UNI>(d1, d2, c, r);
     const real c13 = real(1) / real(3), c43 = real(4) / real(3);
+    // the per-point inputs, whole segments at a time: tot, D_y v, D_z v, q
+    real pt[19][M];
+    {
+        const real *src[19] = {f.tot[0], f.tot[1], f.tot[2], f.tot[3], f.Fy[0], f.Fy[1], f.Fy[2],
+                               f.Fy[3], f.Fy[4], f.Fz[0], f.Fz[1], f.Fz[2], f.Fz[3], f.Fz[4],
+                               f.q[0], f.q[1], f.q[2], f.q[3], f.q[4]};
+#pragma unroll
+        for (int k = 0; k < 19; k++) win.load_seg(src[k], pt[k]);
+    }
     real res[5][M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        const unsigned at = win.mid(t);
         real q[5], Fb[15], ub[3];
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            q[k] = ldo(f.q[k], at);
+            q[k] = pt[14 + k][t];
             Fb[3 * k] = r[5 + k][t];
-            Fb[3 * k + 1] = ldo(f.Fy[k], at);
-            Fb[3 * k + 2] = ldo(f.Fz[k], at);
+            Fb[3 * k + 1] = pt[4 + k][t];
+            Fb[3 * k + 2] = pt[9 + k][t];
         }
-        ub[0] = ldo(f.tot[0], at) + r[0][t] + c43 * r[10][t];
-        ub[1] = ldo(f.tot[1], at) + r[1][t] + r[11][t];
-        ub[2] = ldo(f.tot[2], at) + c13 * r[2][t] - r[3][t] + r[12][t];
-        const real Tb = ldo(f.tot[3], at) - r[4][t] + r[13][t];
+        ub[0] = pt[0][t] + r[0][t] + c43 * r[10][t];
+        ub[1] = pt[1][t] + r[1][t] + r[11][t];
+        ub[2] = pt[2][t] + c13 * r[2][t] - r[3][t] + r[12][t];
+        const real Tb = pt[3][t] - r[4][t] + r[13][t];
         real rp[5];
         chain_point(ph, q, Fb, ub, Tb, rp);
 #pragma unroll
