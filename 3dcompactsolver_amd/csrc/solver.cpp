@@ -124,7 +124,7 @@ struct SolverT : SolverBase {
         for (int f = 0; f < 6; f++) sponge |= c.bc_kind[f] == CS3D_BC_SPONGE;
 
         std::memset(&F, 0, sizeof(F));
-        int nf = 15 + 4 + 4 + 5 + 5 + 4 + (sponge ? 11 : 0);
+        int nf = 15 + 4 + 4 + 5 + 5 + 4 + (sponge ? 15 : 0);
         hc(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         hc(hipMalloc(&mem, (size_t)nf * N * sizeof(real)), "hipMalloc(fields)");
         hc(hipMemsetAsync(mem, 0, (size_t)nf * N * sizeof(real), st), "hipMemset(fields)");
@@ -137,8 +137,8 @@ struct SolverT : SolverBase {
         for (int v = 0; v < 5; v++) F.q[v] = take();
         for (int v = 0; v < 5; v++) F.qk[v] = take();
         for (int v = 0; v < 5; v++) F.acc[v] = take();
-        for (int v = 0; v < 4; v++) F.dX[v] = take();
         for (int v = 0; v < 4; v++) F.dY[v] = take();
+        for (int v = 0; v < 4; v++) F.dZ[v] = take();
         for (int v = 0; v < 5; v++) F.R[v] = take();
         for (int v = 0; v < 5; v++) F.k2[v] = take();
         for (int v = 0; v < 4; v++) F.diag[v] = take();
@@ -146,6 +146,7 @@ struct SolverT : SolverBase {
             F.sigma = take();
             for (int v = 0; v < 5; v++) F.avg[v] = take();
             for (int v = 0; v < 5; v++) F.qstale[v] = take();
+            for (int v = 0; v < 4; v++) F.mix[v] = take();
         }
         hc(hipMalloc(&dsc, sizeof(StepScalars)), "hipMalloc(scalars)");
         hc(hipMemsetAsync(dsc, 0, sizeof(StepScalars), st), "hipMemset(scalars)");
@@ -342,7 +343,8 @@ struct SolverT : SolverBase {
         sa.mode = mode;
         sa.diag = diag;
         sa.stale = (rkStep == 1 && stale_pending) ? 1 : 0;
-        for (int d = 0; d < 3; d++)
+        // y, then z, then x (the last sweep assembles and updates)
+        for (int d : {1, 2, 0})
             timed(d, [&] {
                 hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, D1[d].host.uniform, D1[d].host.M,
                                       sweepG[d], lm[d], F, ph, sa, dsc, st),

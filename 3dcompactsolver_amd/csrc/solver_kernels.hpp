@@ -21,8 +21,9 @@ struct Fields {
     real *acc[5];   // RK accumulator (*2)
     real *qlo[5];   // compensation of q (FP32 compensated mode), else null
     real *acclo[5]; // compensation of acc
-    real *dX[4];    // Ux Vx Wx Tx of the stage
-    real *dY[4];    // Uy Vy Wy Ty
+    real *dY[4];    // Uy Vy Wy Ty of the stage (first sweep)
+    real *dZ[4];    // Uz Vz Wz Tz (middle sweep)
+    real *mix[4];   // stale-primitive D_y U, D_y V, D_z U, D_z W (sponge runs only)
     real *R[5];     // partial right-hand side between sweeps
     real *k2[5];    // dt*RHS (only written in RHS-only mode)
     real *sigma;    // sponge strength (null without sponge)
@@ -62,7 +63,7 @@ struct StageArgs {
     int sponge_face[6];// 1 where a face is a sponge (its plane's RHS is zeroed)
     int Nx, Ny, Nz;
     int compensated;
-    int stale;         // X sweep takes primitives on sponge planes from qstale
+    int stale;         // sweeps take primitives on sponge planes from qstale
 };
 
 // XCD-aware workgroup order: the hardware hands consecutive workgroups to

@@ -106,16 +106,21 @@ __device__ __forceinline__ real pprime(const Phys<real> &ph, const real (&a1)[W]
 template <typename real>
 constexpr int sweep_threads() { return sizeof(real) == 4 ? 1024 : 512; }
 
-template <typename real, bool UNI>
-hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
-                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                        int use_k, const StageArgs &sa, hipStream_t st);
+// the last (x) sweep: lines are contiguous, so small workgroups lose no
+// coalescing and leave each lane the larger register budget its 17 solves
+// and the pointwise assembly need
+constexpr int X_SWEEP_THREADS = 256;
+
 template <typename real, bool UNI>
 hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                        int use_k, hipStream_t st);
+                        int use_k, const StageArgs &sa, hipStream_t st);
 template <typename real, bool UNI>
 hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
+                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
+                        int use_k, const StageArgs &sa, hipStream_t st);
+template <typename real, bool UNI>
+hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         const StageArgs &sa, const StepScalars *sc, hipStream_t st);
 

@@ -202,14 +202,16 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
     if (lg.LX < 1 || lg.threads < 1 || lg.threads > sweep_threads<real>())
         return hipErrorInvalidConfiguration;
     int use_k = sa.rkStep > 1;
+    // sweep order of a stage: y (first), z (middle), x (last, with the
+    // pointwise assembly and the stage update)
     if (which == 0)
-        return uni ? launch_xpass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
-                   : launch_xpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
+        return uni ? launch_xpass<real, true>(d1, d2, lg, lm, F, ph, sa, sc, st)
+                   : launch_xpass<real, false>(d1, d2, lg, lm, F, ph, sa, sc, st);
     if (which == 1)
-        return uni ? launch_ypass<real, true>(d1, d2, lg, lm, F, ph, use_k, st)
-                   : launch_ypass<real, false>(d1, d2, lg, lm, F, ph, use_k, st);
-    return uni ? launch_zpass<real, true>(d1, d2, lg, lm, F, ph, sa, sc, st)
-               : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, sa, sc, st);
+        return uni ? launch_ypass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
+                   : launch_ypass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
+    return uni ? launch_zpass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
+               : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
 }
 
 template <typename real>

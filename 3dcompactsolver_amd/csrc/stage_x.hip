@@ -1,32 +1,43 @@
-// stage_x.hip -- X sweep of one RK stage (lines along x, contiguous in
-// memory, MAP_ROW lanes).
+// stage_x.hip -- last sweep of one RK stage: lines along x (contiguous in
+// memory, MAP_ROW lanes, whole 16-byte chunks), with the pointwise
+// assembly and the RK update fused in.
 //
 // From the stage state: the x Euler fluxes (CSolver_AWS.cpp:861-867) and
 // their first derivatives (:915-922), the first and second x-derivatives of
-// U, V, W, T (:877-912); stores Ux Vx Wx Tx and starts the right-hand side
-// with every term that is linear in an x-derivative.  The 13 line solves run
-// as two batches (5 flux derivatives; 4 first + 4 second derivatives), each
-// batch sharing one pair of exchange barriers.
+// U, V, W, T (:877-912), and the mixed Uxy, Vxy, Uxz, Wxz as x-derivatives
+// of the stored Uy, Vy, Uz, Wz (the reference takes them along y and z of
+// x-derivatives, :991-1003 and :1161-1192; operators along different axes
+// commute).  Then, per point, the terms of solveContinuity .. solveEnergy
+// (:1246-1504) that are products of first derivatives, the sponge source
+// (twice in the energy, :1491/:1495), the zeroing of sponge boundary planes
+// (postStepBCHandling, :1606-1670), and the stage update of
+// RK4::updateConservedData (RK4.hpp:45-111) or TVDRK3::updateConservedData
+// (TVDRK3.hpp:225-260).  Solves: a batch of 9 (5 flux + 4 mixed), then
+// 4 first + 4 second derivatives.
 //
-// The reference's X block forms its fluxes and derivatives from the
+// The reference's x block forms its fluxes and derivatives from the
 // primitive arrays as they stand (CSolver_AWS.cpp:861-912).  At the first
 // stage those were computed (updateNonConservedData, :1913-1925) before
 // updateSponge reset the sponge planes of the state (:1955-2021), so on
 // those planes the primitives belong to the state before the reset while the
 // conserved fluxes use the state after it.  The STALE variant reproduces
 // this from the copy of the pre-reset plane values that the sponge update
-// keeps.
+// keeps, and differentiates the stale-based mixed inputs of the first two
+// sweeps (F.mix).
 #include "stage_common.hpp"
 
 namespace cs3d {
 
-template <typename real, int M, bool UNI, bool STALE>
-__global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
+// QS: the update needs the stage state at the point (sponge source, or the
+// TVDRK3 combinations of stages 2 and 3)
+template <typename real, int M, bool UNI, bool STALE, bool QS>
+__global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
-    Fields<real> F, Phys<real> ph, int use_k, StageArgs sa)
+    Fields<real> F, Phys<real> ph, StageArgs sa, const StepScalars *__restrict__ sc)
 {
     constexpr int HW = 2, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), SWEEP_PLANES)
+    const bool use_k = sa.rkStep > 1;
     const real *q0 = use_k ? F.qk[0] : F.q[0];
     const real *q1 = use_k ? F.qk[1] : F.q[1];
     const real *q2 = use_k ? F.qk[2] : F.q[2];
@@ -42,13 +53,14 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
 #pragma unroll
     for (int t = 0; t < W; t++)
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
+    // lines along x: L = j + Ny k, the point index is i
+    const int lj = Lc % sa.Ny, lk = Lc / sa.Ny;
+    const bool line_plane = (sa.sponge_face[2] && lj == 0) ||
+                            (sa.sponge_face[3] && lj == sa.Ny - 1) ||
+                            (sa.sponge_face[4] && lk == 0) ||
+                            (sa.sponge_face[5] && lk == sa.Nz - 1);
     if (STALE) {
         // primitives on sponge planes from the pre-reset copy
-        const int jj = Lc % sa.Ny, kk = Lc / sa.Ny;
-        const bool line_plane = (sa.sponge_face[2] && jj == 0) ||
-                                (sa.sponge_face[3] && jj == sa.Ny - 1) ||
-                                (sa.sponge_face[4] && kk == 0) ||
-                                (sa.sponge_face[5] && kk == sa.Nz - 1);
 #pragma unroll
         for (int t = 0; t < W; t++) {
             int p = c.s - HW + t;
@@ -62,11 +74,32 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
                      ldo(F.qstale[4], win.at(t)), ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
         }
     }
+    const real c43 = real(4) / real(3), c13 = real(1) / real(3), c23 = real(2) / real(3);
+    real mu[M], amu[M], Uc[M], Vc[M], Wc[M], qs_[5][M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        visc(ph, ph.T0s + Tp[HW + t], mu[t], amu[t]);
+        Uc[t] = U[HW + t];
+        Vc[t] = V[HW + t];
+        Wc[t] = Wv[HW + t];
+        qs_[0][t] = a0[HW + t];
+        qs_[1][t] = a1[HW + t];
+        qs_[2][t] = a2[HW + t];
+        qs_[3][t] = a3[HW + t];
+        qs_[4][t] = a4[HW + t];
+    }
+    // the stored y and z derivatives that are differentiated along x (their
+    // stale-based counterparts in the STALE variant) and their own values
+    const real *mi0 = STALE ? F.mix[0] : F.dY[0];  // Uy
+    const real *mi1 = STALE ? F.mix[1] : F.dY[1];  // Vy
+    const real *mi2 = STALE ? F.mix[2] : F.dZ[0];  // Uz
+    const real *mi3 = STALE ? F.mix[3] : F.dZ[2];  // Wz
+    real Uy[M], Vy[M], uz[M], wz[M];
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
     {
-        // x fluxes; the energy flux (rhoE + p)U = h_ref U + (rhoE' + p')U has
-        // its h_ref part added below as h_ref * Ux
-        real r[5][M], f[W];
+        // x fluxes (the h_ref U part of the energy flux is added as h_ref Ux)
+        // and the mixed derivatives
+        real r[9][M], f[W];
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = (a4[t] + pp[t]) * U[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[0]);
@@ -80,7 +113,23 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = a3[t] * U[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
-        solve_batch<5, kc_deriv<real>(), real, M, UNI>(d1, c, r);
+        win.load(mi0, f);
+#pragma unroll
+        for (int t = 0; t < M; t++) Uy[t] = f[HW + t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
+        win.load(mi1, f);
+#pragma unroll
+        for (int t = 0; t < M; t++) Vy[t] = f[HW + t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
+        win.load(mi2, f);
+#pragma unroll
+        for (int t = 0; t < M; t++) uz[t] = f[HW + t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[7]);
+        win.load(mi3, f);
+#pragma unroll
+        for (int t = 0; t < M; t++) wz[t] = f[HW + t];
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[8]);
+        solve_batch<9, kc_deriv<real>(), real, M, UNI>(d1, c, r);
 #pragma unroll
         for (int t = 0; t < M; t++) {
             RE[t] = -r[0][t];
@@ -88,9 +137,25 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
             R0[t] = -r[2][t];
             Rv[t] = -r[3][t];
             Rw[t] = -r[4][t];
+            const real Uxy = r[5][t], Vxy = r[6][t], Uxz = r[7][t], Wxz = r[8][t];
+            Rv[t] += mu[t] * (c13 * Uxy);
+            RE[t] += mu[t] * (c13 * (Vc[t] * Uxy));
+            Ru[t] += mu[t] * (c13 * Vxy);
+            RE[t] += mu[t] * (c13 * (Uc[t] * Vxy));
+            Rw[t] += mu[t] * (c13 * Uxz);
+            RE[t] += mu[t] * (c13 * (Wc[t] * Uxz));
+            Ru[t] += mu[t] * (c13 * Wxz);
+            RE[t] += mu[t] * (c13 * (Uc[t] * Wxz));
         }
     }
-    // first and second derivatives of T', U, V, W
+    if (STALE) {
+        // the assembly takes the current first derivatives
+        win.load_seg(F.dY[0], Uy);
+        win.load_seg(F.dY[1], Vy);
+        win.load_seg(F.dZ[0], uz);
+        win.load_seg(F.dZ[2], wz);
+    }
+    // first and second x-derivatives of T', U, V, W
     real r[8][M];
     rhs<OP_D1, real, M, HW, UNI>(d1, c, Tp, r[0]);
     rhs<OP_D1, real, M, HW, UNI>(d1, c, U, r[1]);
@@ -101,62 +166,171 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_xpass(
     rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[6]);
     rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[7]);
     solve_batch2<4, 4, kc_deriv<real>(), real, M, UNI>(d1, d2, c, r);
-
-    const real c43 = real(4) / real(3);
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        real mu, amu;
-        visc(ph, ph.T0s + Tp[HW + t], mu, amu);
-        RE[t] += ph.cpPr * (mu * r[4][t]);
+        RE[t] += ph.cpPr * (mu[t] * r[4][t]);
         RE[t] -= ph.h_ref * r[1][t];
-        Ru[t] += mu * (c43 * r[5][t]);
-        RE[t] += mu * (U[HW + t] * (c43 * r[5][t]));
-        Rv[t] += mu * r[6][t];
-        RE[t] += mu * (V[HW + t] * r[6][t]);
-        Rw[t] += mu * r[7][t];
-        RE[t] += mu * (Wv[HW + t] * r[7][t]);
+        Ru[t] += mu[t] * (c43 * r[5][t]);
+        RE[t] += mu[t] * (Uc[t] * (c43 * r[5][t]));
+        Rv[t] += mu[t] * r[6][t];
+        RE[t] += mu[t] * (Vc[t] * r[6][t]);
+        Rw[t] += mu[t] * r[7][t];
+        RE[t] += mu[t] * (Wc[t] * r[7][t]);
     }
-    if (active) {
-        real Ux[M], Vx[M], Wx[M], Tx[M];
+    if (!active) return;
+
+    // the rest of the point values, whole segments at a time
+    real Wy[M], Ty[M], vz[M], Tz[M], Rp[5][M];
+    win.load_seg(F.dY[2], Wy);
+    win.load_seg(F.dY[3], Ty);
+    win.load_seg(F.dZ[1], vz);
+    win.load_seg(F.dZ[3], Tz);
 #pragma unroll
-        for (int t = 0; t < M; t++) {
-            Tx[t] = r[0][t];
-            Ux[t] = r[1][t];
-            Vx[t] = r[2][t];
-            Wx[t] = r[3][t];
+    for (int v = 0; v < 5; v++) win.load_seg(F.R[v], Rp[v]);
+    const real dt = (real)sc->dt;
+    real k[5][M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        const real Tx = r[0][t], Ux = r[1][t], Vx = r[2][t], Wx = r[3][t];
+        const real MuX = amu[t] * Tx, MuY = amu[t] * Ty[t], MuZ = amu[t] * Tz[t];
+        const real Uu = Uc[t], Vv = Vc[t], Ww = Wc[t], m_ = mu[t];
+        const real uy = Uy[t], vy = Vy[t], wy = Wy[t], uz_ = uz[t], vz_ = vz[t], wz_ = wz[t];
+        real su = c43 * MuX * (Ux - real(0.5) * vy - real(0.5) * wz_) + MuY * (uy + Vx) +
+                  MuZ * (Wx + uz_);
+        real sv = c43 * MuY * (vy - real(0.5) * Ux - real(0.5) * wz_) + MuX * (uy + Vx) +
+                  MuZ * (wy + vz_);
+        real sw = c43 * MuZ * (wz_ - real(0.5) * Ux - real(0.5) * vy) + MuX * (Wx + uz_) +
+                  MuY * (wy + vz_);
+        real qn = ph.cpPr * (MuX * Tx + MuY * Ty[t] + MuZ * Tz[t]);
+        real quad = c43 * (Ux * Ux + vy * vy + wz_ * wz_) + uy * uy + uz_ * uz_ + Vx * Vx +
+                    vz_ * vz_ + Wx * Wx + wy * wy - c43 * (Ux * vy + Ux * wz_ + vy * wz_) +
+                    real(2) * (uy * Vx + uz_ * Wx + vz_ * wy);
+        real vt2 = c43 * (Uu * MuX * Ux + Vv * MuY * vy + Ww * MuZ * wz_) -
+                   c23 * Uu * MuX * (vy + wz_) - c23 * Vv * MuY * (Ux + wz_) -
+                   c23 * Ww * MuZ * (Ux + vy) + Uu * MuY * (uy + Vx) + Uu * MuZ * (uz_ + Wx) +
+                   Vv * MuX * (uy + Vx) + Vv * MuZ * (vz_ + wy) + Ww * MuX * (uz_ + Wx) +
+                   Ww * MuY * (vz_ + wy);
+        real rhs_[5];
+        rhs_[0] = Rp[0][t] + R0[t];
+        rhs_[1] = Rp[1][t] + Ru[t] + su;
+        rhs_[2] = Rp[2][t] + Rv[t] + sv;
+        rhs_[3] = Rp[3][t] + Rw[t] + sw;
+        rhs_[4] = Rp[4][t] + RE[t] + qn + m_ * quad + vt2;
+        // table plans: points past the segment's length are never stored
+        // and their offsets may leave the field, so nothing is read there
+        const bool valid = UNI || t < c.len;
+        if (QS && F.sigma && valid) {
+            const unsigned i = win.mid(t);
+            const real sg = ldo(F.sigma, i);
+#pragma unroll
+            for (int v = 0; v < 5; v++) rhs_[v] += sg * (ldo(F.avg[v], i) - qs_[v][t]);
+            rhs_[4] += sg * (ldo(F.avg[4], i) - qs_[4][t]);
         }
-        win.store(F.dX[0], Ux, c.len);
-        win.store(F.dX[1], Vx, c.len);
-        win.store(F.dX[2], Wx, c.len);
-        win.store(F.dX[3], Tx, c.len);
-        win.store(F.R[0], R0, c.len);
-        win.store(F.R[1], Ru, c.len);
-        win.store(F.R[2], Rv, c.len);
-        win.store(F.R[3], Rw, c.len);
-        win.store(F.R[4], RE, c.len);
+        const int ii = c.s + t;
+        const bool plane = line_plane || (sa.sponge_face[0] && ii == 0) ||
+                           (sa.sponge_face[1] && ii == sa.Nx - 1);
+#pragma unroll
+        for (int v = 0; v < 5; v++) k[v][t] = plane ? real(0) : dt * rhs_[v];
+        if (sa.diag && valid) {
+            // per-point inputs of the step diagnostics that come from this
+            // stage's derivatives (CSolver_AWS.cpp:2190-2225)
+            const unsigned i = win.mid(t);
+            const real dil = Ux + vy + wz_;
+            const real S00 = real(0.5) * (Ux + Ux) - c13 * dil;
+            const real S11 = real(0.5) * (vy + vy) - c13 * dil;
+            const real S22 = real(0.5) * (wz_ + wz_) - c13 * dil;
+            const real S01 = real(0.5) * (uy + Vx), S02 = real(0.5) * (uz_ + Wx),
+                       S12 = real(0.5) * (vz_ + wy);
+            real td = S00 * Ux;
+            td += S01 * uy;
+            td += S02 * uz_;
+            td += S01 * Vx;
+            td += S11 * vy;
+            td += S12 * vz_;
+            td += S02 * Wx;
+            td += S12 * wy;
+            td += S22 * wz_;
+            const real wx_ = wy - vz_, wy_ = uz_ - Wx, wzv = Vx - uy;
+            sto(F.diag[0], i, td);
+            sto(F.diag[1], i, wx_ * wx_ + wy_ * wy_ + wzv * wzv);
+            sto(F.diag[2], i, dil);
+            sto(F.diag[3], i, (Ux * Ux + vy * vy + wz_ * wz_) * c13);
+        }
     }
+    if (sa.mode == 1) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) win.store(F.k2[v], k[v], c.len);
+        return;
+    }
+    // stage-update coefficients, uniform over the launch
+    const bool first = sa.rkStep == 1, has_next = sa.rkStep < 4;
+    const real ra = (sa.rkStep == 1 || sa.rkStep == 4) ? real(1) / real(6) : real(1) / real(3);
+    const real rb = sa.rkStep == 3 ? real(1) : real(0.5);
+    const bool tvd_to_k = sa.rkStep < 3;
+    const real ta = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.75) : real(1) / real(3);
+    const real tb = sa.rkStep == 1 ? real(0) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
+    const real tc = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        real qn1[M], o[M];
+        win.load_seg(F.q[v], qn1);
+        if (sa.rk == 4) {
+            // RK4.hpp:45-111: *2 accumulates the stages, *k is the next
+            // stage state
+            if (first) {
+#pragma unroll
+                for (int t = 0; t < M; t++) o[t] = qn1[t];
+            } else {
+                win.load_seg(F.acc[v], o);
+            }
+#pragma unroll
+            for (int t = 0; t < M; t++) o[t] += ra * k[v][t];
+            win.store(F.acc[v], o, c.len);
+            if (has_next) {
+#pragma unroll
+                for (int t = 0; t < M; t++) o[t] = qn1[t] + rb * k[v][t];
+                win.store(F.qk[v], o, c.len);
+            }
+        } else {
+            // TVDRK3.hpp:225-260: stages 1-2 write *k, stage 3 writes *2
+#pragma unroll
+            for (int t = 0; t < M; t++) o[t] = ta * qn1[t] + tb * qs_[v][t] + tc * k[v][t];
+            win.store(tvd_to_k ? F.qk[v] : F.acc[v], o, c.len);
+        }
+    }
+}
+
+template <typename real, bool UNI, bool STALE>
+static hipError_t launch_x3(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
+                            const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
+                            const StageArgs &sa, const StepScalars *sc, hipStream_t st)
+{
+    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    if (F.sigma || sa.rk == 3)
+        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true>), grid, block, lg.shm, st,
+                           d1, d2, lm, F, ph, sa, sc);
+    else
+        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, false>), grid, block, lg.shm, st,
+                           d1, d2, lm, F, ph, sa, sc);
+    return hipGetLastError();
 }
 
 template <typename real, bool UNI>
 hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                        int use_k, const StageArgs &sa, hipStream_t st)
+                        const StageArgs &sa, const StepScalars *sc, hipStream_t st)
 {
-    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
-    if (sa.stale)
-        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,
-                           lm, F, ph, use_k, sa);
-    else
-        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, false>), grid, block, lg.shm, st, d1,
-                           d2, lm, F, ph, use_k, sa);
-    return hipGetLastError();
+    if (lg.threads > X_SWEEP_THREADS) return hipErrorInvalidConfiguration;
+    return sa.stale ? launch_x3<real, UNI, true>(d1, d2, lg, lm, F, ph, sa, sc, st)
+                    : launch_x3<real, UNI, false>(d1, d2, lg, lm, F, ph, sa, sc, st);
 }
 
 #define CS3D_INST_X(real, UNI)                                                                  \
     template hipError_t launch_xpass<real, UNI>(const OpDev<real> *, const OpDev<real> *,          \
                                                 const LaunchGeom &, const LineMap &,               \
-                                                const Fields<real> &, const Phys<real> &, int,     \
-                                                const StageArgs &, hipStream_t);
+                                                const Fields<real> &, const Phys<real> &,          \
+                                                const StageArgs &, const StepScalars *,            \
+                                                hipStream_t);
 CS3D_INST_X(float, true)
 CS3D_INST_X(float, false)
 CS3D_INST_X(double, true)

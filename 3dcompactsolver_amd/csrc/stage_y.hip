@@ -1,20 +1,29 @@
-// stage_y.hip -- Y sweep of one RK stage (lines along y, MAP_COL lanes:
+// stage_y.hip -- first sweep of one RK stage: lines along y (MAP_COL lanes,
 // every load instruction reads consecutive x).
 //
-// y Euler fluxes (CSolver_AWS.cpp:949-963) and their derivatives
-// (:1051-1073), first and second y-derivatives of U, V, W, T (:969-1049),
-// the mixed Uxy, Vxy from the stored Ux, Vx (:991-1003); stores Uy Vy Wy Ty
-// and adds every term linear in a y-derivative to the right-hand side.  The
-// 15 line solves run as two batches (5 flux + 2 mixed first derivatives;
-// 4 first + 4 second derivatives).
+// From the stage state: the y Euler fluxes (CSolver_AWS.cpp:949-963) and
+// their derivatives (:1051-1073), the first and second y-derivatives of U,
+// V, W, T (:969-1049).  Stores Uy Vy Wy Ty and starts the right-hand side
+// with every term linear in one of these.  The mixed derivatives the
+// reference takes along y of x-derivatives (Uxy, Vxy, :991-1003) are taken
+// along x of the stored Uy, Vy in the last sweep: compact operators along
+// different axes commute.  The 13 line solves run as two batches (5 flux
+// derivatives; 4 first + 4 second derivatives).
+//
+// STALE: at the first stage after updateSponge the reference's x block uses
+// primitives that predate the sponge reset on the sponge planes
+// (CSolver_AWS.cpp:861-912 with :1913-1925 and :1955-2021), so its Uxy, Vxy
+// are y-derivatives of x-derivatives of those.  The STALE variant also
+// solves D_y of the stale U and V (into F.mix[0], F.mix[1]) for the last
+// sweep to differentiate along x.
 #include "stage_common.hpp"
 
 namespace cs3d {
 
-template <typename real, int M, bool UNI>
+template <typename real, int M, bool UNI, bool STALE>
 __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
-    Fields<real> F, Phys<real> ph, int use_k)
+    Fields<real> F, Phys<real> ph, int use_k, StageArgs sa)
 {
     constexpr int HW = 2, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), SWEEP_PLANES)
@@ -31,8 +40,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     real(&a4)[W] = a[4];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
     if constexpr (UNI) {
-        // own points from memory, halos through LDS (measured: staging the
-        // two mixed windows below as well is slower for this kernel)
+        // own points from memory, halos through LDS
         load_windows_lds<5>(c, win, [&](int f, unsigned o) {
             return ldo(f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : f == 3 ? q3 : q4, o);
         }, a);
@@ -46,12 +54,12 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
 #pragma unroll
     for (int t = 0; t < W; t++)
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
-    const real c43 = real(4) / real(3), c13 = real(1) / real(3);
-    real R0[M], Ru[M], Rv[M], Rw[M], RE[M], Uxy[M], Vxy[M];
+    const real c43 = real(4) / real(3);
+    real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
     {
         // y fluxes (the h_ref V part of the energy flux is added as h_ref Vy)
-        // and the mixed derivatives of the stored Ux, Vx
-        real r[7][M], f[W];
+        constexpr int NB = STALE ? 7 : 5;
+        real r[NB][M], f[W];
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = (a4[t] + pp[t]) * V[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[0]);
@@ -65,11 +73,37 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = a3[t] * V[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
-        win.load(F.dX[0], f);
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
-        win.load(F.dX[1], f);
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
-        solve_batch<7, kc_deriv<real>(), real, M, UNI>(d1, c, r);
+        if constexpr (STALE) {
+            // U and V with the sponge planes' primitives from the pre-reset
+            // copy; lines along y: L = i + Nx k, the point index is j
+            const int li = Lc % sa.Nx, lk = Lc / sa.Nx;
+            const bool line_plane = (sa.sponge_face[0] && li == 0) ||
+                                    (sa.sponge_face[1] && li == sa.Nx - 1) ||
+                                    (sa.sponge_face[4] && lk == 0) ||
+                                    (sa.sponge_face[5] && lk == sa.Nz - 1);
+            real us[W], vs[W];
+#pragma unroll
+            for (int t = 0; t < W; t++) {
+                int p = c.s - HW + t;
+                p += p < 0 ? n : 0;
+                p -= p >= n ? n : 0;
+                const bool plane = line_plane || (sa.sponge_face[2] && p == 0) ||
+                                   (sa.sponge_face[3] && p == n - 1);
+                us[t] = U[t];
+                vs[t] = V[t];
+                if (plane) {
+                    real i_, u_, v_, w_, p_, t_;
+                    prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
+                         ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
+                         ldo(F.qstale[4], win.at(t)), i_, u_, v_, w_, p_, t_);
+                    us[t] = u_;
+                    vs[t] = v_;
+                }
+            }
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, us, r[5]);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, vs, r[6]);
+        }
+        solve_batch<NB, kc_deriv<real>(), real, M, UNI>(d1, c, r);
 #pragma unroll
         for (int t = 0; t < M; t++) {
             RE[t] = -r[0][t];
@@ -77,8 +111,12 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
             R0[t] = -r[2][t];
             Ru[t] = -r[3][t];
             Rw[t] = -r[4][t];
-            Uxy[t] = r[5][t];
-            Vxy[t] = r[6][t];
+        }
+        if constexpr (STALE) {
+            if (active) {
+                win.store(F.mix[0], r[5], c.len);
+                win.store(F.mix[1], r[6], c.len);
+            }
         }
     }
     // first and second derivatives of T', U, V, W
@@ -92,7 +130,6 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[6]);
     rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[7]);
     solve_batch2<4, 4, kc_deriv<real>(), real, M, UNI>(d1, d2, c, r);
-
 #pragma unroll
     for (int t = 0; t < M; t++) {
         real mu, amu;
@@ -106,38 +143,32 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
         RE[t] += mu * (Vc * (c43 * r[6][t]));
         Rw[t] += mu * r[7][t];
         RE[t] += mu * (Wc * r[7][t]);
-        Rv[t] += mu * (c13 * Uxy[t]);
-        RE[t] += mu * (c13 * (Vc * Uxy[t]));
-        Ru[t] += mu * (c13 * Vxy[t]);
-        RE[t] += mu * (c13 * (Uc * Vxy[t]));
     }
     if (active) {
-#pragma unroll
-        for (int t = 0; t < M; t++) {
-            if (UNI || t < c.len) {
-                const unsigned o = win.mid(t);
-                sto(F.dY[0], o, r[1][t]);
-                sto(F.dY[1], o, r[2][t]);
-                sto(F.dY[2], o, r[3][t]);
-                sto(F.dY[3], o, r[0][t]);
-                sto(F.R[0], o, ldo(F.R[0], o) + R0[t]);
-                sto(F.R[1], o, ldo(F.R[1], o) + Ru[t]);
-                sto(F.R[2], o, ldo(F.R[2], o) + Rv[t]);
-                sto(F.R[3], o, ldo(F.R[3], o) + Rw[t]);
-                sto(F.R[4], o, ldo(F.R[4], o) + RE[t]);
-            }
-        }
+        win.store(F.dY[0], r[1], c.len);
+        win.store(F.dY[1], r[2], c.len);
+        win.store(F.dY[2], r[3], c.len);
+        win.store(F.dY[3], r[0], c.len);
+        win.store(F.R[0], R0, c.len);
+        win.store(F.R[1], Ru, c.len);
+        win.store(F.R[2], Rv, c.len);
+        win.store(F.R[3], Rw, c.len);
+        win.store(F.R[4], RE, c.len);
     }
 }
 
 template <typename real, bool UNI>
 hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                        int use_k, hipStream_t st)
+                        int use_k, const StageArgs &sa, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
-    hipLaunchKernelGGL((k_ypass<real, SEG_DERIV, UNI>), grid, block, lg.shm, st, d1, d2, lm, F,
-                       ph, use_k);
+    if (sa.stale)
+        hipLaunchKernelGGL((k_ypass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,
+                           lm, F, ph, use_k, sa);
+    else
+        hipLaunchKernelGGL((k_ypass<real, SEG_DERIV, UNI, false>), grid, block, lg.shm, st, d1,
+                           d2, lm, F, ph, use_k, sa);
     return hipGetLastError();
 }
 
@@ -145,7 +176,7 @@ hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const Laun
     template hipError_t launch_ypass<real, UNI>(const OpDev<real> *, const OpDev<real> *,          \
                                                 const LaunchGeom &, const LineMap &,               \
                                                 const Fields<real> &, const Phys<real> &, int,     \
-                                                hipStream_t);
+                                                const StageArgs &, hipStream_t);
 CS3D_INST_Y(float, true)
 CS3D_INST_Y(float, false)
 CS3D_INST_Y(double, true)

@@ -36,14 +36,15 @@ import cs3d  # noqa: E402
 
 METRIC = "Mpoint-RK-stages/s at 256^3 FP32 vs FP64 (% HBM roof); adjoint/forward ratio"
 HBM_PEAK = 8.0e12
-# algorithmic field accesses per point, per launch of each kernel class
-# (X sweep 5 reads + 9 writes, Y 12 + 9, Z 28 + 10, filter sweep 5 + 5,
-# post-step reduction 5 reads)
-ACCESSES = {"x_sweep": 14, "y_sweep": 21, "z_sweep": 38, "filter": 10, "reduce": 5}
+# algorithmic field accesses per point, per launch of each kernel class; the
+# sweeps of a stage run y, z, x (first y sweep 5 reads + 9 writes, middle z
+# sweep 12 + 9, last x sweep with the assembly and the RK4 update 28 + 10),
+# filter sweep 5 + 5, post-step reduction 5 reads
+ACCESSES = {"x_sweep": 38, "y_sweep": 14, "z_sweep": 21, "filter": 10, "reduce": 5}
 # HBM accesses per point actually moved per launch (reads, writes), FP32 at
 # 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
 # 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
-MEASURED_TRAFFIC = {"x_sweep": (5.00, 9.00), "y_sweep": (12.00, 9.00), "z_sweep": (30.59, 9.75)}
+MEASURED_TRAFFIC = {"x_sweep": (26.69, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (12.01, 9.01)}
 BYTES_PER_POINT_STAGE = {cs3d.FP32: 322, cs3d.FP64: 644}
 
 
@@ -158,7 +159,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--adjoint-steps", type=int, default=8)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three 256 MiB device copies (the reference kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copy probe: 3 x 256 MiB", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design note, not a measured path", file=sys.stderr)
