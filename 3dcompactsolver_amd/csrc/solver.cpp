@@ -124,7 +124,7 @@ struct SolverT : SolverBase {
         for (int f = 0; f < 6; f++) sponge |= c.bc_kind[f] == CS3D_BC_SPONGE;
 
         std::memset(&F, 0, sizeof(F));
-        int nf = 15 + 4 + 4 + 5 + 5 + 4 + (sponge ? 15 : 0);
+        int nf = 15 + 4 + 4 + 10 + 5 + 4 + (sponge ? 15 : 0);
         hc(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         hc(hipMalloc(&mem, (size_t)nf * N * sizeof(real)), "hipMalloc(fields)");
         hc(hipMemsetAsync(mem, 0, (size_t)nf * N * sizeof(real), st), "hipMemset(fields)");
@@ -140,6 +140,7 @@ struct SolverT : SolverBase {
         for (int v = 0; v < 4; v++) F.dY[v] = take();
         for (int v = 0; v < 4; v++) F.dZ[v] = take();
         for (int v = 0; v < 5; v++) F.R[v] = take();
+        for (int v = 0; v < 5; v++) F.Rz[v] = take();
         for (int v = 0; v < 5; v++) F.k2[v] = take();
         for (int v = 0; v < 4; v++) F.diag[v] = take();
         if (sponge) {

@@ -24,7 +24,8 @@ struct Fields {
     real *dY[4];    // Uy Vy Wy Ty of the stage (first sweep)
     real *dZ[4];    // Uz Vz Wz Tz (middle sweep)
     real *mix[4];   // stale-primitive D_y U, D_y V, D_z U, D_z W (sponge runs only)
-    real *R[5];     // partial right-hand side between sweeps
+    real *R[5];     // partial right-hand side of the first (y) sweep
+    real *Rz[5];    // partial right-hand side of the middle (z) sweep
     real *k2[5];    // dt*RHS (only written in RHS-only mode)
     real *sigma;    // sponge strength (null without sponge)
     real *avg[5];   // sponge running averages (perturbation form)

@@ -186,7 +186,13 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     win.load_seg(F.dZ[1], vz);
     win.load_seg(F.dZ[3], Tz);
 #pragma unroll
-    for (int v = 0; v < 5; v++) win.load_seg(F.R[v], Rp[v]);
+    for (int v = 0; v < 5; v++) {
+        real rz[M];
+        win.load_seg(F.R[v], Rp[v]);
+        win.load_seg(F.Rz[v], rz);
+#pragma unroll
+        for (int t = 0; t < M; t++) Rp[v][t] += rz[t];
+    }
     const real dt = (real)sc->dt;
     real k[5][M];
 #pragma unroll

@@ -4,8 +4,8 @@
 // From the stage state: the z Euler fluxes (CSolver_AWS.cpp:1095-1108) and
 // their derivatives (:1195-1221), the first and second z-derivatives of U,
 // V, W, T (:1113-1159), and the mixed Vyz, Wyz as z-derivatives of the
-// stored Vy, Wy (:1161-1192).  Stores Uz Vz Wz Tz and adds every term
-// linear in one of these to the right-hand side.  Uxz and Wxz are taken
+// stored Vy, Wy (:1161-1192).  Stores Uz Vz Wz Tz and its own partial
+// right-hand side: every term linear in one of these.  Uxz and Wxz are taken
 // along x of the stored Uz, Wz in the last sweep (operators along different
 // axes commute).  Solves: one batch of 13 (5 flux + 4 first + 4 second
 // derivatives), then the 2 mixed ones.
@@ -30,6 +30,8 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     const real *q2 = use_k ? F.qk[2] : F.q[2];
     const real *q3 = use_k ? F.qk[3] : F.q[3];
     const real *q4 = use_k ? F.qk[4] : F.q[4];
+    // the conserved windows are read directly: staging them through LDS
+    // (as the y sweep does) measured slower here
     real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
     win.load(q0, a0);
@@ -175,17 +177,13 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         }
     }
     if (!active) return;
-#pragma unroll
-    for (int t = 0; t < M; t++) {
-        if (UNI || t < c.len) {
-            const unsigned o = win.mid(t);
-            sto(F.R[0], o, ldo(F.R[0], o) + R0[t]);
-            sto(F.R[1], o, ldo(F.R[1], o) + Ru[t]);
-            sto(F.R[2], o, ldo(F.R[2], o) + Rv[t]);
-            sto(F.R[3], o, ldo(F.R[3], o) + Rw[t]);
-            sto(F.R[4], o, ldo(F.R[4], o) + RE[t]);
-        }
-    }
+    // a partial right-hand side of its own (the last sweep adds both): no
+    // read-modify-write on the column path
+    win.store(F.Rz[0], R0, c.len);
+    win.store(F.Rz[1], Ru, c.len);
+    win.store(F.Rz[2], Rv, c.len);
+    win.store(F.Rz[3], Rw, c.len);
+    win.store(F.Rz[4], RE, c.len);
 }
 
 template <typename real, bool UNI>

@@ -38,13 +38,13 @@ METRIC = "Mpoint-RK-stages/s at 256^3 FP32 vs FP64 (% HBM roof); adjoint/forward
 HBM_PEAK = 8.0e12
 # algorithmic field accesses per point, per launch of each kernel class; the
 # sweeps of a stage run y, z, x (first y sweep 5 reads + 9 writes, middle z
-# sweep 12 + 9, last x sweep with the assembly and the RK4 update 28 + 10),
+# sweep 7 + 9, last x sweep with the assembly and the RK4 update 33 + 10),
 # filter sweep 5 + 5, post-step reduction 5 reads
-ACCESSES = {"x_sweep": 38, "y_sweep": 14, "z_sweep": 21, "filter": 10, "reduce": 5}
+ACCESSES = {"x_sweep": 43, "y_sweep": 14, "z_sweep": 16, "filter": 10, "reduce": 5}
 # HBM accesses per point actually moved per launch (reads, writes), FP32 at
 # 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
 # 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
-MEASURED_TRAFFIC = {"x_sweep": (26.69, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (12.01, 9.01)}
+MEASURED_TRAFFIC = {"x_sweep": (31.71, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.03, 9.01)}
 BYTES_PER_POINT_STAGE = {cs3d.FP32: 322, cs3d.FP64: 644}
 
 
