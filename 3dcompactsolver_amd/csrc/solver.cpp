@@ -50,7 +50,11 @@ struct SolverT : SolverBase {
         if (dir == 0) return env_cap("CS3D_SWEEP_X_THREADS", 256);
         return env_cap("CS3D_SWEEP_THREADS", sizeof(real) == 4 ? 1024 : 512);
     }
-    static int filter_max_threads() { return env_cap("CS3D_FILTER_THREADS", 512); }
+    static int filter_max_threads(int dir)
+    {
+        // rows (x) gain nothing from larger workgroups
+        return env_cap("CS3D_FILTER_THREADS", dir == 0 ? 512 : (sizeof(real) == 4 ? 1024 : 512));
+    }
     static int line_terms_max_threads(int dir)
     {
         if (dir == 0) return env_cap("CS3D_ADJ_X_THREADS", 256);
@@ -71,6 +75,7 @@ struct SolverT : SolverBase {
     void *mem = nullptr;
     StepScalars *dsc = nullptr;
     double *dpart = nullptr, *dout = nullptr;
+    double *dfpart = nullptr;  // CFL partials of the last filter sweep
     int nred = 0;
     bool sponge = false;
     bool ic_done = false;
@@ -114,7 +119,7 @@ struct SolverT : SolverBase {
                                   SWEEP_STAGE_FIELDS * D1[d].host.M);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
-                                   filter_max_threads(), 5, sizeof(real));
+                                   filter_max_threads(d), 5, sizeof(real));
             if (filterG[d].LX < 1) throw std::invalid_argument("line too long for the filter");
             lm[d] = xyz_lines(d, Nx, Ny, Nz);
         }
@@ -153,6 +158,9 @@ struct SolverT : SolverBase {
         hc(hipMemsetAsync(dsc, 0, sizeof(StepScalars), st), "hipMemset(scalars)");
         nred = (int)std::min<long>(1024, (N + 255) / 256);
         hc(hipMalloc(&dpart, (size_t)nred * RED_SLOTS_HOST * sizeof(double)), "hipMalloc(partials)");
+        int nfb = 0;
+        for (int d = 0; d < 3; d++) nfb = std::max(nfb, filter_blocks(d));
+        hc(hipMalloc(&dfpart, (size_t)nfb * RED_SLOTS_HOST * sizeof(double)), "hipMalloc(partials)");
         hc(hipMalloc(&dout, 2 * RED_SLOTS_HOST * sizeof(double)), "hipMalloc(reductions)");
 
         std::memset(&base, 0, sizeof(base));
@@ -182,6 +190,7 @@ struct SolverT : SolverBase {
         if (mem) (void)hipFree(mem);
         if (dsc) (void)hipFree(dsc);
         if (dpart) (void)hipFree(dpart);
+        if (dfpart) (void)hipFree(dfpart);
         if (dout) (void)hipFree(dout);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -353,7 +362,10 @@ struct SolverT : SolverBase {
             });
     }
 
-    void filter_sweep(int dir, real *const *in, real *const *out)
+    // workgroups of a filter sweep along dir
+    int filter_blocks(int dir) const { return (lm[dir].nlines + filterG[dir].LX - 1) / filterG[dir].LX; }
+
+    void filter_sweep(int dir, real *const *in, real *const *out, double *part = nullptr)
     {
         Ptr5<real> io;
         for (int v = 0; v < 5; v++) {
@@ -363,13 +375,15 @@ struct SolverT : SolverBase {
         const PlanDev<real> &pl = FL[dir];
         timed(3, [&] {
             hc(launch_filter5<real>(pl.dev, pl.host.uniform, pl.host.M, filterG[dir],
-                                    dir == 0 ? MAP_ROW : MAP_COL, lm[dir], io, st),
+                                    dir == 0 ? MAP_ROW : MAP_COL, lm[dir], io, ph, part, st),
                "filter");
         });
     }
 
-    // filterConservedData (CSolver_AWS.cpp:1682-1893): *2 -> *1
-    void filter_conserved()
+    // filterConservedData (CSolver_AWS.cpp:1682-1893): *2 -> *1.  With
+    // cfl set, the last sweep also leaves the CFL partials of its output;
+    // returns their number of workgroups (0 when it did not).
+    int filter_conserved(bool cfl)
     {
         if (timeStep % cfg.filter_step == 0) {
             filterTimeStep++;
@@ -377,11 +391,15 @@ struct SolverT : SolverBase {
             filter_order(filterTimeStep % 3, o);
             filter_sweep(o[0], F.acc, F.q);
             filter_sweep(o[1], F.q, F.acc);
-            filter_sweep(o[2], F.acc, F.q);
-        } else {
-            // the memcpy of *2 into *1 becomes a swap of the two buffers
-            for (int v = 0; v < 5; v++) std::swap(F.q[v], F.acc[v]);
+            // the partials need 2 doubles per lane of the sweep's LDS
+            const bool fuse = cfl && filterG[o[2]].shm >= 2 * sizeof(double) *
+                                                           (size_t)filterG[o[2]].threads;
+            filter_sweep(o[2], F.acc, F.q, fuse ? dfpart : nullptr);
+            return fuse ? filter_blocks(o[2]) : 0;
         }
+        // the memcpy of *2 into *1 becomes a swap of the two buffers
+        for (int v = 0; v < 5; v++) std::swap(F.q[v], F.acc[v]);
+        return 0;
     }
 
     void sub_step(int rkStep, int rkLast) override
@@ -391,11 +409,15 @@ struct SolverT : SolverBase {
         sweeps(rkStep, 0, diag);
         if (rkStep == 1) stale_pending = false;
         if (rkLast) {
-            filter_conserved();
             // calcDtFromCFL's maximum for the next step, and the step's
-            // diagnostics, from the filtered state before the sponge update
+            // diagnostics, from the filtered state before the sponge update;
+            // without diagnostics the last filter sweep forms the partials
+            const int nb = filter_conserved(!diag);
             timed(4, [&] {
-                hc(launch_reduce<real>(F, ph, N, diag, dpart, nred, dout, dsc, st), "reduce");
+                if (nb)
+                    hc(launch_reduce_final(dfpart, nb, dout, dsc, st), "reduce");
+                else
+                    hc(launch_reduce<real>(F, ph, N, diag, dpart, nred, dout, dsc, st), "reduce");
             });
             diag_valid = diag != 0;
         }

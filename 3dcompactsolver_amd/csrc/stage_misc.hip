@@ -11,11 +11,36 @@ namespace cs3d {
 
 // workgroups of at most 512 lanes: the five fields of a segment stay in
 // registers through the batched solve
-constexpr int FILTER_THREADS = 512;
+// FP32 column filters fit 1024 lanes (32 lines of 32 segments: 128-byte
+// rows); FP64 keeps 512 for its registers
+template <typename real>
+constexpr int filter_threads() { return sizeof(real) == 4 ? 1024 : 512; }
 
-template <typename real, int M, bool UNI, int MAP>
-__global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
-                                                  Ptr5<real> io)
+constexpr int RED_THREADS = 256;
+constexpr int RED_SLOTS = RED_SLOTS_HOST;
+
+// the CFL wave-speed sum of calcDtFromCFL (CSolver_AWS.cpp:513-525) at one
+// point of the state
+template <typename real>
+__device__ __forceinline__ double cfl_speed(const Phys<real> &ph, real rp, real ru, real rv,
+                                            real rw, real re)
+{
+    real irho, U, V, W, pp, Tp;
+    prim(ph, rp, ru, rv, rw, re, irho, U, V, W, pp, Tp);
+    const double rho = (double)ph.rho_ref + (double)rp;
+    const double p = (double)ph.p_ref + (double)pp;
+    const double cs = sqrt((double)ph.gamma * p / rho);
+    return (fabs((double)U) + cs) * (double)ph.idx + (fabs((double)V) + cs) * (double)ph.idy +
+           (fabs((double)W) + cs) * (double)ph.idz;
+}
+
+// RED: the last sweep of the filter also leaves, per workgroup, the CFL
+// maximum and the NaN count of its output (slots 0 and 7 of the
+// k_reduce_state partials; the other slots zero)
+template <typename real, int M, bool UNI, int MAP, bool RED>
+__global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
+                                                  Ptr5<real> io, Phys<real> ph,
+                                                  double *__restrict__ part)
 {
     constexpr int HW = 4, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP, fo, fo->K > kc_filter<real>() ? fo->K : kc_filter<real>(), 5)
@@ -39,6 +64,41 @@ __global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *_
             win.store(io.out[v], fc[v], c.len);
         }
     }
+    if constexpr (RED) {
+        double umax = 0.0, nan = 0.0;
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < M; t++) {
+                if (!UNI && t >= c.len) continue;
+                const double u = cfl_speed(ph, fc[0][t], fc[1][t], fc[2][t], fc[3][t], fc[4][t]);
+                if (u != u)
+                    nan += 1.0;
+                else
+                    umax = fmax(umax, u);
+            }
+        }
+        // tree over the workgroup in the exchange ring's LDS, free once every
+        // lane is past the solve (the workgroup size LX * P need not be a
+        // multiple of the wavefront; the host checks the ring holds
+        // 2 * blockDim doubles)
+        extern __shared__ double red[];
+        const int nt = blockDim.x, tid = threadIdx.x;
+        __syncthreads();
+        red[tid] = umax;
+        red[nt + tid] = nan;
+        __syncthreads();
+        int h = 1;
+        while (2 * h < nt) h *= 2;
+        for (; h > 0; h /= 2) {
+            if (tid < h && tid + h < nt) {
+                red[tid] = fmax(red[tid], red[tid + h]);
+                red[nt + tid] += red[nt + tid + h];
+            }
+            __syncthreads();
+        }
+        if (tid < RED_SLOTS)
+            part[(long)blockIdx.x * RED_SLOTS + tid] = tid == 0 ? red[0] : tid == 7 ? red[nt] : 0.0;
+    }
 }
 
 // -------------------------------------------------------------- reductions
@@ -50,8 +110,6 @@ __global__ __launch_bounds__(FILTER_THREADS) void k_filter5(const OpDev<real> *_
 // counts NaNs; and for calcTurbulenceQuantities (CSolver_AWS.cpp:2040-2095)
 // [8] sum rho' [9] sum rho'^2 [10] sum |u|^2 [11] sum (Ux^2+Vy^2+Wz^2)/3
 // [12] sum of the sound speed [13] sum |w|^2.
-constexpr int RED_THREADS = 256;
-constexpr int RED_SLOTS = RED_SLOTS_HOST;
 
 template <typename real>
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Phys<real> ph,
@@ -214,27 +272,44 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
                : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
 }
 
-template <typename real>
-hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
-                          const LineMap &lm, const Ptr5<real> &io, hipStream_t st)
+template <typename real, bool RED>
+static hipError_t launch_f5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
+                            const LineMap &lm, const Ptr5<real> &io, const Phys<real> &ph,
+                            double *part, hipStream_t st)
 {
-    if (lg.LX < 1 || lg.threads < 1 || lg.threads > FILTER_THREADS)
-        return hipErrorInvalidConfiguration;
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
     const size_t shm = lg.shm;
     if (uni) {
         if (M != SEG_FILTER) return hipErrorInvalidValue;
         if (map == MAP_ROW)
-            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW>), grid, block, shm, st, fo, lm, io);
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW, RED>), grid, block, shm, st, fo, lm, io, ph, part);
         else
-            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL>), grid, block, shm, st, fo, lm, io);
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL, RED>), grid, block, shm, st, fo, lm, io, ph, part);
     } else {
         if (M != SEG_TABLE) return hipErrorInvalidValue;
         if (map == MAP_ROW)
-            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW>), grid, block, shm, st, fo, lm, io);
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW, RED>), grid, block, shm, st, fo, lm, io, ph, part);
         else
-            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL>), grid, block, shm, st, fo, lm, io);
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL, RED>), grid, block, shm, st, fo, lm, io, ph, part);
     }
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
+                          const LineMap &lm, const Ptr5<real> &io, const Phys<real> &ph,
+                          double *part, hipStream_t st)
+{
+    if (lg.LX < 1 || lg.threads < 1 || lg.threads > filter_threads<real>())
+        return hipErrorInvalidConfiguration;
+    return part ? launch_f5<real, true>(fo, uni, M, lg, map, lm, io, ph, part, st)
+                : launch_f5<real, false>(fo, uni, M, lg, map, lm, io, ph, part, st);
+}
+
+hipError_t launch_reduce_final(const double *part, int nb, double *out, StepScalars *sc,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st, part, nb, out, sc);
     return hipGetLastError();
 }
 
@@ -274,7 +349,7 @@ hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const Stag
                                            const StageArgs &, const StepScalars *, hipStream_t); \
     template hipError_t launch_filter5<real>(const OpDev<real> *, int, int, const LaunchGeom &,  \
                                              int, const LineMap &, const Ptr5<real> &,         \
-                                             hipStream_t);                                     \
+                                             const Phys<real> &, double *, hipStream_t);       \
     template hipError_t launch_reduce<real>(const Fields<real> &, const Phys<real> &, long, int, \
                                             double *, int, double *, StepScalars *, hipStream_t); \
     template hipError_t launch_sponge<real>(const Fields<real> &, const Phys<real> &,            \
