@@ -418,7 +418,8 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> 
     }
     solve_batch2<10, 4, KC, real, M, UNI>(d1, d2, c, r);
     const real c13 = real(1) / real(3), c43 = real(4) / real(3);
-    // the per-point inputs, whole segments at a time: tot, D_y v, D_z v, q
+    // the per-point inputs, whole segments at a time: tot + totY, D_y v,
+    // D_z v, q
     real pt[19][M];
     {
         const real *src[19] = {f.tot[0], f.tot[1], f.tot[2], f.tot[3], f.Fy[0], f.Fy[1], f.Fy[2],
@@ -426,6 +427,13 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> 
                                f.q[0], f.q[1], f.q[2], f.q[3], f.q[4]};
 #pragma unroll
         for (int k = 0; k < 19; k++) win.load_seg(src[k], pt[k]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            real ty[M];
+            win.load_seg(f.totY[k], ty);
+#pragma unroll
+            for (int t = 0; t < M; t++) pt[k][t] += ty[t];
+        }
     }
     real res[5][M];
 #pragma unroll

@@ -52,14 +52,14 @@ struct AdjComb {
 // Inputs of the fused x chain kernel and what is made of its result res,
 // the conserved cotangent of the vector-Jacobian product: the x solves
 // (D_x of cU, cV, m1/3, -gxz, -gTx and of v_c; D2_x of 4/3 s1, s2, s3, gq),
-// the y and z parts of the primitive cotangents (tot), the y and z flux
+// the z and y parts of the primitive cotangents (tot, totY), the y and z flux
 // cotangents D_y v_c, D_z v_c, and the state q.  Epilogue: out = res,
 // acc = acc_base + res, vn = ca la + cb res (each skipped when its pointer
 // is null; acc may alias acc_base; vn must not alias v).
 template <typename real>
 struct AdjFinal {
     const real *cU, *cV, *m1, *gxz, *gTx, *s[3], *gq, *v[5];
-    const real *tot[4], *Fy[5], *Fz[5], *q[5];
+    const real *tot[4], *totY[4], *Fy[5], *Fz[5], *q[5];
     real *out[5];
     real *acc[5];
     const real *acc_base[5];

@@ -609,7 +609,7 @@ struct SolverT : SolverBase {
     // forward dt sequence frozen; the stage states of each checkpoint
     // segment are replayed with the fused forward stages and kept.
 
-    static constexpr int ADJ_FIELDS = 41;
+    static constexpr int ADJ_FIELDS = 45;
     std::vector<void *> adjalloc;
     real *A[ADJ_FIELDS] = {};  // vjp work fields
     real *lamb[5] = {}, *vb5[5] = {}, *vb5b[5] = {}, *qb5[5] = {}, *la5[5] = {}, *sc5[5] = {};
@@ -768,10 +768,13 @@ struct SolverT : SolverBase {
         acc(2, tot[1], {one, vb}, none, {{m, g1[G1_YZ]}}, {{one, s[1]}});
         acc(2, tot[2], {one, wb}, none, {{m, g1[G1_WZ]}}, {{c43, s[2]}});
         acc(2, tot[3], {one, Tb}, none, {{m, g1[G1_TZ]}}, {{one, gq}});
-        acc(1, tot[0], {one, tot[0]}, none, {{m, g1[G1_XY]}}, {{one, s[0]}});
-        acc(1, tot[1], {one, tot[1]}, none, {{c13, m3}, {m, g1[G1_VY]}}, {{c43, s[1]}});
-        acc(1, tot[2], {one, tot[2]}, none, {{c13, m2}, {m, g1[G1_YZ]}}, {{one, s[2]}});
-        acc(1, tot[3], {one, tot[3]}, none, {{m, g1[G1_TY]}}, {{one, gq}});
+        // y parts into fields of their own (no read-modify-write on the
+        // column path; the x chain adds them)
+        real **totY = A + 41;
+        acc(1, totY[0], none, none, {{m, g1[G1_XY]}}, {{one, s[0]}});
+        acc(1, totY[1], none, none, {{c13, m3}, {m, g1[G1_VY]}}, {{c43, s[1]}});
+        acc(1, totY[2], none, none, {{c13, m2}, {m, g1[G1_YZ]}}, {{one, s[2]}});
+        acc(1, totY[3], none, none, {{m, g1[G1_TY]}}, {{one, gq}});
         acc(1, cU, {c13, m3}, {m, g1[G1_UX]}, {{c13, s[1]}}, {});
         acc(1, cV, {m, g1[G1_XY]}, none, {{c13, s[0]}}, {});
         // flux cotangents D_y v_c, D_z v_c
@@ -791,7 +794,10 @@ struct SolverT : SolverBase {
             fin.Fz[c] = Fz[c];
             fin.q[c] = Q[c];
         }
-        for (int i = 0; i < 4; i++) fin.tot[i] = tot[i];
+        for (int i = 0; i < 4; i++) {
+            fin.tot[i] = tot[i];
+            fin.totY[i] = totY[i];
+        }
         for (int c = 0; c < 5; c++)
             if (fin.vn[c] == v[c]) throw std::logic_error("adjoint: next cotangent aliases v");
         LaunchGeom g;
