@@ -86,6 +86,8 @@ def lib():
     L.cs3d_rhs_vjp.argtypes = [_vp, _int] + [_dp] * 10
     L.cs3d_adjoint_ke_gradient.argtypes = [_vp, _int, _int] + [_dp] * 4
     L.cs3d_adjoint_release.argtypes = [_vp]
+    L.cs3d_get_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
+    L.cs3d_set_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_stream.argtypes = [_vp]
     L.cs3d_stream.restype = _vp
     L.cs3d_ops_create.argtypes = [_int, _int, _int, ctypes.c_double, ctypes.c_double,
@@ -189,6 +191,16 @@ def config(Nx, Ny, Nz, **kw):
     return c
 
 
+class RunState(ctypes.Structure):
+    """cs3d_run_state"""
+    _fields_ = [("timeStep", _int), ("filterTimeStep", _int), ("time", ctypes.c_double),
+                ("dt", ctypes.c_double), ("cfl_max", ctypes.c_double),
+                ("rho_ref", ctypes.c_double), ("rhoE_ref", ctypes.c_double), ("stale", _int)]
+
+
+DUMP_NAMES = ("rho", "rhoU", "rhoV", "rhoW", "rhoE")
+
+
 class Solver:
     """The GPU solver behind the AbstractCSolver / AbstractRK interfaces.
     Fields are numpy arrays shaped (Nz, Ny, Nx) (XYZ order)."""
@@ -274,6 +286,57 @@ class Solver:
         cnt = (_int * 6)()
         check(lib().cs3d_profile_report(self.h, _dptr(ms), cnt))
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PROFILE_CLASSES)}
+
+    def run_state(self):
+        rs = RunState()
+        check(lib().cs3d_get_run_state(self.h, ctypes.byref(rs)))
+        return rs
+
+    def set_run_state(self, rs):
+        check(lib().cs3d_set_run_state(self.h, ctypes.byref(rs)))
+
+    def dump(self, directory):
+        """dumpSolution's files (CSolver_AWS.cpp:2493-2535: raw FP64 XYZ
+        rho.out.<step> ... rhoE.out.<step>) plus restart.<step> (the run
+        state as text) and, with sponges, the running averages and the
+        stale-plane copy (spongeAvg*.out.<step>, spongeStale*.out.<step>).
+        Returns the step."""
+        rs = self.run_state()
+        n = rs.timeStep
+        sets = [(1, "")]
+        if any(self.cfg.bc_kind[f] == BC_SPONGE for f in range(6)):
+            sets += [(5, "spongeAvg"), (6, "spongeStale")]
+        for set_, pre in sets:
+            for name, a in zip(DUMP_NAMES, self.conserved(set_)):
+                fn = f"{pre}{name}.out.{n}" if pre else f"{name}.out.{n}"
+                a.astype(np.float64).tofile(os.path.join(directory, fn))
+        with open(os.path.join(directory, f"restart.{n}"), "w") as f:
+            for k, _ in RunState._fields_:
+                v = getattr(rs, k)
+                f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
+        return n
+
+    def restart(self, directory, step):
+        """continue from the files dump(directory) wrote at `step`"""
+        rd = lambda fn: np.fromfile(os.path.join(directory, fn), dtype=np.float64).reshape(self.shape)
+        q = [rd(f"{name}.out.{step}") for name in DUMP_NAMES]
+        rs = RunState()
+        with open(os.path.join(directory, f"restart.{step}")) as f:
+            for line in f:
+                k, v = line.split()
+                setattr(rs, k, float(v) if k in ("time", "dt", "cfl_max", "rho_ref", "rhoE_ref")
+                        else int(v))
+        g = self.cfg.gamma
+        rho = q[0]
+        U, V, W = q[1] / rho, q[2] / rho, q[3] / rho
+        p = (g - 1.0) * (q[4] - 0.5 * rho * (U * U + V * V + W * W))
+        self.set_initial_conditions(rho, U, V, W, p)
+        self.set_run_state(rs)
+        self.set_conserved(q, 1)
+        if any(self.cfg.bc_kind[f] == BC_SPONGE for f in range(6)):
+            for set_, pre in ((5, "spongeAvg"), (6, "spongeStale")):
+                self.set_conserved([rd(f"{pre}{name}.out.{step}") for name in DUMP_NAMES], set_)
+        self.set_run_state(rs)  # setting set 1 recomputed the CFL maximum
 
     def rhs_vjp(self, lam, set_=1):
         """(d(dt*RHS)/dQ)^T lam at conserved set `set_`: five fields in, five out"""

@@ -179,7 +179,7 @@ int cs3d_get_conserved(cs3d_solver *s, int set, double *rho, double *rhoU, doubl
                        double *rhoW, double *rhoE)
 {
     CS3D_NEED(s, "cs3d_get_conserved");
-    if (set < 1 || set > 4) return set_error(CS3D_ERR_ARG, "cs3d_get_conserved: bad set");
+    if (set < 1 || set > 6) return set_error(CS3D_ERR_ARG, "cs3d_get_conserved: bad set");
     double *o[5] = {rho, rhoU, rhoV, rhoW, rhoE};
     return guard("cs3d_get_conserved", [&]() -> int {
         s->impl->get_conserved(set, o);
@@ -191,7 +191,7 @@ int cs3d_set_conserved(cs3d_solver *s, int set, const double *rho, const double 
                        const double *rhoV, const double *rhoW, const double *rhoE)
 {
     CS3D_NEED(s, "cs3d_set_conserved");
-    if (set < 1 || set > 4) return set_error(CS3D_ERR_ARG, "cs3d_set_conserved: bad set");
+    if (set < 1 || set > 6) return set_error(CS3D_ERR_ARG, "cs3d_set_conserved: bad set");
     const double *in[5] = {rho, rhoU, rhoV, rhoW, rhoE};
     return guard("cs3d_set_conserved", [&]() -> int {
         s->impl->set_conserved(set, in);
@@ -215,6 +215,26 @@ int cs3d_get_time(cs3d_solver *s, double *time, double *dt, int *timeStep)
     CS3D_NEED(s, "cs3d_get_time");
     return guard("cs3d_get_time", [&]() -> int {
         s->impl->get_time(time, dt, timeStep);
+        return CS3D_OK;
+    });
+}
+
+int cs3d_get_run_state(cs3d_solver *s, cs3d_run_state *rs)
+{
+    CS3D_NEED(s, "cs3d_get_run_state");
+    if (!rs) return set_error(CS3D_ERR_ARG, "cs3d_get_run_state: NULL");
+    return guard("cs3d_get_run_state", [&]() -> int {
+        s->impl->get_run_state(rs);
+        return CS3D_OK;
+    });
+}
+
+int cs3d_set_run_state(cs3d_solver *s, const cs3d_run_state *rs)
+{
+    CS3D_NEED(s, "cs3d_set_run_state");
+    if (!rs) return set_error(CS3D_ERR_ARG, "cs3d_set_run_state: NULL");
+    return guard("cs3d_set_run_state", [&]() -> int {
+        s->impl->set_run_state(*rs);
         return CS3D_OK;
     });
 }

@@ -470,7 +470,7 @@ struct SolverT : SolverBase {
     void get_conserved(int set, double *const *out) override
     {
         need_ic();
-        real *const *src = set == 1 ? F.q : set == 2 ? F.qk : set == 3 ? F.k2 : F.acc;
+        real *const *src = set_fields(set);
         std::vector<real> buf((size_t)N);
         hc(hipStreamSynchronize(st), "sync");
         for (int v = 0; v < 5; v++) {
@@ -481,18 +481,69 @@ struct SolverT : SolverBase {
         }
     }
 
+    // the device fields of a conserved set (1..6, see cs3d.h)
+    real *const *set_fields(int set) const
+    {
+        if ((set == 5 || set == 6) && !sponge)
+            throw std::invalid_argument("sets 5 and 6 exist with sponge faces only");
+        return set == 1 ? F.q : set == 2 ? F.qk : set == 3 ? F.k2 : set == 4 ? F.acc
+               : set == 5 ? F.avg : F.qstale;
+    }
+
     void set_conserved(int set, const double *const *in) override
     {
         need_ic();
         hc(hipStreamSynchronize(st), "sync");
-        std::vector<double> q[5];
+        std::vector<real> buf((size_t)N);
+        real *const *dst = set_fields(set);
         for (int v = 0; v < 5; v++) {
             if (!in[v]) throw std::invalid_argument("set_conserved needs all five fields");
-            q[v].assign(in[v], in[v] + N);
+            const double off = set == 3 ? 0.0 : (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0);
+            for (long i = 0; i < N; i++) buf[i] = (real)(in[v][i] - off);
+            hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
         }
-        int which = set == 1 ? 0 : set == 2 ? 1 : set == 3 ? 2 : 3;
-        upload_state(q, which);
         if (set == 1) hc(launch_reduce<real>(F, ph, N, 0, dpart, nred, dout, dsc, st), "reduce");
+    }
+
+    void get_run_state(cs3d_run_state *rs) override
+    {
+        need_ic();
+        StepScalars h;
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        rs->timeStep = timeStep;
+        rs->filterTimeStep = filterTimeStep;
+        rs->time = h.time;
+        rs->dt = h.dt;
+        rs->cfl_max = h.cfl_max;
+        rs->rho_ref = rho_ref;
+        rs->rhoE_ref = rhoE_ref;
+        rs->stale = stale_pending ? 1 : 0;
+    }
+
+    void set_run_state(const cs3d_run_state &rs) override
+    {
+        need_ic();
+        if (rs.timeStep < 0 || rs.filterTimeStep < 0 || !(rs.rho_ref > 0.0) || !(rs.dt >= 0.0) ||
+            !(rs.cfl_max >= 0.0))
+            throw std::invalid_argument("set_run_state: bad run state");
+        if (rs.stale && !sponge) throw std::invalid_argument("set_run_state: stale planes without sponges");
+        StepScalars h;
+        hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
+        hc(hipStreamSynchronize(st), "sync");
+        h.timeStep = rs.timeStep;
+        h.time = rs.time;
+        h.dt = rs.dt;
+        if (rs.cfl_max > 0.0) h.cfl_max = rs.cfl_max;
+        hc(hipMemcpyAsync(dsc, &h, sizeof(h), hipMemcpyHostToDevice, st), "scalars");
+        timeStep = rs.timeStep;
+        filterTimeStep = rs.filterTimeStep;
+        stale_pending = rs.stale != 0;
+        rho_ref = rs.rho_ref;
+        rhoE_ref = rs.rhoE_ref;
+        build_phys();
+        diag_valid = false;
+        hc(hipStreamSynchronize(st), "sync");
     }
 
     void get_primitives(double *const *out) override

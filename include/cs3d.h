@@ -117,8 +117,10 @@ int cs3d_step(cs3d_solver *s, int nsteps);
 /* the stage right-hand side alone, dt*RHS into the *k2 set (solveEqnSet) */
 int cs3d_rhs(cs3d_solver *s, int rkStep);
 
-/* Host copies of a conserved set: set 1 = *1 (state), 2 = *k, 3 = *k2, 4 = *2.
- * Any output pointer may be NULL. */
+/* Host copies of a conserved set: set 1 = *1 (state), 2 = *k, 3 = *k2, 4 = *2;
+ * with sponge faces also 5 = the sponge running averages (SpongeBC.hpp:
+ * spongeRhoAvg ...) and 6 = the copy of the sponge planes taken before the
+ * last sponge reset.  Any output pointer may be NULL. */
 int cs3d_get_conserved(cs3d_solver *s, int set, double *rho, double *rhoU, double *rhoV,
                        double *rhoW, double *rhoE);
 int cs3d_set_conserved(cs3d_solver *s, int set, const double *rho, const double *rhoU,
@@ -128,6 +130,24 @@ int cs3d_set_conserved(cs3d_solver *s, int set, const double *rho, const double 
 int cs3d_get_primitives(cs3d_solver *s, double *U, double *V, double *W, double *p,
                         double *T, double *mu, double *Amu, double *sos);
 int cs3d_get_time(cs3d_solver *s, double *time, double *dt, int *timeStep);
+
+/* The run state beyond the conserved sets, for restarts (no reference
+ * counterpart: the reference dumps the state, CSolver_AWS.cpp:2493-2535,
+ * but has no reader).  rho_ref, rhoE_ref: the reference state of the
+ * perturbation storage; cfl_max: the CFL maximum the next dt comes from
+ * (taken before the sponge update, so not recomputable from the dumped
+ * state); stale: the next first stage takes sponge-plane primitives from
+ * set 6.  Restart order: cs3d_set_initial_conditions (any field),
+ * cs3d_set_run_state, cs3d_set_conserved for set 1 (and 5, 6 with
+ * sponges), cs3d_set_run_state again (setting set 1 recomputes cfl_max). */
+typedef struct {
+    int timeStep, filterTimeStep;
+    double time, dt, cfl_max;
+    double rho_ref, rhoE_ref;
+    int stale;
+} cs3d_run_state;
+int cs3d_get_run_state(cs3d_solver *s, cs3d_run_state *rs);
+int cs3d_set_run_state(cs3d_solver *s, const cs3d_run_state *rs);
 
 /* When enabled, the last stage of every step keeps the per-point inputs of
  * the step diagnostics (the reference computes them in postStep,

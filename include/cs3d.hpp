@@ -395,7 +395,9 @@ class CSolver_AWS : public AbstractCSolver {
         if (h) cs3d_destroy(h);
         h = nullptr;
         check(cs3d_create(&c, &h), "setInitialConditions (create)");
+        if (restartStep >= 0) loadRestartFields(restartStep);
         check(cs3d_set_initial_conditions(h, rho0, U0, V0, W0, p0), "setInitialConditions");
+        if (restartStep >= 0) applyRestart(restartStep);
         check(cs3d_set_diagnostics(h, tgvDiagnostics || turbDiagnostics), "set_diagnostics");
         if (useTiming) check(cs3d_profile_enable(h, 1), "profile_enable");
         endFlag = false;
@@ -490,28 +492,135 @@ class CSolver_AWS : public AbstractCSolver {
     }
 
     // dumpSolution (CSolver_AWS.cpp:2493-2535): every dumpStep steps, raw
-    // FP64 XYZ arrays rho.out.<step> ... rhoE.out.<step>
+    // FP64 XYZ arrays rho.out.<step> ... rhoE.out.<step>; in addition the
+    // run state restartSolution needs (restart.<step>, text) and, with
+    // sponges, their averages and stale-plane copy (spongeAvg*.out.<step>,
+    // spongeStale*.out.<step>)
     void dumpSolution()
     {
         if (ts->dumpStep <= 0 || timeStep % ts->dumpStep != 0) return;
         check(cs3d_get_conserved(h, 1, rho1, rhoU1, rhoV1, rhoW1, rhoE1), "dumpSolution");
-        const char *names[5] = {"rho", "rhoU", "rhoV", "rhoW", "rhoE"};
         double *data[5] = {rho1, rhoU1, rhoV1, rhoW1, rhoE1};
-        for (int v = 0; v < 5; v++) {
-            std::string fn = outDir + "/" + names[v] + ".out." + std::to_string(timeStep);
-            FILE *fp = std::fopen(fn.c_str(), "wb");
-            if (!fp) {
-                std::cerr << " > cannot open " << fn << std::endl;
-                std::abort();
+        for (int v = 0; v < 5; v++) writeField(fieldName("", v), data[v]);
+        if (hasSponge()) {
+            std::vector<double> buf(5 * (size_t)dom->N);
+            double *b[5];
+            for (int v = 0; v < 5; v++) b[v] = buf.data() + v * (size_t)dom->N;
+            for (int set = 5; set <= 6; set++) {
+                check(cs3d_get_conserved(h, set, b[0], b[1], b[2], b[3], b[4]), "dumpSolution");
+                for (int v = 0; v < 5; v++)
+                    writeField(fieldName(set == 5 ? "spongeAvg" : "spongeStale", v), b[v]);
             }
-            std::fwrite(data[v], sizeof(double), (size_t)dom->N, fp);
-            std::fclose(fp);
+        }
+        cs3d_run_state rs;
+        check(cs3d_get_run_state(h, &rs), "dumpSolution");
+        std::ofstream f(outDir + "/restart." + std::to_string(timeStep));
+        f << std::setprecision(17) << "timeStep " << rs.timeStep << "\n"
+          << "filterTimeStep " << rs.filterTimeStep << "\n"
+          << "time " << rs.time << "\n"
+          << "dt " << rs.dt << "\n"
+          << "cfl_max " << rs.cfl_max << "\n"
+          << "rho_ref " << rs.rho_ref << "\n"
+          << "rhoE_ref " << rs.rhoE_ref << "\n"
+          << "stale " << rs.stale << "\n";
+    }
+
+    // No reference counterpart: with restartStep >= 0, setInitialConditions
+    // continues from the files dumpSolution wrote at that step in outDir
+    // instead of reading the IC arrays.
+    int restartStep = -1;
+
+  private:
+    // the dumped state as IC arrays (for the handle's set-up)
+    void loadRestartFields(int step)
+    {
+        const size_t N = (size_t)dom->N;
+        double *data[5] = {rho1, rhoU1, rhoV1, rhoW1, rhoE1};
+        for (int v = 0; v < 5; v++) readField(fieldName("", v, step), data[v]);
+        const double g = ig->gamma;
+        for (size_t i = 0; i < N; i++) {
+            const double r = rho1[i], u = rhoU1[i] / r, v = rhoV1[i] / r, w = rhoW1[i] / r;
+            rho0[i] = r;
+            U0[i] = u;
+            V0[i] = v;
+            W0[i] = w;
+            p0[i] = (g - 1.0) * (rhoE1[i] - 0.5 * r * (u * u + v * v + w * w));
         }
     }
+
+    // the dumped run state and fields into the handle
+    void applyRestart(int step)
+    {
+        const size_t N = (size_t)dom->N;
+        cs3d_run_state rs{};
+        std::ifstream f(outDir + "/restart." + std::to_string(step));
+        std::string key;
+        while (f >> key) {
+            if (key == "timeStep") f >> rs.timeStep;
+            else if (key == "filterTimeStep") f >> rs.filterTimeStep;
+            else if (key == "time") f >> rs.time;
+            else if (key == "dt") f >> rs.dt;
+            else if (key == "cfl_max") f >> rs.cfl_max;
+            else if (key == "rho_ref") f >> rs.rho_ref;
+            else if (key == "rhoE_ref") f >> rs.rhoE_ref;
+            else if (key == "stale") f >> rs.stale;
+        }
+        check(cs3d_set_run_state(h, &rs), "restartSolution");
+        check(cs3d_set_conserved(h, 1, rho1, rhoU1, rhoV1, rhoW1, rhoE1), "restartSolution");
+        if (hasSponge()) {
+            std::vector<double> buf(5 * N);
+            double *b[5];
+            for (int v = 0; v < 5; v++) b[v] = buf.data() + v * N;
+            for (int set = 5; set <= 6; set++) {
+                for (int v = 0; v < 5; v++)
+                    readField(fieldName(set == 5 ? "spongeAvg" : "spongeStale", v, step), b[v]);
+                check(cs3d_set_conserved(h, set, b[0], b[1], b[2], b[3], b[4]), "restartSolution");
+            }
+        }
+        // setting the state recomputed the CFL maximum; the dumped one stands
+        check(cs3d_set_run_state(h, &rs), "restartSolution");
+        timeStep = rs.timeStep;
+        time = rs.time;
+        dt = rs.dt;
+    }
+
+  public:
 
     cs3d_solver *handle() { return h; }
 
   private:
+    bool hasSponge() const
+    {
+        const int kinds[6] = {bc->bcX0, bc->bcX1, bc->bcY0, bc->bcY1, bc->bcZ0, bc->bcZ1};
+        for (int k : kinds)
+            if (k == BC::SPONGE) return true;
+        return false;
+    }
+    std::string fieldName(const char *prefix, int v, int step = -1) const
+    {
+        const char *names[5] = {"rho", "rhoU", "rhoV", "rhoW", "rhoE"};
+        return outDir + "/" + prefix + names[v] + ".out." +
+               std::to_string(step < 0 ? timeStep : step);
+    }
+    void writeField(const std::string &fn, const double *d) const
+    {
+        FILE *fp = std::fopen(fn.c_str(), "wb");
+        if (!fp || std::fwrite(d, sizeof(double), (size_t)dom->N, fp) != (size_t)dom->N) {
+            std::cerr << " > cannot write " << fn << std::endl;
+            std::abort();
+        }
+        std::fclose(fp);
+    }
+    void readField(const std::string &fn, double *d) const
+    {
+        FILE *fp = std::fopen(fn.c_str(), "rb");
+        if (!fp || std::fread(d, sizeof(double), (size_t)dom->N, fp) != (size_t)dom->N) {
+            std::cerr << " > cannot read " << fn << std::endl;
+            std::abort();
+        }
+        std::fclose(fp);
+    }
+
     cs3d_solver *h = nullptr;
     int rkOrder = 4;
     std::vector<double> store;

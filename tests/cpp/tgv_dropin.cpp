@@ -1,7 +1,9 @@
 // Taylor-Green driver written against the C++ drop-in classes exactly as a
 // reference driver is written against the reference's (3DCompact.cpp:40-110,
 // taylorgreen_vortex drivers): Domain, BC, TimeStepping, CSolver_AWS, RK4.
-// Usage: tgv_dropin N steps precision(0 FP64 | 1 FP32) outdir
+// Usage: tgv_dropin N steps precision(0 FP64 | 1 FP32) outdir [dumpStep
+// [restartStep]]: steps is the last step (maxTimeStep); with restartStep
+// the run continues from the files a dump at that step left in outdir.
 // Prints the final volume-averaged kinetic energy.
 #define CS3D_DROPIN_GLOBAL
 #include "cs3d.hpp"
@@ -16,18 +18,21 @@ int main(int argc, char **argv)
     const int steps = argc > 2 ? std::atoi(argv[2]) : 3;
     const int prec = argc > 3 ? std::atoi(argv[3]) : 0;
     const char *out = argc > 4 ? argv[4] : ".";
+    const int dumpStep = argc > 5 ? std::atoi(argv[5]) : 1000000;
+    const int restart = argc > 6 ? std::atoi(argv[6]) : -1;
     const double L = 2.0 * M_PI - 2.0 * M_PI / N;
     Domain *dom = new Domain(N, N, N, L, L, L);
     BC *bc = new BC(BC::PERIODIC_SOLVE, BC::PERIODIC, BC::PERIODIC, BC::PERIODIC_SOLVE,
                     BC::PERIODIC, BC::PERIODIC, BC::PERIODIC_SOLVE, BC::PERIODIC, BC::PERIODIC);
     TimeStepping *ts = new TimeStepping(TimeStepping::CONST_CFL, 0.5, steps, 1e30, 1, 1000000,
-                                        1000000);
+                                        dumpStep);
     const double mu_ref = (1.0 / 1600.0) / std::pow(100.0, 0.76);
     CSolver_AWS *cs = new CSolver_AWS(dom, bc, ts, 0.49, mu_ref, 8, false);
     cs->ig->Pr = 0.71;
     cs->precision = prec;
     cs->tgvDiagnostics = true;
     cs->outDir = out;
+    cs->restartStep = restart;
     // Taylor-Green initial condition (Mach 0.1, isothermal density)
     const double p0 = 1.0 / (1.4 * 0.1 * 0.1);
     for (int k = 0; k < N; k++)

@@ -68,3 +68,30 @@ def test_dropin_driver_matches_c_abi(tmp_path):
         # dilprime = rms(div u) is cancellation-dominated on this field
         assert abs(rows[s][6] - ref[6]) <= 1e-6 * abs(ref[6]), (s, rows[s][6], ref[6])
     assert abs(ke_cpp - g.kinetic_energy()) <= 1e-12 * abs(ke_cpp)
+
+
+@pytest.mark.gpu
+def test_dropin_dump_and_restart(tmp_path):
+    """a driver run of 6 steps against one that stops at step 3 (dumping
+    every 3 steps) and a second driver that restarts from that dump
+    (CSolver_AWS::restartStep) and runs to step 6"""
+    exe = BIN if os.access(BIN, os.X_OK) else _build()
+    full, part = tmp_path / "full", tmp_path / "part"
+    full.mkdir()
+    part.mkdir()
+    run = lambda *a: subprocess.run([exe, *map(str, a)], capture_output=True, text=True,
+                                    timeout=120)
+    r_full = run(24, 6, 0, full, 3)
+    r_a = run(24, 3, 0, part, 3)
+    assert (part / "rho.out.3").exists() and (part / "restart.3").exists()
+    r_b = run(24, 6, 0, part, 3, 3)
+    for r in (r_full, r_a, r_b):
+        assert r.returncode == 0, r.stderr
+    f_full, f_b = r_full.stdout.split(), r_b.stdout.split()
+    assert int(f_full[1]) == int(f_b[1]) == 6
+    assert abs(float(f_full[3]) - float(f_b[3])) <= 1e-13 * float(f_full[3])
+    assert abs(float(f_full[5]) - float(f_b[5])) <= 1e-13 * float(f_full[5])
+    for name in ("rho", "rhoU", "rhoE"):
+        a = np.fromfile(full / f"{name}.out.6")
+        b = np.fromfile(part / f"{name}.out.6")
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a)), name

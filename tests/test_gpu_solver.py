@@ -194,3 +194,41 @@ def test_fp32_kinetic_energy_tracks_fp64_1000_steps():
         ke[prec] = g.kinetic_energy()
         print("KE", prec, ke[prec], flush=True)
     assert abs(ke[cs3d.FP32] - ke[cs3d.FP64]) <= 1e-5 * abs(ke[cs3d.FP64])
+
+
+@pytest.mark.parametrize("prec,sponge", [(cs3d.FP64, False), (cs3d.FP32, False),
+                                         (cs3d.FP64, True)])
+def test_dump_and_restart_continue_the_run(tmp_path, prec, sponge):
+    """dump after 3 steps, restart a fresh solver from the files, 3 more
+    steps: the same state as 6 uninterrupted steps.  The dump holds full
+    FP64 values while the solver stores perturbations of a reference state,
+    so the restart reproduces the run to rounding (1e-13 FP64, 2e-6 FP32)."""
+    if sponge:
+        dims = (24, 16, 16)
+        kw = dict(Lx=6.0, Ly=4.0, Lz=4.0, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+                  filter_step=2, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 1, 1, 0, 0),
+                  sponge_frac=(0.25, 0.25, 0.25))
+        icf = random_ic((16, 16, 24), 6, amp=0.2)
+    else:
+        N = 24
+        kw, L = tgv_kw(N)
+        kw["filter_step"] = 2
+        dims = (N, N, N)
+        icf = tgv_ic(N, L)
+    Nx, Ny, Nz = dims
+    a = cs3d.Solver(cs3d.config(Nx, Ny, Nz, precision=prec, **kw))
+    a.set_initial_conditions(*icf)
+    a.step(3)
+    n = a.dump(str(tmp_path))
+    assert n == 3
+    a.step(3)
+    b = cs3d.Solver(cs3d.config(Nx, Ny, Nz, precision=prec, **kw))
+    b.restart(str(tmp_path), 3)
+    b.step(3)
+    ta, tb = a.time(), b.time()
+    assert ta[2] == tb[2] == 6
+    assert abs(ta[0] - tb[0]) <= 1e-13 * abs(ta[0])
+    errs = [relerr(x, y) for x, y in zip(b.conserved(1), a.conserved(1))]
+    print(f"restart prec={prec} sponge={sponge}: " + " ".join(f"{e:.2e}" for e in errs),
+          flush=True)
+    assert max(errs) <= (1e-13 if prec == cs3d.FP64 else 2e-6)
