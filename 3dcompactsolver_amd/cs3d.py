@@ -86,6 +86,7 @@ def lib():
     L.cs3d_rhs_vjp.argtypes = [_vp, _int] + [_dp] * 10
     L.cs3d_adjoint_ke_gradient.argtypes = [_vp, _int, _int] + [_dp] * 4
     L.cs3d_adjoint_release.argtypes = [_vp]
+    L.cs3d_diag_shear_layer.argtypes = [_vp, _int, _dp]
     L.cs3d_get_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_set_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_stream.argtypes = [_vp]
@@ -286,6 +287,16 @@ class Solver:
         cnt = (_int * 6)()
         check(lib().cs3d_profile_report(self.h, _dptr(ms), cnt))
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.PROFILE_CLASSES)}
+
+    SHEAR_FILES = ("rhoprime2", "uprime2", "vprime2", "wprime2", "pprime2", "tprime2",
+                   "uvprime", "uwprime", "vwprime", "uAvg", "rhoAvg")
+
+    def diag_shear_layer(self, fix_q12=False):
+        """{file name: Ny values} of shearLayerInfoCalc for the current state"""
+        out = np.zeros(11 * self.cfg.Ny)
+        check(lib().cs3d_diag_shear_layer(self.h, 1 if fix_q12 else 0, _dptr(out)))
+        return {n: out[i * self.cfg.Ny:(i + 1) * self.cfg.Ny].copy()
+                for i, n in enumerate(self.SHEAR_FILES)}
 
     def run_state(self):
         rs = RunState()

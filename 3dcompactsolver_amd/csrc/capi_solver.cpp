@@ -219,6 +219,16 @@ int cs3d_get_time(cs3d_solver *s, double *time, double *dt, int *timeStep)
     });
 }
 
+int cs3d_diag_shear_layer(cs3d_solver *s, int fix_q12, double *out)
+{
+    CS3D_NEED(s, "cs3d_diag_shear_layer");
+    if (!out) return set_error(CS3D_ERR_ARG, "cs3d_diag_shear_layer: NULL");
+    return guard("cs3d_diag_shear_layer", [&]() -> int {
+        s->impl->shear_profiles(fix_q12, out);
+        return CS3D_OK;
+    });
+}
+
 int cs3d_get_run_state(cs3d_solver *s, cs3d_run_state *rs)
 {
     CS3D_NEED(s, "cs3d_get_run_state");

@@ -505,6 +505,50 @@ struct SolverT : SolverBase {
         if (set == 1) hc(launch_reduce<real>(F, ph, N, 0, dpart, nred, dout, dsc, st), "reduce");
     }
 
+    // shearLayerInfoCalc (CSolver_AWS.cpp:2259-2456): x-z plane means and
+    // second moments per y index, out = 11 rows of Ny values in the order of
+    // its files.  Q12: the reference adds v'w' into uwprime and leaves
+    // vwprime zero; fix_q12 gives each its own.
+    void shear_profiles(int fix_q12, double *out) override
+    {
+        need_ic();
+        const size_t nrow = (size_t)Ny * Nz * 9;
+        double *drows = nullptr, *dmean = nullptr;
+        hc(hipMalloc(&drows, nrow * sizeof(double)), "hipMalloc(profiles)");
+        hc(hipMalloc(&dmean, (size_t)Ny * 6 * sizeof(double)), "hipMalloc(profiles)");
+        std::vector<double> rows(nrow), mean((size_t)Ny * 6), mom((size_t)Ny * 9);
+        auto planes = [&](int pass, std::vector<double> &dst, int nq) {
+            hc(launch_plane_rows<real>(F, ph, Nx, Ny, Nz, pass, dmean, drows, st), "profiles");
+            hc(hipMemcpyAsync(rows.data(), drows, nrow * sizeof(double), hipMemcpyDeviceToHost, st),
+               "profiles");
+            hc(hipStreamSynchronize(st), "sync");
+            const double inv = 1.0 / ((double)Nx * Nz);
+            for (int j = 0; j < Ny; j++)
+                for (int q = 0; q < nq; q++) {
+                    double s = 0.0;
+                    for (int k = 0; k < Nz; k++) s += rows[((size_t)k * Ny + j) * 9 + q];
+                    dst[(size_t)j * nq + q] = s * inv;
+                }
+        };
+        planes(0, mean, 6);
+        hc(hipMemcpyAsync(dmean, mean.data(), mean.size() * sizeof(double), hipMemcpyHostToDevice,
+                          st),
+           "profiles");
+        planes(1, mom, 9);
+        (void)hipFree(drows);
+        (void)hipFree(dmean);
+        for (int j = 0; j < Ny; j++) {
+            const double *m = &mom[(size_t)j * 9];
+            double *o = out;
+            for (int q = 0; q < 6; q++) o[q * Ny + j] = m[q];  // rho' u' v' w' p' T' squared
+            o[6 * Ny + j] = m[6];                               // u'v'
+            o[7 * Ny + j] = fix_q12 ? m[7] : m[7] + m[8];       // u'w' (+ v'w', Q12)
+            o[8 * Ny + j] = fix_q12 ? m[8] : 0.0;               // v'w'
+            o[9 * Ny + j] = mean[(size_t)j * 6 + 1];            // uAvg
+            o[10 * Ny + j] = mean[(size_t)j * 6 + 0];           // rhoAvg
+        }
+    }
+
     void get_run_state(cs3d_run_state *rs) override
     {
         need_ic();

@@ -107,6 +107,11 @@ template <typename real>
 hipError_t launch_reduce(const Fields<real> &F, const Phys<real> &ph, long N, int diag,
                          double *part, int nb, double *out, StepScalars *sc, hipStream_t st);
 hipError_t launch_dt(StepScalars *sc, double cfl, double dt_fixed, hipStream_t st);
+// per-(j, k) row sums of the plane statistics (profiles.hip): 9 doubles a
+// row; pass 0 means (6 used), pass 1 second moments about `mean` (6 per j)
+template <typename real>
+hipError_t launch_plane_rows(const Fields<real> &F, const Phys<real> &ph, int Nx, int Ny, int Nz,
+                             int pass, const double *mean, double *rows, hipStream_t st);
 template <typename real>
 hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,
                          const StepScalars *sc, double avgT, double epsP, double spongeP,

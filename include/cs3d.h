@@ -162,6 +162,13 @@ int cs3d_diag_taylor_green(cs3d_solver *s, double out[7]);
  * taylorReyn3, turbMach, meanTurbDiss, uprime, kolNu, rhoprime, dilprime,
  * vortprime, meanKineticEng */
 int cs3d_diag_turbulence(cs3d_solver *s, double out[11]);
+/* shearLayerInfoCalc (CSolver_AWS.cpp:2259-2456) of the current state, FP64:
+ * out = 11 rows of Ny values, each row the x-z plane statistic per y index,
+ * in the order of its files: rhoprime2, uprime2, vprime2, wprime2, pprime2,
+ * tprime2, uvprime, uwprime, vwprime, uAvg, rhoAvg.  fix_q12 = 0 keeps the
+ * reference's accumulation of v'w' into uwprime (vwprime zero); 1 separates
+ * them. */
+int cs3d_diag_shear_layer(cs3d_solver *s, int fix_q12, double *out);
 /* volume-averaged kinetic energy of the current state, FP64 */
 int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
 

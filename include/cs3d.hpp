@@ -334,6 +334,9 @@ class CSolver_AWS : public AbstractCSolver {
     bool tgvDiagnostics = false;     // append taylorgreen.out every step
     bool turbDiagnostics = false;    // append turbdata.out every step
     std::string outDir = ".";
+    // shearLayerInfoCalc's files every 25 steps, as the reference's postStep
+    // writes them for every run; fixShearQ12 separates v'w' from u'w' (Q12)
+    bool shearLayerOutput = true, fixShearQ12 = false;
 
     CSolver_AWS(Domain *dom, BC *bc, TimeStepping *ts, double alphaF, double mu_ref,
                 int blocksize, bool useTiming)
@@ -419,9 +422,10 @@ class CSolver_AWS : public AbstractCSolver {
     // dumpSolution, checkEnd
     void postStep() override
     {
-        check(cs3d_post_step(h), "postStep");
         check(cs3d_get_time(h, &time, &dt, &timeStep), "get_time");
         ts->dt = dt;
+        if (shearLayerOutput) shearLayerInfoCalc();
+        check(cs3d_post_step(h), "postStep");
         if (tgvDiagnostics) calcTaylorGreenQuantities();
         if (turbDiagnostics) calcTurbulenceQuantities();
         checkSolution();
@@ -440,6 +444,25 @@ class CSolver_AWS : public AbstractCSolver {
         for (int i = 0; i < 7; i++) f << o[i] << (i < 6 ? " " : "");
         f << std::endl;
     }
+    // shearLayerInfoCalc (CSolver_AWS.cpp:2259-2456): every 25 steps, x-z
+    // plane statistics per y index appended to rhoprime2.out ... rhoAvg.out
+    // (a line: time, then the Ny values, a blank after each)
+    void shearLayerInfoCalc()
+    {
+        if (timeStep % 25 != 0) return;
+        const int Ny = dom->Ny;
+        std::vector<double> o(11 * (size_t)Ny);
+        check(cs3d_diag_shear_layer(h, fixShearQ12 ? 1 : 0, o.data()), "shearLayerInfoCalc");
+        const char *files[11] = {"rhoprime2", "uprime2", "vprime2", "wprime2", "pprime2", "tprime2",
+                                 "uvprime",   "uwprime", "vwprime", "uAvg",    "rhoAvg"};
+        for (int f = 0; f < 11; f++) {
+            std::ofstream out(outDir + "/" + files[f] + ".out", std::ios::app);
+            out << std::setprecision(17) << time << " ";
+            for (int j = 0; j < Ny; j++) out << o[(size_t)f * Ny + j] << " ";
+            out << std::endl;
+        }
+    }
+
     // calcTurbulenceQuantities (CSolver_AWS.cpp:2033-2173): turbdata.out in
     // the reference's format (:2154-2164, a blank after every column)
     void calcTurbulenceQuantities()

@@ -232,3 +232,44 @@ def test_dump_and_restart_continue_the_run(tmp_path, prec, sponge):
     print(f"restart prec={prec} sponge={sponge}: " + " ".join(f"{e:.2e}" for e in errs),
           flush=True)
     assert max(errs) <= (1e-13 if prec == cs3d.FP64 else 2e-6)
+
+
+@pytest.mark.parametrize("prec", [cs3d.FP64, cs3d.FP32])
+def test_shear_layer_profiles_closed_forms(prec):
+    """shearLayerInfoCalc's plane statistics on a field whose x-z means and
+    moments are exact: s_x = (-1)^i, s_z = (-1)^k (even Nx, Nz);
+    U = a(y) + c s_x, V = d s_z, W = e s_x + f s_z, rho = 1 + g(y), p
+    uniform.  Q12: the reference's uwprime is u'w' + v'w' = c e + d f and its
+    vwprime 0; fixed: c e and d f."""
+    Nx, Ny, Nz = 16, 12, 8
+    kw = dict(Lx=6.0, Ly=4.0, Lz=4.0, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+              filter_step=1, rk=4)
+    y = np.arange(Ny) * (4.0 / (Ny - 1))
+    sx = (-1.0) ** np.arange(Nx)
+    sz = (-1.0) ** np.arange(Nz)
+    Sx = np.broadcast_to(sx[None, None, :], (Nz, Ny, Nx))
+    Sz = np.broadcast_to(sz[:, None, None], (Nz, Ny, Nx))
+    a = 0.3 * np.tanh(y - 2.0)
+    gy = 0.05 * np.cos(y)
+    c, d, e, f = 0.1, 0.07, 0.05, 0.03
+    U = a[None, :, None] + c * Sx
+    V = d * Sz
+    W = e * Sx + f * Sz
+    rho = np.broadcast_to(1.0 + gy[None, :, None], (Nz, Ny, Nx)).copy()
+    p = np.full((Nz, Ny, Nx), 1.0 / 1.4)
+    g = cs3d.Solver(cs3d.config(Nx, Ny, Nz, precision=prec, **kw))
+    g.set_initial_conditions(rho, U, V, W, p)
+    tol = 1e-12 if prec == cs3d.FP64 else 2e-6
+    for fix in (False, True):
+        o = g.diag_shear_layer(fix_q12=fix)
+        assert np.allclose(o["uAvg"], a, atol=tol, rtol=0)
+        assert np.allclose(o["rhoAvg"], 1.0 + gy, atol=tol, rtol=0)
+        assert np.allclose(o["uprime2"], c * c, atol=tol, rtol=tol)
+        assert np.allclose(o["vprime2"], d * d, atol=tol, rtol=tol)
+        assert np.allclose(o["wprime2"], e * e + f * f, atol=tol, rtol=tol)
+        assert np.allclose(o["uvprime"], 0.0, atol=tol)
+        assert np.allclose(o["rhoprime2"], 0.0, atol=tol)
+        assert np.allclose(o["uwprime"], c * e + (d * f if not fix else 0.0), atol=tol, rtol=tol)
+        assert np.allclose(o["vwprime"], d * f if fix else 0.0, atol=tol, rtol=tol)
+        # p is uniform and T varies in y only: both moments vanish to rounding
+        assert np.all(o["pprime2"] < 1e-10) and np.all(o["tprime2"] < 1e-10)
