@@ -44,6 +44,7 @@ class Config(ctypes.Structure):
         ("sponge_frac", ctypes.c_double * 3), ("sponge_strength", ctypes.c_double),
         ("sponge_avgT", ctypes.c_double), ("sponge_epsP", ctypes.c_double),
         ("dt_fixed", ctypes.c_double),
+        ("wall_vel", (ctypes.c_double * 3) * 6),
     ]
 
 
@@ -187,6 +188,12 @@ def config(Nx, Ny, Nz, **kw):
             arr = getattr(c, k)
             for i, x in enumerate(v):
                 arr[i] = x
+        elif k == "wall_vel":
+            # {face: (U, V, W)} or a 6 x 3 sequence, faces X0, X1, Y0, Y1, Z0, Z1
+            items = v.items() if isinstance(v, dict) else enumerate(v)
+            for f, uvw in items:
+                for m, x in enumerate(uvw):
+                    c.wall_vel[f][m] = x
         else:
             setattr(c, k, v)
     return c

@@ -43,9 +43,13 @@ int check_config(const cs3d_config *c)
         int k = c->bc_kind[f];
         if (k < CS3D_BC_PERIODIC || k > CS3D_BC_INLET)
             return set_error(CS3D_ERR_ARG, "cs3d_create: bad bc_kind");
-        if (k == CS3D_BC_ADIABATIC_WALL || k == CS3D_BC_MOVING_ADIABATIC_WALL || k == CS3D_BC_INLET)
+        if (k == CS3D_BC_INLET)
             return set_error(CS3D_ERR_UNSUPPORTED,
-                             "cs3d_create: wall and inlet boundaries are not on the GPU path");
+                             "cs3d_create: the reference has no inlet boundary handling");
+        const int n = f < 2 ? c->Nx : f < 4 ? c->Ny : c->Nz;
+        if ((k == CS3D_BC_ADIABATIC_WALL || k == CS3D_BC_MOVING_ADIABATIC_WALL) && n < 7)
+            return set_error(CS3D_ERR_ARG,
+                             "cs3d_create: a wall needs at least 7 points along its direction");
     }
     if (c->rk != CS3D_RK4 && c->rk != CS3D_TVDRK3)
         return set_error(CS3D_ERR_ARG, "cs3d_create: rk must be 3 or 4");

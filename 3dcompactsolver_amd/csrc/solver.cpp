@@ -169,7 +169,10 @@ struct SolverT : SolverBase {
         base.Ny = Ny;
         base.Nz = Nz;
         base.compensated = 0;
-        for (int f = 0; f < 6; f++) base.sponge_face[f] = c.bc_kind[f] == CS3D_BC_SPONGE;
+        for (int f = 0; f < 6; f++) {
+            base.sponge_face[f] = c.bc_kind[f] == CS3D_BC_SPONGE;
+            base.wall_face[f] = is_wall(f);
+        }
         if (sponge) upload_sigma();
         hc(hipStreamSynchronize(st), "init");
     }
@@ -246,13 +249,42 @@ struct SolverT : SolverBase {
     }
 
     // ---------------------------------------------------------------- IC
+    bool is_wall(int f) const
+    {
+        return cfg.bc_kind[f] == CS3D_BC_ADIABATIC_WALL ||
+               cfg.bc_kind[f] == CS3D_BC_MOVING_ADIABATIC_WALL;
+    }
+    bool any_wall() const
+    {
+        for (int f = 0; f < 6; f++)
+            if (is_wall(f)) return true;
+        return false;
+    }
+    // the wall faces in the reference's order: adiabatic X0..Z1, then
+    // moving X0..Z1
+    std::vector<int> wall_order() const
+    {
+        std::vector<int> o;
+        for (int kind : {CS3D_BC_ADIABATIC_WALL, CS3D_BC_MOVING_ADIABATIC_WALL})
+            for (int f = 0; f < 6; f++)
+                if (cfg.bc_kind[f] == kind) o.push_back(f);
+        return o;
+    }
+    // wall velocity of face f: a moving wall's tangential components
+    void wall_velocity(int f, double vel[3]) const
+    {
+        for (int m = 0; m < 3; m++)
+            vel[m] = (cfg.bc_kind[f] == CS3D_BC_MOVING_ADIABATIC_WALL && m != f / 2)
+                         ? cfg.wall_vel[f][m]
+                         : 0.0;
+    }
+
     void set_ic(const double *rho0, const double *U0, const double *V0, const double *W0,
                 const double *p0) override
     {
         // setInitialConditions (CSolver_AWS.cpp:207-234)
         std::vector<double> q[5];
         for (int v = 0; v < 5; v++) q[v].resize((size_t)N);
-        double srho = 0.0, sE = 0.0;
         for (long i = 0; i < N; i++) {
             double rho = rho0[i], U = U0[i], V = V0[i], W = W0[i], p = p0[i];
             q[0][i] = rho;
@@ -260,7 +292,11 @@ struct SolverT : SolverBase {
             q[2][i] = rho * V;
             q[3][i] = rho * W;
             q[4][i] = p / (cfg.gamma - 1.0) + 0.5 * rho * (U * U + V * V + W * W);
-            srho += rho;
+        }
+        if (any_wall()) ic_walls(q, rho0, U0, V0, W0, p0);
+        double srho = 0.0, sE = 0.0;
+        for (long i = 0; i < N; i++) {
+            srho += q[0][i];
             sE += q[4][i];
         }
         rho_ref = srho / (double)N;
@@ -281,6 +317,58 @@ struct SolverT : SolverBase {
         hc(launch_reduce<real>(F, ph, N, 0, dpart, nred, dout, dsc, st), "reduce");
         hc(hipStreamSynchronize(st), "set_ic");
         ic_done = true;
+    }
+
+    // the wall branches of setInitialConditions (CSolver_AWS.cpp:235-495):
+    // on each wall face in turn the wall velocity and T from calcNeumann
+    // (Utils.hpp:52) of the six points inward, then p = rho R T and rhoE
+    // everywhere from the corrected primitives
+    void ic_walls(std::vector<double> *q, const double *rho0, const double *U0, const double *V0,
+                  const double *W0, const double *p0)
+    {
+        std::vector<double> U(U0, U0 + N), V(V0, V0 + N), W(W0, W0 + N), T((size_t)N);
+        for (long i = 0; i < N; i++) T[i] = p0[i] / (rho0[i] * R_gas);
+        const long sxy = (long)Nx * Ny;
+        for (int f : wall_order()) {
+            const int dir = f / 2, hi = f % 2;
+            double vel[3];
+            wall_velocity(f, vel);
+            const long stride = dir == 0 ? 1 : dir == 1 ? Nx : sxy;
+            const long in = hi ? -stride : stride;
+            for (int k = 0; k < Nz; k++)
+                for (int j = 0; j < Ny; j++)
+                    for (int i = 0; i < Nx; i++) {
+                        const int idx = dir == 0 ? i : dir == 1 ? j : k;
+                        const int n = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
+                        if (idx != (hi ? n - 1 : 0)) continue;
+                        const long ip = k * sxy + (long)j * Nx + i;
+                        U[ip] = vel[0];
+                        V[ip] = vel[1];
+                        W[ip] = vel[2];
+                        for (int m = 0; m < 3; m++) q[1 + m][ip] = q[0][ip] * vel[m];
+                        T[ip] = (T[ip + in] * 360.0 - T[ip + 2 * in] * 450.0 +
+                                 T[ip + 3 * in] * 400.0 - T[ip + 4 * in] * 225.0 +
+                                 T[ip + 5 * in] * 72.0 - T[ip + 6 * in] * 10.0) /
+                                147.0;
+                    }
+        }
+        for (long i = 0; i < N; i++) {
+            const double rho = q[0][i], p = rho * R_gas * T[i];
+            q[4][i] = p / (cfg.gamma - 1.0) + 0.5 * rho * (U[i] * U[i] + V[i] * V[i] + W[i] * W[i]);
+        }
+    }
+
+    // preStepBCHandling (CSolver_AWS.cpp:553-826) on the stage state
+    void pre_step_walls(const StageArgs &sa)
+    {
+        for (int f : wall_order()) {
+            WallFace w;
+            w.dir = f / 2;
+            w.hi = f % 2;
+            wall_velocity(f, w.vel);
+            w.R = R_gas;
+            timed(5, [&] { hc(launch_wall_face<real>(F, ph, sa, w, st), "walls"); });
+        }
     }
 
     // q holds full conserved variables (FP64); stored in perturbation form
@@ -353,6 +441,7 @@ struct SolverT : SolverBase {
         sa.mode = mode;
         sa.diag = diag;
         sa.stale = (rkStep == 1 && stale_pending) ? 1 : 0;
+        if (any_wall()) pre_step_walls(sa);
         // y, then z, then x (the last sweep assembles and updates)
         for (int d : {1, 2, 0})
             timed(d, [&] {
@@ -725,6 +814,7 @@ struct SolverT : SolverBase {
             if (cfg.bc_type[d] != CS3D_PERIODIC_SOLVE)
                 throw std::invalid_argument("the adjoint needs periodic lines in every direction");
         if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
+        if (any_wall()) throw std::invalid_argument("the adjoint does not cover walls");
         if (!A[0]) {
             for (int i = 0; i < ADJ_FIELDS; i++) A[i] = adj_alloc_field();
             real **sets[6] = {lamb, vb5, vb5b, qb5, la5, sc5};

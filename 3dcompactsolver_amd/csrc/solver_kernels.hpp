@@ -65,6 +65,14 @@ struct StageArgs {
     int Nx, Ny, Nz;
     int compensated;
     int stale;         // sweeps take primitives on sponge planes from qstale
+    int wall_face[6];  // 1 where a face is a wall (adiabatic or moving)
+};
+
+// one wall face for k_wall_face (walls.hip)
+struct WallFace {
+    int dir, hi;       // normal direction, 0: the i/j/k = 0 face, 1: the last
+    double vel[3];     // wall velocity, normal component zero
+    double R;          // gas constant
 };
 
 // XCD-aware workgroup order: the hardware hands consecutive workgroups to
@@ -112,6 +120,10 @@ hipError_t launch_dt(StepScalars *sc, double cfl, double dt_fixed, hipStream_t s
 template <typename real>
 hipError_t launch_plane_rows(const Fields<real> &F, const Phys<real> &ph, int Nx, int Ny, int Nz,
                              int pass, const double *mean, double *rows, hipStream_t st);
+// preStepBCHandling of one wall face on the stage state (walls.hip)
+template <typename real>
+hipError_t launch_wall_face(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,
+                            const WallFace &w, hipStream_t st);
 template <typename real>
 hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,
                          const StepScalars *sc, double avgT, double epsP, double spongeP,

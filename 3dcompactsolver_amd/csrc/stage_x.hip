@@ -9,8 +9,8 @@
 // x-derivatives, :991-1003 and :1161-1192; operators along different axes
 // commute).  Then, per point, the terms of solveContinuity .. solveEnergy
 // (:1246-1504) that are products of first derivatives, the sponge source
-// (twice in the energy, :1491/:1495), the zeroing of sponge boundary planes
-// (postStepBCHandling, :1606-1670), and the stage update of
+// (twice in the energy, :1491/:1495), the wall and sponge plane rules of
+// postStepBCHandling (:1506-1670), and the stage update of
 // RK4::updateConservedData (RK4.hpp:45-111) or TVDRK3::updateConservedData
 // (TVDRK3.hpp:225-260).  Solves: a batch of 9 (5 flux + 4 mixed), then
 // 4 first + 4 second derivatives.
@@ -59,6 +59,13 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
                             (sa.sponge_face[3] && lj == sa.Ny - 1) ||
                             (sa.sponge_face[4] && lk == 0) ||
                             (sa.sponge_face[5] && lk == sa.Nz - 1);
+    // whole x lines on a y or z wall: the normal direction of the last such
+    // face (z faces come after y faces), -1 when none
+    const int line_wall = ((sa.wall_face[4] && lk == 0) || (sa.wall_face[5] && lk == sa.Nz - 1))
+                              ? 2
+                          : ((sa.wall_face[2] && lj == 0) || (sa.wall_face[3] && lj == sa.Ny - 1))
+                              ? 1
+                              : -1;
     if (STALE) {
         // primitives on sponge planes from the pre-reset copy
 #pragma unroll
@@ -237,6 +244,19 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
                            (sa.sponge_face[1] && ii == sa.Nx - 1);
 #pragma unroll
         for (int v = 0; v < 5; v++) k[v][t] = plane ? real(0) : dt * rhs_[v];
+        // wall planes keep only the normal continuity flux term of the last
+        // wall face through the point (faces X0..Z1 in turn); a sponge plane
+        // stays zero
+        int wn = line_wall;
+        if (wn < 0 && ((sa.wall_face[0] && ii == 0) || (sa.wall_face[1] && ii == sa.Nx - 1)))
+            wn = 0;
+        if (wn >= 0 && !plane) {
+            real cn = R0[t];
+            if (wn > 0) cn = valid ? ldo(wn == 1 ? F.R[0] : F.Rz[0], win.mid(t)) : real(0);
+            k[0][t] = dt * cn;
+#pragma unroll
+            for (int v = 1; v < 5; v++) k[v][t] = real(0);
+        }
         if (sa.diag && valid) {
             // per-point inputs of the step diagnostics that come from this
             // stage's derivatives (CSolver_AWS.cpp:2190-2225)

@@ -1,0 +1,119 @@
+// walls.hip -- adiabatic and moving walls before the sweeps of a stage
+// (preStepBCHandling, CSolver_AWS.cpp:553-826).
+//
+// Every point of a wall face plane of the stage state (*1 at the first
+// stage, *k after) gets the wall velocity in its momenta (zero, or the
+// moving wall's tangential components, CSolver_AWS.hpp:105-107) and the
+// energy of p = rho R T_w, T_w = calcNeumann (Utils.hpp:52) of the
+// temperatures of the six points inward: the value that makes the one-sided
+// 7-point normal derivative vanish.  The density is kept.  The sweeps then
+// form the wall primitives from these conserved values, as the reference
+// differentiates its wall-corrected primitive arrays.
+//
+// One launch per face, in the reference's order (adiabatic faces X0..Z1,
+// then moving faces X0..Z1): a point on an edge reads the values an earlier
+// face left and keeps the last face's.
+//
+// At the first stage after a sponge update the reference's temperature
+// array on the sponge face planes still belongs to the state before the
+// update (see stage_x.hip); the neighbour temperatures are taken from the
+// pre-update copy there, and a wall point on such a plane writes its new
+// values into that copy as well, since the reference overwrites its
+// primitive arrays at wall points.
+#include "stage_common.hpp"
+
+namespace cs3d {
+
+__device__ __forceinline__ bool sponge_plane(const StageArgs &sa, int i, int j, int k)
+{
+    return (sa.sponge_face[0] && i == 0) || (sa.sponge_face[1] && i == sa.Nx - 1) ||
+           (sa.sponge_face[2] && j == 0) || (sa.sponge_face[3] && j == sa.Ny - 1) ||
+           (sa.sponge_face[4] && k == 0) || (sa.sponge_face[5] && k == sa.Nz - 1);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph, StageArgs sa,
+                                                   WallFace w)
+{
+    const int Nx = sa.Nx, Ny = sa.Ny, Nz = sa.Nz;
+    const int na = w.dir == 0 ? Ny : Nx, nb = w.dir == 2 ? Ny : Nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int a = t % na, b = t / na;
+    int ijk[3];
+    if (w.dir == 0) {
+        ijk[0] = w.hi ? Nx - 1 : 0;
+        ijk[1] = a;
+        ijk[2] = b;
+    } else if (w.dir == 1) {
+        ijk[0] = a;
+        ijk[1] = w.hi ? Ny - 1 : 0;
+        ijk[2] = b;
+    } else {
+        ijk[0] = a;
+        ijk[1] = b;
+        ijk[2] = w.hi ? Nz - 1 : 0;
+    }
+    const long sxy = (long)Nx * Ny;
+    const long ip = (long)ijk[2] * sxy + (long)ijk[1] * Nx + ijk[0];
+    const long stride = w.dir == 0 ? 1 : w.dir == 1 ? (long)Nx : sxy;
+    const long in = w.hi ? -stride : stride;
+    real *const *q = sa.rkStep > 1 ? F.qk : F.q;
+    // temperatures (perturbation form T - T0s) of the six points inward;
+    // the wall direction holds at least 7 points
+    real Tn[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const long o = ip + (long)(m + 1) * in;
+        int nijk[3] = {ijk[0], ijk[1], ijk[2]};
+        nijk[w.dir] += w.hi ? -(m + 1) : (m + 1);
+        const bool st = sa.stale && sponge_plane(sa, nijk[0], nijk[1], nijk[2]);
+        real *const *src = st ? F.qstale : q;
+        real ir, U, V, Wv, pp;
+        prim(ph, src[0][o], src[1][o], src[2][o], src[3][o], src[4][o], ir, U, V, Wv, pp, Tn[m]);
+    }
+    // calcNeumann: its weights sum to one, so it acts on T - T0s alike
+    const real Tw = (Tn[0] * real(360) - Tn[1] * real(450) + Tn[2] * real(400) -
+                     Tn[3] * real(225) + Tn[4] * real(72) - Tn[5] * real(10)) /
+                    real(147);
+    const real rp = q[0][ip];
+    const real rho = ph.rho_ref + rp;
+    const real u = (real)w.vel[0], v = (real)w.vel[1], ww = (real)w.vel[2];
+    // p - p_ref = rho R T_w - rho_ref R T0s
+    const real pp = rp * ph.RT0s + rho * (real)w.R * Tw;
+    // rhoE - rhoE_ref (p_ref = (gamma - 1) rhoE_ref)
+    const real re = pp / ph.gm1 + real(0.5) * rho * (u * u + v * v + ww * ww);
+    const real m1 = rho * u, m2 = rho * v, m3 = rho * ww;
+    q[1][ip] = m1;
+    q[2][ip] = m2;
+    q[3][ip] = m3;
+    q[4][ip] = re;
+    if (sa.stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) {
+        F.qstale[0][ip] = rp;
+        F.qstale[1][ip] = m1;
+        F.qstale[2][ip] = m2;
+        F.qstale[3][ip] = m3;
+        F.qstale[4][ip] = re;
+    }
+}
+
+template <typename real>
+hipError_t launch_wall_face(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,
+                            const WallFace &w, hipStream_t st)
+{
+    if (w.dir < 0 || w.dir > 2) return hipErrorInvalidValue;
+    const int n = w.dir == 0 ? sa.Nx : w.dir == 1 ? sa.Ny : sa.Nz;
+    if (n < 7) return hipErrorInvalidValue;
+    const long pts = w.dir == 0 ? (long)sa.Ny * sa.Nz
+                                : w.dir == 1 ? (long)sa.Nx * sa.Nz : (long)sa.Nx * sa.Ny;
+    const int nb = (int)((pts + 255) / 256);
+    hipLaunchKernelGGL((k_wall_face<real>), dim3(nb), dim3(256), 0, st, F, ph, sa, w);
+    return hipGetLastError();
+}
+
+template hipError_t launch_wall_face<float>(const Fields<float> &, const Phys<float> &,
+                                            const StageArgs &, const WallFace &, hipStream_t);
+template hipError_t launch_wall_face<double>(const Fields<double> &, const Phys<double> &,
+                                             const StageArgs &, const WallFace &, hipStream_t);
+
+}  // namespace cs3d

@@ -32,7 +32,11 @@ enum {
 
 /* BC.hpp:10 BCType */
 enum { CS3D_PERIODIC_SOLVE = 0, CS3D_DIRICHLET_SOLVE = 1 };
-/* BC.hpp:11 BCKind */
+/* BC.hpp:11 BCKind.  Walls (CSolver_AWS.cpp:553-826, 1506-1605): no-slip
+ * or moving, adiabatic (the wall temperature makes the one-sided 7-point
+ * normal derivative vanish), at least 7 points along the wall's direction.
+ * INLET is declared by the reference but handled nowhere in it; cs3d_create
+ * refuses it. */
 enum {
     CS3D_BC_PERIODIC = 0,
     CS3D_BC_SPONGE = 1,
@@ -80,6 +84,11 @@ typedef struct {
     double sponge_avgT;       /* 15.0 */
     double sponge_epsP;       /* 0.005 */
     double dt_fixed;          /* > 0: constant dt instead of the CFL dt */
+    /* moving-wall velocity (U, V, W) per face X0, X1, Y0, Y1, Z0, Z1, used
+     * on CS3D_BC_MOVING_ADIABATIC_WALL faces: the tangential components are
+     * CSolver_AWS's X0WallV, X0WallW, ..., Z1WallV (CSolver_AWS.hpp:105-107,
+     * default 0); the normal component is ignored (always 0 on a wall) */
+    double wall_vel[6][3];
 } cs3d_config;
 
 typedef struct cs3d_solver cs3d_solver;

@@ -329,6 +329,10 @@ class CSolver_AWS : public AbstractCSolver {
     int blocksize;
     double time = 0.0, dt = 0.0;
     int timeStep = 0;
+    // moving-wall velocities (CSolver_AWS.hpp:105-107, zero by default)
+    double X0WallV = 0.0, X0WallW = 0.0, X1WallV = 0.0, X1WallW = 0.0;
+    double Y0WallU = 0.0, Y0WallW = 0.0, Y1WallU = 0.0, Y1WallW = 0.0;
+    double Z0WallU = 0.0, Z0WallV = 0.0, Z1WallU = 0.0, Z1WallV = 0.0;
     // GPU-path options (no reference counterpart)
     int precision = CS3D_FP64;       // CS3D_FP32: FP32 bulk arithmetic
     bool tgvDiagnostics = false;     // append taylorgreen.out every step
@@ -395,6 +399,11 @@ class CSolver_AWS : public AbstractCSolver {
         c.Pr = ig->Pr;
         c.rk = rkOrder == 3 ? CS3D_TVDRK3 : CS3D_RK4;
         c.precision = precision;
+        const double wv[6][3] = {{0.0, X0WallV, X0WallW}, {0.0, X1WallV, X1WallW},
+                                 {Y0WallU, 0.0, Y0WallW}, {Y1WallU, 0.0, Y1WallW},
+                                 {Z0WallU, Z0WallV, 0.0}, {Z1WallU, Z1WallV, 0.0}};
+        for (int f = 0; f < 6; f++)
+            for (int m = 0; m < 3; m++) c.wall_vel[f][m] = wv[f][m];
         if (h) cs3d_destroy(h);
         h = nullptr;
         check(cs3d_create(&c, &h), "setInitialConditions (create)");

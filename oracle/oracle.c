@@ -465,14 +465,23 @@ or_solver *or_create(const or_config *cfg)
     or_solver *s = (or_solver *)calloc(1, sizeof(or_solver));
     s->c = *cfg;
     const or_config *c = &s->c;
+    /* BC.hpp:11 declares INLET, but CSolver_AWS handles no such face */
     for (int f = 0; f < 6; f++)
-        if (c->bc_kind[f] == OR_ADIABATIC_WALL ||
-            c->bc_kind[f] == OR_MOVING_ADIABATIC_WALL ||
-            c->bc_kind[f] == OR_INLET) {
-            fprintf(stderr, "oracle: wall/inlet boundaries are not restated\n");
+        if (c->bc_kind[f] == OR_INLET) {
+            fprintf(stderr, "oracle: inlet boundaries have no reference handling\n");
             free(s);
             return NULL;
         }
+    /* the wall value takes the 6 points next to the face */
+    for (int f = 0; f < 6; f++) {
+        int n = f < 2 ? c->Nx : f < 4 ? c->Ny : c->Nz;
+        if ((c->bc_kind[f] == OR_ADIABATIC_WALL ||
+             c->bc_kind[f] == OR_MOVING_ADIABATIC_WALL) && n < 7) {
+            fprintf(stderr, "oracle: a wall direction needs at least 7 points\n");
+            free(s);
+            return NULL;
+        }
+    }
     s->N = (long)c->Nx * c->Ny * c->Nz;
     /* Domain.hpp:30-45 */
     s->x = alloc_field(c->Nx);
@@ -548,7 +557,93 @@ static inline double ig_rhoE(const or_solver *s, double rho, double p,
     return p / (s->c.gamma - 1.0) + 0.5 * rho * (U * U + V * V + W * W);
 }
 
-/* CSolver_AWS.cpp:199-234, 486-495 (the wall branches are not restated) */
+/* Utils.hpp:52 */
+double or_calc_neumann(double f1, double f2, double f3, double f4, double f5,
+                       double f6)
+{
+    return (f1 * 360.0 - f2 * 450.0 + f3 * 400.0 - f4 * 225.0 + f5 * 72.0 -
+            f6 * 10.0) /
+           147.0;
+}
+
+static int is_wall(int kind)
+{
+    return kind == OR_ADIABATIC_WALL || kind == OR_MOVING_ADIABATIC_WALL;
+}
+
+/* One wall face f (X0, X1, Y0, Y1, Z0, Z1) of the state qs and the
+ * primitive arrays: the wall velocity (zero, or the moving wall's
+ * tangential components) in U, V, W and the momenta, and T from calcNeumann
+ * of the 6 points inward.  pre = 1 (preStepBCHandling,
+ * CSolver_AWS.cpp:553-826) also sets p = rho R T and rhoE on the face;
+ * pre = 0 is the initial-condition form (:235-484), which recomputes p and
+ * rhoE everywhere afterwards. */
+static void wall_face(or_solver *s, int f, double **qs, int pre)
+{
+    const or_config *c = &s->c;
+    const int Nx = c->Nx, Ny = c->Ny, Nz = c->Nz;
+    const int dir = f / 2, hi = f % 2;
+    const long sz = (long)Nx * Ny;
+    const long stride = dir == 0 ? 1 : dir == 1 ? Nx : sz;
+    const long in = hi ? -stride : stride;
+    const int moving = c->bc_kind[f] == OR_MOVING_ADIABATIC_WALL;
+    double uw[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < 3; m++)
+        if (moving && m != dir) uw[m] = c->wall_vel[f][m];
+    const int na = dir == 0 ? Ny : Nx, nb = dir == 2 ? Ny : Nz;
+    for (int b = 0; b < nb; b++)
+        for (int a = 0; a < na; a++) {
+            int i, j, k;
+            if (dir == 0) {
+                i = hi ? Nx - 1 : 0; j = a; k = b;
+            } else if (dir == 1) {
+                i = a; j = hi ? Ny - 1 : 0; k = b;
+            } else {
+                i = a; j = b; k = hi ? Nz - 1 : 0;
+            }
+            long ip = (long)k * sz + (long)j * Nx + i;
+            double rho = qs[0][ip];
+            U_[ip] = uw[0];
+            V_[ip] = uw[1];
+            W_[ip] = uw[2];
+            for (int m = 0; m < 3; m++)
+                qs[1 + m][ip] = (moving && m != dir) ? rho * uw[m] : 0.0;
+            T_[ip] = or_calc_neumann(T_[ip + in], T_[ip + 2 * in], T_[ip + 3 * in],
+                                     T_[ip + 4 * in], T_[ip + 5 * in],
+                                     T_[ip + 6 * in]);
+            if (pre) {
+                P_[ip] = rho * s->R_gas * T_[ip];
+                qs[4][ip] = ig_rhoE(s, rho, P_[ip], U_[ip], V_[ip], W_[ip]);
+            }
+        }
+}
+
+/* the faces in the reference's order: adiabatic walls X0..Z1, then moving
+ * walls X0..Z1 (an edge takes the last face's values) */
+static int wall_faces(or_solver *s, double **qs, int pre)
+{
+    int any = 0;
+    for (int f = 0; f < 6; f++)
+        if (s->c.bc_kind[f] == OR_ADIABATIC_WALL) {
+            wall_face(s, f, qs, pre);
+            any = 1;
+        }
+    for (int f = 0; f < 6; f++)
+        if (s->c.bc_kind[f] == OR_MOVING_ADIABATIC_WALL) {
+            wall_face(s, f, qs, pre);
+            any = 1;
+        }
+    return any;
+}
+
+/* preStepBCHandling, CSolver_AWS.cpp:553-826: on the stage's state (*1 at
+ * the first stage, *k after) */
+static void pre_step_bc(or_solver *s)
+{
+    wall_faces(s, s->rkStep == 1 ? s->q[0] : s->q[1], 1);
+}
+
+/* CSolver_AWS.cpp:199-504 */
 void or_set_ic(or_solver *s, const double *rho0, const double *U0,
                const double *V0, const double *W0, const double *p0)
 {
@@ -572,6 +667,16 @@ void or_set_ic(or_solver *s, const double *rho0, const double *U0,
         MU_[ip] = ig_mu(s, T_[ip]);
         AMU_[ip] = ig_amu(s, T_[ip]);
         SOS_[ip] = ig_sos(s, q1[0][ip], P_[ip]);
+    }
+    if (wall_faces(s, q1, 0)) {
+        /* :486-495: pressure, speed of sound and energy everywhere from the
+         * (wall-corrected) temperature */
+#pragma omp parallel for
+        for (long ip = 0; ip < N; ip++) {
+            P_[ip] = q1[0][ip] * s->R_gas * T_[ip];
+            SOS_[ip] = ig_sos(s, q1[0][ip], P_[ip]);
+            q1[4][ip] = ig_rhoE(s, q1[0][ip], P_[ip], U_[ip], V_[ip], W_[ip]);
+        }
     }
     if (s->sponge) {
         for (int v = 0; v < 5; v++)
@@ -791,12 +896,30 @@ static void solve_eqn_set(or_solver *s)
     }
 }
 
-/* postStepBCHandling sponge planes, CSolver_AWS.cpp:1606-1670 */
+/* postStepBCHandling, CSolver_AWS.cpp:1506-1670: wall planes keep the
+ * normal continuity flux term only (X0..Z1 in turn), then sponge planes are
+ * zeroed */
 static void post_step_bc(or_solver *s)
 {
     const or_config *c = &s->c;
     int Nx = c->Nx, Ny = c->Ny, Nz = c->Nz;
     double **k2 = s->q[2];
+    const double dt = s->dt;
+    for (int f = 0; f < 6; f++) {
+        if (!is_wall(c->bc_kind[f])) continue;
+        int dir = f / 2, hi = f % 2;
+        const double *cont = s->d[dir == 0 ? OR_D_contX : dir == 1 ? OR_D_contY : OR_D_contZ];
+        for (int k = 0; k < Nz; k++)
+            for (int j = 0; j < Ny; j++)
+                for (int i = 0; i < Nx; i++) {
+                    int idx = dir == 0 ? i : dir == 1 ? j : k;
+                    int n = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
+                    if (idx != (hi ? n - 1 : 0)) continue;
+                    long ip = (long)k * Nx * Ny + (long)j * Nx + i;
+                    k2[0][ip] = -dt * cont[ip];
+                    for (int v = 1; v < 5; v++) k2[v][ip] = 0.0;
+                }
+    }
     for (int f = 0; f < 6; f++) {
         if (c->bc_kind[f] != OR_SPONGE) continue;
         int dir = f / 2, hi = f % 2;
@@ -815,6 +938,7 @@ static void post_step_bc(or_solver *s)
 void or_rhs(or_solver *s, int rkStep)
 {
     s->rkStep = rkStep;
+    pre_step_bc(s);
     pre_step_derivatives(s);
     solve_eqn_set(s);
     post_step_bc(s);
@@ -913,6 +1037,7 @@ void or_sub_step(or_solver *s, int rkStep, int rkLast)
 {
     s->rkStep = rkStep;
     s->rkLast = rkLast;
+    pre_step_bc(s);
     pre_step_derivatives(s);
     solve_eqn_set(s);
     post_step_bc(s);

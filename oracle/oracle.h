@@ -15,6 +15,10 @@
  *     wavenumber / transfer-function closed forms and polynomial exactness.
  *   - ideal gas, sponge sigma, RK4/TVDRK3 updates: against the reference
  *     headers compiled into oracle/_ref.
+ *   - calcNeumann: against the reference's Utils.hpp compiled into
+ *     oracle/_ref and polynomial exactness; wall handling: quiescent-box
+ *     invariance, the no-slip/adiabatic values on the wall planes, and
+ *     mass conservation in a closed box.
  *   - RHS assembly / full step: invariants (uniform state -> zero RHS, mass
  *     conservation, filter commutation), Taylor-Green closed forms at t=0, and
  *     the reference's recorded taylorgreen.out first row (64^3).
@@ -76,6 +80,10 @@ typedef struct {
     double sponge_strength;     /* 6.0 */
     double sponge_avgT;         /* 15.0 */
     double sponge_epsP;         /* 0.005 */
+    /* moving-wall velocity (U, V, W) per face X0..Z1: the tangential
+     * components are CSolver_AWS.hpp:105-107 (X0WallV, X0WallW, ...,
+     * default 0); the normal component is ignored */
+    double wall_vel[6][3];
 } or_config;
 
 void or_config_default(or_config *c, int Nx, int Ny, int Nz);
@@ -124,6 +132,11 @@ void or_diag_turb(or_solver *s, double out[11]);
 
 /* sponge strength field (SpongeBC.hpp:35-154), XYZ, length N (NULL if none) */
 const double *or_sponge_sigma(or_solver *s);
+
+/* calcNeumann (Utils.hpp:52): the boundary value that makes the one-sided
+ * 7-point first derivative vanish, from the 6 values next to the boundary */
+double or_calc_neumann(double f1, double f2, double f3, double f4, double f5,
+                       double f6);
 
 #ifdef __cplusplus
 }

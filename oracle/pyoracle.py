@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _dp = ctypes.POINTER(ctypes.c_double)
 
 PERIODIC_SOLVE, DIRICHLET_SOLVE = 0, 1
-PERIODIC, SPONGE = 0, 1
+PERIODIC, SPONGE, ADIABATIC_WALL, MOVING_ADIABATIC_WALL = 0, 1, 2, 3
 
 
 def _ptr(a):
@@ -32,6 +32,7 @@ class OrConfig(ctypes.Structure):
         ("rk", ctypes.c_int), ("sponge_frac", ctypes.c_double * 3),
         ("sponge_strength", ctypes.c_double), ("sponge_avgT", ctypes.c_double),
         ("sponge_epsP", ctypes.c_double),
+        ("wall_vel", (ctypes.c_double * 3) * 6),
     ]
 
 
@@ -93,6 +94,8 @@ def _load(path):
                                       ctypes.POINTER(_dp)]
     L.or_field_op.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_double,
                                                    ctypes.c_int, _dp, _dp]
+    L.or_calc_neumann.restype = ctypes.c_double
+    L.or_calc_neumann.argtypes = [ctypes.c_double] * 6
     return L
 
 
@@ -141,6 +144,11 @@ def field_op(kind, direction, bctype, field, dh=1.0, alpha_f=0.49, ref_layout=Fa
     lib().or_field_op(kind, direction, bctype, Nx, Ny, Nz, dh, alpha_f,
                       1 if ref_layout else 0, _ptr(f), _ptr(out))
     return out
+
+
+def calc_neumann(f):
+    """calcNeumann of the 6 values next to a boundary"""
+    return lib().or_calc_neumann(*[float(x) for x in f])
 
 
 def rk_update(rk, rk_step, arrays):
@@ -261,6 +269,12 @@ def make_config(Nx, Ny, Nz, **kw):
             arr = getattr(c, k)
             for i, x in enumerate(v):
                 arr[i] = x
+        elif k == "wall_vel":
+            # {face: (U, V, W)} or a 6 x 3 sequence
+            items = v.items() if isinstance(v, dict) else enumerate(v)
+            for f, uvw in items:
+                for m, x in enumerate(uvw):
+                    c.wall_vel[f][m] = x
         else:
             setattr(c, k, v)
     return c
@@ -289,5 +303,7 @@ def ref_lib():
         L.ref_sponge_sigma.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [
             ctypes.POINTER(ctypes.c_int), _dp]
         L.ref_rk_update.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(_dp)]
+        L.ref_calc_neumann.restype = ctypes.c_double
+        L.ref_calc_neumann.argtypes = [ctypes.c_double] * 6
         _ref = L
     return _ref

@@ -175,4 +175,10 @@ void ref_rk_update(int rk, int rkStep, int Nx, int Ny, int Nz,
     }
 }
 
+// Utils.hpp:52
+double ref_calc_neumann(double f1, double f2, double f3, double f4, double f5, double f6)
+{
+    return calcNeumann(f1, f2, f3, f4, f5, f6);
+}
+
 }  // extern "C"

@@ -51,10 +51,16 @@ def _create(cfg):
     return rc, h
 
 
-def test_rejects_walls_before_touching_the_gpu(hiplib):
-    c = cs3d.config(16, 16, 16, bc_type=(0, 1, 0), bc_kind=(0, 0, 2, 2, 0, 0))
+def test_rejects_inlet_and_short_wall_lines_before_touching_the_gpu(hiplib):
+    # BC.hpp:11 declares INLET; CSolver_AWS handles no such face
+    c = cs3d.config(16, 16, 16, bc_type=(0, 1, 0), bc_kind=(0, 0, 4, 2, 0, 0))
     rc, h = _create(c)
     assert rc == cs3d.ERR_UNSUPPORTED and not h
+    assert b"inlet" in hiplib.cs3d_last_error()
+    # the wall temperature takes the 6 points next to the wall
+    c = cs3d.config(16, 6, 16, bc_type=(0, 1, 0), bc_kind=(0, 0, 2, 2, 0, 0))
+    rc, h = _create(c)
+    assert rc == cs3d.ERR_ARG and not h
     assert b"wall" in hiplib.cs3d_last_error()
 
 
