@@ -28,8 +28,9 @@
 
 namespace cs3d {
 
-// QS: the update needs the stage state at the point (sponge source, or the
-// TVDRK3 combinations of stages 2 and 3)
+// QS: the stage state at the point stays in registers for the update (the
+// sponge source, the TVDRK3 combinations of stages 2 and 3, and *1 itself
+// at the first stage, where the stage state is *1)
 template <typename real, int M, bool UNI, bool STALE, bool QS>
 __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
@@ -45,6 +46,8 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const real *q4 = use_k ? F.qk[4] : F.q[4];
     real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
+    // (halos taken from neighbouring lanes with ds_bpermute instead of the
+    // chunk loads measured 15% slower: the permutes wait for each load)
     win.load(q0, a0);
     win.load(q1, a1);
     win.load(q2, a2);
@@ -299,7 +302,12 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         real qn1[M], o[M];
-        win.load_seg(F.q[v], qn1);
+        if (QS && first) {
+#pragma unroll
+            for (int t = 0; t < M; t++) qn1[t] = qs_[v][t];
+        } else {
+            win.load_seg(F.q[v], qn1);
+        }
         if (sa.rk == 4) {
             // RK4.hpp:45-111: *2 accumulates the stages, *k is the next
             // stage state
@@ -332,7 +340,7 @@ static hipError_t launch_x3(const OpDev<real> *d1, const OpDev<real> *d2, const 
                             const StageArgs &sa, const StepScalars *sc, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
-    if (F.sigma || sa.rk == 3)
+    if (F.sigma || sa.rk == 3 || sa.rkStep == 1)
         hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true>), grid, block, lg.shm, st,
                            d1, d2, lm, F, ph, sa, sc);
     else

@@ -8,8 +8,9 @@ isothermal density; FP32 bulk arithmetic.  The same workload in FP64 is
 timed beside it.  Inputs are resident in HBM before the timed region.
 
 value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
-% roof = value * 322 B / 8 TB/s (322 B = algorithmic bytes per point-stage
-         of RK4 + filter in FP32, 644 B in FP64).
+% roof = value * 307 B / 8 TB/s (307 B = algorithmic bytes per point-stage
+         of RK4 + filter in FP32, 614 B in FP64: 307 field accesses per
+         point and step, listed in STAGE_ACCESSES).
 roofline = the dominant kernel class, timed with HIP events recorded on the
          solver's own stream around every launch of the timed region.
 adjoint = cost of the kinetic-energy gradient through 8 RK4 steps of the
@@ -36,16 +37,31 @@ import cs3d  # noqa: E402
 
 METRIC = "Mpoint-RK-stages/s at 256^3 FP32 vs FP64 (% HBM roof); adjoint/forward ratio"
 HBM_PEAK = 8.0e12
-# algorithmic field accesses per point, per launch of each kernel class; the
-# sweeps of a stage run y, z, x (first y sweep 5 reads + 9 writes, middle z
-# sweep 7 + 9, last x sweep with the assembly and the RK4 update 33 + 10),
-# filter sweep 5 + 5, post-step reduction 5 reads
-ACCESSES = {"x_sweep": 43, "y_sweep": 14, "z_sweep": 16, "filter": 10, "reduce": 5}
+# algorithmic field accesses per point of every launch in one RK4 step with
+# the filter (no sponge).  The sweeps of a stage run y, z, x: the y sweep
+# reads the 5 conserved fields and writes Uy Vy Wy Ty and its 5 partial
+# right-hand sides (5 + 9); the z sweep also reads Vy, Wy and writes
+# Uz Vz Wz Tz and its own 5 partial sums (7 + 9); the x sweep reads the
+# conserved fields, Uy Vy Uz Wz, Wy Ty Vz Tz and both partial sums (23),
+# then *1 and *2 from the second stage on (at the first stage *1 is the
+# stage state it already holds, and *2 starts from it), and writes *2 and,
+# before the last stage, *k.  Each filter sweep reads and writes the 5
+# conserved fields; the CFL reduction rides on the last one.
+STAGE_ACCESSES = {
+    "y_sweep": [14, 14, 14, 14],
+    "z_sweep": [16, 16, 16, 16],
+    "x_sweep": [23 + 10, 33 + 10, 33 + 10, 33 + 5],
+    "filter": [10, 10, 10],
+}
+# per launch, averaged over the launches of a step
+ACCESSES = {k: sum(v) / len(v) for k, v in STAGE_ACCESSES.items()}
+ACCESSES_PER_STEP = sum(sum(v) for v in STAGE_ACCESSES.values())  # 307
 # HBM accesses per point actually moved per launch (reads, writes), FP32 at
 # 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
 # 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
 MEASURED_TRAFFIC = {"x_sweep": (31.71, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.03, 9.01)}
-BYTES_PER_POINT_STAGE = {cs3d.FP32: 322, cs3d.FP64: 644}
+# bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
+BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP * 4 / 4, cs3d.FP64: ACCESSES_PER_STEP * 8 / 4}
 
 
 def tgv_ic(N, L, mach=0.1):
@@ -113,7 +129,7 @@ def roofline(prof, N, prec):
     best = max((k for k in ACCESSES), key=lambda k: prof[k][0])
     tot_ms, cnt = prof[best]
     avg_s = tot_ms / cnt * 1e-3
-    alg_bytes = ACCESSES[best] * s * N ** 3
+    alg_bytes = int(ACCESSES[best] * s * N ** 3)
     achieved = alg_bytes / avg_s / 1e9
     traffic = None
     if prec == cs3d.FP32 and N == 256 and best in MEASURED_TRAFFIC:
