@@ -189,6 +189,12 @@ __device__ __forceinline__ void sto(real *__restrict__ f, unsigned boff, real v)
 // 16-byte vector width VW): the window, widened to whole VW-aligned chunks,
 // is read with one 16-byte load per chunk, and the segment is stored the same
 // way.  With n a multiple of VW no chunk straddles the periodic wrap.
+//
+// Interleaved fields: a pair field holds two values per point (a, b) and a
+// quad field four, at point index * 2 (* 4) elements, so the offsets above
+// scale by 2 (4).  Column lanes then move 2 (4) values per point with one
+// load or store, which widens the contiguous run of a wavefront row from
+// 16 lines * 4 bytes to 128 (256) bytes.
 template <typename real, int M, int HW, bool VEC>
 struct Win {
     static constexpr int W = M + 2 * HW;
@@ -279,7 +285,146 @@ struct Win {
                 if (t < len) sto(f, off[HW + t], r[t]);
         }
     }
+
+    // ---- pair fields
+    typedef real vec2 __attribute__((ext_vector_type(2)));
+    typedef real vec4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ const char *pair_at(const real *f, unsigned o)
+    {
+        return (const char *)f + 2ull * o;
+    }
+    // the window of both values
+    __device__ __forceinline__ void load_pair(const real *__restrict__ f, real (&wa)[W],
+                                              real (&wb)[W]) const
+    {
+        if (VEC) {
+            // a chunk of VW points is 2 * VW values: two 16-byte loads
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const vec v = *(const vec *)(pair_at(f, off[c]) + 16 * h);
+#pragma unroll
+                    for (int e = 0; e < VW; e++) {
+                        const int t = c * VW + h * (VW / 2) + e / 2 - D;
+                        if (t >= 0 && t < W) {
+                            if (e % 2)
+                                wb[t] = v[e];
+                            else
+                                wa[t] = v[e];
+                        }
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < W; t++) {
+                const vec2 v = *(const vec2 *)pair_at(f, off[t]);
+                wa[t] = v[0];
+                wb[t] = v[1];
+            }
+        }
+    }
+    // the segment's own M points of both values
+    __device__ __forceinline__ void load_seg_pair(const real *__restrict__ f, real (&ra)[M],
+                                                  real (&rb)[M]) const
+    {
+        if (VEC) {
+#pragma unroll
+            for (int c = 0; c < M / VW; c++) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const vec v = *(const vec *)(pair_at(f, off[LO / VW + c]) + 16 * h);
+#pragma unroll
+                    for (int e = 0; e < VW; e++) {
+                        const int t = c * VW + h * (VW / 2) + e / 2;
+                        if (e % 2)
+                            rb[t] = v[e];
+                        else
+                            ra[t] = v[e];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; t++) {
+                const vec2 v = *(const vec2 *)pair_at(f, off[HW + t]);
+                ra[t] = v[0];
+                rb[t] = v[1];
+            }
+        }
+    }
+    __device__ __forceinline__ void store_pair(real *__restrict__ f, const real (&ra)[M],
+                                               const real (&rb)[M], int len) const
+    {
+        if (VEC) {
+#pragma unroll
+            for (int c = 0; c < M / VW; c++) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    vec v;
+#pragma unroll
+                    for (int e = 0; e < VW; e++) {
+                        const int t = c * VW + h * (VW / 2) + e / 2;
+                        v[e] = e % 2 ? rb[t] : ra[t];
+                    }
+                    *(vec *)(pair_at(f, off[LO / VW + c]) + 16 * h) = v;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; t++)
+                if (t < len) {
+                    vec2 v;
+                    v[0] = ra[t];
+                    v[1] = rb[t];
+                    *(vec2 *)pair_at(f, off[HW + t]) = v;
+                }
+        }
+    }
+
+    // ---- quad fields (one point = 4 values: one 16-byte load in FP32)
+    static __device__ __forceinline__ const char *quad_at(const real *f, unsigned o)
+    {
+        return (const char *)f + 4ull * o;
+    }
+    __device__ __forceinline__ void load_seg_quad(const real *__restrict__ f, real (&r0)[M],
+                                                  real (&r1)[M], real (&r2)[M],
+                                                  real (&r3)[M]) const
+    {
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            const vec4 v = *(const vec4 *)quad_at(f, mid(t));
+            r0[t] = v[0];
+            r1[t] = v[1];
+            r2[t] = v[2];
+            r3[t] = v[3];
+        }
+    }
+    __device__ __forceinline__ void store_quad(real *__restrict__ f, const real (&r0)[M],
+                                               const real (&r1)[M], const real (&r2)[M],
+                                               const real (&r3)[M], int len) const
+    {
+#pragma unroll
+        for (int t = 0; t < M; t++)
+            if (VEC || t < len) {
+                vec4 v;
+                v[0] = r0[t];
+                v[1] = r1[t];
+                v[2] = r2[t];
+                v[3] = r3[t];
+                *(vec4 *)quad_at(f, mid(t)) = v;
+            }
+    }
 };
+
+// one value of an interleaved field at the point of single-field byte
+// offset o: component k of N
+template <int N, typename real>
+__device__ __forceinline__ real ldo_part(const real *__restrict__ f, unsigned o, int k)
+{
+    return *(const real *)((const char *)f + (unsigned long long)N * o + (unsigned)k * sizeof(real));
+}
 
 // Windows of NF fields for lanes of uniform periodic MAP_COL plans, without
 // re-reading the neighbours' points: each lane loads its own M points of
@@ -288,10 +433,10 @@ struct Win {
 // neighbouring segments' slots (cyclic).  The staging area is the exchange
 // buffer, NF * M * lanes elements; the function starts and ends with a
 // barrier, so it may be called between solve batches.
-template <int NF, typename real, int M, int HW, bool VEC, typename Src>
-__device__ __forceinline__ void load_windows_lds(const Lane<real, MAP_COL> &c,
-                                                 const Win<real, M, HW, VEC> &win, Src src,
-                                                 real (&w)[NF][M + 2 * HW])
+// Core of load_windows_lds: pt(t, v) loads the NF values of own point t.
+template <int NF, typename real, int M, int HW, typename Pt>
+__device__ __forceinline__ void load_windows_lds_pt(const Lane<real, MAP_COL> &c, Pt pt,
+                                                    real (&w)[NF][M + 2 * HW])
 {
     static_assert(HW <= M, "halo wider than a segment");
     real *st = c.xb;
@@ -299,12 +444,13 @@ __device__ __forceinline__ void load_windows_lds(const Lane<real, MAP_COL> &c,
     const int me = c.g * c.LX + c.line;
     __syncthreads();
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
+    for (int t = 0; t < M; t++) {
+        real v[NF];
+        pt(t, v);
 #pragma unroll
-        for (int t = 0; t < M; t++) {
-            const real v = src(f, win.mid(t));
-            w[f][HW + t] = v;
-            st[(f * M + t) * T + me] = v;
+        for (int f = 0; f < NF; f++) {
+            w[f][HW + t] = v[f];
+            st[(f * M + t) * T + me] = v[f];
         }
     }
     __syncthreads();
@@ -320,6 +466,32 @@ __device__ __forceinline__ void load_windows_lds(const Lane<real, MAP_COL> &c,
         }
     }
     __syncthreads();
+}
+
+template <int NF, typename real, int M, int HW, bool VEC, typename Src>
+__device__ __forceinline__ void load_windows_lds(const Lane<real, MAP_COL> &c,
+                                                 const Win<real, M, HW, VEC> &win, Src src,
+                                                 real (&w)[NF][M + 2 * HW])
+{
+    load_windows_lds_pt<NF, real, M, HW>(c, [&](int t, real (&v)[NF]) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) v[f] = src(f, win.mid(t));
+    }, w);
+}
+
+// the windows of both values of a pair field, one load per own point
+template <typename real, int M, int HW, bool VEC>
+__device__ __forceinline__ void load_windows_lds_pair(const Lane<real, MAP_COL> &c,
+                                                      const Win<real, M, HW, VEC> &win,
+                                                      const real *__restrict__ f,
+                                                      real (&w)[2][M + 2 * HW])
+{
+    typedef typename Win<real, M, HW, VEC>::vec2 vec2;
+    load_windows_lds_pt<2, real, M, HW>(c, [&](int t, real (&v)[2]) {
+        const vec2 x = *(const vec2 *)Win<real, M, HW, VEC>::pair_at(f, win.mid(t));
+        v[0] = x[0];
+        v[1] = x[1];
+    }, w);
 }
 
 // LDS elements per lane that load_windows_lds stages for NF fields

@@ -130,22 +130,31 @@ struct SolverT : SolverBase {
 
         std::memset(&F, 0, sizeof(F));
         int nf = 15 + 4 + 4 + 10 + 5 + 4 + (sponge ? 15 : 0);
+        // fields start on 512-byte boundaries (the vector and interleaved
+        // accesses need 16/32-byte alignment whatever N is)
+        const size_t Np = ((size_t)N + 63) / 64 * 64;
         hc(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
-        hc(hipMalloc(&mem, (size_t)nf * N * sizeof(real)), "hipMalloc(fields)");
-        hc(hipMemsetAsync(mem, 0, (size_t)nf * N * sizeof(real), st), "hipMemset(fields)");
+        hc(hipMalloc(&mem, (size_t)nf * Np * sizeof(real)), "hipMalloc(fields)");
+        hc(hipMemsetAsync(mem, 0, (size_t)nf * Np * sizeof(real), st), "hipMemset(fields)");
         real *p = (real *)mem;
-        auto take = [&]() {
+        auto take = [&](int width = 1) {
             real *r = p;
-            p += N;
+            p += (size_t)width * Np;
             return r;
         };
         for (int v = 0; v < 5; v++) F.q[v] = take();
         for (int v = 0; v < 5; v++) F.qk[v] = take();
         for (int v = 0; v < 5; v++) F.acc[v] = take();
-        for (int v = 0; v < 4; v++) F.dY[v] = take();
-        for (int v = 0; v < 4; v++) F.dZ[v] = take();
-        for (int v = 0; v < 5; v++) F.R[v] = take();
-        for (int v = 0; v < 5; v++) F.Rz[v] = take();
+        // interleaved intermediates: a pair field takes two fields' space,
+        // a quad four
+        F.dYut = take(2);
+        F.dYvw = take(2);
+        F.dZuw = take(2);
+        F.dZvt = take(2);
+        F.Ry4 = take(4);
+        F.RyE = take();
+        F.Rz4 = take(4);
+        F.RzE = take();
         for (int v = 0; v < 5; v++) F.k2[v] = take();
         for (int v = 0; v < 4; v++) F.diag[v] = take();
         if (sponge) {

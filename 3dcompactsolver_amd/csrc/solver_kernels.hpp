@@ -21,11 +21,18 @@ struct Fields {
     real *acc[5];   // RK accumulator (*2)
     real *qlo[5];   // compensation of q (FP32 compensated mode), else null
     real *acclo[5]; // compensation of acc
-    real *dY[4];    // Uy Vy Wy Ty of the stage (first sweep)
-    real *dZ[4];    // Uz Vz Wz Tz (middle sweep)
+    // intermediates of a stage, interleaved (pairs and quads, see Win):
+    // column sweeps write 2 or 4 values per point with one store
+    real *dYut;     // (Uy, Ty) pairs of the stage (first sweep)
+    real *dYvw;     // (Vy, Wy) pairs
+    real *dZuw;     // (Uz, Wz) pairs (middle sweep)
+    real *dZvt;     // (Vz, Tz) pairs
     real *mix[4];   // stale-primitive D_y U, D_y V, D_z U, D_z W (sponge runs only)
-    real *R[5];     // partial right-hand side of the first (y) sweep
-    real *Rz[5];    // partial right-hand side of the middle (z) sweep
+    real *Ry4;      // partial right-hand side of the first (y) sweep: quads of
+                    // the continuity and momentum parts
+    real *RyE;      // its energy part
+    real *Rz4;      // the same for the middle (z) sweep
+    real *RzE;
     real *k2[5];    // dt*RHS (only written in RHS-only mode)
     real *sigma;    // sponge strength (null without sponge)
     real *avg[5];   // sponge running averages (perturbation form)

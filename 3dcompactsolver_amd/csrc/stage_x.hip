@@ -100,11 +100,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     }
     // the stored y and z derivatives that are differentiated along x (their
     // stale-based counterparts in the STALE variant) and their own values
-    const real *mi0 = STALE ? F.mix[0] : F.dY[0];  // Uy
-    const real *mi1 = STALE ? F.mix[1] : F.dY[1];  // Vy
-    const real *mi2 = STALE ? F.mix[2] : F.dZ[0];  // Uz
-    const real *mi3 = STALE ? F.mix[3] : F.dZ[2];  // Wz
-    real Uy[M], Vy[M], uz[M], wz[M];
+    real Uy[M], Vy[M], uz[M], wz[M], Wy[M], Ty[M];
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
     {
         // x fluxes (the h_ref U part of the energy flux is added as h_ref Ux)
@@ -123,22 +119,41 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = a3[t] * U[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
-        win.load(mi0, f);
+        if constexpr (STALE) {
+            win.load(F.mix[0], f);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
+            win.load(F.mix[1], f);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
+            win.load(F.mix[2], f);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[7]);
+            win.load(F.mix[3], f);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[8]);
+        } else {
+            // Uy with Ty, Vy with Wy, Uz with Wz
+            real fb[W];
+            win.load_pair(F.dYut, f, fb);
 #pragma unroll
-        for (int t = 0; t < M; t++) Uy[t] = f[HW + t];
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
-        win.load(mi1, f);
+            for (int t = 0; t < M; t++) {
+                Uy[t] = f[HW + t];
+                Ty[t] = fb[HW + t];
+            }
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[5]);
+            win.load_pair(F.dYvw, f, fb);
 #pragma unroll
-        for (int t = 0; t < M; t++) Vy[t] = f[HW + t];
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
-        win.load(mi2, f);
+            for (int t = 0; t < M; t++) {
+                Vy[t] = f[HW + t];
+                Wy[t] = fb[HW + t];
+            }
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[6]);
+            win.load_pair(F.dZuw, f, fb);
 #pragma unroll
-        for (int t = 0; t < M; t++) uz[t] = f[HW + t];
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[7]);
-        win.load(mi3, f);
-#pragma unroll
-        for (int t = 0; t < M; t++) wz[t] = f[HW + t];
-        rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[8]);
+            for (int t = 0; t < M; t++) {
+                uz[t] = f[HW + t];
+                wz[t] = fb[HW + t];
+            }
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[7]);
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, fb, r[8]);
+        }
         solve_batch<9, kc_deriv<real>(), real, M, UNI>(d1, c, r);
 #pragma unroll
         for (int t = 0; t < M; t++) {
@@ -160,10 +175,9 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     }
     if (STALE) {
         // the assembly takes the current first derivatives
-        win.load_seg(F.dY[0], Uy);
-        win.load_seg(F.dY[1], Vy);
-        win.load_seg(F.dZ[0], uz);
-        win.load_seg(F.dZ[2], wz);
+        win.load_seg_pair(F.dYut, Uy, Ty);
+        win.load_seg_pair(F.dYvw, Vy, Wy);
+        win.load_seg_pair(F.dZuw, uz, wz);
     }
     // first and second x-derivatives of T', U, V, W
     real r[8][M];
@@ -190,18 +204,18 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     if (!active) return;
 
     // the rest of the point values, whole segments at a time
-    real Wy[M], Ty[M], vz[M], Tz[M], Rp[5][M];
-    win.load_seg(F.dY[2], Wy);
-    win.load_seg(F.dY[3], Ty);
-    win.load_seg(F.dZ[1], vz);
-    win.load_seg(F.dZ[3], Tz);
+    real vz[M], Tz[M], Rp[5][M];
+    win.load_seg_pair(F.dZvt, vz, Tz);
+    {
+        real rz[5][M];
+        win.load_seg_quad(F.Ry4, Rp[0], Rp[1], Rp[2], Rp[3]);
+        win.load_seg(F.RyE, Rp[4]);
+        win.load_seg_quad(F.Rz4, rz[0], rz[1], rz[2], rz[3]);
+        win.load_seg(F.RzE, rz[4]);
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-        real rz[M];
-        win.load_seg(F.R[v], Rp[v]);
-        win.load_seg(F.Rz[v], rz);
+        for (int v = 0; v < 5; v++)
 #pragma unroll
-        for (int t = 0; t < M; t++) Rp[v][t] += rz[t];
+            for (int t = 0; t < M; t++) Rp[v][t] += rz[v][t];
     }
     const real dt = (real)sc->dt;
     real k[5][M];
@@ -255,7 +269,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             wn = 0;
         if (wn >= 0 && !plane) {
             real cn = R0[t];
-            if (wn > 0) cn = valid ? ldo(wn == 1 ? F.R[0] : F.Rz[0], win.mid(t)) : real(0);
+            if (wn > 0) cn = valid ? ldo_part<4>(wn == 1 ? F.Ry4 : F.Rz4, win.mid(t), 0) : real(0);
             k[0][t] = dt * cn;
 #pragma unroll
             for (int v = 1; v < 5; v++) k[v][t] = real(0);

@@ -3,8 +3,9 @@
 //
 // From the stage state: the y Euler fluxes (CSolver_AWS.cpp:949-963) and
 // their derivatives (:1051-1073), the first and second y-derivatives of U,
-// V, W, T (:969-1049).  Stores Uy Vy Wy Ty and starts the right-hand side
-// with every term linear in one of these.  The mixed derivatives the
+// V, W, T (:969-1049).  Stores Uy Vy Wy Ty (as the pairs (Uy, Ty),
+// (Vy, Wy)) and starts the right-hand side with every term linear in one of
+// these (continuity and momenta as one quad per point, energy apart).  The mixed derivatives the
 // reference takes along y of x-derivatives (Uxy, Vxy, :991-1003) are taken
 // along x of the stored Uy, Vy in the last sweep: compact operators along
 // different axes commute.  The 13 line solves run as two batches (5 flux
@@ -145,15 +146,10 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
         RE[t] += mu * (Wc * r[7][t]);
     }
     if (active) {
-        win.store(F.dY[0], r[1], c.len);
-        win.store(F.dY[1], r[2], c.len);
-        win.store(F.dY[2], r[3], c.len);
-        win.store(F.dY[3], r[0], c.len);
-        win.store(F.R[0], R0, c.len);
-        win.store(F.R[1], Ru, c.len);
-        win.store(F.R[2], Rv, c.len);
-        win.store(F.R[3], Rw, c.len);
-        win.store(F.R[4], RE, c.len);
+        win.store_pair(F.dYut, r[1], r[0], c.len);
+        win.store_pair(F.dYvw, r[2], r[3], c.len);
+        win.store_quad(F.Ry4, R0, Ru, Rv, Rw, c.len);
+        win.store(F.RyE, RE, c.len);
     }
 }
 

@@ -4,8 +4,10 @@
 // From the stage state: the z Euler fluxes (CSolver_AWS.cpp:1095-1108) and
 // their derivatives (:1195-1221), the first and second z-derivatives of U,
 // V, W, T (:1113-1159), and the mixed Vyz, Wyz as z-derivatives of the
-// stored Vy, Wy (:1161-1192).  Stores Uz Vz Wz Tz and its own partial
-// right-hand side: every term linear in one of these.  Uxz and Wxz are taken
+// stored Vy, Wy (:1161-1192).  Stores Uz Vz Wz Tz (as the pairs (Uz, Wz),
+// (Vz, Tz)) and its own partial right-hand side (continuity and momenta as
+// one quad per point, energy apart): every term linear in one of these.
+// Uxz and Wxz are taken
 // along x of the stored Uz, Wz in the last sweep (operators along different
 // axes commute).  Solves: one batch of 13 (5 flux + 4 first + 4 second
 // derivatives), then the 2 mixed ones.
@@ -131,29 +133,20 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         RE[t] += mu[t] * (Wc[t] * (c43 * r[12][t]));
     }
     if (active) {
-        win.store(F.dZ[0], r[6], c.len);
-        win.store(F.dZ[1], r[7], c.len);
-        win.store(F.dZ[2], r[8], c.len);
-        win.store(F.dZ[3], r[5], c.len);
+        win.store_pair(F.dZuw, r[6], r[8], c.len);
+        win.store_pair(F.dZvt, r[7], r[5], c.len);
     }
     {
         // Vyz, Wyz from the stored Vy, Wy (and the stale D_z U, D_z W)
         constexpr int NM = 2 + NS;
         real m[NM][M];
-        if constexpr (UNI) {
-            real mw[2][W];
-            load_windows_lds<2>(c, win, [&](int fi, unsigned o) {
-                return ldo(fi == 0 ? F.dY[1] : F.dY[2], o);
-            }, mw);
+        real mw[2][W];
+        if constexpr (UNI)
+            load_windows_lds_pair(c, win, F.dYvw, mw);
+        else
+            win.load_pair(F.dYvw, mw[0], mw[1]);
 #pragma unroll
-            for (int i = 0; i < 2; i++) rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[i], m[i]);
-        } else {
-            real f[W];
-            win.load(F.dY[1], f);
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m[0]);
-            win.load(F.dY[2], f);
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, f, m[1]);
-        }
+        for (int i = 0; i < 2; i++) rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[i], m[i]);
         if constexpr (STALE) {
 #pragma unroll
             for (int t = 0; t < M; t++) {
@@ -179,11 +172,8 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     if (!active) return;
     // a partial right-hand side of its own (the last sweep adds both): no
     // read-modify-write on the column path
-    win.store(F.Rz[0], R0, c.len);
-    win.store(F.Rz[1], Ru, c.len);
-    win.store(F.Rz[2], Rv, c.len);
-    win.store(F.Rz[3], Rw, c.len);
-    win.store(F.Rz[4], RE, c.len);
+    win.store_quad(F.Rz4, R0, Ru, Rv, Rw, c.len);
+    win.store(F.RzE, RE, c.len);
 }
 
 template <typename real, bool UNI>
