@@ -30,30 +30,38 @@ static void hc(hipError_t e, const char *what)
 }
 
 // Workgroup size caps of the sweeps and the filter sweeps.  The environment
-// variables CS3D_SWEEP_THREADS / CS3D_FILTER_THREADS lower them for tuning;
-// values outside [64, cap] are ignored.
-static int env_cap(const char *name, int cap)
+// variables CS3D_SWEEP_THREADS, CS3D_SWEEP_X_THREADS, CS3D_FILTER_THREADS
+// and CS3D_FILTER_X_THREADS set them for tuning; values outside [64, cap]
+// are ignored.
+static int env_cap(const char *name, int cap, int dflt = 0)
 {
+    if (dflt <= 0) dflt = cap;
     const char *v = std::getenv(name);
-    if (!v) return cap;
+    if (!v) return dflt;
     int x = std::atoi(v);
-    return (x >= 64 && x <= cap) ? x : cap;
+    return (x >= 64 && x <= cap) ? x : dflt;
 }
 
 template <typename real>
 struct SolverT : SolverBase {
     // Y/Z sweeps want 16+ lines per workgroup (64-byte rows of x); the X
-    // sweep reads whole lines and runs best with small workgroups that let
-    // several resident per CU overlap their load and solve phases
-    static int sweep_max_threads(int dir)
+    // sweep reads whole lines and runs best with the smallest workgroups:
+    // one line, or as many as fill one wavefront when a line has fewer than
+    // 64 segments (P).  Independent workgroups of one wavefront overlap
+    // their load and solve phases on a CU (256^3 FP32: 64 threads 6% faster
+    // than 256).
+    static int sweep_max_threads(int dir, int P)
     {
-        if (dir == 0) return env_cap("CS3D_SWEEP_X_THREADS", 256);
+        if (dir == 0)
+            return env_cap("CS3D_SWEEP_X_THREADS", X_SWEEP_THREADS,
+                           std::min(X_SWEEP_THREADS, std::max(64, P)));
         return env_cap("CS3D_SWEEP_THREADS", sizeof(real) == 4 ? 1024 : 512);
     }
     static int filter_max_threads(int dir)
     {
         // rows (x) gain nothing from larger workgroups
-        return env_cap("CS3D_FILTER_THREADS", dir == 0 ? 512 : (sizeof(real) == 4 ? 1024 : 512));
+        if (dir == 0) return env_cap("CS3D_FILTER_X_THREADS", 512);
+        return env_cap("CS3D_FILTER_THREADS", sizeof(real) == 4 ? 1024 : 512);
     }
     static int line_terms_max_threads(int dir)
     {
@@ -115,7 +123,7 @@ struct SolverT : SolverBase {
                 throw std::invalid_argument("inconsistent derivative plans");
             const int P = D1[d].host.P;
             const int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
-            sweepG[d] = line_geom(P, kr, sweep_max_threads(d), SWEEP_PLANES, sizeof(real),
+            sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
                                   SWEEP_STAGE_FIELDS * D1[d].host.M);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
