@@ -63,9 +63,14 @@ struct SolverT : SolverBase {
         if (dir == 0) return env_cap("CS3D_FILTER_X_THREADS", 512);
         return env_cap("CS3D_FILTER_THREADS", sizeof(real) == 4 ? 1024 : 512);
     }
-    static int line_terms_max_threads(int dir)
+    // the adjoint's x kernels take one line per workgroup, as the x sweep
+    static int adj_x_threads(int P)
     {
-        if (dir == 0) return env_cap("CS3D_ADJ_X_THREADS", 256);
+        return env_cap("CS3D_ADJ_X_THREADS", ADJ_X_THREADS, std::min(ADJ_X_THREADS, std::max(64, P)));
+    }
+    static int line_terms_max_threads(int dir, int P)
+    {
+        if (dir == 0) return adj_x_threads(P);
         return env_cap("CS3D_ADJ_THREADS", line_terms_threads<real>());
     }
 
@@ -151,7 +156,7 @@ struct SolverT : SolverBase {
             return r;
         };
         for (int v = 0; v < 5; v++) F.q[v] = take();
-        for (int v = 0; v < 5; v++) F.qk[v] = take();
+        for (int v = 0; v < 5; v++) F.qk[v] = F.qkw[v] = take();
         for (int v = 0; v < 5; v++) F.acc[v] = take();
         // interleaved intermediates: a pair field takes two fields' space,
         // a quad four
@@ -848,7 +853,7 @@ struct SolverT : SolverBase {
         if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
             throw std::logic_error("derivative plans of one direction differ in segmentation");
         const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
-        LaunchGeom g = line_geom(h1.P, kr, line_terms_max_threads(dir), n1 + n2, sizeof(real));
+        LaunchGeom g = line_geom(h1.P, kr, line_terms_max_threads(dir, h1.P), n1 + n2, sizeof(real));
         if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
         hc(launch_line_terms<real>(n1, n2, acc, D1[dir].dev, D2[dir].dev, h1.uniform, h1.M, g,
                                    dir == 0 ? MAP_ROW : MAP_COL, lm[dir], a, st),
@@ -911,7 +916,7 @@ struct SolverT : SolverBase {
         const OpHost &h1 = D1[0].host, &h2 = D2[0].host;
         if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
             throw std::logic_error("derivative plans of one direction differ in segmentation");
-        g = line_geom(h1.P, std::max(std::max(h1.K, h2.K), kc_deriv<real>()), ADJ_X_THREADS,
+        g = line_geom(h1.P, std::max(std::max(h1.K, h2.K), kc_deriv<real>()), adj_x_threads(h1.P),
                       planes, sizeof(real));
         if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
     }
@@ -1063,10 +1068,18 @@ struct SolverT : SolverBase {
             r.order = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
         }
         if (stg) copy5(stg[0], F.q);
+        // the last sweep of stage s < 4 writes stage state s + 1 straight
+        // into the store, and stage s + 1 reads it there
+        real *own[5];
+        for (int c = 0; c < 5; c++) own[c] = F.qk[c];
         for (int s = 1; s <= 4; s++) {
+            if (stg && s < 4)
+                for (int c = 0; c < 5; c++) F.qkw[c] = stg[s][c];
             sub_step(s, s == 4);
-            if (stg && s < 4) copy5(stg[s], F.qk);
+            if (stg && s < 4)
+                for (int c = 0; c < 5; c++) F.qk[c] = stg[s][c];
         }
+        for (int c = 0; c < 5; c++) F.qk[c] = F.qkw[c] = own[c];
         post_step();
         return r;
     }

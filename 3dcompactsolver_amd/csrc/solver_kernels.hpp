@@ -18,6 +18,8 @@ template <typename real>
 struct Fields {
     real *q[5];     // state (*1)
     real *qk[5];    // stage state (*k)
+    real *qkw[5];   // where the last sweep writes the next stage state:
+                    // qk itself, or a stage-state store of the adjoint
     real *acc[5];   // RK accumulator (*2)
     real *qlo[5];   // compensation of q (FP32 compensated mode), else null
     real *acclo[5]; // compensation of acc

@@ -337,13 +337,13 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             if (has_next) {
 #pragma unroll
                 for (int t = 0; t < M; t++) o[t] = qn1[t] + rb * k[v][t];
-                win.store(F.qk[v], o, c.len);
+                win.store(F.qkw[v], o, c.len);
             }
         } else {
             // TVDRK3.hpp:225-260: stages 1-2 write *k, stage 3 writes *2
 #pragma unroll
             for (int t = 0; t < M; t++) o[t] = ta * qn1[t] + tb * qs_[v][t] + tc * k[v][t];
-            win.store(tvd_to_k ? F.qk[v] : F.acc[v], o, c.len);
+            win.store(tvd_to_k ? F.qkw[v] : F.acc[v], o, c.len);
         }
     }
 }

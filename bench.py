@@ -14,8 +14,10 @@ value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
 roofline = the dominant kernel class, timed with HIP events recorded on the
          solver's own stream around every launch of the timed region.
 adjoint = cost of the kinetic-energy gradient through 8 RK4 steps of the
-         same workload (forward pass, checkpointed replay, reverse sweep)
-         against 8 forward steps.
+         same workload against 8 forward steps: the forward pass keeps every
+         stage state in HBM (32 states, 10.7 GB at 256^3 FP32), then the
+         reverse sweep; --adjoint-checkpoint K instead keeps checkpoints
+         every K steps and replays segments.
 cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
          with the reference's flags) timed on this host's cores on a
          bounded sample.
@@ -175,6 +177,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--adjoint-steps", type=int, default=8)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=0,
+                    help="steps between adjoint checkpoints (default: all steps, i.e. every "
+                         "stage state of the run stays in HBM and nothing is recomputed)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 256 MiB device copies (the reference kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -206,7 +211,8 @@ def main():
                 "roofline": roof64,
                 "kinetic_energy": ke64,
                 "fp32_ke_rel_err": abs(ke32 - ke64) / abs(ke64)}
-    adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, 4, ms32)
+    ck = args.adjoint_checkpoint if args.adjoint_checkpoint > 0 else args.adjoint_steps
+    adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, ck, ms32)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_grid, args.cpu_steps)
     line = {
         "metric": METRIC,
