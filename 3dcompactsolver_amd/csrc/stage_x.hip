@@ -179,6 +179,22 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         win.load_seg_pair(F.dYvw, Vy, Wy);
         win.load_seg_pair(F.dZuw, uz, wz);
     }
+    // the rest of the point values, whole segments at a time, issued before
+    // the second solve batch so that they arrive during it (inactive lanes
+    // address the last line and read in bounds)
+    real vz[M], Tz[M], Rp[5][M];
+    win.load_seg_pair(F.dZvt, vz, Tz);
+    {
+        real rz[5][M];
+        win.load_seg_quad(F.Ry4, Rp[0], Rp[1], Rp[2], Rp[3]);
+        win.load_seg(F.RyE, Rp[4]);
+        win.load_seg_quad(F.Rz4, rz[0], rz[1], rz[2], rz[3]);
+        win.load_seg(F.RzE, rz[4]);
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+            for (int t = 0; t < M; t++) Rp[v][t] += rz[v][t];
+    }
     // first and second x-derivatives of T', U, V, W
     real r[8][M];
     rhs<OP_D1, real, M, HW, UNI>(d1, c, Tp, r[0]);
@@ -203,20 +219,6 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     }
     if (!active) return;
 
-    // the rest of the point values, whole segments at a time
-    real vz[M], Tz[M], Rp[5][M];
-    win.load_seg_pair(F.dZvt, vz, Tz);
-    {
-        real rz[5][M];
-        win.load_seg_quad(F.Ry4, Rp[0], Rp[1], Rp[2], Rp[3]);
-        win.load_seg(F.RyE, Rp[4]);
-        win.load_seg_quad(F.Rz4, rz[0], rz[1], rz[2], rz[3]);
-        win.load_seg(F.RzE, rz[4]);
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-#pragma unroll
-            for (int t = 0; t < M; t++) Rp[v][t] += rz[v][t];
-    }
     const real dt = (real)sc->dt;
     real k[5][M];
 #pragma unroll
