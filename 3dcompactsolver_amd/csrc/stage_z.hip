@@ -41,6 +41,18 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
     win.load(q2, a2);
     win.load(q3, a3);
     win.load(q4, a4);
+    // the own points of (Vy, Wy) for the mixed derivatives, issued now so
+    // they arrive during the first solve batch (not in the STALE variant,
+    // whose registers are full)
+    constexpr bool PRE = UNI && !STALE;
+    real pv[2][M];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            pv[0][t] = ldo_part<2>(F.dYvw, win.mid(t), 0);
+            pv[1][t] = ldo_part<2>(F.dYvw, win.mid(t), 1);
+        }
+    }
 #pragma unroll
     for (int t = 0; t < W; t++)
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
@@ -141,7 +153,12 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         constexpr int NM = 2 + NS;
         real m[NM][M];
         real mw[2][W];
-        if constexpr (UNI)
+        if constexpr (PRE)
+            load_windows_lds_pt<2, real, M, HW>(c, [&](int t, real(&v)[2]) {
+                v[0] = pv[0][t];
+                v[1] = pv[1][t];
+            }, mw);
+        else if constexpr (UNI)
             load_windows_lds_pair(c, win, F.dYvw, mw);
         else
             win.load_pair(F.dYvw, mw[0], mw[1]);
