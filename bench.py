@@ -61,7 +61,7 @@ ACCESSES_PER_STEP = sum(sum(v) for v in STAGE_ACCESSES.values())  # 307
 # HBM accesses per point actually moved per launch (reads, writes), FP32 at
 # 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
 # 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
-MEASURED_TRAFFIC = {"x_sweep": (31.71, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.03, 9.01)}
+MEASURED_TRAFFIC = {"x_sweep": (30.50, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.01, 9.00)}
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
 BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP * 4 / 4, cs3d.FP64: ACCESSES_PER_STEP * 8 / 4}
 
