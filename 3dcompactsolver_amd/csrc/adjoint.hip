@@ -195,7 +195,6 @@ __global__ __launch_bounds__(line_terms_threads<real>()) void k_line_terms(
     LineTerms<real> a)
 {
     constexpr int HW = 2, W = M + 2 * HW, N = N1 + N2;
-    constexpr int KC = kc_deriv<real>();
     CS3D_LANE_SETUP(MAP, d1, sweep_kr(d1, d2), N)
     (void)n;
     (void)per;
@@ -209,7 +208,7 @@ __global__ __launch_bounds__(line_terms_threads<real>()) void k_line_terms(
         else
             rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[t]);
     }
-    solve_batch2<N1, N2, KC, real, M, UNI>(d1, d2, c, r);
+    solve_d12<N1, N2, real, M, UNI>(d1, d2, c, r);
     if (ACC) {
         real o[M];
 #pragma unroll
@@ -298,7 +297,6 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xseed(const OpDev<real> *
                                                              AdjSeedOut<real> o, Phys<real> ph)
 {
     constexpr int HW = 2, W = M + 2 * HW;
-    constexpr int KC = kc_deriv<real>();
     CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 12)
     (void)n;
     (void)per;
@@ -317,7 +315,7 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xseed(const OpDev<real> *
             if (f < 4) rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[8 + f]);
         }
     }
-    solve_batch2<8, 4, KC, real, M, UNI>(d1, d2, c, r);
+    solve_d12<8, 4, real, M, UNI>(d1, d2, c, r);
     // the remaining per-point inputs, whole segments at a time
     enum { P_MU, P_A, P_V1, P_V2, P_V3, P_VE, P_WY, P_TY, P_VZ, P_TZ, P_VYZ, P_WYZ, P_D2 };
     real pt[P_D2 + 8][M];
@@ -394,7 +392,6 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> 
                                                               Phys<real> ph)
 {
     constexpr int HW = 2, W = M + 2 * HW;
-    constexpr int KC = kc_deriv<real>();
     CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 14)
     (void)n;
     (void)per;
@@ -416,7 +413,7 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> 
             rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[10 + k]);
         }
     }
-    solve_batch2<10, 4, KC, real, M, UNI>(d1, d2, c, r);
+    solve_d12<10, 4, real, M, UNI>(d1, d2, c, r);
     const real c13 = real(1) / real(3), c43 = real(4) / real(3);
     // the per-point inputs, whole segments at a time: tot + totY, D_y v,
     // D_z v, q

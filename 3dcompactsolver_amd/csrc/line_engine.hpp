@@ -683,7 +683,7 @@ __device__ __forceinline__ const real *carry_weights(const OpDev<real> *__restri
 // address.  Uniform plans store zero weights past K, so on lines of at least
 // KR >= KC segments their first KC terms run unrolled without branches;
 // terms past KC, shorter lines and table plans run in a plain loop.
-template <int N1, int N2, bool FWD, int KC, bool UNI, typename real, int MAP>
+template <int N1, int N2, bool FWD, int KC1, int KC2, bool UNI, typename real, int MAP>
 __device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 + N2],
                                          real (&out)[N1 + N2], const real *__restrict__ w1,
                                          int K1, const real *__restrict__ w2, int K2, int cyc)
@@ -705,34 +705,45 @@ __device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 
     for (int i = 0; i < N; i++) out[i] = 0;
     constexpr int dir = FWD ? -S : S;
     const real *p = b + prim;
-    // the unrolled reads reach KC positions back/forward; every one of them
-    // was written only when the line has at least KR >= KC segments
+    // the unrolled reads reach KC1 (KC2) positions back/forward; every one
+    // of them was written only when the line has at least KR >= KC segments
     const bool unrolled = UNI && c.P >= c.KR;
-    const int k0 = unrolled ? KC : 0;
+    const int k1 = unrolled ? KC1 : 0, k2 = unrolled ? KC2 : 0;
     if (unrolled) {
 #pragma unroll
-        for (int k = 1; k <= KC; k++) {
-            const real wa = w1[k - 1], wb = w2[k - 1];
+        for (int k = 1; k <= KC1; k++) {
+            const real wk = w1[k - 1];
 #pragma unroll
-            for (int i = 0; i < N; i++) out[i] += (i < N1 ? wa : wb) * p[dir * k + i];
+            for (int i = 0; i < N1; i++) out[i] += wk * p[dir * k + i];
+        }
+        if constexpr (N2 > 0) {
+#pragma unroll
+            for (int k = 1; k <= KC2; k++) {
+                const real wk = w2[k - 1];
+#pragma unroll
+                for (int i = N1; i < N; i++) out[i] += wk * p[dir * k + i];
+            }
         }
     }
-    for (int k = k0 + 1; k <= K1; k++) {
+    for (int k = k1 + 1; k <= K1; k++) {
         const real wk = w1[k - 1];
 #pragma unroll
         for (int i = 0; i < N1; i++) out[i] += wk * p[dir * k + i];
     }
-    for (int k = k0 + 1; k <= K2; k++) {
-        const real wk = w2[k - 1];
+    if constexpr (N2 > 0) {
+        for (int k = k2 + 1; k <= K2; k++) {
+            const real wk = w2[k - 1];
 #pragma unroll
-        for (int i = N1; i < N; i++) out[i] += wk * p[dir * k + i];
+            for (int i = N1; i < N; i++) out[i] += wk * p[dir * k + i];
+        }
     }
 }
 
 // N1 solves with operator 1 and N2 with operator 2, r holding B f on entry
-// and A^-1 B f on exit; two barriers for the whole batch.  KC: carry terms
-// unrolled on uniform plans (the lane's ring margin must be at least KC).
-template <int N1, int N2, int KC, typename real, int M, bool UNI, int MAP>
+// and A^-1 B f on exit; two barriers for the whole batch.  KC1, KC2: carry
+// terms unrolled on uniform plans for each operator (the lane's ring margin
+// must be at least both).
+template <int N1, int N2, int KC1, int KC2, typename real, int M, bool UNI, int MAP>
 __device__ __forceinline__ void solve_batch2(const OpDev<real> *__restrict__ op1,
                                              const OpDev<real> *__restrict__ op2,
                                              Lane<real, MAP> &c, real (&r)[N1 + N2][M])
@@ -742,12 +753,12 @@ __device__ __forceinline__ void solve_batch2(const OpDev<real> *__restrict__ op1
 #pragma unroll
     for (int i = 0; i < N; i++)
         e[i] = fwd_local<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i]);
-    exchange<N1, N2, true, KC, UNI>(c, e, C, carry_weights<real, UNI, true>(op1, c), op1->K,
+    exchange<N1, N2, true, KC1, KC2, UNI>(c, e, C, carry_weights<real, UNI, true>(op1, c), op1->K,
                            carry_weights<real, UNI, true>(op2, c), op2->K, op1->cyclic);
 #pragma unroll
     for (int i = 0; i < N; i++)
         e[i] = bwd_local<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i], C[i]);
-    exchange<N1, N2, false, KC, UNI>(c, e, C, carry_weights<real, UNI, false>(op1, c), op1->K,
+    exchange<N1, N2, false, KC1, KC2, UNI>(c, e, C, carry_weights<real, UNI, false>(op1, c), op1->K,
                             carry_weights<real, UNI, false>(op2, c), op2->K, op1->cyclic);
 #pragma unroll
     for (int i = 0; i < N; i++) finish<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i], C[i]);
@@ -757,7 +768,7 @@ template <int N, int KC, typename real, int M, bool UNI, int MAP>
 __device__ __forceinline__ void solve_batch(const OpDev<real> *__restrict__ op,
                                             Lane<real, MAP> &c, real (&r)[N][M])
 {
-    solve_batch2<N, 0, KC, real, M, UNI, MAP>(op, op, c, r);
+    solve_batch2<N, 0, KC, KC, real, M, UNI, MAP>(op, op, c, r);
 }
 
 }  // namespace cs3d

@@ -32,9 +32,9 @@ void PlanDev<real>::release()
 
 OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32)
 {
-    // cross-segment terms below this weight are dropped (far below the
-    // rounding of the working precision)
-    const double tol = fp32 ? 1e-10 : 1e-20;
+    // cross-segment terms below this weight are dropped: a sixth (FP32) and
+    // a tenth (FP64) of the unit roundoff, relative to the carried value
+    const double tol = fp32 ? 1e-8 : 1e-17;
     // uniform kernels only for periodic lines that split into equal
     // segments of the uniform length; everything else takes the table kernels
     const int MU = kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
@@ -155,7 +155,9 @@ __global__ __launch_bounds__(1024) void k_field_op(const OpDev<real> *__restrict
     extern __shared__ char smem_raw[];
     real *xb = (real *)smem_raw;
     constexpr int HW = KIND == OP_FILTER ? 4 : 2, W = M + 2 * HW;
-    constexpr int KC = KIND == OP_FILTER ? kc_filter<real>() : kc_deriv<real>();
+    constexpr int KC = KIND == OP_FILTER ? kc_filter<real>()
+                       : KIND == OP_D2   ? kc_d2<real>()
+                                         : kc_deriv<real>();
     const int P = op->P;
     const int LX = blockDim.x / P;
     Lane<real, MAP> c = make_lane<real, MAP>(op, P, LX, op->K > KC ? op->K : KC, 1, xb);

@@ -27,13 +27,17 @@ constexpr int SEG_FILTER = 8;
 constexpr int SEG_TABLE = 4;
 
 // Carry terms unrolled by the kernels on uniform plans (any further terms
-// run in a loop).  The derivative plans keep 6 (FP32) / 12 (FP64) terms; the
-// filter's grow as alpha_f approaches 1/2 (15 / 29 at 0.49 with 8-point
-// segments).  A lane's ring margin is max(K, KC).
+// run in a loop), one count per operator: the K its periodic 4-point plan
+// keeps at the truncation tolerance of plan_for, 5 (FP32) / 11 (FP64) for
+// the first derivative, 3 / 6 for the second; the filter's grow as alpha_f
+// approaches 1/2 (12 / 25 at 0.49 with 8-point segments).  A lane's ring
+// margin is max(K, KC) over its operators.
 template <typename real>
-constexpr int kc_deriv() { return sizeof(real) == 4 ? 6 : 12; }
+constexpr int kc_deriv() { return sizeof(real) == 4 ? 5 : 11; }
 template <typename real>
-constexpr int kc_filter() { return sizeof(real) == 4 ? 16 : 32; }
+constexpr int kc_d2() { return sizeof(real) == 4 ? 3 : 6; }
+template <typename real>
+constexpr int kc_filter() { return sizeof(real) == 4 ? 12 : 25; }
 
 template <typename real>
 struct PlanDev {

@@ -79,6 +79,15 @@ __device__ __forceinline__ void prim(const Phys<real> &ph, real rp, real ru, rea
     Win<real, M, HW, vec_windows<real, M, MAPV, UNI>()> win;                            \
     win.init((unsigned)lm.base(Lc), (unsigned)lm.sp, c.s, n, per);
 
+// N1 first-derivative and N2 second-derivative solves
+template <int N1, int N2, typename real, int M, bool UNI, int MAP>
+__device__ __forceinline__ void solve_d12(const OpDev<real> *__restrict__ d1,
+                                          const OpDev<real> *__restrict__ d2,
+                                          Lane<real, MAP> &c, real (&r)[N1 + N2][M])
+{
+    solve_batch2<N1, N2, kc_deriv<real>(), kc_d2<real>(), real, M, UNI, MAP>(d1, d2, c, r);
+}
+
 // ring margin of a sweep that uses both derivative operators
 template <typename real>
 __device__ __forceinline__ int sweep_kr(const OpDev<real> *d1, const OpDev<real> *d2)

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[5]);
     rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[6]);
     rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[7]);
-    solve_batch2<4, 4, kc_deriv<real>(), real, M, UNI>(d1, d2, c, r);
+    solve_d12<4, 4, real, M, UNI>(d1, d2, c, r);
 #pragma unroll
     for (int t = 0; t < M; t++) {
         real mu, amu;

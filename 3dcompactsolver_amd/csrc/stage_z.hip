@@ -126,7 +126,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         Vc[t] = V[HW + t];
         Wc[t] = Wv[HW + t];
     }
-    solve_batch2<9, 4, kc_deriv<real>(), real, M, UNI>(d1, d2, c, r);
+    solve_d12<9, 4, real, M, UNI>(d1, d2, c, r);
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
