@@ -8,8 +8,7 @@
 // these (continuity and momenta as one quad per point, energy apart).  The mixed derivatives the
 // reference takes along y of x-derivatives (Uxy, Vxy, :991-1003) are taken
 // along x of the stored Uy, Vy in the last sweep: compact operators along
-// different axes commute.  The 13 line solves run as two batches (5 flux
-// derivatives; 4 first + 4 second derivatives).
+// different axes commute.  The 13 line solves run as one batch.
 //
 // STALE: at the first stage after updateSponge the reference's x block uses
 // primitives that predate the sponge reset on the sponge planes
@@ -57,10 +56,16 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
     const real c43 = real(4) / real(3);
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
+    // the 5 y fluxes (the h_ref V part of the energy flux is added as
+    // h_ref Vy), STALE: the first derivatives of the stale U, V, then the
+    // first and second derivatives of T', U, V, W.  One batch of 13 solves;
+    // the STALE variant's 15 exceed the exchange ring's planes and run as
+    // 7 + 8.
+    constexpr int NS = STALE ? 2 : 0, I1 = 5 + NS, N1 = I1 + 4;
+    static_assert((STALE ? 8 : N1 + 4) <= SWEEP_PLANES, "batch wider than the exchange ring");
+    real r[N1 + 4][M];
     {
-        // y fluxes (the h_ref V part of the energy flux is added as h_ref Vy)
-        constexpr int NB = STALE ? 7 : 5;
-        real r[NB][M], f[W];
+        real f[W];
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = (a4[t] + pp[t]) * V[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[0]);
@@ -74,80 +79,77 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
 #pragma unroll
         for (int t = 0; t < W; t++) f[t] = a3[t] * V[t];
         rhs<OP_D1, real, M, HW, UNI>(d1, c, f, r[4]);
-        if constexpr (STALE) {
-            // U and V with the sponge planes' primitives from the pre-reset
-            // copy; lines along y: L = i + Nx k, the point index is j
-            const int li = Lc % sa.Nx, lk = Lc / sa.Nx;
-            const bool line_plane = (sa.sponge_face[0] && li == 0) ||
-                                    (sa.sponge_face[1] && li == sa.Nx - 1) ||
-                                    (sa.sponge_face[4] && lk == 0) ||
-                                    (sa.sponge_face[5] && lk == sa.Nz - 1);
-            real us[W], vs[W];
-#pragma unroll
-            for (int t = 0; t < W; t++) {
-                int p = c.s - HW + t;
-                p += p < 0 ? n : 0;
-                p -= p >= n ? n : 0;
-                const bool plane = line_plane || (sa.sponge_face[2] && p == 0) ||
-                                   (sa.sponge_face[3] && p == n - 1);
-                us[t] = U[t];
-                vs[t] = V[t];
-                if (plane) {
-                    real i_, u_, v_, w_, p_, t_;
-                    prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
-                         ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
-                         ldo(F.qstale[4], win.at(t)), i_, u_, v_, w_, p_, t_);
-                    us[t] = u_;
-                    vs[t] = v_;
-                }
-            }
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, us, r[5]);
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, vs, r[6]);
-        }
-        solve_batch<NB, kc_deriv<real>(), real, M, UNI>(d1, c, r);
-#pragma unroll
-        for (int t = 0; t < M; t++) {
-            RE[t] = -r[0][t];
-            Rv[t] = -r[1][t];
-            R0[t] = -r[2][t];
-            Ru[t] = -r[3][t];
-            Rw[t] = -r[4][t];
-        }
-        if constexpr (STALE) {
-            if (active) {
-                win.store(F.mix[0], r[5], c.len);
-                win.store(F.mix[1], r[6], c.len);
-            }
-        }
     }
-    // first and second derivatives of T', U, V, W
-    real r[8][M];
-    rhs<OP_D1, real, M, HW, UNI>(d1, c, Tp, r[0]);
-    rhs<OP_D1, real, M, HW, UNI>(d1, c, U, r[1]);
-    rhs<OP_D1, real, M, HW, UNI>(d1, c, V, r[2]);
-    rhs<OP_D1, real, M, HW, UNI>(d1, c, Wv, r[3]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, Tp, r[4]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[5]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[6]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[7]);
-    solve_d12<4, 4, real, M, UNI>(d1, d2, c, r);
+    if constexpr (STALE) {
+        // U and V with the sponge planes' primitives from the pre-reset
+        // copy; lines along y: L = i + Nx k, the point index is j
+        const int li = Lc % sa.Nx, lk = Lc / sa.Nx;
+        const bool line_plane = (sa.sponge_face[0] && li == 0) ||
+                                (sa.sponge_face[1] && li == sa.Nx - 1) ||
+                                (sa.sponge_face[4] && lk == 0) ||
+                                (sa.sponge_face[5] && lk == sa.Nz - 1);
+        real us[W], vs[W];
+#pragma unroll
+        for (int t = 0; t < W; t++) {
+            int p = c.s - HW + t;
+            p += p < 0 ? n : 0;
+            p -= p >= n ? n : 0;
+            const bool plane = line_plane || (sa.sponge_face[2] && p == 0) ||
+                               (sa.sponge_face[3] && p == n - 1);
+            us[t] = U[t];
+            vs[t] = V[t];
+            if (plane) {
+                real i_, u_, v_, w_, p_, t_;
+                prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
+                     ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
+                     ldo(F.qstale[4], win.at(t)), i_, u_, v_, w_, p_, t_);
+                us[t] = u_;
+                vs[t] = v_;
+            }
+        }
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, us, r[5]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, vs, r[6]);
+    }
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, Tp, r[I1 + 0]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, U, r[I1 + 1]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, V, r[I1 + 2]);
+    rhs<OP_D1, real, M, HW, UNI>(d1, c, Wv, r[I1 + 3]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Tp, r[N1 + 0]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[N1 + 1]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[N1 + 2]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[N1 + 3]);
+    if constexpr (STALE) {
+        solve_batch<I1, kc_deriv<real>(), real, M, UNI>(d1, c, *(real(*)[I1][M])r);
+        solve_d12<4, 4, real, M, UNI>(d1, d2, c, *(real(*)[8][M])(r + I1));
+        if (active) {
+            win.store(F.mix[0], r[5], c.len);
+            win.store(F.mix[1], r[6], c.len);
+        }
+    } else {
+        solve_d12<N1, 4, real, M, UNI>(d1, d2, c, r);
+    }
 #pragma unroll
     for (int t = 0; t < M; t++) {
+        RE[t] = -r[0][t];
+        Rv[t] = -r[1][t];
+        R0[t] = -r[2][t];
+        Ru[t] = -r[3][t];
+        Rw[t] = -r[4][t];
         real mu, amu;
         visc(ph, ph.T0s + Tp[HW + t], mu, amu);
         const real Uc = U[HW + t], Vc = V[HW + t], Wc = Wv[HW + t];
-        RE[t] += ph.cpPr * (mu * r[4][t]);
-        Ru[t] += mu * r[5][t];
-        RE[t] += mu * (Uc * r[5][t]);
-        RE[t] -= ph.h_ref * r[2][t];
-        Rv[t] += mu * (c43 * r[6][t]);
-        RE[t] += mu * (Vc * (c43 * r[6][t]));
-        Rw[t] += mu * r[7][t];
-        RE[t] += mu * (Wc * r[7][t]);
+        RE[t] += ph.cpPr * (mu * r[N1][t]);
+        Ru[t] += mu * r[N1 + 1][t];
+        RE[t] += mu * (Uc * r[N1 + 1][t]);
+        RE[t] -= ph.h_ref * r[I1 + 2][t];
+        Rv[t] += mu * (c43 * r[N1 + 2][t]);
+        RE[t] += mu * (Vc * (c43 * r[N1 + 2][t]));
+        Rw[t] += mu * r[N1 + 3][t];
+        RE[t] += mu * (Wc * r[N1 + 3][t]);
     }
     if (active) {
-        win.store_pair(F.dYut, r[1], r[0], c.len);
-        win.store_pair(F.dYvw, r[2], r[3], c.len);
+        win.store_pair(F.dYut, r[I1 + 1], r[I1 + 0], c.len);
+        win.store_pair(F.dYvw, r[I1 + 2], r[I1 + 3], c.len);
         win.store_quad(F.Ry4, R0, Ru, Rv, Rw, c.len);
         win.store(F.RyE, RE, c.len);
     }

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         Vc[t] = V[HW + t];
         Wc[t] = Wv[HW + t];
     }
+    static_assert(13 <= SWEEP_PLANES && 2 + NS <= SWEEP_PLANES, "batch wider than the exchange ring");
     solve_d12<9, 4, real, M, UNI>(d1, d2, c, r);
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
 #pragma unroll
