@@ -112,14 +112,16 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
                         const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                         hipStream_t st);
 // part non-null: the sweep also leaves per-workgroup CFL partials there
-// (one k_reduce_state slot set per workgroup) for launch_reduce_final
+// (maximum and NaN count, two doubles per workgroup) for launch_reduce_cfl
 template <typename real>
 hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
                           const LineMap &lm, const Ptr5<real> &io, const Phys<real> &ph,
                           double *part, hipStream_t st);
 constexpr int RED_SLOTS_HOST = 16;
-hipError_t launch_reduce_final(const double *part, int nb, double *out, StepScalars *sc,
-                               hipStream_t st);
+// the filter's CFL partials into the reduction outputs (slot 0 the CFL
+// maximum, slot 7 the NaN count) and StepScalars::cfl_max
+hipError_t launch_reduce_cfl(const double *part, int nb, double *out, StepScalars *sc,
+                             hipStream_t st);
 template <typename real>
 hipError_t launch_reduce(const Fields<real> &F, const Phys<real> &ph, long N, int diag,
                          double *part, int nb, double *out, StepScalars *sc, hipStream_t st);

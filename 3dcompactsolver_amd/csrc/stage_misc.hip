@@ -35,8 +35,7 @@ __device__ __forceinline__ double cfl_speed(const Phys<real> &ph, real rp, real 
 }
 
 // RED: the last sweep of the filter also leaves, per workgroup, the CFL
-// maximum and the NaN count of its output (slots 0 and 7 of the
-// k_reduce_state partials; the other slots zero)
+// maximum and the NaN count of its output (two doubles, for k_reduce_cfl)
 template <typename real, int M, bool UNI, int MAP, bool RED>
 __global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
                                                   Ptr5<real> io, Phys<real> ph,
@@ -96,8 +95,10 @@ __global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<
             }
             __syncthreads();
         }
-        if (tid < RED_SLOTS)
-            part[(long)blockIdx.x * RED_SLOTS + tid] = tid == 0 ? red[0] : tid == 7 ? red[nt] : 0.0;
+        if (tid == 0) {
+            part[2 * (long)blockIdx.x] = red[0];
+            part[2 * (long)blockIdx.x + 1] = red[nt];
+        }
     }
 }
 
@@ -192,6 +193,38 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__re
         for (int s = 0; s < RED_SLOTS; s++) out[s] = sh[s][0];
         sc->cfl_max = sh[0][0];
         if (sh[7][0] > 0.0 || sh[0][0] > 1e300) sc->nonfinite = 1;
+    }
+}
+
+// The filter's CFL partials (maximum, NaN count per workgroup) into the
+// k_reduce_final outputs: slot 0 the maximum, slot 7 the NaN count, the
+// other slots zero.
+constexpr int CFL_THREADS = 1024;
+__global__ __launch_bounds__(CFL_THREADS) void k_reduce_cfl(const double *__restrict__ part, int nb,
+                                                            double *__restrict__ out,
+                                                            StepScalars *sc)
+{
+    __shared__ double sm[2][CFL_THREADS];
+    double mx = 0.0, nan = 0.0;
+    for (int b = threadIdx.x; b < nb; b += CFL_THREADS) {
+        mx = fmax(mx, part[2 * (long)b]);
+        nan += part[2 * (long)b + 1];
+    }
+    sm[0][threadIdx.x] = mx;
+    sm[1][threadIdx.x] = nan;
+    __syncthreads();
+    for (int w = CFL_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sm[0][threadIdx.x] = fmax(sm[0][threadIdx.x], sm[0][threadIdx.x + w]);
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < RED_SLOTS)
+        out[threadIdx.x] = threadIdx.x == 0 ? sm[0][0] : threadIdx.x == 7 ? sm[1][0] : 0.0;
+    if (threadIdx.x == 0) {
+        sc->cfl_max = sm[0][0];
+        if (sm[1][0] > 0.0 || sm[0][0] > 1e300) sc->nonfinite = 1;
     }
 }
 
@@ -306,10 +339,10 @@ hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeo
                 : launch_f5<real, false>(fo, uni, M, lg, map, lm, io, ph, part, st);
 }
 
-hipError_t launch_reduce_final(const double *part, int nb, double *out, StepScalars *sc,
-                               hipStream_t st)
+hipError_t launch_reduce_cfl(const double *part, int nb, double *out, StepScalars *sc,
+                             hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st, part, nb, out, sc);
+    hipLaunchKernelGGL(k_reduce_cfl, dim3(1), dim3(CFL_THREADS), 0, st, part, nb, out, sc);
     return hipGetLastError();
 }
 
