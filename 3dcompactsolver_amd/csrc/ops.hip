@@ -30,23 +30,23 @@ void PlanDev<real>::release()
     tables = nullptr;
 }
 
-OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32)
+OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32, int seg)
 {
     // cross-segment terms below this weight are dropped: a sixth (FP32) and
     // a tenth (FP64) of the unit roundoff, relative to the carried value
     const double tol = fp32 ? 1e-8 : 1e-17;
     // uniform kernels only for periodic lines that split into equal
     // segments of the uniform length; everything else takes the table kernels
-    const int MU = kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
+    const int MU = seg > 0 ? seg : kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
     if (periodic && n % MU == 0) return build_plan(kind, periodic, n, dh, alpha_f, MU, tol, true);
     return build_plan(kind, periodic, n, dh, alpha_f, SEG_TABLE, tol, false);
 }
 
 template <typename real>
-void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f)
+void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f, int seg)
 {
     pl.release();
-    OpHost h = plan_for(kind, periodic, n, dh, alpha_f, sizeof(real) == 4);
+    OpHost h = plan_for(kind, periodic, n, dh, alpha_f, sizeof(real) == 4, seg);
     pl.host = h;
     OpDev<real> d;
     std::memset(&d, 0, sizeof(d));
@@ -220,8 +220,8 @@ hipError_t launch_field_op(const PlanDev<real> &pl, int map, const LineMap &lm,
 
 template struct PlanDev<float>;
 template struct PlanDev<double>;
-template void make_plan<float>(PlanDev<float> &, int, int, int, double, double);
-template void make_plan<double>(PlanDev<double> &, int, int, int, double, double);
+template void make_plan<float>(PlanDev<float> &, int, int, int, double, double, int);
+template void make_plan<double>(PlanDev<double> &, int, int, int, double, double, int);
 template hipError_t launch_field_op<float>(const PlanDev<float> &, int, const LineMap &,
                                            const float *, float *, hipStream_t);
 template hipError_t launch_field_op<double>(const PlanDev<double> &, int, const LineMap &,

@@ -17,7 +17,9 @@ OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f, int 
                   bool allow_uniform);
 // the plan the library uses for an operator in a precision (segment length
 // and truncation chosen as the kernels expect)
-OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32);
+// seg > 0: the uniform segment length (default: SEG_FILTER for the filter,
+// SEG_DERIV for the derivatives)
+OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32, int seg = 0);
 
 // Points per lane.  Derivatives keep 4 points per lane in both precisions
 // (the fused sweeps hold many windows at once); the periodic filter keeps 8;
@@ -38,6 +40,9 @@ template <typename real>
 constexpr int kc_d2() { return sizeof(real) == 4 ? 3 : 6; }
 template <typename real>
 constexpr int kc_filter() { return sizeof(real) == 4 ? 12 : 25; }
+// the filter on 4-point segments (fused into the x sweep): 23 / 49 at 0.49
+template <typename real>
+constexpr int kc_filter_x() { return sizeof(real) == 4 ? 23 : 49; }
 
 template <typename real>
 struct PlanDev {
@@ -56,7 +61,8 @@ struct PlanDev {
 // std::invalid_argument for unsupported sizes, std::runtime_error on HIP
 // failures.
 template <typename real>
-void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f);
+void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f,
+               int seg = 0);
 
 // out = op(in) on every line of lm; map is MAP_ROW for contiguous lines.
 template <typename real>

@@ -81,6 +81,11 @@ struct SolverT : SolverBase {
     long N;
     double dxyz[3];
     PlanDev<real> D1[3], D2[3], FL[3];
+    // the x filter on the derivative segmentation, fused into the last
+    // stage's x sweep when x lines are uniform and periodic (fx_ok, FP32;
+    // CS3D_NO_FILTER_FUSION in the environment turns it off)
+    PlanDev<real> FLX;
+    bool fx_ok = false;
     LaunchGeom sweepG[3], filterG[3];
     LineMap lm[3];
     Fields<real> F;
@@ -119,6 +124,15 @@ struct SolverT : SolverBase {
             throw std::invalid_argument("grid too large: every field must stay below 4 GiB");
         const int dims[3] = {Nx, Ny, Nz};
         const double Ls[3] = {c.Lx, c.Ly, c.Lz};
+        for (int d = 0; d < 3; d++) dxyz[d] = (1.0 / ((double)dims[d] - 1.0)) * Ls[d];
+        // FP32 only: in FP64 the 4-point filter needs 49 carry terms and the
+        // fused sweep costs as much as the separate one (measured 256^3:
+        // FP32 step -1.2%, FP64 +0.2%)
+        if (sizeof(real) == 4 && c.bc_type[0] == CS3D_PERIODIC_SOLVE && Nx % SEG_DERIV == 0 &&
+            !std::getenv("CS3D_NO_FILTER_FUSION")) {
+            make_plan(FLX, OP_FILTER, 1, Nx, dxyz[0], c.alpha_f, SEG_DERIV);
+            fx_ok = FLX.host.uniform && FLX.host.M == SEG_DERIV;
+        }
         for (int d = 0; d < 3; d++) {
             // Domain.hpp:30-45
             dxyz[d] = (1.0 / ((double)dims[d] - 1.0)) * Ls[d];
@@ -129,7 +143,11 @@ struct SolverT : SolverBase {
             if (D1[d].host.P != D2[d].host.P || D1[d].host.uniform != D2[d].host.uniform)
                 throw std::invalid_argument("inconsistent derivative plans");
             const int P = D1[d].host.P;
-            const int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
+            int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
+            if (d == 0 && fx_ok) {
+                fx_ok = D1[0].host.uniform && FLX.host.P == P;
+                if (fx_ok) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
+            }
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
                                   SWEEP_STAGE_FIELDS * D1[d].host.M);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
@@ -213,6 +231,7 @@ struct SolverT : SolverBase {
             D2[d].release();
             FL[d].release();
         }
+        FLX.release();
         for (void *p : adjalloc) (void)hipFree(p);
         if (mem) (void)hipFree(mem);
         if (dsc) (void)hipFree(dsc);
@@ -458,7 +477,9 @@ struct SolverT : SolverBase {
         timeStep++;
     }
 
-    void sweeps(int rkStep, int mode, int diag)
+    // fuse: the last stage of a filtering step runs the x filter in its
+    // x sweep (fx_ok), writing the x-filtered state to *1
+    void sweeps(int rkStep, int mode, int diag, bool fuse = false)
     {
         StageArgs sa = base;
         sa.rkStep = rkStep;
@@ -469,7 +490,8 @@ struct SolverT : SolverBase {
         // y, then z, then x (the last sweep assembles and updates)
         for (int d : {1, 2, 0})
             timed(d, [&] {
-                hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, D1[d].host.uniform, D1[d].host.M,
+                hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, (fuse && d == 0) ? FLX.dev : nullptr,
+                                      D1[d].host.uniform, D1[d].host.M,
                                       sweepG[d], lm[d], F, ph, sa, dsc, st),
                    "sweep");
             });
@@ -496,13 +518,13 @@ struct SolverT : SolverBase {
     // filterConservedData (CSolver_AWS.cpp:1682-1893): *2 -> *1.  With
     // cfl set, the last sweep also leaves the CFL partials of its output;
     // returns their number of workgroups (0 when it did not).
-    int filter_conserved(bool cfl)
+    int filter_conserved(bool cfl, bool fused = false)
     {
         if (timeStep % cfg.filter_step == 0) {
             filterTimeStep++;
             int o[3];
-            filter_order(filterTimeStep % 3, o);
-            filter_sweep(o[0], F.acc, F.q);
+            filter_order(filterTimeStep % 3, o, fused);
+            if (!fused) filter_sweep(o[0], F.acc, F.q);
             filter_sweep(o[1], F.q, F.acc);
             // the partials need 2 doubles per lane of the sweep's LDS
             const bool fuse = cfl && filterG[o[2]].shm >= 2 * sizeof(double) *
@@ -519,13 +541,14 @@ struct SolverT : SolverBase {
     {
         need_ic();
         int diag = (diag_on && rkLast) ? 1 : 0;
-        sweeps(rkStep, 0, diag);
+        const bool fuse = fx_ok && rkLast && timeStep % cfg.filter_step == 0;
+        sweeps(rkStep, 0, diag, fuse);
         if (rkStep == 1) stale_pending = false;
         if (rkLast) {
             // calcDtFromCFL's maximum for the next step, and the step's
             // diagnostics, from the filtered state before the sponge update;
             // without diagnostics the last filter sweep forms the partials
-            const int nb = filter_conserved(!diag);
+            const int nb = filter_conserved(!diag, fuse);
             timed(4, [&] {
                 if (nb)
                     hc(launch_reduce_cfl(dfpart, nb, dout, dsc, st), "reduce");
@@ -1017,11 +1040,23 @@ struct SolverT : SolverBase {
     }
 
     // the direction order filter_conserved uses for filter count m
-    static void filter_order(int m, int o[3])
+    // filterConservedData's round robin (filter count m: X Y Z, Y Z X, Z X Y);
+    // fused: x first (in the last x sweep), the other two in their order.
+    // Filters along different axes commute, so the orders differ by
+    // rounding only.
+    static void filter_order(int m, int o[3], bool fused = false)
     {
         if (m == 1) { o[0] = 0; o[1] = 1; o[2] = 2; }
         else if (m == 2) { o[0] = 1; o[1] = 2; o[2] = 0; }
         else { o[0] = 2; o[1] = 0; o[2] = 1; }
+        if (fused) {
+            int r[2], n = 0;
+            for (int i = 0; i < 3; i++)
+                if (o[i] != 0) r[n++] = o[i];
+            o[0] = 0;
+            o[1] = r[0];
+            o[2] = r[1];
+        }
     }
 
     // the transposed filter of filter count m: the three sweeps in reverse
@@ -1029,7 +1064,7 @@ struct SolverT : SolverBase {
     void filter_transposed(real *const *in, real *const *out, real *const *scratch, int m)
     {
         int o[3];
-        filter_order(m, o);
+        filter_order(m, o, fx_ok);
         filter_sweep(o[2], in, out);
         filter_sweep(o[1], out, scratch);
         filter_sweep(o[0], scratch, out);

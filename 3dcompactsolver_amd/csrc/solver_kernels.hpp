@@ -107,7 +107,8 @@ struct Ptr5 {
 
 // launchers (stage_*.hip)
 template <typename real>
-hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
+                        const OpDev<real> *fx, int uni, int M,
                         const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
                         const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                         hipStream_t st);

@@ -96,6 +96,16 @@ __device__ __forceinline__ int sweep_kr(const OpDev<real> *d1, const OpDev<real>
     return k > kc_deriv<real>() ? k : kc_deriv<real>();
 }
 
+// ring margin of the x sweep that also runs the fused x filter
+template <typename real>
+__device__ __forceinline__ int xfilt_kr(const OpDev<real> *d1, const OpDev<real> *d2,
+                                        const OpDev<real> *fx)
+{
+    int k = sweep_kr(d1, d2);
+    k = fx->K > k ? fx->K : k;
+    return kc_filter_x<real>() > k ? kc_filter_x<real>() : k;
+}
+
 // q windows a0..a4 (perturbation form) and ir = 1/rho at window index t
 #define CS3D_U(t) (a1[t] * ir[t])
 #define CS3D_V(t) (a2[t] * ir[t])
@@ -128,9 +138,11 @@ template <typename real, bool UNI>
 hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, const StageArgs &sa, hipStream_t st);
+// fx non-null: the fused x filter of the step's last stage (stage_x.hip)
 template <typename real, bool UNI>
-hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
-                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                        const StageArgs &sa, const StepScalars *sc, hipStream_t st);
+hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const OpDev<real> *fx,
+                        const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
+                        const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
+                        hipStream_t st);
 
 }  // namespace cs3d

@@ -284,7 +284,8 @@ __global__ void k_sponge(Fields<real> F, Phys<real> ph, StageArgs sa, const Step
 // ---------------------------------------------------------- host launchers
 
 template <typename real>
-hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
+                        const OpDev<real> *fx, int uni, int M,
                         const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
                         const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                         hipStream_t st)
@@ -296,8 +297,8 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
     // sweep order of a stage: y (first), z (middle), x (last, with the
     // pointwise assembly and the stage update)
     if (which == 0)
-        return uni ? launch_xpass<real, true>(d1, d2, lg, lm, F, ph, sa, sc, st)
-                   : launch_xpass<real, false>(d1, d2, lg, lm, F, ph, sa, sc, st);
+        return uni ? launch_xpass<real, true>(d1, d2, fx, lg, lm, F, ph, sa, sc, st)
+                   : launch_xpass<real, false>(d1, d2, fx, lg, lm, F, ph, sa, sc, st);
     if (which == 1)
         return uni ? launch_ypass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
                    : launch_ypass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
@@ -376,7 +377,8 @@ hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const Stag
 }
 
 #define CS3D_INST(real)                                                                          \
-    template hipError_t launch_sweep<real>(int, const OpDev<real> *, const OpDev<real> *, int, int, \
+    template hipError_t launch_sweep<real>(int, const OpDev<real> *, const OpDev<real> *,         \
+                                           const OpDev<real> *, int, int,                        \
                                            const LaunchGeom &, const LineMap &,                 \
                                            const Fields<real> &, const Phys<real> &,            \
                                            const StageArgs &, const StepScalars *, hipStream_t); \

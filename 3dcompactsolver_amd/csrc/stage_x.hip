@@ -31,13 +31,17 @@ namespace cs3d {
 // QS: the stage state at the point stays in registers for the update (the
 // sponge source, the TVDRK3 combinations of stages 2 and 3, and *1 itself
 // at the first stage, where the stage state is *1)
-template <typename real, int M, bool UNI, bool STALE, bool QS>
+// FILT: last stage of a step that filters, on uniform periodic x lines: the
+// new state is filtered along x in place of the x filter sweep and written
+// to *1 (fx: the x filter with this sweep's segmentation)
+template <typename real, int M, bool UNI, bool STALE, bool QS, bool FILT>
 __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
-    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
-    Fields<real> F, Phys<real> ph, StageArgs sa, const StepScalars *__restrict__ sc)
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2,
+    const OpDev<real> *__restrict__ fx, LineMap lm, Fields<real> F, Phys<real> ph, StageArgs sa,
+    const StepScalars *__restrict__ sc)
 {
     constexpr int HW = 2, W = M + 2 * HW;
-    CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), SWEEP_PLANES)
+    CS3D_LANE_SETUP(MAP_ROW, d1, FILT ? xfilt_kr(d1, d2, fx) : sweep_kr(d1, d2), SWEEP_PLANES)
     const bool use_k = sa.rkStep > 1;
     const real *q0 = use_k ? F.qk[0] : F.q[0];
     const real *q1 = use_k ? F.qk[1] : F.q[1];
@@ -315,6 +319,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const real ta = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.75) : real(1) / real(3);
     const real tb = sa.rkStep == 1 ? real(0) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
     const real tc = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
+    real fin[FILT ? 5 : 1][M];  // the step's new state (FILT)
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         real qn1[M], o[M];
@@ -335,7 +340,12 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             }
 #pragma unroll
             for (int t = 0; t < M; t++) o[t] += ra * k[v][t];
-            win.store(F.acc[v], o, c.len);
+            if constexpr (FILT) {
+#pragma unroll
+                for (int t = 0; t < M; t++) fin[v][t] = o[t];
+            } else {
+                win.store(F.acc[v], o, c.len);
+            }
             if (has_next) {
 #pragma unroll
                 for (int t = 0; t < M; t++) o[t] = qn1[t] + rb * k[v][t];
@@ -345,39 +355,94 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             // TVDRK3.hpp:225-260: stages 1-2 write *k, stage 3 writes *2
 #pragma unroll
             for (int t = 0; t < M; t++) o[t] = ta * qn1[t] + tb * qs_[v][t] + tc * k[v][t];
-            win.store(tvd_to_k ? F.qkw[v] : F.acc[v], o, c.len);
+            if constexpr (FILT) {
+#pragma unroll
+                for (int t = 0; t < M; t++) fin[v][t] = o[t];
+            } else {
+                win.store(tvd_to_k ? F.qkw[v] : F.acc[v], o, c.len);
+            }
+        }
+    }
+    if constexpr (FILT) {
+        // filterConservedData's x sweep (CSolver_AWS.cpp:1682-1893) on the
+        // new state in registers: out = f - A^-1 (A - B) f.  The halo (four
+        // points on each side: the whole neighbouring segments) comes
+        // through the LDS ring, free past the solves; lines are periodic
+        // and uniform, so the neighbours are segments g -+ 1 mod P.
+        static_assert(M == 4, "the fused filter takes whole neighbouring segments as halo");
+        constexpr int FH = 4, FW = M + 2 * FH;
+        real *stg = c.xb;
+        const int T = blockDim.x, me = threadIdx.x;
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+            for (int t = 0; t < M; t++) stg[(v * M + t) * T + me] = fin[v][t];
+        __syncthreads();
+        const int ml = c.line * P + (c.g == 0 ? P - 1 : c.g - 1);
+        const int mr = c.line * P + (c.g == P - 1 ? 0 : c.g + 1);
+        real r[5][M];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            real w[FW];
+#pragma unroll
+            for (int t = 0; t < M; t++) {
+                w[t] = stg[(v * M + t) * T + ml];
+                w[FH + t] = fin[v][t];
+                w[FH + M + t] = stg[(v * M + t) * T + mr];
+            }
+            rhs<OP_FILTER, real, M, FH, true>(fx, c, w, r[v]);
+        }
+        solve_batch<5, kc_filter_x<real>(), real, M, true>(fx, c, r);
+        if (active) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+#pragma unroll
+                for (int t = 0; t < M; t++) fin[v][t] -= r[v][t];
+                win.store(F.q[v], fin[v], c.len);
+            }
         }
     }
 }
 
-template <typename real, bool UNI, bool STALE>
-static hipError_t launch_x3(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
-                            const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                            const StageArgs &sa, const StepScalars *sc, hipStream_t st)
+template <typename real, bool UNI, bool STALE, bool FILT>
+static hipError_t launch_x4(const OpDev<real> *d1, const OpDev<real> *d2, const OpDev<real> *fx,
+                            const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
+                            const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
+                            hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
     if (F.sigma || sa.rk == 3 || sa.rkStep == 1)
-        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true>), grid, block, lg.shm, st,
-                           d1, d2, lm, F, ph, sa, sc);
+        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true, FILT>), grid, block, lg.shm,
+                           st, d1, d2, fx, lm, F, ph, sa, sc);
     else
-        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, false>), grid, block, lg.shm, st,
-                           d1, d2, lm, F, ph, sa, sc);
+        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, false, FILT>), grid, block,
+                           lg.shm, st, d1, d2, fx, lm, F, ph, sa, sc);
     return hipGetLastError();
 }
 
 template <typename real, bool UNI>
-hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
-                        const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
-                        const StageArgs &sa, const StepScalars *sc, hipStream_t st)
+hipError_t launch_xpass(const OpDev<real> *d1, const OpDev<real> *d2, const OpDev<real> *fx,
+                        const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
+                        const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
+                        hipStream_t st)
 {
     if (lg.threads > X_SWEEP_THREADS) return hipErrorInvalidConfiguration;
-    return sa.stale ? launch_x3<real, UNI, true>(d1, d2, lg, lm, F, ph, sa, sc, st)
-                    : launch_x3<real, UNI, false>(d1, d2, lg, lm, F, ph, sa, sc, st);
+    if (fx) {
+        // the fused filter: uniform periodic lines only, never a stale
+        // stage (the last stage of a step), never the right-hand side alone
+        if (!UNI || sa.stale || sa.mode != 0) return hipErrorInvalidValue;
+        if constexpr (UNI)
+            return launch_x4<real, true, false, true>(d1, d2, fx, lg, lm, F, ph, sa, sc, st);
+    }
+    return sa.stale ? launch_x4<real, UNI, true, false>(d1, d2, nullptr, lg, lm, F, ph, sa, sc, st)
+                    : launch_x4<real, UNI, false, false>(d1, d2, nullptr, lg, lm, F, ph, sa, sc, st);
 }
 
 #define CS3D_INST_X(real, UNI)                                                                  \
     template hipError_t launch_xpass<real, UNI>(const OpDev<real> *, const OpDev<real> *,          \
-                                                const LaunchGeom &, const LineMap &,               \
+                                                const OpDev<real> *, const LaunchGeom &,           \
+                                                const LineMap &,                                   \
                                                 const Fields<real> &, const Phys<real> &,          \
                                                 const StageArgs &, const StepScalars *,            \
                                                 hipStream_t);
