@@ -8,9 +8,10 @@ isothermal density; FP32 bulk arithmetic.  The same workload in FP64 is
 timed beside it.  Inputs are resident in HBM before the timed region.
 
 value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
-% roof = value * 307 B / 8 TB/s (307 B = algorithmic bytes per point-stage
-         of RK4 + filter in FP32, 614 B in FP64: 307 field accesses per
-         point and step, listed in STAGE_ACCESSES).
+% roof = value * bytes per point-stage / 8 TB/s: the algorithmic field
+         accesses per point and step (STAGE_ACCESSES: 297 in FP32, where
+         the x filter runs inside the last x sweep, 307 in FP64) times the
+         element size, over 4 stages.
 roofline = the dominant kernel class, timed with HIP events recorded on the
          solver's own stream around every launch of the timed region.
 adjoint = cost of the kinetic-energy gradient through 8 RK4 steps of the
@@ -47,23 +48,32 @@ HBM_PEAK = 8.0e12
 # conserved fields, Uy Vy Uz Wz, Wy Ty Vz Tz and both partial sums (23),
 # then *1 and *2 from the second stage on (at the first stage *1 is the
 # stage state it already holds, and *2 starts from it), and writes *2 and,
-# before the last stage, *k.  Each filter sweep reads and writes the 5
-# conserved fields; the CFL reduction rides on the last one.
-STAGE_ACCESSES = {
-    "y_sweep": [14, 14, 14, 14],
-    "z_sweep": [16, 16, 16, 16],
-    "x_sweep": [23 + 10, 33 + 10, 33 + 10, 33 + 5],
-    "filter": [10, 10, 10],
-}
+# before the last stage, *k (at the last stage of an FP32 step the filtered
+# new state goes to *1 instead of *2: the x filter runs inside the sweep).
+# Each filter sweep reads and writes the 5 conserved fields; the CFL
+# reduction rides on the last one.  FP32 runs two filter sweeps (y, z),
+# FP64 three.
+def stage_accesses(prec):
+    return {
+        "y_sweep": [14, 14, 14, 14],
+        "z_sweep": [16, 16, 16, 16],
+        "x_sweep": [23 + 10, 33 + 10, 33 + 10, 33 + 5],
+        "filter": [10, 10] if prec == cs3d.FP32 else [10, 10, 10],
+    }
+
+
+STAGE_ACCESSES = {p: stage_accesses(p) for p in (cs3d.FP32, cs3d.FP64)}
 # per launch, averaged over the launches of a step
-ACCESSES = {k: sum(v) / len(v) for k, v in STAGE_ACCESSES.items()}
-ACCESSES_PER_STEP = sum(sum(v) for v in STAGE_ACCESSES.values())  # 307
+ACCESSES = {p: {k: sum(v) / len(v) for k, v in a.items()} for p, a in STAGE_ACCESSES.items()}
+# 297 (FP32) / 307 (FP64)
+ACCESSES_PER_STEP = {p: sum(sum(v) for v in a.values()) for p, a in STAGE_ACCESSES.items()}
 # HBM accesses per point actually moved per launch (reads, writes), FP32 at
 # 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
 # 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
 MEASURED_TRAFFIC = {"x_sweep": (30.50, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.01, 9.00)}
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
-BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP * 4 / 4, cs3d.FP64: ACCESSES_PER_STEP * 8 / 4}
+BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP[cs3d.FP32] * 4 / 4,
+                         cs3d.FP64: ACCESSES_PER_STEP[cs3d.FP64] * 8 / 4}
 
 
 def tgv_ic(N, L, mach=0.1):
@@ -128,10 +138,11 @@ def run_adjoint(N, prec, ic, nsteps, checkpoint, fwd_ms):
 
 def roofline(prof, N, prec):
     s = 4 if prec == cs3d.FP32 else 8
-    best = max((k for k in ACCESSES), key=lambda k: prof[k][0])
+    acc = ACCESSES[prec]
+    best = max((k for k in acc), key=lambda k: prof[k][0])
     tot_ms, cnt = prof[best]
     avg_s = tot_ms / cnt * 1e-3
-    alg_bytes = int(ACCESSES[best] * s * N ** 3)
+    alg_bytes = int(acc[best] * s * N ** 3)
     achieved = alg_bytes / avg_s / 1e9
     traffic = None
     if prec == cs3d.FP32 and N == 256 and best in MEASURED_TRAFFIC:
