@@ -114,8 +114,10 @@ def test_emulated_plan_matches_oracle(n, bc, kind, prec):
     f = rng.standard_normal(n)
     ref = po.line_op(kind, bc, f, dh=dh, alpha_f=0.49)
     got = emulate(p, kind, f)
-    # FP32 plans drop cross-segment terms below 1e-10; FP64 plans below 1e-20
-    tol = 1e-9 if prec == cs3d.FP32 else 1e-13
+    # FP32 plans drop cross-segment terms below 1e-8 (a sixth of FP32's unit
+    # roundoff), FP64 plans below 1e-17; the emulation runs in FP64, so the
+    # difference is the truncation alone
+    tol = 2e-8 if prec == cs3d.FP32 else 1e-13
     assert relerr(got, ref) < tol
 
 
