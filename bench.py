@@ -22,6 +22,8 @@ adjoint = cost of the kinetic-energy gradient through 8 RK4 steps of the
 cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
          with the reference's flags) timed on this host's cores on a
          bounded sample.
+hbm_copy_GBps = a 1 GiB device-to-device hipMemcpy, read + write bytes per
+         second: the practical HBM ceiling beside the 8 TB/s peak.
 
 Only one GPU: --gpus N with N > 1 exits with a message.
 """
@@ -154,6 +156,44 @@ def roofline(prof, N, prec):
                   "avg_launch_ms": round(tot_ms / cnt, 4)}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip() + f" ({os.cpu_count()} logical CPUs)"
+    except OSError:
+        pass
+    return "unknown"
+
+
+def copy_bandwidth():
+    """device-to-device copy of 1 GiB through the HIP runtime (5 timed after
+    2 warm-up): the practical HBM ceiling beside the 8 TB/s peak, in bytes
+    read + written per second"""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        return None
+    try:
+        hip.hipMemset(a, 0, ctypes.c_size_t(n))
+        for _ in range(2):
+            hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)
+        hip.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t0) / 5
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return round(2 * n / dt / 1e9, 1)
+
+
 def cpu_baseline(N, steps):
     """FP64 OpenMP restatement of the reference (liboracle_fast.so, reference
     compiler flags), TGV N^3, one warm-up step then `steps` timed steps."""
@@ -171,7 +211,7 @@ def cpu_baseline(N, steps):
     mpts = N ** 3 * 4 * steps / (t1 - t0) / 1e6
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(mpts, 3), "unit": "Mpoint-RK-stages/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": cpu_model(),
             "sample": f"Taylor-Green {N}^3 FP64 RK4 + filter, {steps} steps after 1 warm-up "
                       f"({t1 - t0:.1f} s of CPU time on {threads} OpenMP threads)"}
 
@@ -225,6 +265,7 @@ def main():
     ck = args.adjoint_checkpoint if args.adjoint_checkpoint > 0 else args.adjoint_steps
     adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, ck, ms32)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_grid, args.cpu_steps)
+    copy_bw = copy_bandwidth()
     line = {
         "metric": METRIC,
         "value": round(v32, 1),
@@ -246,6 +287,7 @@ def main():
         "fp64": fp64,
         "adjoint": adj,
         "roofline": roof,
+        "hbm_copy_GBps": copy_bw,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
