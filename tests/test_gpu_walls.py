@@ -116,3 +116,17 @@ def test_couette_fp32_tracks_fp64():
     errs = [relerr(out[cs3d.FP32][v], out[cs3d.FP64][v]) for v in (0, 1, 4)]
     print("couette fp32 vs fp64: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
     assert max(errs) < 1e-5
+
+
+def test_adjoint_refuses_walls():
+    """the adjoint does not cover walls (here on periodic z lines, which the
+    reference allows, so that the periodic-lines check passes)"""
+    dims = (20, 18, 16)
+    kw = dict(Lx=2.0, Ly=2.0, Lz=2.0, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+              filter_step=1, rk=4, bc_type=(0, 0, 0), bc_kind=(0, 0, 0, 0, ADIA, MOVE),
+              wall_vel=VEL)
+    g = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP64, **kw))
+    g.set_initial_conditions(*random_ic((16, 18, 20), 3, amp=0.1))
+    with pytest.raises(cs3d.CS3DError, match="wall"):
+        g.ke_gradient(1, 1)
+    g.close()
