@@ -221,7 +221,12 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         Rw[t] += mu[t] * r[7][t];
         RE[t] += mu[t] * (Wc[t] * r[7][t]);
     }
-    if (!active) return;
+    // inactive lanes (a partial last workgroup) stop here, except where the
+    // fused filter still needs every lane at its barriers; they then skip
+    // every store
+    if constexpr (!FILT) {
+        if (!active) return;
+    }
 
     const real dt = (real)sc->dt;
     real k[5][M];
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
 #pragma unroll
             for (int v = 1; v < 5; v++) k[v][t] = real(0);
         }
-        if (sa.diag && valid) {
+        if (sa.diag && valid && active) {
             // per-point inputs of the step diagnostics that come from this
             // stage's derivatives (CSolver_AWS.cpp:2190-2225)
             const unsigned i = win.mid(t);
