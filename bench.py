@@ -69,10 +69,13 @@ STAGE_ACCESSES = {p: stage_accesses(p) for p in (cs3d.FP32, cs3d.FP64)}
 ACCESSES = {p: {k: sum(v) / len(v) for k, v in a.items()} for p, a in STAGE_ACCESSES.items()}
 # 297 (FP32) / 307 (FP64)
 ACCESSES_PER_STEP = {p: sum(sum(v) for v in a.values()) for p, a in STAGE_ACCESSES.items()}
-# HBM accesses per point actually moved per launch (reads, writes), FP32 at
-# 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a
-# 256 MiB copy (profiles/hbm_traffic_tgv256_fp32.md)
-MEASURED_TRAFFIC = {"x_sweep": (30.50, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.01, 9.00)}
+# HBM accesses per point actually moved per launch (reads, writes) at 256^3,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 256 MiB
+# copy (profiles/hbm_traffic_tgv256_fp32.md, profiles/hbm_traffic_tgv256_fp64.md)
+MEASURED_TRAFFIC = {
+    cs3d.FP32: {"x_sweep": (30.50, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.01, 9.00)},
+    cs3d.FP64: {"x_sweep": (30.50, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.02, 9.00)},
+}
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
 BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP[cs3d.FP32] * 4 / 4,
                          cs3d.FP64: ACCESSES_PER_STEP[cs3d.FP64] * 8 / 4}
@@ -147,8 +150,8 @@ def roofline(prof, N, prec):
     alg_bytes = int(acc[best] * s * N ** 3)
     achieved = alg_bytes / avg_s / 1e9
     traffic = None
-    if prec == cs3d.FP32 and N == 256 and best in MEASURED_TRAFFIC:
-        traffic = int(sum(MEASURED_TRAFFIC[best]) * s * N ** 3)
+    if N == 256 and best in MEASURED_TRAFFIC[prec]:
+        traffic = int(sum(MEASURED_TRAFFIC[prec][best]) * s * N ** 3)
     return best, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
                   "unit": "GB/s", "frac": round(achieved / (HBM_PEAK / 1e9), 4),
                   "traffic": traffic, "kernel": best,
@@ -231,6 +234,8 @@ def main():
     ap.add_argument("--adjoint-checkpoint", type=int, default=0,
                     help="steps between adjoint checkpoints (default: all steps, i.e. every "
                          "stage state of the run stays in HBM and nothing is recomputed)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="run only the FP64 leg and print its step time (for profiling it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 256 MiB device copies (the reference kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -251,6 +256,12 @@ def main():
     N = args.grid
     kw, L = tgv_kw(N)
     ic = tgv_ic(N, L)
+    if args.fp64_only:
+        ms64, v64, prof64, _ = run_gpu(N, cs3d.FP64, args.steps, args.warmup, ic)
+        print(json.dumps({"fp64_ms_per_step": round(ms64, 3), "value": round(v64, 1),
+                          "kernel_ms_per_step": {k: round(v[0] / args.steps, 3)
+                                                 for k, v in prof64.items()}}), flush=True)
+        return
     ms32, v32, prof32, ke32 = run_gpu(N, cs3d.FP32, args.steps, args.warmup, ic)
     best, roof = roofline(prof32, N, cs3d.FP32)
     fp64 = None
