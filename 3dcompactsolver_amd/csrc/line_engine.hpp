@@ -180,6 +180,16 @@ __device__ __forceinline__ void sto(real *__restrict__ f, unsigned boff, real v)
     *(real *)((char *)f + boff) = v;
 }
 
+// a store, non-temporal when NT
+template <bool NT, typename T>
+__device__ __forceinline__ void st_hint(T v, T *p)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 // A lane's window f[s-HW .. s+M+HW) of one line.
 //
 // Scalar form: the byte offset of every window point (periodic lines wrap,
@@ -268,7 +278,11 @@ struct Win {
             for (int t = 0; t < M; t++) r[t] = ldo(f, off[HW + t]);
         }
     }
-    // store the segment's M values (the first len of them on table plans)
+    // store the segment's M values (the first len of them on table plans).
+    // NT: non-temporal stores, for results the next kernels read from HBM
+    // anyway (the sweeps' outputs: 4% faster FP32 steps, 3% FP64), so that
+    // they do not displace the windows being read from the caches
+    template <bool NT = false>
     __device__ __forceinline__ void store(real *__restrict__ f, const real (&r)[M], int len) const
     {
         if (VEC) {
@@ -277,12 +291,12 @@ struct Win {
                 vec v;
 #pragma unroll
                 for (int e = 0; e < VW; e++) v[e] = r[c * VW + e];
-                *(vec *)((char *)f + off[LO / VW + c]) = v;
+                st_hint<NT>(v, (vec *)((char *)f + off[LO / VW + c]));
             }
         } else {
 #pragma unroll
             for (int t = 0; t < M; t++)
-                if (t < len) sto(f, off[HW + t], r[t]);
+                if (t < len) st_hint<NT>(r[t], (real *)((char *)f + off[HW + t]));
         }
     }
 
@@ -354,6 +368,7 @@ struct Win {
             }
         }
     }
+    template <bool NT = false>
     __device__ __forceinline__ void store_pair(real *__restrict__ f, const real (&ra)[M],
                                                const real (&rb)[M], int len) const
     {
@@ -368,7 +383,7 @@ struct Win {
                         const int t = c * VW + h * (VW / 2) + e / 2;
                         v[e] = e % 2 ? rb[t] : ra[t];
                     }
-                    *(vec *)(pair_at(f, off[LO / VW + c]) + 16 * h) = v;
+                    st_hint<NT>(v, (vec *)(pair_at(f, off[LO / VW + c]) + 16 * h));
                 }
             }
         } else {
@@ -378,7 +393,7 @@ struct Win {
                     vec2 v;
                     v[0] = ra[t];
                     v[1] = rb[t];
-                    *(vec2 *)pair_at(f, off[HW + t]) = v;
+                    st_hint<NT>(v, (vec2 *)pair_at(f, off[HW + t]));
                 }
         }
     }
@@ -401,6 +416,7 @@ struct Win {
             r3[t] = v[3];
         }
     }
+    template <bool NT = false>
     __device__ __forceinline__ void store_quad(real *__restrict__ f, const real (&r0)[M],
                                                const real (&r1)[M], const real (&r2)[M],
                                                const real (&r3)[M], int len) const
@@ -413,7 +429,7 @@ struct Win {
                 v[1] = r1[t];
                 v[2] = r2[t];
                 v[3] = r3[t];
-                *(vec4 *)quad_at(f, mid(t)) = v;
+                st_hint<NT>(v, (vec4 *)quad_at(f, mid(t)));
             }
     }
 };

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     }
     if (sa.mode == 1) {
 #pragma unroll
-        for (int v = 0; v < 5; v++) win.store(F.k2[v], k[v], c.len);
+        for (int v = 0; v < 5; v++) win.template store<true>(F.k2[v], k[v], c.len);
         return;
     }
     // stage-update coefficients, uniform over the launch
@@ -349,12 +349,12 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
 #pragma unroll
                 for (int t = 0; t < M; t++) fin[v][t] = o[t];
             } else {
-                win.store(F.acc[v], o, c.len);
+                win.template store<true>(F.acc[v], o, c.len);
             }
             if (has_next) {
 #pragma unroll
                 for (int t = 0; t < M; t++) o[t] = qn1[t] + rb * k[v][t];
-                win.store(F.qkw[v], o, c.len);
+                win.template store<true>(F.qkw[v], o, c.len);
             }
         } else {
             // TVDRK3.hpp:225-260: stages 1-2 write *k, stage 3 writes *2
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
 #pragma unroll
                 for (int t = 0; t < M; t++) fin[v][t] = o[t];
             } else {
-                win.store(tvd_to_k ? F.qkw[v] : F.acc[v], o, c.len);
+                win.template store<true>(tvd_to_k ? F.qkw[v] : F.acc[v], o, c.len);
             }
         }
     }
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             for (int v = 0; v < 5; v++) {
 #pragma unroll
                 for (int t = 0; t < M; t++) fin[v][t] -= r[v][t];
-                win.store(F.q[v], fin[v], c.len);
+                win.template store<true>(F.q[v], fin[v], c.len);
             }
         }
     }

@@ -117,8 +117,8 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
         solve_batch<I1, kc_deriv<real>(), real, M, UNI>(d1, c, *(real(*)[I1][M])r);
         solve_d12<4, 4, real, M, UNI>(d1, d2, c, *(real(*)[8][M])(r + I1));
         if (active) {
-            win.store(F.mix[0], r[5], c.len);
-            win.store(F.mix[1], r[6], c.len);
+            win.template store<true>(F.mix[0], r[5], c.len);
+            win.template store<true>(F.mix[1], r[6], c.len);
         }
     } else {
         solve_d12<N1, 4, real, M, UNI>(d1, d2, c, r);
@@ -143,10 +143,10 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
         RE[t] += mu * (Wc * r[N1 + 3][t]);
     }
     if (active) {
-        win.store_pair(F.dYut, r[I1 + 1], r[I1 + 0], c.len);
-        win.store_pair(F.dYvw, r[I1 + 2], r[I1 + 3], c.len);
-        win.store_quad(F.Ry4, R0, Ru, Rv, Rw, c.len);
-        win.store(F.RyE, RE, c.len);
+        win.template store_pair<true>(F.dYut, r[I1 + 1], r[I1 + 0], c.len);
+        win.template store_pair<true>(F.dYvw, r[I1 + 2], r[I1 + 3], c.len);
+        win.template store_quad<true>(F.Ry4, R0, Ru, Rv, Rw, c.len);
+        win.template store<true>(F.RyE, RE, c.len);
     }
 }
 

@@ -146,8 +146,8 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         RE[t] += mu[t] * (Wc[t] * (c43 * r[12][t]));
     }
     if (active) {
-        win.store_pair(F.dZuw, r[6], r[8], c.len);
-        win.store_pair(F.dZvt, r[7], r[5], c.len);
+        win.template store_pair<true>(F.dZuw, r[6], r[8], c.len);
+        win.template store_pair<true>(F.dZvt, r[7], r[5], c.len);
     }
     {
         // Vyz, Wyz from the stored Vy, Wy (and the stale D_z U, D_z W)
@@ -182,16 +182,16 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
         }
         if constexpr (STALE) {
             if (active) {
-                win.store(F.mix[2], m[2], c.len);
-                win.store(F.mix[3], m[3], c.len);
+                win.template store<true>(F.mix[2], m[2], c.len);
+                win.template store<true>(F.mix[3], m[3], c.len);
             }
         }
     }
     if (!active) return;
     // a partial right-hand side of its own (the last sweep adds both): no
     // read-modify-write on the column path
-    win.store_quad(F.Rz4, R0, Ru, Rv, Rw, c.len);
-    win.store(F.RzE, RE, c.len);
+    win.template store_quad<true>(F.Rz4, R0, Ru, Rv, Rw, c.len);
+    win.template store<true>(F.RzE, RE, c.len);
 }
 
 template <typename real, bool UNI>
