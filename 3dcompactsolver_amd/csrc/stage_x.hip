@@ -324,6 +324,10 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const real ta = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.75) : real(1) / real(3);
     const real tb = sa.rkStep == 1 ? real(0) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
     const real tc = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
+    // RK4's last stage takes only the accumulator, not the step's base
+    // state: the fused-filter sweep, which runs only there, skips that load
+    // (a run-time skip in the other sweeps measured slower in FP64)
+    const bool need_base = !FILT || sa.rk != 4 || has_next || first;
     real fin[FILT ? 5 : 1][M];  // the step's new state (FILT)
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -331,8 +335,11 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         if (QS && first) {
 #pragma unroll
             for (int t = 0; t < M; t++) qn1[t] = qs_[v][t];
-        } else {
+        } else if (need_base) {
             win.load_seg(F.q[v], qn1);
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; t++) qn1[t] = real(0);
         }
         if (sa.rk == 4) {
             // RK4.hpp:45-111: *2 accumulates the stages, *k is the next
