@@ -780,6 +780,96 @@ __device__ __forceinline__ void solve_batch2(const OpDev<real> *__restrict__ op1
     for (int i = 0; i < N; i++) finish<real, M, UNI, MAP>(i < N1 ? op1 : op2, c, r[i], C[i]);
 }
 
+// ------------------------------------------------------------ wavefront carries
+//
+// Lines of exactly 64 uniform periodic segments with MAP_ROW lanes: one line
+// is one wavefront, segment g on lane g.  The forward carry into segment g,
+// sum_{k>=1} uphi[k-1] e_{g-k} (cyclic), is then the cyclic weighted scan
+// sum_{i>=0} lam^i x_{g-i} of x_g = e_{g-1}: a weighted inclusive scan inside
+// each row of 16 lanes (four DPP row shifts, zero past the row's edge), the
+// four row totals (readlane), and each row total once per pass round the
+// line, weighted lam^(distance) / (1 - lam^64).  Backward the same mirrored
+// (x_g = b_{g+1}, row shifts to the left, totals at each row's first lane).
+// No LDS, no barrier, and the sum is not truncated.
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float read_lane(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double read_lane(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// this lane's weights of the four row totals, forward (w[0]) and backward
+template <typename real>
+__device__ __forceinline__ void wave_row_weights(const OpDev<real> *__restrict__ op, int lane,
+                                                 real (&w)[2][4])
+{
+    const int r = lane >> 4, j = lane & 15;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        w[0][q] = op->wrow[0][j + 1 + 16 * ((r - q - 1) & 3)];
+        w[1][q] = op->wrow[1][16 - j + 16 * ((q - r - 1) & 3)];
+    }
+}
+
+template <bool FWD, typename real>
+__device__ __forceinline__ real wave_carry(real v, const real *__restrict__ wl,
+                                           const real (&wq)[4], int lane)
+{
+    real x = __shfl(v, FWD ? (lane + 63) & 63 : (lane + 1) & 63, 64);
+    constexpr int B = FWD ? 0x110 : 0x100;  // DPP row_shr:n / row_shl:n
+    x += wl[0] * dpp_move<B + 1>(x);
+    x += wl[1] * dpp_move<B + 2>(x);
+    x += wl[2] * dpp_move<B + 4>(x);
+    x += wl[3] * dpp_move<B + 8>(x);
+    real out = x;
+#pragma unroll
+    for (int q = 0; q < 4; q++) out += wq[q] * read_lane(x, FWD ? 16 * q + 15 : 16 * q);
+    return out;
+}
+
+// solve_batch2 on lines of 64 uniform periodic segments, one per wavefront
+// (every lane of the wavefront must take part)
+template <int N1, int N2, typename real, int M, int MAP>
+__device__ __forceinline__ void solve_batch_wave(const OpDev<real> *__restrict__ op1,
+                                                 const OpDev<real> *__restrict__ op2,
+                                                 Lane<real, MAP> &c, real (&r)[N1 + N2][M])
+{
+    static_assert(MAP == MAP_ROW, "segment g must sit on lane g");
+    constexpr int N = N1 + N2;
+    const int lane = threadIdx.x & 63;
+    real w1[2][4], w2[2][4];
+    wave_row_weights(op1, lane, w1);
+    if constexpr (N2 > 0) wave_row_weights(op2, lane, w2);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const OpDev<real> *op = i < N1 ? op1 : op2;
+        const real e = fwd_local<real, M, true, MAP>(op, c, r[i]);
+        const real C = wave_carry<true>(e, op->wlam[0], i < N1 ? w1[0] : w2[0], lane);
+        const real b = bwd_local<real, M, true, MAP>(op, c, r[i], C);
+        const real D = wave_carry<false>(b, op->wlam[1], i < N1 ? w1[1] : w2[1], lane);
+        finish<real, M, true, MAP>(op, c, r[i], D);
+    }
+}
+
 template <int N, int KC, typename real, int M, bool UNI, int MAP>
 __device__ __forceinline__ void solve_batch(const OpDev<real> *__restrict__ op,
                                             Lane<real, MAP> &c, real (&r)[N][M])

@@ -78,6 +78,25 @@ void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, doub
             d.uphi[k] = (real)h.phi[k];
             d.upsi[k] = (real)h.psi[k];
         }
+        if (h.cyclic) {
+            // segment transfer factors: the products of -m and of mu over
+            // one segment (plan.cpp's Pf, Pb)
+            const double lam[2] = {h.pf[h.M - 1], h.pb[0]};
+            for (int s = 0; s < 2; s++) {
+                double p = lam[s];
+                for (int i = 0; i < 4; i++) {
+                    d.wlam[s][i] = (real)p;
+                    p *= p;
+                }
+                double l64 = 1.0;
+                for (int k = 0; k < 64; k++) l64 *= lam[s];
+                double pk = 1.0;
+                for (int k = 0; k <= 64; k++) {
+                    d.wrow[s][k] = (real)(pk / (1.0 - l64));
+                    pk *= lam[s];
+                }
+            }
+        }
     }
     // tables: m iw mu pf pb (n each), phi psi (P*K each), sstart slen (P)
     size_t nreal = 5 * (size_t)h.n + 2 * (size_t)h.P * h.K;

@@ -62,6 +62,13 @@ struct OpDev {
     real upb[MAX_SEG];   // backward fix-up, local index t
     real uphi[KMAX];     // C_g = sum_k uphi[k-1] e_{g-k}
     real upsi[KMAX];     // D_g = sum_k upsi[k-1] b_{g+k}
+    // the same carries summed over a whole line of 64 segments held by one
+    // wavefront (uniform periodic plans): with lam the segment transfer
+    // factor of the forward (0) or backward (1) recurrence,
+    // uphi[k-1] = lam^(k-1) / (1 - lam^P); wlam[d][i] = lam^(2^i) and
+    // wrow[d][k] = lam^k / (1 - lam^64)
+    real wlam[2][4];
+    real wrow[2][65];
     // table path (device pointers into the plan's own allocation)
     const real *m, *iw, *mu, *pf, *pb;  // per point, length n
     const real *phi, *psi;              // [P][K]

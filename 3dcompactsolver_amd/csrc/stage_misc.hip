@@ -82,6 +82,32 @@ __global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<
         // 2 * blockDim doubles)
         extern __shared__ double red[];
         const int nt = blockDim.x, tid = threadIdx.x;
+        if (nt % 64 == 0) {
+            // whole wavefronts: across each wavefront by lane shuffles (no
+            // barrier), then one value per wavefront through LDS
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                umax = fmax(umax, __shfl_xor(umax, o));
+                nan += __shfl_xor(nan, o);
+            }
+            const int nw = nt / 64;
+            __syncthreads();
+            if ((tid & 63) == 0) {
+                red[tid / 64] = umax;
+                red[nw + tid / 64] = nan;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double a = red[0], b = red[nw];
+                for (int i = 1; i < nw; i++) {
+                    a = fmax(a, red[i]);
+                    b += red[nw + i];
+                }
+                part[2 * (long)blockIdx.x] = a;
+                part[2 * (long)blockIdx.x + 1] = b;
+            }
+            return;
+        }
         __syncthreads();
         red[tid] = umax;
         red[nt + tid] = nan;

@@ -1,0 +1,77 @@
+"""Lines of 256 points along x: a line is 64 segments of 4 points, one
+wavefront.  The x filter fused into the last FP32 stage then takes its
+cross-segment carries from a wavefront scan instead of the LDS ring.
+Against the oracle: FP64 within 1e-12 per field after every step (x lines
+of 64 segments through the LDS ring, TVDRK3 and RK4); FP32 with the fused
+filter within FP32 rounding of the same steps, and against the same grid
+run with the filter unfused (truncated LDS carries)."""
+import numpy as np
+import pytest
+
+import cs3d
+import pyoracle as po
+from helpers import random_ic, relerr
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (256, 8, 8)
+KW = dict(Lx=8.0, Ly=0.25, Lz=0.25, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+          filter_step=1, bc_type=(0, 0, 0))
+
+
+def ic():
+    Nx, Ny, Nz = DIMS
+    rho, U, V, W, p = random_ic((Nz, Ny, Nx), 21, amp=0.2)
+    # y and z lines of 8 points: keep the state smooth along them
+    x = np.arange(Nx) * (KW["Lx"] / (Nx - 1))
+    U = U + 0.3 * np.sin(2 * np.pi * x / KW["Lx"])[None, None, :]
+    return rho, U, V, W, p
+
+
+def check(tag, got, ref, tol):
+    errs = [relerr(got[v], ref[v]) for v in range(5)]
+    print(f"{tag}: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) < tol, (tag, errs)
+
+
+@pytest.mark.parametrize("rk", [4, 3])
+def test_x256_fp64_matches_oracle(rk):
+    o = po.Solver(po.make_config(*DIMS, rk=rk, **KW))
+    g = cs3d.Solver(cs3d.config(*DIMS, precision=cs3d.FP64, rk=rk, **KW))
+    q0 = ic()
+    o.set_ic(*q0)
+    g.set_initial_conditions(*q0)
+    for n in range(1, 4):
+        o.step()
+        g.step(1)
+        check(f"x256 fp64 rk{rk} step {n}", g.conserved(1), o.conserved(1), 1e-12)
+    o.close()
+    g.close()
+
+
+def test_x256_fp32_fused_filter_matches_oracle(monkeypatch):
+    """FP32 with the x filter fused into the last stage (its carries from
+    the wavefront scan) against the oracle, and against the same FP32 run
+    with the filter in separate sweeps (truncated LDS carries)"""
+    o = po.Solver(po.make_config(*DIMS, rk=4, **KW))
+    q0 = ic()
+    o.set_ic(*q0)
+    for _ in range(3):
+        o.step()
+    ref = o.conserved(1)
+    o.close()
+    out = {}
+    for fused in (True, False):
+        if fused:
+            monkeypatch.delenv("CS3D_NO_FILTER_FUSION", raising=False)
+        else:
+            monkeypatch.setenv("CS3D_NO_FILTER_FUSION", "1")
+        g = cs3d.Solver(cs3d.config(*DIMS, precision=cs3d.FP32, rk=4, **KW))
+        g.set_initial_conditions(*q0)
+        g.step(3)
+        out[fused] = g.conserved(1)
+        g.close()
+    # rho' and rhoE' are perturbations of O(0.01) on an O(1) state
+    check("x256 fp32 fused vs oracle", out[True], ref, 2e-5)
+    check("x256 fp32 unfused vs oracle", out[False], ref, 2e-5)
+    check("x256 fp32 fused vs unfused", out[True], out[False], 2e-6)
