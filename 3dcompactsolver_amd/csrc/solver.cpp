@@ -82,7 +82,9 @@ struct SolverT : SolverBase {
     double dxyz[3];
     PlanDev<real> D1[3], D2[3], FL[3];
     // the x filter on the derivative segmentation, fused into the last
-    // stage's x sweep when x lines are uniform and periodic (fx_ok, FP32;
+    // stage's x sweep when x lines are uniform and periodic (fx_ok; in FP64
+    // only on lines of 64 segments, whose filter carries come from the
+    // wavefront scan: through the LDS ring its 4-point plan needs 49 terms;
     // CS3D_NO_FILTER_FUSION in the environment turns it off)
     PlanDev<real> FLX;
     bool fx_ok = false;
@@ -125,11 +127,12 @@ struct SolverT : SolverBase {
         const int dims[3] = {Nx, Ny, Nz};
         const double Ls[3] = {c.Lx, c.Ly, c.Lz};
         for (int d = 0; d < 3; d++) dxyz[d] = (1.0 / ((double)dims[d] - 1.0)) * Ls[d];
-        // FP32 only: in FP64 the 4-point filter needs 49 carry terms and the
-        // fused sweep costs as much as the separate one (measured 256^3:
-        // FP32 step -1.2%, FP64 +0.2%)
-        if (sizeof(real) == 4 && c.bc_type[0] == CS3D_PERIODIC_SOLVE && Nx % SEG_DERIV == 0 &&
-            !std::getenv("CS3D_NO_FILTER_FUSION")) {
+        // FP64 only on lines of 64 segments: through the LDS ring its 4-point
+        // filter needs 49 carry terms and the fused sweep costs as much as
+        // the separate one (measured 256^3 with the ring: FP32 step -1.2%,
+        // FP64 +0.2%)
+        if (c.bc_type[0] == CS3D_PERIODIC_SOLVE && Nx % SEG_DERIV == 0 &&
+            (sizeof(real) == 4 || Nx == 64 * SEG_DERIV) && !std::getenv("CS3D_NO_FILTER_FUSION")) {
             make_plan(FLX, OP_FILTER, 1, Nx, dxyz[0], c.alpha_f, SEG_DERIV);
             fx_ok = FLX.host.uniform && FLX.host.M == SEG_DERIV;
         }
@@ -146,7 +149,7 @@ struct SolverT : SolverBase {
             int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
             if (d == 0 && fx_ok) {
                 fx_ok = D1[0].host.uniform && FLX.host.P == P;
-                if (fx_ok) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
+                if (fx_ok && P != 64) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
             }
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
                                   SWEEP_STAGE_FIELDS * D1[d].host.M);

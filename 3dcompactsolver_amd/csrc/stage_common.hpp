@@ -96,12 +96,15 @@ __device__ __forceinline__ int sweep_kr(const OpDev<real> *d1, const OpDev<real>
     return k > kc_deriv<real>() ? k : kc_deriv<real>();
 }
 
-// ring margin of the x sweep that also runs the fused x filter
+// ring margin of the x sweep that also runs the fused x filter: on lines
+// of 64 segments the filter's carries come from the wavefront scan and do
+// not use the ring (the host sizes the ring the same way)
 template <typename real>
 __device__ __forceinline__ int xfilt_kr(const OpDev<real> *d1, const OpDev<real> *d2,
                                         const OpDev<real> *fx)
 {
     int k = sweep_kr(d1, d2);
+    if (fx->P == 64) return k;
     k = fx->K > k ? fx->K : k;
     return kc_filter_x<real>() > k ? kc_filter_x<real>() : k;
 }

@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the compact RHS pipeline on one MI355X.
 
-Workload (one "step" = one RK4 time step = 4 RHS stages + the 3-sweep filter):
+Workload (one "step" = one RK4 time step = 4 RHS stages + the filter in x, y, z):
 Taylor-Green vortex, 256^3 periodic on [0, 2pi)^3, RK4, 8th-order filter
 every step (alpha_f 0.49), CFL 0.5, Re 1600, Mach 0.1, gamma 1.4, Pr 0.71,
 isothermal density; FP32 bulk arithmetic.  The same workload in FP64 is
@@ -9,8 +9,8 @@ timed beside it.  Inputs are resident in HBM before the timed region.
 
 value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
 % roof = value * bytes per point-stage / 8 TB/s: the algorithmic field
-         accesses per point and step (STAGE_ACCESSES: 292 in FP32, where
-         the x filter runs inside the last x sweep, 307 in FP64) times the
+         accesses per point and step (STAGE_ACCESSES: 292, the x filter
+         running inside the last x sweep) times the
          element size, over 4 stages.
 roofline = the dominant kernel class, timed with HIP events recorded on the
          solver's own stream around every launch of the timed region.
@@ -50,32 +50,32 @@ HBM_PEAK = 8.0e12
 # conserved fields, Uy Vy Uz Wz, Wy Ty Vz Tz and both partial sums (23),
 # then *1 and *2 from the second stage on (at the first stage *1 is the
 # stage state it already holds, and *2 starts from it), and writes *2 and,
-# before the last stage, *k.  At the last stage of an FP32 step the x filter
-# runs inside the sweep: it reads only *2 (not *1) and writes the filtered
-# new state to *1 instead of *2.
+# before the last stage, *k.  At the last stage of the step the x filter
+# runs inside the sweep (x lines of 256 points: 64 segments, one wavefront
+# each; FP32 also at other lengths): it reads only *2 (not *1) and writes the
+# filtered new state to *1 instead of *2.
 # Each filter sweep reads and writes the 5 conserved fields; the CFL
-# reduction rides on the last one.  FP32 runs two filter sweeps (y, z),
-# FP64 three.
+# reduction rides on the last one: two filter sweeps (y, z).
 def stage_accesses(prec):
     return {
         "y_sweep": [14, 14, 14, 14],
         "z_sweep": [16, 16, 16, 16],
-        "x_sweep": [23 + 10, 33 + 10, 33 + 10, (28 if prec == cs3d.FP32 else 33) + 5],
-        "filter": [10, 10] if prec == cs3d.FP32 else [10, 10, 10],
+        "x_sweep": [23 + 10, 33 + 10, 33 + 10, 28 + 5],
+        "filter": [10, 10],
     }
 
 
 STAGE_ACCESSES = {p: stage_accesses(p) for p in (cs3d.FP32, cs3d.FP64)}
 # per launch, averaged over the launches of a step
 ACCESSES = {p: {k: sum(v) / len(v) for k, v in a.items()} for p, a in STAGE_ACCESSES.items()}
-# 292 (FP32) / 307 (FP64)
+# 292 in both precisions
 ACCESSES_PER_STEP = {p: sum(sum(v) for v in a.values()) for p, a in STAGE_ACCESSES.items()}
 # HBM accesses per point actually moved per launch (reads, writes) at 256^3,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 256 MiB
 # copy (profiles/hbm_traffic_tgv256_fp32.md, profiles/hbm_traffic_tgv256_fp64.md)
 MEASURED_TRAFFIC = {
     cs3d.FP32: {"x_sweep": (29.25, 8.75), "y_sweep": (5.00, 9.41), "z_sweep": (7.00, 9.43)},
-    cs3d.FP64: {"x_sweep": (30.50, 8.75), "y_sweep": (5.00, 9.00), "z_sweep": (7.02, 9.00)},
+    cs3d.FP64: {"x_sweep": (29.25, 9.28), "y_sweep": (5.00, 9.45), "z_sweep": (7.00, 9.38)},
 }
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
 BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP[cs3d.FP32] * 4 / 4,

@@ -1,10 +1,10 @@
 """Lines of 256 points along x: a line is 64 segments of 4 points, one
-wavefront.  The x filter fused into the last FP32 stage then takes its
-cross-segment carries from a wavefront scan instead of the LDS ring.
-Against the oracle: FP64 within 1e-12 per field after every step (x lines
-of 64 segments through the LDS ring, TVDRK3 and RK4); FP32 with the fused
-filter within FP32 rounding of the same steps, and against the same grid
-run with the filter unfused (truncated LDS carries)."""
+wavefront.  The x filter, fused into the last stage in both precisions on
+such lines, then takes its cross-segment carries from a wavefront scan
+instead of the LDS ring.  Against the oracle: FP64 within 1e-12 per field
+after every step (TVDRK3 and RK4); FP32 within FP32 rounding of the same
+steps, and against the same grid run with the filter unfused (truncated
+LDS carries)."""
 import numpy as np
 import pytest
 
