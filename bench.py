@@ -74,7 +74,7 @@ ACCESSES_PER_STEP = {p: sum(sum(v) for v in a.values()) for p, a in STAGE_ACCESS
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 256 MiB
 # copy (profiles/hbm_traffic_tgv256_fp32.md, profiles/hbm_traffic_tgv256_fp64.md)
 MEASURED_TRAFFIC = {
-    cs3d.FP32: {"x_sweep": (29.25, 8.75), "y_sweep": (5.00, 9.41), "z_sweep": (7.00, 9.43)},
+    cs3d.FP32: {"x_sweep": (29.25, 8.75), "y_sweep": (5.00, 9.40), "z_sweep": (7.00, 9.44)},
     cs3d.FP64: {"x_sweep": (29.25, 9.28), "y_sweep": (5.00, 9.45), "z_sweep": (7.00, 9.38)},
 }
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
