@@ -10,7 +10,7 @@ import pytest
 
 import cs3d
 import pyoracle as po
-from helpers import random_ic, relerr
+from helpers import random_ic, relerr, shear_ic
 
 pytestmark = pytest.mark.gpu
 
@@ -75,3 +75,26 @@ def test_x256_fp32_fused_filter_matches_oracle(monkeypatch):
     check("x256 fp32 fused vs oracle", out[True], ref, 2e-5)
     check("x256 fp32 unfused vs oracle", out[False], ref, 2e-5)
     check("x256 fp32 fused vs unfused", out[True], out[False], 2e-6)
+
+
+@pytest.mark.parametrize("prec", [cs3d.FP64, cs3d.FP32])
+def test_x256_sponges_matches_oracle(prec):
+    """256-point periodic x lines with sponges on both (non-periodic) y
+    faces, filter every second step: the fused x filter with the sponge
+    forcing, and the stale-primitive first stage after each sponge update"""
+    dims = (256, 24, 8)
+    kw = dict(Lx=32.0, Ly=10.0, Lz=2.0, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+              filter_step=2, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 1, 1, 0, 0),
+              sponge_frac=(0.25, 0.15, 0.25))
+    q0 = shear_ic(*dims, 10.0, 5)
+    o = po.Solver(po.make_config(*dims, **kw))
+    o.set_ic(*q0)
+    g = cs3d.Solver(cs3d.config(*dims, precision=prec, **kw))
+    g.set_initial_conditions(*q0)
+    tol = 1e-12 if prec == cs3d.FP64 else 2e-5
+    for n in range(1, 5):
+        o.step()
+        g.step(1)
+        check(f"x256 sponges prec={prec} step {n}", g.conserved(1), o.conserved(1), tol)
+    o.close()
+    g.close()
