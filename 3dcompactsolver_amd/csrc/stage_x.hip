@@ -227,23 +227,26 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     if constexpr (!FILT) {
         if (!active) return;
     }
-    // FP32 without the fused filter: the update's inputs (the step's base
-    // state, the RK4 accumulator) issued before the pointwise assembly, so
-    // that they arrive during it (3% faster steps; elsewhere the registers
-    // are not there: the fused-filter sweep would lose its second wavefront
-    // per SIMD, FP64 would spill)
-    constexpr bool HOIST = !FILT && sizeof(real) == 4;
-    real qb[HOIST ? 5 : 1][M], ab[HOIST ? 5 : 1][M];
+    // FP32: the update's inputs (the step's base state, the RK4
+    // accumulator) issued before the pointwise assembly, so that they arrive
+    // during it; the fused-filter sweep, which runs at RK4's last stage,
+    // takes only the accumulator.  FP64 keeps the late loads: its registers
+    // are full and it would spill.
+    constexpr bool HOIST = sizeof(real) == 4;
+    constexpr bool HOIST_BASE = HOIST && !FILT;
+    real qb[HOIST_BASE ? 5 : 1][M], ab[HOIST ? 5 : 1][M];
     if constexpr (HOIST) {
         if (sa.mode != 1) {
             const bool first_stage = sa.rkStep == 1;
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                if (QS && first_stage) {
+                if constexpr (HOIST_BASE) {
+                    if (QS && first_stage) {
 #pragma unroll
-                    for (int t = 0; t < M; t++) qb[v][t] = qs_[v][t];
-                } else {
-                    win.load_seg(F.q[v], qb[v]);
+                        for (int t = 0; t < M; t++) qb[v][t] = qs_[v][t];
+                    } else {
+                        win.load_seg(F.q[v], qb[v]);
+                    }
                 }
                 if (sa.rk == 4 && !first_stage) win.load_seg(F.acc[v], ab[v]);
             }
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         real qn1[M], o[M];
-        if constexpr (HOIST) {
+        if constexpr (HOIST_BASE) {
 #pragma unroll
             for (int t = 0; t < M; t++) qn1[t] = qb[v][t];
         } else if (QS && first) {
