@@ -281,7 +281,9 @@ struct Win {
     // store the segment's M values (the first len of them on table plans).
     // NT: non-temporal stores, for results the next kernels read from HBM
     // anyway (the sweeps' outputs: 4% faster FP32 steps, 3% FP64), so that
-    // they do not displace the windows being read from the caches
+    // they do not displace the windows being read from the caches (with
+    // ordinary stores in the y sweep alone, the z sweep after it runs 11%
+    // slower)
     template <bool NT = false>
     __device__ __forceinline__ void store(real *__restrict__ f, const real (&r)[M], int len) const
     {
