@@ -177,3 +177,54 @@ def test_filter_low_wavenumber_gain_in_fp32():
     f = np.sin(2 * np.pi * i / n) + 0.3 * np.cos(2 * np.pi * i / n)
     gain = np.dot(emulate(q, 0, f), f) / np.dot(f, f)
     assert abs(gain - 1) < 1e-12
+
+
+def wave_scan(e, lam, fwd):
+    """the device's wavefront carry scan (wave_carry) on 64 lanes, in FP64:
+    rotate by one lane, weighted inclusive scan inside rows of 16 with zero
+    fill, the four row totals, per-lane weights lam^d / (1 - lam^64)"""
+    x = np.roll(e, 1 if fwd else -1)
+    for d in (1, 2, 4, 8):
+        s = np.zeros(64)
+        for g in range(64):
+            j = g % 16
+            if fwd and j >= d:
+                s[g] = x[g - d]
+            if not fwd and j + d <= 15:
+                s[g] = x[g + d]
+        x = x + lam ** d * s
+    tot = [x[16 * q + 15] if fwd else x[16 * q] for q in range(4)]
+    out = x.copy()
+    for g in range(64):
+        r, j = g // 16, g % 16
+        for q in range(4):
+            k = j + 1 + 16 * ((r - q - 1) % 4) if fwd else 16 - j + 16 * ((q - r - 1) % 4)
+            out[g] += lam ** k / (1 - lam ** 64) * tot[q]
+    return out
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("prec", [cs3d.FP64, cs3d.FP32])
+def test_wavefront_scan_premise(kind, prec):
+    """Lines of 64 uniform periodic segments take their carries from a
+    wavefront scan (x lines of 256 points; the x filter fused into the last
+    stage runs on the 4-point derivative segmentation).  Its premise, on the
+    library's own tables: the carry weights are geometric, phi[k-1] =
+    lam^(k-1) / (1 - lam^P) with lam = pf[M-1] forward and psi with
+    pb[0] backward; and the scan's lane arithmetic equals the direct cyclic
+    sums with the kept weights, to their truncation."""
+    n = 256 if kind else 512   # the filter's own plans use 8-point segments
+    p = plan(kind, 0, n, 0.05, 0.49, prec)
+    M, P, K = p["M"], p["P"], p["K"]
+    assert p["uniform"] and p["cyclic"] and P == 64
+    lf, lb = p["pf"][M - 1], p["pb"][0]
+    k = np.arange(K)
+    np.testing.assert_allclose(p["phi"][0], lf ** k / (1 - lf ** P), rtol=1e-13, atol=0)
+    np.testing.assert_allclose(p["psi"][0], lb ** k / (1 - lb ** P), rtol=1e-13, atol=0)
+    rng = np.random.default_rng(kind)
+    e = rng.standard_normal(64)
+    C = np.array([sum(p["phi"][g, i] * e[(g - 1 - i) % 64] for i in range(K)) for g in range(64)])
+    D = np.array([sum(p["psi"][g, i] * e[(g + 1 + i) % 64] for i in range(K)) for g in range(64)])
+    tol = 1e-7 if prec == cs3d.FP32 else 1e-15   # the tables' truncation
+    assert np.abs(wave_scan(e, lf, True) - C).max() < tol * np.abs(e).sum()
+    assert np.abs(wave_scan(e, lb, False) - D).max() < tol * np.abs(e).sum()
