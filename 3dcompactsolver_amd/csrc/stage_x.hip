@@ -227,12 +227,15 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     if constexpr (!FILT) {
         if (!active) return;
     }
-    // FP32: the update's inputs (the step's base state, the RK4
-    // accumulator) issued before the pointwise assembly, so that they arrive
-    // during it; the fused-filter sweep, which runs at RK4's last stage,
-    // takes only the accumulator.  FP64 keeps the late loads: its registers
-    // are full and it would spill.
-    constexpr bool HOIST = sizeof(real) == 4;
+    // the update's inputs (the step's base state, the RK4 accumulator)
+    // issued before the pointwise assembly, so that they arrive during it
+    // (FP32 3.5% and FP64 3.4% faster steps; FP64, at one wavefront per
+    // SIMD, holds them in AGPRs).  The fused-filter sweep, which runs at
+    // RK4's last stage, takes only the accumulator.  Not in the FP64 STALE
+    // sweep: there the extra registers overflow the AGPRs into scratch, and
+    // that build gave wrong results (7e-8 against the oracle) where the
+    // late loads give 1e-15.
+    constexpr bool HOIST = !(STALE && sizeof(real) == 8);
     constexpr bool HOIST_BASE = HOIST && !FILT;
     real qb[HOIST_BASE ? 5 : 1][M], ab[HOIST ? 5 : 1][M];
     if constexpr (HOIST) {
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const real tc = sa.rkStep == 1 ? real(1) : sa.rkStep == 2 ? real(0.25) : real(2) / real(3);
     // RK4's last stage takes only the accumulator, not the step's base
     // state: the fused-filter sweep, which runs only there, skips that load
-    // (a run-time skip in the other sweeps measured slower in FP64)
+    // (TVDRK3's last stage still takes it, here)
     const bool need_base = !FILT || sa.rk != 4 || has_next || first;
     real fin[FILT ? 5 : 1][M];  // the step's new state (FILT)
 #pragma unroll
