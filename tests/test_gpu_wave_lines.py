@@ -2,9 +2,9 @@
 wavefront.  The x filter, fused into the last stage in both precisions on
 such lines, then takes its cross-segment carries from a wavefront scan
 instead of the LDS ring.  Against the oracle: FP64 within 1e-12 per field
-after every step (TVDRK3 and RK4); FP32 within FP32 rounding of the same
-steps, and against the same grid run with the filter unfused (truncated
-LDS carries)."""
+after every step (TVDRK3 and RK4, with y sponges, with y walls); FP32
+within FP32 rounding of the same steps, and against the same grid run with
+the filter unfused (truncated LDS carries)."""
 import numpy as np
 import pytest
 
@@ -96,5 +96,29 @@ def test_x256_sponges_matches_oracle(prec):
         o.step()
         g.step(1)
         check(f"x256 sponges prec={prec} step {n}", g.conserved(1), o.conserved(1), tol)
+    o.close()
+    g.close()
+
+
+@pytest.mark.parametrize("prec", [cs3d.FP64, cs3d.FP32])
+def test_x256_walls_matches_oracle(prec):
+    """256-point periodic x lines between an adiabatic and a moving wall in
+    y (z periodic), filter every step: the wall rows of the x sweep with the
+    fused x filter on 64-segment lines"""
+    dims = (256, 16, 12)
+    kw = dict(Lx=16.0, Ly=1.5, Lz=1.0, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+              filter_step=1, rk=4, bc_type=(0, 1, 0),
+              bc_kind=(0, 0, cs3d.BC_ADIABATIC_WALL, cs3d.BC_MOVING_ADIABATIC_WALL, 0, 0),
+              wall_vel={3: (0.1, 0.0, 0.05)})
+    q0 = random_ic((12, 16, 256), 13, amp=0.1)
+    o = po.Solver(po.make_config(*dims, **kw))
+    o.set_ic(*q0)
+    g = cs3d.Solver(cs3d.config(*dims, precision=prec, **kw))
+    g.set_initial_conditions(*q0)
+    tol = 1e-12 if prec == cs3d.FP64 else 2e-5
+    for n in range(1, 4):
+        o.step()
+        g.step(1)
+        check(f"x256 walls prec={prec} step {n}", g.conserved(1), o.conserved(1), tol)
     o.close()
     g.close()
