@@ -439,10 +439,15 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             }
             rhs<OP_FILTER, real, M, FH, true>(fx, c, w, r[v]);
         }
-        if (c.P == 64)
+        if (c.P == 64) {
             solve_batch_wave<5, 0, real, M, MAP_ROW>(fx, fx, c, r);
-        else
+        } else {
+            // the exchange ring starts at the staging area: every lane of
+            // the workgroup must have read its halos before any lane writes
+            // the ring (c.P is uniform, so the barrier is too)
+            __syncthreads();
             solve_batch<5, kc_filter_x<real>(), real, M, true>(fx, c, r);
+        }
         if (active) {
 #pragma unroll
             for (int v = 0; v < 5; v++) {

@@ -284,6 +284,11 @@ def make_config(Nx, Ny, Nz, **kw):
 _ref = None
 
 
+def ref_built():
+    """whether _ref/libref.so exists (without loading it)"""
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref.so"))
+
+
 def ref_lib():
     """The reference's own Utils/Derivatives/Filter compiled into
     _ref/libref.so, or None when it was not built (e.g. on a machine

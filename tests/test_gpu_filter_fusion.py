@@ -17,17 +17,20 @@ from helpers import random_ic, relerr, tgv_ic
 pytestmark = pytest.mark.gpu
 
 
-def run(fused, nsteps, filter_step, dims=(64, 64, 64), rk=4):
+def run(fused, nsteps, filter_step, dims=(64, 64, 64), rk=4, env=None):
     N = 64
     L = 2 * np.pi - 2 * np.pi / N
     kw = dict(Lx=L, Ly=L, Lz=L, alpha_f=0.49, mu_ref=(1 / 1600) / 100 ** 0.76, Pr=0.71,
               cfl=0.5, filter_step=filter_step, rk=rk)
+    env = dict(env or {})
     if not fused:
-        os.environ["CS3D_NO_FILTER_FUSION"] = "1"
+        env["CS3D_NO_FILTER_FUSION"] = "1"
+    os.environ.update(env)
     try:
         g = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP32, **kw))
     finally:
-        os.environ.pop("CS3D_NO_FILTER_FUSION", None)
+        for k in env:
+            os.environ.pop(k, None)
     if dims == (64, 64, 64):
         g.set_initial_conditions(*tgv_ic(N, L))
     else:
@@ -62,3 +65,19 @@ def test_fused_x_filter_partial_workgroup(rk):
     errs = [relerr(a[v], b[v]) for v in range(5)]
     print(f"rk{rk} {dims}: fused vs separate " + " ".join(f"{e:.1e}" for e in errs), flush=True)
     assert max(errs) < 5e-6
+
+
+@pytest.mark.parametrize("dims,env", [((512, 8, 8), None),
+                                      ((64, 16, 16), {"CS3D_SWEEP_X_THREADS": "256"})])
+def test_fused_x_filter_multi_wavefront(dims, env):
+    """Workgroups of several wavefronts on the fused filter's LDS path:
+    512-point x lines (128 segments: one line of two wavefronts per
+    workgroup) and 64-point lines (16 segments) in 256-lane workgroups of 16
+    lines.  Every wavefront must have
+    read its halos from the staging area before any of them writes the
+    exchange ring that shares that LDS."""
+    a = run(True, 6, 1, dims, env=env)
+    b = run(False, 6, 1, dims, env=env)
+    errs = [relerr(a[v], b[v]) for v in range(5)]
+    print(f"{dims} {env}: fused vs separate " + " ".join(f"{e:.1e}" for e in errs), flush=True)
+    assert max(errs) < 2e-6
