@@ -89,6 +89,9 @@ struct SolverT : SolverBase {
     PlanDev<real> FLX;
     bool fx_ok = false;
     LaunchGeom sweepG[3], filterG[3];
+    // the z sweep's geometry for its variants with a smaller workgroup
+    // limit (zpass_threads: FP32 table plans and the stale stage)
+    LaunchGeom sweepGz_small;
     LineMap lm[3];
     Fields<real> F;
     Phys<real> ph;
@@ -154,6 +157,15 @@ struct SolverT : SolverBase {
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
                                   SWEEP_STAGE_FIELDS * D1[d].host.M);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
+            if (d == 2) {
+                const int cap = zpass_threads<real, false, true>();
+                sweepGz_small = line_geom(P, kr, std::min(sweep_max_threads(d, P), cap),
+                                          SWEEP_PLANES, sizeof(real),
+                                          SWEEP_STAGE_FIELDS * D1[d].host.M);
+                if (sweepGz_small.LX < 1)
+                    throw std::invalid_argument("line too long for the fused sweeps");
+                if (!D1[d].host.uniform) sweepG[d] = sweepGz_small;
+            }
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
                                    filter_max_threads(d), 5, sizeof(real));
             if (filterG[d].LX < 1) throw std::invalid_argument("line too long for the filter");
@@ -493,9 +505,10 @@ struct SolverT : SolverBase {
         // y, then z, then x (the last sweep assembles and updates)
         for (int d : {1, 2, 0})
             timed(d, [&] {
+                const LaunchGeom &g = (d == 2 && sa.stale) ? sweepGz_small : sweepG[d];
                 hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, (fuse && d == 0) ? FLX.dev : nullptr,
                                       D1[d].host.uniform, D1[d].host.M,
-                                      sweepG[d], lm[d], F, ph, sa, dsc, st),
+                                      g, lm[d], F, ph, sa, dsc, st),
                    "sweep");
             });
     }

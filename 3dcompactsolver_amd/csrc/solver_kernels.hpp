@@ -99,6 +99,22 @@ constexpr int SWEEP_PLANES = 13;
 // most fields whose windows a sweep kernel stages through LDS at once
 constexpr int SWEEP_STAGE_FIELDS = 5;
 
+// workgroup size limit of the sweeps: FP32 fits 4 waves/SIMD (128 VGPRs),
+// FP64 holds twice the registers and runs 2 waves/SIMD
+template <typename real>
+constexpr int sweep_threads() { return sizeof(real) == 4 ? 1024 : 512; }
+
+// the z sweep's FP32 variants for table plans (ragged or non-periodic z
+// lines) and for the stale stage carry more live values than 128 VGPRs: at
+// 1024 lanes they spilled 12-80 bytes per lane to scratch, so they run at
+// most 512 lanes (256 VGPRs); the host launches them with a matching
+// geometry
+template <typename real, bool UNI, bool STALE>
+constexpr int zpass_threads()
+{
+    return (sizeof(real) == 4 && (!UNI || STALE)) ? 512 : sweep_threads<real>();
+}
+
 template <typename real>
 struct Ptr5 {
     const real *in[5];

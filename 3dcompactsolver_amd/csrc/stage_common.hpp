@@ -123,11 +123,6 @@ __device__ __forceinline__ real pprime(const Phys<real> &ph, const real (&a1)[W]
     return ph.gm1 * (a4[t] - real(0.5) * (a1[t] * U + a2[t] * V + a3[t] * Wv));
 }
 
-// workgroup size limit of the sweeps: FP32 fits 4 waves/SIMD (128 VGPRs),
-// FP64 holds twice the registers and runs 2 waves/SIMD
-template <typename real>
-constexpr int sweep_threads() { return sizeof(real) == 4 ? 1024 : 512; }
-
 // the last (x) sweep: lines are contiguous, so small workgroups lose no
 // coalescing and leave each lane the larger register budget its 17 solves
 // and the pointwise assembly need

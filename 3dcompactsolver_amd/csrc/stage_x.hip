@@ -231,11 +231,11 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     // issued before the pointwise assembly, so that they arrive during it
     // (FP32 3.5% and FP64 3.4% faster steps; FP64, at one wavefront per
     // SIMD, holds them in AGPRs).  The fused-filter sweep, which runs at
-    // RK4's last stage, takes only the accumulator.  Not in the FP64 STALE
-    // sweep: there the extra registers overflow the AGPRs into scratch, and
-    // that build gave wrong results (7e-8 against the oracle) where the
-    // late loads give 1e-15.
-    constexpr bool HOIST = !(STALE && sizeof(real) == 8);
+    // RK4's last stage, takes only the accumulator.  The STALE sweep runs
+    // only at a step's first stage (the launcher checks it), where the base
+    // state is the stage state already in registers and RK4 has no
+    // accumulator yet: it has nothing to issue early.
+    constexpr bool HOIST = !STALE;
     constexpr bool HOIST_BASE = HOIST && !FILT;
     real qb[HOIST_BASE ? 5 : 1][M], ab[HOIST ? 5 : 1][M];
     if constexpr (HOIST) {
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         return;
     }
     // stage-update coefficients, uniform over the launch
-    const bool first = sa.rkStep == 1, has_next = sa.rkStep < 4;
+    const bool first = STALE || sa.rkStep == 1, has_next = sa.rkStep < 4;
     const real ra = (sa.rkStep == 1 || sa.rkStep == 4) ? real(1) / real(6) : real(1) / real(3);
     const real rb = sa.rkStep == 3 ? real(1) : real(0.5);
     const bool tvd_to_k = sa.rkStep < 3;
@@ -466,7 +466,10 @@ static hipError_t launch_x4(const OpDev<real> *d1, const OpDev<real> *d2, const 
                             hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
-    if (F.sigma || sa.rk == 3 || sa.rkStep == 1)
+    // the stale primitives exist only at the first stage after a sponge
+    // update, and the STALE kernel takes the first stage as given
+    if (STALE && sa.rkStep != 1) return hipErrorInvalidValue;
+    if (STALE || F.sigma || sa.rk == 3 || sa.rkStep == 1)
         hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true, FILT>), grid, block, lg.shm,
                            st, d1, d2, fx, lm, F, ph, sa, sc);
     else

@@ -21,7 +21,7 @@
 namespace cs3d {
 
 template <typename real, int M, bool UNI, bool STALE>
-__global__ __launch_bounds__(sweep_threads<real>()) void k_zpass(
+__global__ __launch_bounds__((zpass_threads<real, UNI, STALE>())) void k_zpass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     Fields<real> F, Phys<real> ph, int use_k, StageArgs sa)
 {
@@ -200,6 +200,8 @@ hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const Laun
                         int use_k, const StageArgs &sa, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    const int cap = sa.stale ? zpass_threads<real, UNI, true>() : zpass_threads<real, UNI, false>();
+    if (lg.threads > cap) return hipErrorInvalidConfiguration;
     if (sa.stale)
         hipLaunchKernelGGL((k_zpass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,
                            lm, F, ph, use_k, sa);

@@ -21,6 +21,15 @@
 
 #define MAXLINE 2048
 
+/* Mutation switch for the oracle's own tests: tests/test_oracle_sponge.py
+ * compiles private copies of this file with OR_PLANT set to 1..7, each
+ * carrying one plausible mistake in the sponge path, and shows that its
+ * closed-form checks reject every one.  Every library the product is checked
+ * against is built with OR_PLANT = 0. */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 /* ------------------------------------------------------------------ */
 /* Line solvers                                                         */
 /* ------------------------------------------------------------------ */
@@ -837,7 +846,8 @@ static void solve_eqn_set(or_solver *s)
         double spg[5] = {0, 0, 0, 0, 0};
         if (s->sponge)
             for (int v = 0; v < 5; v++)
-                spg[v] = s->sigma[ip] * (s->spgAvg[v][ip] - qs[v][ip]);
+                spg[v] = OR_PLANT == 2 ? s->sigma[ip] * (qs[v][ip] - s->spgAvg[v][ip])
+                                       : s->sigma[ip] * (s->spgAvg[v][ip] - qs[v][ip]);
         double Ux = d[OR_D_Ux][ip], Uy = d[OR_D_Uy][ip], Uz = d[OR_D_Uz][ip];
         double Vx = d[OR_D_Vx][ip], Vy = d[OR_D_Vy][ip], Vz = d[OR_D_Vz][ip];
         double Wx = d[OR_D_Wx][ip], Wy = d[OR_D_Wy][ip], Wz = d[OR_D_Wz][ip];
@@ -892,7 +902,7 @@ static void solve_eqn_set(or_solver *s)
                      V * MuZ * (Vz + Wy) + W * MuX * (Uz + Wx) +
                      W * MuY * (Vz + Wy);
         double eul = -d[OR_D_eX][ip] - d[OR_D_eY][ip] - d[OR_D_eZ][ip] + spg[4];
-        k2[4][ip] = dt * (qtemp + vt1 + vt2 + eul + spg[4]);
+        k2[4][ip] = dt * (qtemp + vt1 + vt2 + eul + (OR_PLANT == 1 ? 0.0 : spg[4]));
     }
 }
 
@@ -921,7 +931,7 @@ static void post_step_bc(or_solver *s)
                 }
     }
     for (int f = 0; f < 6; f++) {
-        if (c->bc_kind[f] != OR_SPONGE) continue;
+        if (c->bc_kind[f] != OR_SPONGE || OR_PLANT == 5) continue;
         int dir = f / 2, hi = f % 2;
         for (int k = 0; k < Nz; k++)
             for (int j = 0; j < Ny; j++)
@@ -1053,21 +1063,23 @@ void or_post_step(or_solver *s)
     const or_config *c = &s->c;
     long N = s->N;
     double **q1 = s->q[0];
-    double eps = 1.0 / (c->sponge_avgT / s->dt + 1.0);
+    double eps = OR_PLANT == 3 ? s->dt / c->sponge_avgT : 1.0 / (c->sponge_avgT / s->dt + 1.0);
     double spP = 1.0 / c->gamma;
+    double epsP = OR_PLANT == 4 ? 1.0 - c->sponge_epsP : c->sponge_epsP;
     double **A = s->spgAvg;
 #pragma omp parallel for
     for (long ip = 0; ip < N; ip++) {
         for (int v = 0; v < 5; v++) A[v][ip] += eps * (q1[v][ip] - A[v][ip]);
-        A[4][ip] = c->sponge_epsP * A[4][ip] +
-                   (1.0 - c->sponge_epsP) *
+        double **K = OR_PLANT == 7 ? q1 : A;
+        A[4][ip] = epsP * A[4][ip] +
+                   (1.0 - epsP) *
                        (spP / (c->gamma - 1.0) +
-                        0.5 * (A[1][ip] * A[1][ip] + A[2][ip] * A[2][ip] +
-                               A[3][ip] * A[3][ip]) / A[0][ip]);
+                        0.5 * (K[1][ip] * K[1][ip] + K[2][ip] * K[2][ip] +
+                               K[3][ip] * K[3][ip]) / K[0][ip]);
     }
     int Nx = c->Nx, Ny = c->Ny, Nz = c->Nz;
     for (int f = 0; f < 6; f++) {
-        if (c->bc_kind[f] != OR_SPONGE) continue;
+        if (c->bc_kind[f] != OR_SPONGE || OR_PLANT == 6) continue;
         int dir = f / 2, hi = f % 2;
         for (int k = 0; k < Nz; k++)
             for (int j = 0; j < Ny; j++)
@@ -1098,6 +1110,7 @@ double *or_field(or_solver *s, int set, int which) { return s->q[set - 1][which]
 double *or_prim(or_solver *s, int which) { return s->prim[which]; }
 double *or_deriv(or_solver *s, int which) { return s->d[which]; }
 const double *or_sponge_sigma(or_solver *s) { return s->sponge ? s->sigma : NULL; }
+double *or_sponge_avg(or_solver *s, int which) { return s->sponge ? s->spgAvg[which] : NULL; }
 
 /* calcTaylorGreenQuantities, CSolver_AWS.cpp:2175-2257 (Q8: derivative
  * arrays are those of the last stage, rho and U those after the filter) */

@@ -132,6 +132,10 @@ void or_diag_turb(or_solver *s, double out[11]);
 
 /* sponge strength field (SpongeBC.hpp:35-154), XYZ, length N (NULL if none) */
 const double *or_sponge_sigma(or_solver *s);
+/* sponge running average of conserved variable `which` (0..4: rho, rhoU,
+ * rhoV, rhoW, rhoE; spongeRhoAvg .. spongeRhoEAvg of SpongeBC.hpp), XYZ,
+ * length N, writable (NULL if no sponge) */
+double *or_sponge_avg(or_solver *s, int which);
 
 /* calcNeumann (Utils.hpp:52): the boundary value that makes the one-sided
  * 7-point first derivative vanish, from the 6 values next to the boundary */

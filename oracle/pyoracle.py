@@ -40,6 +40,11 @@ _lib = None
 _fast = None
 
 
+def load_path(path):
+    """an oracle library built elsewhere (the mutation tests' copies)"""
+    return _load(path)
+
+
 def lib(fast=False):
     """Load liboracle.so (strict) or liboracle_fast.so (reference flags)."""
     global _lib, _fast
@@ -82,6 +87,8 @@ def _load(path):
     L.or_diag_turb.argtypes = [ctypes.c_void_p, _dp]
     L.or_sponge_sigma.restype = _dp
     L.or_sponge_sigma.argtypes = [ctypes.c_void_p]
+    L.or_sponge_avg.restype = _dp
+    L.or_sponge_avg.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.or_solve_tri.argtypes = [_dp] * 5 + [_dp, ctypes.c_int]
     L.or_cyclic.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
                             _dp, ctypes.c_int, _dp]
@@ -162,8 +169,8 @@ def rk_update(rk, rk_step, arrays):
 class Solver:
     """The oracle's solver (FP64, CPU).  Field arrays are (Nz, Ny, Nx)."""
 
-    def __init__(self, cfg: OrConfig, fast=False):
-        self.L = lib(fast)
+    def __init__(self, cfg: OrConfig, fast=False, library=None):
+        self.L = library if library is not None else lib(fast)
         self.cfg = cfg
         self.h = self.L.or_create(ctypes.byref(cfg))
         if not self.h:
@@ -241,6 +248,16 @@ class Solver:
         out = np.zeros(11)
         self.L.or_diag_turb(self.h, _ptr(out))
         return out
+
+    def sponge_avg(self):
+        """the five sponge running averages (rho .. rhoE), or None"""
+        if not self.L.or_sponge_avg(self.h, 0):
+            return None
+        return [self._view(self.L.or_sponge_avg(self.h, v)).copy() for v in range(5)]
+
+    def set_sponge_avg(self, arrays):
+        for v in range(5):
+            self._view(self.L.or_sponge_avg(self.h, v))[...] = arrays[v]
 
     def sponge_sigma(self):
         p = self.L.or_sponge_sigma(self.h)

@@ -12,6 +12,21 @@ def relerr(a, b):
     return float(np.abs(a - b).max() / scale)
 
 
+def state_errs(got, ref, momentum_scale=False):
+    """max-norm errors of the five conserved fields.  Each field is taken
+    relative to its own maximum, or, with momentum_scale, the three momenta
+    relative to the largest momentum of the three: a component that stays
+    near zero by symmetry (rho W of the Taylor-Green vortex) is then judged
+    against the flow's momentum, not against its own rounding-level size."""
+    errs = [relerr(got[v], ref[v]) for v in range(5)]
+    if momentum_scale:
+        m = max(np.abs(np.asarray(ref[v], dtype=np.float64)).max() for v in (1, 2, 3))
+        for v in (1, 2, 3):
+            d = np.asarray(got[v], dtype=np.float64) - np.asarray(ref[v], dtype=np.float64)
+            errs[v] = float(np.abs(d).max() / m)
+    return errs
+
+
 def grid(N, L):
     """x_i = i*L/(N-1) (Domain.hpp:30-45)"""
     return np.arange(N) * (L / (N - 1.0))

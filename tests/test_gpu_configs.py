@@ -6,7 +6,7 @@
   kinetic energy within 1e-5 relative after 1000 steps).
 - Decaying isotropic turbulence 128^3 with TVDRK3 from the restated
   generator (ic.py): the FP64 GPU path against the oracle for two steps,
-  FP32 against FP64 after two steps (2e-5, as for Taylor-Green) and after
+  FP32 against FP64 after two steps (2e-6, as for Taylor-Green) and after
   500 steps, where the turbulent field has amplified the rounding
   difference (measured 2.2e-5 in the kinetic energy; bound 1e-4).
 - Temporal shear layer 512 x 128 x 42, y non-periodic with 15% sponges,
@@ -85,7 +85,7 @@ def test_hit128_tvdrk3():
     _, q32, _ = run((N, N, N), kw, icf, cs3d.FP32, 2)
     errs = [relerr(a, b) for a, b in zip(q32, q64)]
     print("HIT fp32 vs fp64, 2 steps: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
-    assert max(errs) <= 2e-5
+    assert max(errs) <= 2e-6
     k64, q64, _ = run((N, N, N), kw, icf, cs3d.FP64, 500)
     k32, q32, _ = run((N, N, N), kw, icf, cs3d.FP32, 500)
     rel = abs(k32 - k64) / abs(k64)

@@ -37,7 +37,9 @@ def _from_ref(f, d):
 def test_ops_match_oracle(dims, bc, prec):
     rng = np.random.default_rng(sum(dims) + bc)
     Nx, Ny, Nz = dims
-    tol = 1e-12 if prec == cs3d.FP64 else 1e-5
+    # FP32: about 3x the largest measured error (2.7e-6, second derivatives
+    # of random fields)
+    tol = 1e-12 if prec == cs3d.FP64 else 8e-6
     for d in range(3):
         n = dims[d]
         dh = (1.0 / (n - 1.0)) * LENGTHS[d]
@@ -51,9 +53,11 @@ def test_ops_match_oracle(dims, bc, prec):
             ref = po.field_op(kind, d, bc, f, dh=dh, alpha_f=0.49)
             op = flt if kind == 0 else der
             got = op.apply(kind, f)
-            assert relerr(got, ref) < tol, (d, kind)
             got_ref_layout = op.apply(kind, _to_ref(f, d), cs3d.LAYOUT_REFERENCE)
-            assert relerr(_from_ref(got_ref_layout, d), ref) < tol, (d, kind)
+            e1, e2 = relerr(got, ref), relerr(_from_ref(got_ref_layout, d), ref)
+            print(f"  errors {e1:.2e} {e2:.2e}", flush=True)
+            assert e1 < tol, (d, kind)
+            assert e2 < tol, (d, kind)
 
 
 def test_ops_reject_short_lines():

@@ -8,13 +8,15 @@ import pytest
 
 import cs3d
 import pyoracle as po
-from helpers import random_ic, relerr, shear_ic, tgv_ic
+from helpers import random_ic, relerr, shear_ic, state_errs, tgv_ic
 
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12      # per field, max-norm relative, after a step
 FP64_RHS_TOL = 1e-10  # dt*RHS itself (continuity cancels to O(Mach^2))
-FP32_TOL = 2e-5       # FP32 state vs the FP64 oracle after a few steps
+# FP32 state vs the FP64 oracle over the first steps, momenta against the
+# flow's momentum scale: about 3x the largest measured error (8e-7)
+FP32_TOL = 2e-6
 
 
 def pair(dims, ic, prec, **kw):
@@ -27,7 +29,7 @@ def pair(dims, ic, prec, **kw):
 
 
 def check_fields(tag, got, ref, tol):
-    errs = [relerr(got[v], ref[v]) for v in range(5)]
+    errs = state_errs(got, ref, momentum_scale=tol >= FP32_TOL)
     print(f"{tag}: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
     assert max(errs) < tol, (tag, errs)
 
@@ -76,7 +78,8 @@ def test_tgv_steps(prec):
     for n in range(2, 4):  # filter round robin YZX, ZXY
         o.step()
         g.step(1)
-        check_fields(f"tgv prec={prec} step {n}", g.conserved(1), o.conserved(1), tol * 10)
+        check_fields(f"tgv prec={prec} step {n}", g.conserved(1), o.conserved(1),
+                     tol * (10 if prec == cs3d.FP64 else 1))
     t, dt, ts = g.time()
     assert ts == 3 and abs(t - o.time) < 1e-15 + 1e-6 * o.time
 
@@ -150,7 +153,8 @@ def test_shear_layer_sponge_nonperiodic(prec):
             g.sub_step(s, s == 4)
         o.post_step()
         g.post_step()
-        check_fields(f"shear prec={prec} step {n}", g.conserved(1), o.conserved(1), tol * 10)
+        check_fields(f"shear prec={prec} step {n}", g.conserved(1), o.conserved(1),
+                     tol * (10 if prec == cs3d.FP64 else 1))
 
 
 def test_x_and_z_sponges_fp64():
