@@ -90,6 +90,7 @@ def lib():
     L.cs3d_diag_shear_layer.argtypes = [_vp, _int, _dp]
     L.cs3d_get_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_set_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
+    L.cs3d_stream_probe.argtypes = [_int, ctypes.c_size_t, _int, _dp, _dp]
     L.cs3d_stream.argtypes = [_vp]
     L.cs3d_stream.restype = _vp
     L.cs3d_ops_create.argtypes = [_int, _int, _int, ctypes.c_double, ctypes.c_double,
@@ -381,3 +382,12 @@ class Solver:
         v = ctypes.c_double()
         check(lib().cs3d_kinetic_energy(self.h, ctypes.byref(v)))
         return v.value
+
+
+def stream_probe(kind="copy", nbytes=1 << 30, reps=5):
+    """(best, mean) HBM rate in GB/s (bytes read + written) of the library's
+    16-byte-vector copy or triad kernel over arrays of nbytes each"""
+    best, mean = ctypes.c_double(), ctypes.c_double()
+    check(lib().cs3d_stream_probe({"copy": 0, "triad": 1}[kind], nbytes, reps,
+                                  ctypes.byref(best), ctypes.byref(mean)))
+    return best.value, mean.value

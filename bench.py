@@ -8,22 +8,26 @@ isothermal density; FP32 bulk arithmetic.  The same workload in FP64 is
 timed beside it.  Inputs are resident in HBM before the timed region.
 
 value  = point-stages per second (N^3 * 4 * steps / wall time), FP32.
-% roof = value * bytes per point-stage / 8 TB/s: the algorithmic field
-         accesses per point and step (STAGE_ACCESSES: 292, the x filter
-         running inside the last x sweep) times the
-         element size, over 4 stages.
+% roof = value * bytes per point-stage / 8 TB/s, on two bases:
+         hbm_roof_frac_fixed_model: the fixed split-sweep model of the
+         path, 322 field accesses per point and RK4 step with the filter
+         (73 per stage + 30 for the filter), 322 B per point-stage in FP32
+         and 644 B in FP64 -- the basis of the 60% target;
+         hbm_roof_frac: this build's own accesses (STAGE_ACCESSES: 292 per
+         step, the x filter running inside the last x sweep).
 roofline = the dominant kernel class, timed with HIP events recorded on the
          solver's own stream around every launch of the timed region.
-adjoint = cost of the kinetic-energy gradient through 8 RK4 steps of the
-         same workload against 8 forward steps: the forward pass keeps every
-         stage state in HBM (32 states, 10.7 GB at 256^3 FP32), then the
-         reverse sweep; --adjoint-checkpoint K instead keeps checkpoints
-         every K steps and replays segments.
+adjoint = cost of the kinetic-energy gradient through 40 RK4 steps of the
+         same workload with a checkpoint every 20 steps: the forward pass,
+         the replay of the first segment from its checkpoint, and the
+         reverse sweep, against 40 forward steps (reverse_over_forward =
+         (replay + reverse) / forward).
 cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
-         with the reference's flags) timed on this host's cores on a
-         bounded sample.
-hbm_copy_GBps = a 1 GiB device-to-device hipMemcpy, read + write bytes per
-         second: the practical HBM ceiling beside the 8 TB/s peak.
+         with the reference's flags) timed on this host's cores at 256^3
+         (1 warm-up + 3 steps), with a 128^3 sample beside it.
+hbm_stream_GBps = the library's hand-written 16-byte-vector copy and triad
+         kernels over 1 GiB arrays, bytes read + written per second: the
+         HBM rate plain streams reach on this GPU, beside the 8 TB/s peak.
 
 Only one GPU: --gpus N with N > 1 exits with a message.
 """
@@ -80,6 +84,18 @@ MEASURED_TRAFFIC = {
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
 BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP[cs3d.FP32] * 4 / 4,
                          cs3d.FP64: ACCESSES_PER_STEP[cs3d.FP64] * 8 / 4}
+# the fixed split-sweep model: per RK4 stage X 5 + 9, Y 12 + 9, Z with the
+# assembly and update 28 + 10 = 73 accesses; the filter 3 sweeps x 5 fields
+# x (read + write) = 30 per step: 4 * 73 + 30 = 322 per point and step
+FIXED_MODEL_ACCESSES_PER_STEP = 4 * 73 + 30
+FIXED_BYTES_PER_POINT_STAGE = {cs3d.FP32: FIXED_MODEL_ACCESSES_PER_STEP * 4 / 4,
+                               cs3d.FP64: FIXED_MODEL_ACCESSES_PER_STEP * 8 / 4}
+
+
+def roof_fracs(value, prec):
+    """(fixed-model fraction, own-model fraction) of the 8 TB/s peak"""
+    return (round(value * 1e6 * FIXED_BYTES_PER_POINT_STAGE[prec] / HBM_PEAK, 4),
+            round(value * 1e6 * BYTES_PER_POINT_STAGE[prec] / HBM_PEAK, 4))
 
 
 def tgv_ic(N, L, mach=0.1):
@@ -171,36 +187,20 @@ def cpu_model():
     return "unknown"
 
 
-def copy_bandwidth():
-    """device-to-device copy of 1 GiB through the HIP runtime (5 timed after
-    2 warm-up): the practical HBM ceiling beside the 8 TB/s peak, in bytes
-    read + written per second"""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    n = 1 << 30
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or \
-            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
-        return None
-    try:
-        hip.hipMemset(a, 0, ctypes.c_size_t(n))
-        for _ in range(2):
-            hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
-        hip.hipDeviceSynchronize()
-        t0 = time.perf_counter()
-        for _ in range(5):
-            hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)
-        hip.hipDeviceSynchronize()
-        dt = (time.perf_counter() - t0) / 5
-    finally:
-        hip.hipFree(a)
-        hip.hipFree(b)
-    return round(2 * n / dt / 1e9, 1)
+def stream_bandwidth():
+    """the library's copy and triad kernels over 1 GiB arrays (5 timed
+    after 2 warm-up launches): best and mean GB/s"""
+    out = {}
+    for kind in ("copy", "triad"):
+        best, mean = cs3d.stream_probe(kind, 1 << 30, 5)
+        out[kind] = {"best": round(best, 1), "mean": round(mean, 1)}
+    return out
 
 
 def cpu_baseline(N, steps):
     """FP64 OpenMP restatement of the reference (liboracle_fast.so, reference
-    compiler flags), TGV N^3, one warm-up step then `steps` timed steps."""
+    compiler flags), TGV N^3, one warm-up step then `steps` timed steps.
+    The oracle is the CPU baseline here, never the measured GPU path."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     kw, L = tgv_kw(N)
@@ -228,13 +228,15 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-grid", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-grid", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-small-steps", type=int, default=6,
+                    help="timed steps of the 128^3 CPU sample beside the main one (0: none)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
-    ap.add_argument("--adjoint-steps", type=int, default=8)
-    ap.add_argument("--adjoint-checkpoint", type=int, default=0,
-                    help="steps between adjoint checkpoints (default: all steps, i.e. every "
-                         "stage state of the run stays in HBM and nothing is recomputed)")
+    ap.add_argument("--adjoint-steps", type=int, default=40)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=20,
+                    help="steps between adjoint checkpoints; 0: all steps (every stage state "
+                         "of the run stays in HBM and nothing is recomputed)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="run only the FP64 leg and print its step time (for profiling it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -242,13 +244,9 @@ def main():
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print("copy probe: 3 x 256 MiB", flush=True)
+        # 2 untimed + 3 timed copies of 256 MiB by the library's copy kernel
+        best, _ = cs3d.stream_probe("copy", 256 << 20, 3)
+        print(f"copy probe: 5 x 256 MiB, best {best:.0f} GB/s", flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -269,15 +267,23 @@ def main():
     if not args.no_fp64:
         ms64, v64, prof64, ke64 = run_gpu(N, cs3d.FP64, args.steps, args.warmup, ic)
         _, roof64 = roofline(prof64, N, cs3d.FP64)
+        fixed64, own64 = roof_fracs(v64, cs3d.FP64)
         fp64 = {"value": round(v64, 1), "ms_per_step": round(ms64, 3),
-                "hbm_roof_frac": round(v64 * 1e6 * BYTES_PER_POINT_STAGE[cs3d.FP64] / HBM_PEAK, 4),
+                "hbm_roof_frac_fixed_model": fixed64,
+                "hbm_roof_frac": own64,
                 "roofline": roof64,
                 "kinetic_energy": ke64,
                 "fp32_ke_rel_err": abs(ke32 - ke64) / abs(ke64)}
     ck = args.adjoint_checkpoint if args.adjoint_checkpoint > 0 else args.adjoint_steps
     adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, ck, ms32)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_grid, args.cpu_steps)
-    copy_bw = copy_bandwidth()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_grid, args.cpu_steps)
+        if args.cpu_small_steps > 0 and args.cpu_grid != 128:
+            small = cpu_baseline(128, args.cpu_small_steps)
+            cpu["at_128"] = {k: small[k] for k in ("value", "unit", "sample")}
+    stream_bw = stream_bandwidth()
+    fixed32, own32 = roof_fracs(v32, cs3d.FP32)
     line = {
         "metric": METRIC,
         "value": round(v32, 1),
@@ -294,12 +300,13 @@ def main():
         "config": {"workload": f"Taylor-Green vortex {N}^3 periodic, RK4, filter every step "
                                "(alpha_f 0.49), CFL 0.5, Re 1600, Mach 0.1, FP32 bulk",
                    "grid": [N, N, N], "rk": "RK4", "filter_step": 1},
-        "hbm_roof_frac": round(v32 * 1e6 * BYTES_PER_POINT_STAGE[cs3d.FP32] / HBM_PEAK, 4),
+        "hbm_roof_frac_fixed_model": fixed32,
+        "hbm_roof_frac": own32,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof32.items()},
         "fp64": fp64,
         "adjoint": adj,
         "roofline": roof,
-        "hbm_copy_GBps": copy_bw,
+        "hbm_stream_GBps": stream_bw,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -221,6 +221,16 @@ void *cs3d_stream(cs3d_solver *s);
 int cs3d_profile_enable(cs3d_solver *s, int enabled);
 int cs3d_profile_report(cs3d_solver *s, double ms[6], int counts[6]);
 
+/* The HBM rate this GPU sustains on plain streams, from hand-written
+ * 16-byte-vector kernels (no reference counterpart: a measured ceiling for
+ * the sweeps' roofline fractions, and a copy of known size to calibrate
+ * HBM traffic counters).  kind 0: copy b = a (2 arrays), 1: triad
+ * a = b + s c (3 arrays) of `bytes` each; 2 untimed then `reps` timed
+ * launches on the null stream, HIP events around each.  Bytes read plus
+ * written per second (1e9), the best launch and the mean.  Allocates and
+ * frees its arrays. */
+int cs3d_stream_probe(int kind, size_t bytes, int reps, double *best_gbps, double *mean_gbps);
+
 /*
  * Standalone compact operators: Derivatives(dom, bcType, dir)
  * (Derivatives.hpp:54) + calc1stDerivField / calc2ndDerivField
