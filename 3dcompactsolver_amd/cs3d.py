@@ -88,6 +88,7 @@ def lib():
     L.cs3d_adjoint_ke_gradient.argtypes = [_vp, _int, _int] + [_dp] * 4
     L.cs3d_adjoint_release.argtypes = [_vp]
     L.cs3d_diag_shear_layer.argtypes = [_vp, _int, _dp]
+    L.cs3d_diag_shear_layer_noalloc.argtypes = [_vp, _int, _dp]
     L.cs3d_get_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_set_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_stream_probe.argtypes = [_int, ctypes.c_size_t, _int, _dp, _dp]

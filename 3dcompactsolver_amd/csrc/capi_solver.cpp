@@ -233,6 +233,16 @@ int cs3d_diag_shear_layer(cs3d_solver *s, int fix_q12, double *out)
     });
 }
 
+int cs3d_diag_shear_layer_noalloc(cs3d_solver *s, int fix_q12, double *out)
+{
+    CS3D_NEED(s, "cs3d_diag_shear_layer_noalloc");
+    if (!out) return set_error(CS3D_ERR_ARG, "cs3d_diag_shear_layer_noalloc: NULL");
+    return guard("cs3d_diag_shear_layer_noalloc", [&]() -> int {
+        s->impl->shear_profiles_noalloc(fix_q12, out);
+        return CS3D_OK;
+    });
+}
+
 int cs3d_get_run_state(cs3d_solver *s, cs3d_run_state *rs)
 {
     CS3D_NEED(s, "cs3d_get_run_state");

@@ -701,6 +701,16 @@ struct SolverT : SolverBase {
         }
     }
 
+    void shear_profiles_noalloc(int fix_q12, double *out) override
+    {
+        need_ic();
+        if ((long)Ny * Nz > SHEAR_MAX_ROWS || Ny > SHEAR_MAX_NY)
+            throw std::invalid_argument("grid too large for the profiles without allocation: Ny * Nz "
+                                        "must be at most " + std::to_string(SHEAR_MAX_ROWS) +
+                                        " and Ny at most " + std::to_string(SHEAR_MAX_NY));
+        hc(shear_profiles_static<real>(F, ph, Nx, Ny, Nz, fix_q12, out, st), "profiles");
+    }
+
     void get_run_state(cs3d_run_state *rs) override
     {
         need_ic();

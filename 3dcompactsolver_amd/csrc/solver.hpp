@@ -26,6 +26,8 @@ struct SolverBase {
     virtual void get_time(double *time, double *dt, int *ts) = 0;
     virtual void get_run_state(cs3d_run_state *rs) = 0;
     virtual void shear_profiles(int fix_q12, double *out) = 0;
+    // the same without allocating (device memory or host buffers)
+    virtual void shear_profiles_noalloc(int fix_q12, double *out) = 0;
     virtual void set_run_state(const cs3d_run_state &rs) = 0;
     virtual void set_diag(int on) = 0;
     virtual int diag_tgv(double out[7]) = 0;

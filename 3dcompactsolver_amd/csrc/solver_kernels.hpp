@@ -148,6 +148,17 @@ hipError_t launch_dt(StepScalars *sc, double cfl, double dt_fixed, hipStream_t s
 template <typename real>
 hipError_t launch_plane_rows(const Fields<real> &F, const Phys<real> &ph, int Nx, int Ny, int Nz,
                              int pass, const double *mean, double *rows, hipStream_t st);
+// The shear-layer profiles without any allocation (profiles.hip): the same
+// row sums into a file-scope device array, reduced over the planes on the
+// device, the 11 rows of cs3d_diag_shear_layer copied into host_out (11 Ny
+// doubles).  Grids with Ny * Nz above SHEAR_MAX_ROWS or Ny above
+// SHEAR_MAX_NY are refused (hipErrorInvalidValue); calls that overlap on
+// one GPU share its array and race.
+constexpr long SHEAR_MAX_ROWS = 131072;
+constexpr int SHEAR_MAX_NY = 8192;
+template <typename real>
+hipError_t shear_profiles_static(const Fields<real> &F, const Phys<real> &ph, int Nx, int Ny,
+                                 int Nz, int fix_q12, double *host_out, hipStream_t st);
 // preStepBCHandling of one wall face on the stage state (walls.hip)
 template <typename real>
 hipError_t launch_wall_face(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,

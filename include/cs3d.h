@@ -178,6 +178,15 @@ int cs3d_diag_turbulence(cs3d_solver *s, double out[11]);
  * reference's accumulation of v'w' into uwprime (vwprime zero); 1 separates
  * them. */
 int cs3d_diag_shear_layer(cs3d_solver *s, int fix_q12, double *out);
+/* No counterpart in the reference: cs3d_diag_shear_layer for callers that
+ * cannot allocate memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same statistics and output,
+ * with no hipMalloc, hipFree or host buffer: the work arrays live in a
+ * file-scope device array of 10.6 MiB that each GPU holds until the
+ * process ends.  Grids with Ny * Nz above 131072 or Ny above 8192 are
+ * refused with CS3D_ERR_ARG.  Calls that overlap on one GPU share that
+ * array and race; calls on one solver handle never overlap. */
+int cs3d_diag_shear_layer_noalloc(cs3d_solver *s, int fix_q12, double *out);
 /* volume-averaged kinetic energy of the current state, FP64 */
 int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
 
