@@ -21,13 +21,14 @@
 namespace cs3d {
 
 template <typename real>
-__global__ void k_adj_prim(Ptr5<real> q, Phys<real> ph, AdjPrim<real> o, long N)
+__global__ void k_adj_prim(QSet<real> q, Phys<real> ph, AdjPrim<real> o, long N)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long)gridDim.x * blockDim.x) {
         // the forward's own primitive and viscosity evaluations
-        real ir, U, V, W, pp, Tp, mu, amu;
-        prim(ph, q.in[0][i], q.in[1][i], q.in[2][i], q.in[3][i], q.in[4][i], ir, U, V, W, pp, Tp);
+        real ir, U, V, W, pp, Tp, mu, amu, q5[5];
+        qset_load(q, i, q5);
+        prim(ph, q5[0], q5[1], q5[2], q5[3], q5[4], ir, U, V, W, pp, Tp);
         visc(ph, ph.T0s + Tp, mu, amu);
         o.U[i] = U;
         o.V[i] = V;
@@ -419,11 +420,12 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> 
     // D_z v, q
     real pt[19][M];
     {
-        const real *src[19] = {f.tot[0], f.tot[1], f.tot[2], f.tot[3], f.Fy[0], f.Fy[1], f.Fy[2],
-                               f.Fy[3], f.Fy[4], f.Fz[0], f.Fz[1], f.Fz[2], f.Fz[3], f.Fz[4],
-                               f.q[0], f.q[1], f.q[2], f.q[3], f.q[4]};
+        const real *src[14] = {f.tot[0], f.tot[1], f.tot[2], f.tot[3], f.Fy[0], f.Fy[1], f.Fy[2],
+                               f.Fy[3], f.Fy[4], f.Fz[0], f.Fz[1], f.Fz[2], f.Fz[3], f.Fz[4]};
 #pragma unroll
-        for (int k = 0; k < 19; k++) win.load_seg(src[k], pt[k]);
+        for (int k = 0; k < 14; k++) win.load_seg(src[k], pt[k]);
+        win.load_seg_quad(f.q.m, pt[14], pt[15], pt[16], pt[17]);
+        win.load_seg(f.q.e, pt[18]);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             real ty[M];
@@ -517,13 +519,15 @@ hipError_t launch_adj_xfinal(const OpDev<real> *d1, const OpDev<real> *d2, int u
 }
 
 template <typename real>
-__global__ void k_adj_ke_seed(Ptr5<real> q, Phys<real> ph, Ptr5<real> out, long N)
+__global__ void k_adj_ke_seed(QSet<real> q, Phys<real> ph, Ptr5<real> out, long N)
 {
     const real iN = real(1) / real(N);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long)gridDim.x * blockDim.x) {
-        const real rho = ph.rho_ref + q.in[0][i];
-        const real u = q.in[1][i] / rho, v = q.in[2][i] / rho, w = q.in[3][i] / rho;
+        real q5[5];
+        qset_load(q, i, q5);
+        const real rho = ph.rho_ref + q5[0];
+        const real u = q5[1] / rho, v = q5[2] / rho, w = q5[3] / rho;
         out.out[0][i] = -real(0.5) * (u * u + v * v + w * w) * iN;
         out.out[1][i] = u * iN;
         out.out[2][i] = v * iN;
@@ -533,16 +537,18 @@ __global__ void k_adj_ke_seed(Ptr5<real> q, Phys<real> ph, Ptr5<real> out, long 
 }
 
 template <typename real>
-__global__ void k_adj_to_velocity(Ptr5<real> q, Ptr5<real> qb, Phys<real> ph, real *gU, real *gV,
+__global__ void k_adj_to_velocity(QSet<real> q, Ptr5<real> qb, Phys<real> ph, real *gU, real *gV,
                                   real *gW, long N)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long)gridDim.x * blockDim.x) {
-        const real rho = ph.rho_ref + q.in[0][i];
+        real q5[5];
+        qset_load(q, i, q5);
+        const real rho = ph.rho_ref + q5[0];
         const real Eb = qb.in[4][i];
-        gU[i] = rho * (qb.in[1][i] + (q.in[1][i] / rho) * Eb);
-        gV[i] = rho * (qb.in[2][i] + (q.in[2][i] / rho) * Eb);
-        gW[i] = rho * (qb.in[3][i] + (q.in[3][i] / rho) * Eb);
+        gU[i] = rho * (qb.in[1][i] + (q5[1] / rho) * Eb);
+        gV[i] = rho * (qb.in[2][i] + (q5[2] / rho) * Eb);
+        gW[i] = rho * (qb.in[3][i] + (q5[3] / rho) * Eb);
     }
 }
 
@@ -554,7 +560,7 @@ static dim3 grid_for(long N)
 }
 
 template <typename real>
-hipError_t launch_adj_prim(const Ptr5<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
+hipError_t launch_adj_prim(const QSet<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
                            long N, hipStream_t st)
 {
     hipLaunchKernelGGL((k_adj_prim<real>), grid_for(N), dim3(256), 0, st, q, ph, o, N);
@@ -567,14 +573,14 @@ hipError_t launch_adj_lincomb(const AdjComb<real> &a, long N, hipStream_t st)
     return hipGetLastError();
 }
 template <typename real>
-hipError_t launch_adj_ke_seed(const Ptr5<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
+hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
                               long N, hipStream_t st)
 {
     hipLaunchKernelGGL((k_adj_ke_seed<real>), grid_for(N), dim3(256), 0, st, q, ph, out, N);
     return hipGetLastError();
 }
 template <typename real>
-hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
+hipError_t launch_adj_to_velocity(const QSet<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
                                   real *gU, real *gV, real *gW, long N, hipStream_t st)
 {
     hipLaunchKernelGGL((k_adj_to_velocity<real>), grid_for(N), dim3(256), 0, st, q, qb, ph, gU, gV,
@@ -583,7 +589,7 @@ hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, con
 }
 
 #define CS3D_INST_ADJ(real)                                                                       \
-    template hipError_t launch_adj_prim<real>(const Ptr5<real> &, const Phys<real> &,              \
+    template hipError_t launch_adj_prim<real>(const QSet<real> &, const Phys<real> &,              \
                                               const AdjPrim<real> &, long, hipStream_t);         \
     template hipError_t launch_adj_xseed<real>(const OpDev<real> *, const OpDev<real> *, int, int, \
                                                const LaunchGeom &, const LineMap &,              \
@@ -598,9 +604,9 @@ hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, con
                                                 int, const LaunchGeom &, const LineMap &,        \
                                                 const AdjFinal<real> &, const Phys<real> &,      \
                                                 hipStream_t);                                    \
-    template hipError_t launch_adj_ke_seed<real>(const Ptr5<real> &, const Phys<real> &,           \
+    template hipError_t launch_adj_ke_seed<real>(const QSet<real> &, const Phys<real> &,           \
                                                  const Ptr5<real> &, long, hipStream_t);         \
-    template hipError_t launch_adj_to_velocity<real>(const Ptr5<real> &, const Ptr5<real> &,       \
+    template hipError_t launch_adj_to_velocity<real>(const QSet<real> &, const Ptr5<real> &,       \
                                                      const Phys<real> &, real *, real *, real *, \
                                                      long, hipStream_t);
 CS3D_INST_ADJ(float)

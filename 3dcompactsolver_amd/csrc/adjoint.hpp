@@ -59,7 +59,8 @@ struct AdjComb {
 template <typename real>
 struct AdjFinal {
     const real *cU, *cV, *m1, *gxz, *gTx, *s[3], *gq, *v[5];
-    const real *tot[4], *totY[4], *Fy[5], *Fz[5], *q[5];
+    const real *tot[4], *totY[4], *Fy[5], *Fz[5];
+    QSet<real> q;
     real *out[5];
     real *acc[5];
     const real *acc_base[5];
@@ -93,7 +94,7 @@ struct LineTerms {
 };
 
 template <typename real>
-hipError_t launch_adj_prim(const Ptr5<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
+hipError_t launch_adj_prim(const QSet<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
                            long N, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_xseed(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
@@ -106,14 +107,14 @@ hipError_t launch_adj_xfinal(const OpDev<real> *d1, const OpDev<real> *d2, int u
                              const LaunchGeom &g, const LineMap &lm, const AdjFinal<real> &f,
                              const Phys<real> &ph, hipStream_t st);
 template <typename real>
-hipError_t launch_adj_ke_seed(const Ptr5<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
+hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
                               long N, hipStream_t st);
 template <typename real>
 hipError_t launch_line_terms(int n1, int n2, bool acc, const OpDev<real> *d1,
                              const OpDev<real> *d2, int uni, int M, const LaunchGeom &g, int map,
                              const LineMap &lm, const LineTerms<real> &a, hipStream_t st);
 template <typename real>
-hipError_t launch_adj_to_velocity(const Ptr5<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
+hipError_t launch_adj_to_velocity(const QSet<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
                                   real *gU, real *gV, real *gW, long N, hipStream_t st);
 
 }  // namespace cs3d

@@ -405,6 +405,19 @@ struct Win {
     {
         return (const char *)f + 4ull * o;
     }
+    // the window of a quad field: one vector load per window point
+    __device__ __forceinline__ void load_quad(const real *__restrict__ f, real (&w0)[W],
+                                              real (&w1)[W], real (&w2)[W], real (&w3)[W]) const
+    {
+#pragma unroll
+        for (int t = 0; t < W; t++) {
+            const vec4 v = *(const vec4 *)quad_at(f, at(t));
+            w0[t] = v[0];
+            w1[t] = v[1];
+            w2[t] = v[2];
+            w3[t] = v[3];
+        }
+    }
     __device__ __forceinline__ void load_seg_quad(const real *__restrict__ f, real (&r0)[M],
                                                   real (&r1)[M], real (&r2)[M],
                                                   real (&r3)[M]) const
@@ -417,6 +430,18 @@ struct Win {
             r2[t] = v[2];
             r3[t] = v[3];
         }
+    }
+    // one point's quad (t < len on table plans: the caller checks)
+    template <bool NT = false>
+    __device__ __forceinline__ void store_quad_at(real *__restrict__ f, int t, real r0, real r1,
+                                                  real r2, real r3) const
+    {
+        vec4 v;
+        v[0] = r0;
+        v[1] = r1;
+        v[2] = r2;
+        v[3] = r3;
+        st_hint<NT>(v, (vec4 *)quad_at(f, mid(t)));
     }
     template <bool NT = false>
     __device__ __forceinline__ void store_quad(real *__restrict__ f, const real (&r0)[M],

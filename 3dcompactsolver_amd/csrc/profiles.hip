@@ -29,9 +29,10 @@ __global__ __launch_bounds__(PROF_THREADS) void k_plane_rows(Fields<real> F, Phy
         for (int q = 0; q < 6; q++) m[q] = mean[(long)j * 6 + q];
     for (int i = threadIdx.x; i < Nx; i += PROF_THREADS) {
         const long ii = base + i;
-        real irho, U, V, W, pp, Tp;
-        prim(ph, F.q[0][ii], F.q[1][ii], F.q[2][ii], F.q[3][ii], F.q[4][ii], irho, U, V, W, pp, Tp);
-        const double v[6] = {(double)ph.rho_ref + (double)F.q[0][ii], (double)U, (double)V,
+        real irho, U, V, W, pp, Tp, q5[5];
+        qset_load(F.q, ii, q5);
+        prim(ph, q5[0], q5[1], q5[2], q5[3], q5[4], irho, U, V, W, pp, Tp);
+        const double v[6] = {(double)ph.rho_ref + (double)q5[0], (double)U, (double)V,
                              (double)W, (double)ph.p_ref + (double)pp,
                              (double)ph.T0s + (double)Tp};
         if (pass == 0) {

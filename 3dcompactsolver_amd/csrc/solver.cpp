@@ -89,9 +89,9 @@ struct SolverT : SolverBase {
     PlanDev<real> FLX;
     bool fx_ok = false;
     LaunchGeom sweepG[3], filterG[3];
-    // the z sweep's geometry for its variants with a smaller workgroup
-    // limit (zpass_threads: FP32 table plans and the stale stage)
-    LaunchGeom sweepGz_small;
+    // the column sweeps' geometry for their variants with a smaller
+    // workgroup limit (col_small)
+    LaunchGeom sweepG_small[3];
     LineMap lm[3];
     Fields<real> F;
     Phys<real> ph;
@@ -157,14 +157,13 @@ struct SolverT : SolverBase {
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
                                   SWEEP_STAGE_FIELDS * D1[d].host.M);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
-            if (d == 2) {
-                const int cap = zpass_threads<real, false, true>();
-                sweepGz_small = line_geom(P, kr, std::min(sweep_max_threads(d, P), cap),
-                                          SWEEP_PLANES, sizeof(real),
-                                          SWEEP_STAGE_FIELDS * D1[d].host.M);
-                if (sweepGz_small.LX < 1)
+            sweepG_small[d] = sweepG[d];
+            if (d > 0) {
+                sweepG_small[d] = line_geom(P, kr, std::min(sweep_max_threads(d, P), 512),
+                                            SWEEP_PLANES, sizeof(real),
+                                            SWEEP_STAGE_FIELDS * D1[d].host.M);
+                if (sweepG_small[d].LX < 1)
                     throw std::invalid_argument("line too long for the fused sweeps");
-                if (!D1[d].host.uniform) sweepG[d] = sweepGz_small;
             }
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
                                    filter_max_threads(d), 5, sizeof(real));
@@ -190,9 +189,16 @@ struct SolverT : SolverBase {
             p += (size_t)width * Np;
             return r;
         };
-        for (int v = 0; v < 5; v++) F.q[v] = take();
-        for (int v = 0; v < 5; v++) F.qk[v] = F.qkw[v] = take();
-        for (int v = 0; v < 5; v++) F.acc[v] = take();
+        // a conserved set: quads (rho', rhoU, rhoV, rhoW) and rhoE'
+        auto take_set = [&]() {
+            QSet<real> q5;
+            q5.m = take(4);
+            q5.e = take();
+            return q5;
+        };
+        F.q = take_set();
+        F.qk = F.qkw = take_set();
+        F.acc = take_set();
         // interleaved intermediates: a pair field takes two fields' space,
         // a quad four
         F.dYut = take(2);
@@ -207,8 +213,8 @@ struct SolverT : SolverBase {
         for (int v = 0; v < 4; v++) F.diag[v] = take();
         if (sponge) {
             F.sigma = take();
-            for (int v = 0; v < 5; v++) F.avg[v] = take();
-            for (int v = 0; v < 5; v++) F.qstale[v] = take();
+            F.avg = take_set();
+            F.qstale = take_set();
             for (int v = 0; v < 4; v++) F.mix[v] = take();
         }
         hc(hipMalloc(&dsc, sizeof(StepScalars)), "hipMalloc(scalars)");
@@ -361,10 +367,7 @@ struct SolverT : SolverBase {
         rhoE_ref = sE / (double)N;
         build_phys();
         upload_state(q, 0);
-        if (sponge)
-            for (int v = 0; v < 5; v++)
-                hc(hipMemcpyAsync(F.avg[v], F.q[v], (size_t)N * sizeof(real), hipMemcpyDeviceToDevice, st),
-                   "sponge averages");
+        if (sponge) copy_set(F.avg, F.q);
         StepScalars z;
         std::memset(&z, 0, sizeof(z));
         hc(hipMemcpyAsync(dsc, &z, sizeof(z), hipMemcpyHostToDevice, st), "scalars");
@@ -432,13 +435,18 @@ struct SolverT : SolverBase {
     // q holds full conserved variables (FP64); stored in perturbation form
     void upload_state(const std::vector<double> *q, int which)
     {
-        std::vector<real> buf((size_t)N);
-        real *const *dst = which == 0 ? F.q : which == 1 ? F.qk : which == 2 ? F.k2 : F.acc;
-        for (int v = 0; v < 5; v++) {
-            double off = which == 2 ? 0.0 : (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0);
-            for (long i = 0; i < N; i++) buf[i] = (real)(q[v][i] - off);
-            hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
-        }
+        const double *in[5];
+        for (int v = 0; v < 5; v++) in[v] = q[v].data();
+        set_conserved_raw(which == 0 ? 1 : which == 1 ? 2 : which == 2 ? 3 : 4, in);
+    }
+
+    // device copy of a conserved set
+    void copy_set(const QSet<real> &dst, const QSet<real> &src)
+    {
+        hc(hipMemcpyAsync(dst.m, src.m, 4 * (size_t)N * sizeof(real), hipMemcpyDeviceToDevice, st),
+           "copy");
+        hc(hipMemcpyAsync(dst.e, src.e, (size_t)N * sizeof(real), hipMemcpyDeviceToDevice, st),
+           "copy");
     }
 
     // ---------------------------------------------------------------- timing
@@ -505,7 +513,8 @@ struct SolverT : SolverBase {
         // y, then z, then x (the last sweep assembles and updates)
         for (int d : {1, 2, 0})
             timed(d, [&] {
-                const LaunchGeom &g = (d == 2 && sa.stale) ? sweepGz_small : sweepG[d];
+                const bool small = d > 0 && col_small<real>(d, D1[d].host.uniform, sa.stale);
+                const LaunchGeom &g = small ? sweepG_small[d] : sweepG[d];
                 hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, (fuse && d == 0) ? FLX.dev : nullptr,
                                       D1[d].host.uniform, D1[d].host.M,
                                       g, lm[d], F, ph, sa, dsc, st),
@@ -516,13 +525,27 @@ struct SolverT : SolverBase {
     // workgroups of a filter sweep along dir
     int filter_blocks(int dir) const { return (lm[dir].nlines + filterG[dir].LX - 1) / filterG[dir].LX; }
 
-    void filter_sweep(int dir, real *const *in, real *const *out, double *part = nullptr)
+    void filter_sweep(int dir, const QSet<real> &in, const QSet<real> &out, double *part = nullptr)
     {
-        Ptr5<real> io;
+        FiltIO<real> io{};
+        io.set = 1;
+        io.qin = in;
+        io.qout = out;
+        filter_launch(dir, io, part);
+    }
+    // five separate fields (the adjoint's cotangents)
+    void filter_sweep(int dir, real *const *in, real *const *out)
+    {
+        FiltIO<real> io{};
+        io.set = 0;
         for (int v = 0; v < 5; v++) {
-            io.in[v] = in[v];
-            io.out[v] = out[v];
+            io.sep.in[v] = in[v];
+            io.sep.out[v] = out[v];
         }
+        filter_launch(dir, io, nullptr);
+    }
+    void filter_launch(int dir, const FiltIO<real> &io, double *part)
+    {
         const PlanDev<real> &pl = FL[dir];
         timed(3, [&] {
             hc(launch_filter5<real>(pl.dev, pl.host.uniform, pl.host.M, filterG[dir],
@@ -549,7 +572,7 @@ struct SolverT : SolverBase {
             return fuse ? filter_blocks(o[2]) : 0;
         }
         // the memcpy of *2 into *1 becomes a swap of the two buffers
-        for (int v = 0; v < 5; v++) std::swap(F.q[v], F.acc[v]);
+        std::swap(F.q, F.acc);
         return 0;
     }
 
@@ -622,39 +645,73 @@ struct SolverT : SolverBase {
     void get_conserved(int set, double *const *out) override
     {
         need_ic();
-        real *const *src = set_fields(set);
-        std::vector<real> buf((size_t)N);
         hc(hipStreamSynchronize(st), "sync");
-        for (int v = 0; v < 5; v++) {
-            if (!out[v]) continue;
-            hc(hipMemcpy(buf.data(), src[v], (size_t)N * sizeof(real), hipMemcpyDeviceToHost), "download");
-            double off = set == 3 ? 0.0 : (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0);
-            for (long i = 0; i < N; i++) out[v][i] = (double)buf[i] + off;
+        if (set == 3) {
+            // dt*RHS: five separate fields, no reference offset
+            std::vector<real> buf((size_t)N);
+            for (int v = 0; v < 5; v++) {
+                if (!out[v]) continue;
+                hc(hipMemcpy(buf.data(), F.k2[v], (size_t)N * sizeof(real), hipMemcpyDeviceToHost),
+                   "download");
+                for (long i = 0; i < N; i++) out[v][i] = (double)buf[i];
+            }
+            return;
         }
+        const QSet<real> &src = set_of(set);
+        std::vector<real> m(4 * (size_t)N), e((size_t)N);
+        hc(hipMemcpy(m.data(), src.m, m.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+        hc(hipMemcpy(e.data(), src.e, e.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+        for (int v = 0; v < 4; v++) {
+            if (!out[v]) continue;
+            const double off = v == 0 ? rho_ref : 0.0;
+            for (long i = 0; i < N; i++) out[v][i] = (double)m[4 * i + v] + off;
+        }
+        if (out[4])
+            for (long i = 0; i < N; i++) out[4][i] = (double)e[i] + rhoE_ref;
     }
 
-    // the device fields of a conserved set (1..6, see cs3d.h)
-    real *const *set_fields(int set) const
+    // the device fields of a conserved set (1, 2, 4, 5, 6; see cs3d.h)
+    const QSet<real> &set_of(int set) const
     {
         if ((set == 5 || set == 6) && !sponge)
             throw std::invalid_argument("sets 5 and 6 exist with sponge faces only");
-        return set == 1 ? F.q : set == 2 ? F.qk : set == 3 ? F.k2 : set == 4 ? F.acc
-               : set == 5 ? F.avg : F.qstale;
+        if (set < 1 || set > 6 || set == 3) throw std::invalid_argument("bad conserved set");
+        return set == 1 ? F.q : set == 2 ? F.qk : set == 4 ? F.acc : set == 5 ? F.avg : F.qstale;
     }
 
     void set_conserved(int set, const double *const *in) override
     {
         need_ic();
-        hc(hipStreamSynchronize(st), "sync");
-        std::vector<real> buf((size_t)N);
-        real *const *dst = set_fields(set);
-        for (int v = 0; v < 5; v++) {
-            if (!in[v]) throw std::invalid_argument("set_conserved needs all five fields");
-            const double off = set == 3 ? 0.0 : (v == 0 ? rho_ref : v == 4 ? rhoE_ref : 0.0);
-            for (long i = 0; i < N; i++) buf[i] = (real)(in[v][i] - off);
-            hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
-        }
+        set_conserved_raw(set, in);
         if (set == 1) hc(launch_reduce<real>(F, ph, N, 0, dpart, nred, dout, dsc, st), "reduce");
+    }
+
+    // full conserved values in (FP64), stored in perturbation form
+    void set_conserved_raw(int set, const double *const *in)
+    {
+        hc(hipStreamSynchronize(st), "sync");
+        for (int v = 0; v < 5; v++)
+            if (!in[v]) throw std::invalid_argument("set_conserved needs all five fields");
+        if (set == 3) {
+            std::vector<real> buf((size_t)N);
+            for (int v = 0; v < 5; v++) {
+                for (long i = 0; i < N; i++) buf[i] = (real)in[v][i];
+                hc(hipMemcpy(F.k2[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice),
+                   "upload");
+            }
+            return;
+        }
+        const QSet<real> &dst = set_of(set);
+        std::vector<real> m(4 * (size_t)N), e((size_t)N);
+        for (long i = 0; i < N; i++) {
+            m[4 * i + 0] = (real)(in[0][i] - rho_ref);
+            m[4 * i + 1] = (real)in[1][i];
+            m[4 * i + 2] = (real)in[2][i];
+            m[4 * i + 3] = (real)in[3][i];
+            e[i] = (real)(in[4][i] - rhoE_ref);
+        }
+        hc(hipMemcpy(dst.m, m.data(), m.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
+        hc(hipMemcpy(dst.e, e.data(), e.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
     }
 
     // shearLayerInfoCalc (CSolver_AWS.cpp:2259-2456): x-z plane means and
@@ -870,12 +927,12 @@ struct SolverT : SolverBase {
     std::vector<void *> adjalloc;
     real *A[ADJ_FIELDS] = {};  // vjp work fields
     real *lamb[5] = {}, *vb5[5] = {}, *vb5b[5] = {}, *qb5[5] = {}, *la5[5] = {}, *sc5[5] = {};
-    std::vector<std::array<real *, 5>> adjpool;  // gradient states kept across calls
+    std::vector<QSet<real>> adjpool;  // gradient states kept across calls
 
-    real *adj_alloc_field()
+    real *adj_alloc_field(int width = 1)
     {
         void *p = nullptr;
-        hc(hipMalloc(&p, (size_t)N * sizeof(real)), "hipMalloc(adjoint)");
+        hc(hipMalloc(&p, (size_t)width * N * sizeof(real)), "hipMalloc(adjoint)");
         adjalloc.push_back(p);
         return (real *)p;
     }
@@ -974,13 +1031,11 @@ struct SolverT : SolverBase {
 
     // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
     // with v = dt * lam given; fin says where res goes (AdjFinal).
-    void vjp(real *const *Q, real *const *v, AdjFinal<real> fin)
+    void vjp(const QSet<real> &Q, real *const *v, AdjFinal<real> fin)
     {
-        Ptr5<real> q;
-        for (int c = 0; c < 5; c++) q.in[c] = Q[c];
         real *U = A[0], *V = A[1], *W = A[2], *Tp = A[3];
         AdjPrim<real> pr{U, V, W, Tp, A[4], A[5]};
-        hc(launch_adj_prim<real>(q, ph, pr, N, st), "adjoint primitives");
+        hc(launch_adj_prim<real>(Q, ph, pr, N, st), "adjoint primitives");
         // y and z derivative fields (U, V, W, T order) and the mixed D_z Vy, D_z Wy
         real **dy1 = A + 6, **dy2 = A + 10, **dz1 = A + 14, **dz2 = A + 18;
         real *Vyz = A[22], *Wyz = A[23];
@@ -1050,8 +1105,8 @@ struct SolverT : SolverBase {
             fin.v[c] = v[c];
             fin.Fy[c] = Fy[c];
             fin.Fz[c] = Fz[c];
-            fin.q[c] = Q[c];
         }
+        fin.q = Q;
         for (int i = 0; i < 4; i++) {
             fin.tot[i] = tot[i];
             fin.totY[i] = totY[i];
@@ -1096,14 +1151,6 @@ struct SolverT : SolverBase {
         filter_sweep(o[0], scratch, out);
     }
 
-    void copy5(real *const *dst, real *const *src)
-    {
-        for (int c = 0; c < 5; c++)
-            hc(hipMemcpyAsync(dst[c], src[c], (size_t)N * sizeof(real), hipMemcpyDeviceToDevice,
-                              st),
-               "copy");
-    }
-
     // one step on the fused forward path from F.q; stg (when given) receives
     // the stage states q, Q2, Q3, Q4.  Replays (dt > 0) restore the step
     // counters the filter round robin reads.
@@ -1112,7 +1159,7 @@ struct SolverT : SolverBase {
         int timeStep, filterTimeStep, order;  // counters before the step; order -1: no filter
     };
 
-    StepRec fused_step(const StepRec *replay, real *const (*stg)[5])
+    StepRec fused_step(const StepRec *replay, const QSet<real> *stg)
     {
         StepRec r;
         if (replay) {
@@ -1130,26 +1177,23 @@ struct SolverT : SolverBase {
             r.dt = hs.dt;
             r.order = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
         }
-        if (stg) copy5(stg[0], F.q);
+        if (stg) copy_set(stg[0], F.q);
         // the last sweep of stage s < 4 writes stage state s + 1 straight
         // into the store, and stage s + 1 reads it there
-        real *own[5];
-        for (int c = 0; c < 5; c++) own[c] = F.qk[c];
+        const QSet<real> own = F.qk;
         for (int s = 1; s <= 4; s++) {
-            if (stg && s < 4)
-                for (int c = 0; c < 5; c++) F.qkw[c] = stg[s][c];
+            if (stg && s < 4) F.qkw = stg[s];
             sub_step(s, s == 4);
-            if (stg && s < 4)
-                for (int c = 0; c < 5; c++) F.qk[c] = stg[s][c];
+            if (stg && s < 4) F.qk = stg[s];
         }
-        for (int c = 0; c < 5; c++) F.qk[c] = F.qkw[c] = own[c];
+        F.qk = F.qkw = own;
         post_step();
         return r;
     }
 
     // lam (cotangent of q_{n+1}) -> cotangent of q_n for the step whose
     // stage states are stg
-    void adjoint_step(real *const (*stg)[5], const StepRec &r, real *const *lam)
+    void adjoint_step(const QSet<real> *stg, const StepRec &r, real *const *lam)
     {
         const real rdt = (real)r.dt;
         real *const *la = lam;
@@ -1238,23 +1282,23 @@ struct SolverT : SolverBase {
         size_t used = 0;
         auto state = [&]() {
             if (used == adjpool.size()) {
-                std::array<real *, 5> a;
-                for (auto &p : a) p = adj_alloc_field();
+                QSet<real> a;
+                a.m = adj_alloc_field(4);
+                a.e = adj_alloc_field();
                 adjpool.push_back(a);
             }
             return adjpool[used++];
         };
-        std::vector<std::array<real *, 5>> ck(nseg), stg(4 * (size_t)K);
+        std::vector<QSet<real>> ck(nseg), stg(4 * (size_t)K);
         for (auto &c : stg) c = state();
-        std::array<real *, 5> fin = state();
+        const QSet<real> fin = state();
         for (auto &c : ck) c = state();
-        auto stages = [&](int i) { return (real *const(*)[5])stg[4 * (size_t)i].data(); };
-        static_assert(sizeof(std::array<real *, 5>) == 5 * sizeof(real *), "packed state");
+        auto stages = [&](int i) { return (const QSet<real> *)&stg[4 * (size_t)i]; };
         std::vector<StepRec> rec(nsteps);
         // forward pass on the fused path: checkpoints, and the stage states
         // of the last segment
         for (int n = 0; n < nsteps; n++) {
-            if (n % K == 0) copy5(ck[n / K].data(), F.q);
+            if (n % K == 0) copy_set(ck[n / K], F.q);
             rec[n] = fused_step(nullptr, n >= last0 ? stages(n - last0) : nullptr);
         }
         StepScalars after;
@@ -1262,36 +1306,30 @@ struct SolverT : SolverBase {
         hc(hipStreamSynchronize(st), "sync");
         const int ts_end = timeStep, fts_end = filterTimeStep;
         if (J) hc(kinetic_energy(J) == CS3D_OK ? hipSuccess : hipErrorUnknown, "kinetic energy");
-        copy5(fin.data(), F.q);
+        copy_set(fin, F.q);
         // cotangent of the final state
-        Ptr5<real> qf, lo;
-        for (int v = 0; v < 5; v++) {
-            qf.in[v] = F.q[v];
-            lo.out[v] = lamb[v];
-        }
-        hc(launch_adj_ke_seed<real>(qf, ph, lo, N, st), "adjoint seed");
+        Ptr5<real> lo;
+        for (int v = 0; v < 5; v++) lo.out[v] = lamb[v];
+        hc(launch_adj_ke_seed<real>(F.q, ph, lo, N, st), "adjoint seed");
         // reverse, segment by segment
         for (int sgi = nseg - 1; sgi >= 0; sgi--) {
             const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
             if (sgi != nseg - 1) {
-                copy5(F.q, ck[sgi].data());
+                copy_set(F.q, ck[sgi]);
                 for (int n = n0; n < n1; n++) fused_step(&rec[n], stages(n - n0));
             }
             for (int n = n1 - 1; n >= n0; n--) adjoint_step(stages(n - n0), rec[n], lamb);
         }
         // to the initial primitive velocities at fixed rho and p
-        Ptr5<real> q0, lb;
-        for (int v = 0; v < 5; v++) {
-            q0.in[v] = ck[0][v];
-            lb.in[v] = lamb[v];
-        }
-        hc(launch_adj_to_velocity<real>(q0, lb, ph, qb5[0], qb5[1], qb5[2], N, st),
+        Ptr5<real> lb;
+        for (int v = 0; v < 5; v++) lb.in[v] = lamb[v];
+        hc(launch_adj_to_velocity<real>(ck[0], lb, ph, qb5[0], qb5[1], qb5[2], N, st),
            "velocity gradient");
         if (gU) download(qb5[0], gU);
         if (gV) download(qb5[1], gV);
         if (gW) download(qb5[2], gW);
         // back to the final state of the forward pass
-        copy5(F.q, fin.data());
+        copy_set(F.q, fin);
         timeStep = ts_end;
         filterTimeStep = fts_end;
         hc(hipMemcpyAsync(dsc, &after, sizeof(after), hipMemcpyHostToDevice, st), "scalars");

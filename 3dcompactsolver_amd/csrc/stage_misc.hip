@@ -36,31 +36,52 @@ __device__ __forceinline__ double cfl_speed(const Phys<real> &ph, real rp, real 
 
 // RED: the last sweep of the filter also leaves, per workgroup, the CFL
 // maximum and the NaN count of its output (two doubles, for k_reduce_cfl)
-template <typename real, int M, bool UNI, int MAP, bool RED>
+// SET: the five fields are a conserved set (quads + rhoE'), else separate
+template <typename real, int M, bool UNI, int MAP, bool RED, bool SET>
 __global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
-                                                  Ptr5<real> io, Phys<real> ph,
+                                                  FiltIO<real> io, Phys<real> ph,
                                                   double *__restrict__ part)
 {
     constexpr int HW = 4, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP, fo, fo->K > kc_filter<real>() ? fo->K : kc_filter<real>(), 5)
     real r[5][M], fc[5][M];
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        real w[W];
-        win.load(io.in[v], w);
+    // B f of one window, and the segment's own values, for
+    // out = f - A^-1 (A - B) f (held in registers: read again after the
+    // solve they come from HBM)
+    auto one = [&](int v, const real(&w)[W]) {
         rhs<OP_FILTER, real, M, HW, UNI>(fo, c, w, r[v]);
-        // the segment's own values, for out = f - A^-1 (A - B) f (held in
-        // registers: read again after the solve they come from HBM)
 #pragma unroll
         for (int t = 0; t < M; t++) fc[v][t] = w[HW + t];
+    };
+    if constexpr (SET) {
+        real w0[W], w1[W], w2[W], w3[W], w4[W];
+        win.load_quad(io.qin.m, w0, w1, w2, w3);
+        one(0, w0);
+        one(1, w1);
+        one(2, w2);
+        one(3, w3);
+        win.load(io.qin.e, w4);
+        one(4, w4);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            real w[W];
+            win.load(io.sep.in[v], w);
+            one(v, w);
+        }
     }
     solve_batch<5, kc_filter<real>(), real, M, UNI>(fo, c, r);
     if (active) {
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
+        for (int v = 0; v < 5; v++)
 #pragma unroll
             for (int t = 0; t < M; t++) fc[v][t] -= r[v][t];
-            win.store(io.out[v], fc[v], c.len);
+        if constexpr (SET) {
+            win.store_quad(io.qout.m, fc[0], fc[1], fc[2], fc[3], c.len);
+            win.store(io.qout.e, fc[4], c.len);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++) win.store(io.sep.out[v], fc[v], c.len);
         }
     }
     if constexpr (RED) {
@@ -147,9 +168,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Ph
     double acc[RED_SLOTS] = {};
     for (long i = (long)blockIdx.x * RED_THREADS + threadIdx.x; i < N;
          i += (long)gridDim.x * RED_THREADS) {
-        real irho, U, V, W, pp, Tp;
-        prim(ph, F.q[0][i], F.q[1][i], F.q[2][i], F.q[3][i], F.q[4][i], irho, U, V, W, pp, Tp);
-        double rho = (double)ph.rho_ref + (double)F.q[0][i];
+        real irho, U, V, W, pp, Tp, q5[5];
+        qset_load(F.q, i, q5);
+        prim(ph, q5[0], q5[1], q5[2], q5[3], q5[4], irho, U, V, W, pp, Tp);
+        double rho = (double)ph.rho_ref + (double)q5[0];
         double p = (double)ph.p_ref + (double)pp;
         double cs = sqrt((double)ph.gamma * p / rho);
         double u = (fabs((double)U) + cs) * (double)ph.idx +
@@ -173,7 +195,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_state(Fields<real> F, Ph
                 acc[3] += 0.5 * rho * w2;
                 acc[4] += m * rho * w2;
                 acc[5] += dv * dv;
-                const double rp = F.q[0][i];
+                const double rp = q5[0];
                 acc[8] += rp;
                 acc[9] += rp * rp;
                 acc[10] += uv;
@@ -280,29 +302,24 @@ __global__ void k_sponge(Fields<real> F, Phys<real> ph, StageArgs sa, const Step
     const double eps = 1.0 / (avgT / sc->dt + 1.0);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long)gridDim.x * blockDim.x) {
-        real a[5];
+        real a[5], qv[5];
+        qset_load(F.avg, i, a);
+        qset_load(F.q, i, qv);
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
-            a[v] = F.avg[v][i];
-            a[v] += (real)eps * (F.q[v][i] - a[v]);
-        }
+        for (int v = 0; v < 5; v++) a[v] += (real)eps * (qv[v] - a[v]);
         const real rho = ph.rho_ref + a[0];
         const real target = (real)(spongeP / (double)ph.gm1) - ph.rhoE_ref +
                             real(0.5) * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / rho;
         a[4] = (real)epsP * a[4] + (real)(1.0 - epsP) * target;
-#pragma unroll
-        for (int v = 0; v < 5; v++) F.avg[v][i] = a[v];
+        qset_store(F.avg, i, a);
         const int ii = (int)(i % sa.Nx), jj = (int)((i / sa.Nx) % sa.Ny),
                   kk = (int)(i / ((long)sa.Nx * sa.Ny));
         const bool plane = (sa.sponge_face[0] && ii == 0) || (sa.sponge_face[1] && ii == sa.Nx - 1) ||
                            (sa.sponge_face[2] && jj == 0) || (sa.sponge_face[3] && jj == sa.Ny - 1) ||
                            (sa.sponge_face[4] && kk == 0) || (sa.sponge_face[5] && kk == sa.Nz - 1);
         if (plane) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                F.qstale[v][i] = F.q[v][i];
-                F.q[v][i] = a[v];
-            }
+            qset_store(F.qstale, i, qv);
+            qset_store(F.q, i, a);
         }
     }
 }
@@ -332,9 +349,9 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
                : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
 }
 
-template <typename real, bool RED>
+template <typename real, bool RED, bool SET>
 static hipError_t launch_f5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
-                            const LineMap &lm, const Ptr5<real> &io, const Phys<real> &ph,
+                            const LineMap &lm, const FiltIO<real> &io, const Phys<real> &ph,
                             double *part, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
@@ -342,28 +359,31 @@ static hipError_t launch_f5(const OpDev<real> *fo, int uni, int M, const LaunchG
     if (uni) {
         if (M != SEG_FILTER) return hipErrorInvalidValue;
         if (map == MAP_ROW)
-            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW, RED>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
         else
-            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL, RED>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
     } else {
         if (M != SEG_TABLE) return hipErrorInvalidValue;
         if (map == MAP_ROW)
-            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW, RED>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
         else
-            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL, RED>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
     }
     return hipGetLastError();
 }
 
 template <typename real>
 hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
-                          const LineMap &lm, const Ptr5<real> &io, const Phys<real> &ph,
+                          const LineMap &lm, const FiltIO<real> &io, const Phys<real> &ph,
                           double *part, hipStream_t st)
 {
     if (lg.LX < 1 || lg.threads < 1 || lg.threads > filter_threads<real>())
         return hipErrorInvalidConfiguration;
-    return part ? launch_f5<real, true>(fo, uni, M, lg, map, lm, io, ph, part, st)
-                : launch_f5<real, false>(fo, uni, M, lg, map, lm, io, ph, part, st);
+    // the CFL partials come from the state: a conserved set only
+    if (part && !io.set) return hipErrorInvalidValue;
+    if (!io.set) return launch_f5<real, false, false>(fo, uni, M, lg, map, lm, io, ph, part, st);
+    return part ? launch_f5<real, true, true>(fo, uni, M, lg, map, lm, io, ph, part, st)
+                : launch_f5<real, false, true>(fo, uni, M, lg, map, lm, io, ph, part, st);
 }
 
 hipError_t launch_reduce_cfl(const double *part, int nb, double *out, StepScalars *sc,
@@ -409,7 +429,7 @@ hipError_t launch_sponge(const Fields<real> &F, const Phys<real> &ph, const Stag
                                            const Fields<real> &, const Phys<real> &,            \
                                            const StageArgs &, const StepScalars *, hipStream_t); \
     template hipError_t launch_filter5<real>(const OpDev<real> *, int, int, const LaunchGeom &,  \
-                                             int, const LineMap &, const Ptr5<real> &,         \
+                                             int, const LineMap &, const FiltIO<real> &,       \
                                              const Phys<real> &, double *, hipStream_t);       \
     template hipError_t launch_reduce<real>(const Fields<real> &, const Phys<real> &, long, int, \
                                             double *, int, double *, StepScalars *, hipStream_t); \

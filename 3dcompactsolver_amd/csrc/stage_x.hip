@@ -43,20 +43,22 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     constexpr int HW = 2, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP_ROW, d1, FILT ? xfilt_kr(d1, d2, fx) : sweep_kr(d1, d2), SWEEP_PLANES)
     const bool use_k = sa.rkStep > 1;
-    const real *q0 = use_k ? F.qk[0] : F.q[0];
-    const real *q1 = use_k ? F.qk[1] : F.q[1];
-    const real *q2 = use_k ? F.qk[2] : F.q[2];
-    const real *q3 = use_k ? F.qk[3] : F.q[3];
-    const real *q4 = use_k ? F.qk[4] : F.q[4];
+    const QSet<real> qs = use_k ? F.qk : F.q;
     real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
     // (halos taken from neighbouring lanes with ds_bpermute instead of the
     // chunk loads measured 15% slower: the permutes wait for each load)
-    win.load(q0, a0);
-    win.load(q1, a1);
-    win.load(q2, a2);
-    win.load(q3, a3);
-    win.load(q4, a4);
+    win.load_quad(qs.m, a0, a1, a2, a3);
+    win.load(qs.e, a4);
+    // a conserved set's values on this lane's segment
+    auto load_set_seg = [&](const QSet<real> &q5, real(&r)[5][M]) {
+        win.load_seg_quad(q5.m, r[0], r[1], r[2], r[3]);
+        win.load_seg(q5.e, r[4]);
+    };
+    auto store_set_seg = [&](const QSet<real> &q5, const real(&r)[5][M]) {
+        win.template store_quad<false>(q5.m, r[0], r[1], r[2], r[3], c.len);
+        win.template store<true>(q5.e, r[4], c.len);
+    };
 #pragma unroll
     for (int t = 0; t < W; t++)
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
@@ -82,10 +84,11 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             p -= p >= n ? n : 0;
             const bool plane = line_plane || (sa.sponge_face[0] && p == 0) ||
                                (sa.sponge_face[1] && p == n - 1);
-            if (plane)
-                prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
-                     ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
-                     ldo(F.qstale[4], win.at(t)), ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
+            if (plane) {
+                real qo[5];
+                qset_load_at(F.qstale, win.at(t), qo);
+                prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
+            }
         }
     }
     const real c43 = real(4) / real(3), c13 = real(1) / real(3), c23 = real(2) / real(3);
@@ -229,30 +232,30 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     }
     // the update's inputs (the step's base state, the RK4 accumulator)
     // issued before the pointwise assembly, so that they arrive during it
-    // (FP32 3.5% and FP64 3.4% faster steps; FP64, at one wavefront per
-    // SIMD, holds them in AGPRs).  The fused-filter sweep, which runs at
+    // (FP32 3.5% faster steps).  The fused-filter sweep, which runs at
     // RK4's last stage, takes only the accumulator.  The STALE sweep runs
     // only at a step's first stage (the launcher checks it), where the base
     // state is the stage state already in registers and RK4 has no
-    // accumulator yet: it has nothing to issue early.
-    constexpr bool HOIST = !STALE;
+    // accumulator yet: it has nothing to issue early.  FP64, at one
+    // wavefront per SIMD with its registers full, loads them late: issued
+    // early they overflow into scratch.
+    constexpr bool HOIST = !STALE && sizeof(real) == 4;
     constexpr bool HOIST_BASE = HOIST && !FILT;
     real qb[HOIST_BASE ? 5 : 1][M], ab[HOIST ? 5 : 1][M];
     if constexpr (HOIST) {
         if (sa.mode != 1) {
             const bool first_stage = sa.rkStep == 1;
+            if constexpr (HOIST_BASE) {
+                if (QS && first_stage) {
 #pragma unroll
-            for (int v = 0; v < 5; v++) {
-                if constexpr (HOIST_BASE) {
-                    if (QS && first_stage) {
+                    for (int v = 0; v < 5; v++)
 #pragma unroll
                         for (int t = 0; t < M; t++) qb[v][t] = qs_[v][t];
-                    } else {
-                        win.load_seg(F.q[v], qb[v]);
-                    }
+                } else {
+                    load_set_seg(F.q, qb);
                 }
-                if (sa.rk == 4 && !first_stage) win.load_seg(F.acc[v], ab[v]);
             }
+            if (sa.rk == 4 && !first_stage) load_set_seg(F.acc, ab);
         }
     }
 
@@ -291,9 +294,11 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         if (QS && F.sigma && valid) {
             const unsigned i = win.mid(t);
             const real sg = ldo(F.sigma, i);
+            real av[5];
+            qset_load_at(F.avg, i, av);
 #pragma unroll
-            for (int v = 0; v < 5; v++) rhs_[v] += sg * (ldo(F.avg[v], i) - qs_[v][t]);
-            rhs_[4] += sg * (ldo(F.avg[4], i) - qs_[4][t]);
+            for (int v = 0; v < 5; v++) rhs_[v] += sg * (av[v] - qs_[v][t]);
+            rhs_[4] += sg * (av[4] - qs_[4][t]);
         }
         const int ii = c.s + t;
         const bool plane = line_plane || (sa.sponge_face[0] && ii == 0) ||
@@ -357,58 +362,71 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     // (TVDRK3's last stage still takes it, here)
     const bool need_base = !FILT || sa.rk != 4 || has_next || first;
     real fin[FILT ? 5 : 1][M];  // the step's new state (FILT)
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        real qn1[M], o[M];
+    // the step's base state (*1) and RK4's accumulator where they are not in
+    // registers already (the hoisted loads above, or the stage state at the
+    // first stage)
+    real ql[HOIST_BASE ? 1 : 5][M], al[HOIST ? 1 : 5][M];
+    if constexpr (!HOIST_BASE) {
+        if (!(QS && first) && need_base) load_set_seg(F.q, ql);
+    }
+    if constexpr (!HOIST) {
+        if (sa.rk == 4 && !first) load_set_seg(F.acc, al);
+    }
+    auto base = [&](int v, int t) -> real {
         if constexpr (HOIST_BASE) {
-#pragma unroll
-            for (int t = 0; t < M; t++) qn1[t] = qb[v][t];
-        } else if (QS && first) {
-#pragma unroll
-            for (int t = 0; t < M; t++) qn1[t] = qs_[v][t];
-        } else if (need_base) {
-            win.load_seg(F.q[v], qn1);
+            return qb[v][t];
         } else {
-#pragma unroll
-            for (int t = 0; t < M; t++) qn1[t] = real(0);
+            if (QS && first) return qs_[v][t];
+            return need_base ? ql[v][t] : real(0);
         }
+    };
+    auto accum = [&](int v, int t) -> real {
+        if constexpr (HOIST)
+            return ab[v][t];
+        else
+            return al[v][t];
+    };
+    // point by point: each point's quad goes out as soon as it is formed;
+    // the rhoE' values of the segment go out together at the end.  The quad
+    // stores are ordinary ones: a wavefront's store of one point per lane
+    // covers every fourth 16 bytes, which the cache merges with the other
+    // three points' stores; as non-temporal stores the FP32 x sweep took
+    // 37% longer (FP64 44%)
+    real eA[M], eQ[M];
+    const QSet<real> out_a = (sa.rk == 4 || !tvd_to_k) ? F.acc : F.qkw;
+    const bool store_q = sa.rk == 4 && has_next;
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        const bool st_ok = UNI || t < c.len;
+        real b[5], a[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) b[v] = base(v, t);
         if (sa.rk == 4) {
             // RK4.hpp:45-111: *2 accumulates the stages, *k is the next
             // stage state
-            if (first) {
 #pragma unroll
-                for (int t = 0; t < M; t++) o[t] = qn1[t];
-            } else if constexpr (HOIST) {
+            for (int v = 0; v < 5; v++) a[v] = (first ? b[v] : accum(v, t)) + ra * k[v][t];
+            if (store_q) {
 #pragma unroll
-                for (int t = 0; t < M; t++) o[t] = ab[v][t];
-            } else {
-                win.load_seg(F.acc[v], o);
-            }
-#pragma unroll
-            for (int t = 0; t < M; t++) o[t] += ra * k[v][t];
-            if constexpr (FILT) {
-#pragma unroll
-                for (int t = 0; t < M; t++) fin[v][t] = o[t];
-            } else {
-                win.template store<true>(F.acc[v], o, c.len);
-            }
-            if (has_next) {
-#pragma unroll
-                for (int t = 0; t < M; t++) o[t] = qn1[t] + rb * k[v][t];
-                win.template store<true>(F.qkw[v], o, c.len);
+                for (int v = 0; v < 5; v++) b[v] += rb * k[v][t];
+                if (st_ok) win.template store_quad_at<false>(F.qkw.m, t, b[0], b[1], b[2], b[3]);
+                eQ[t] = b[4];
             }
         } else {
             // TVDRK3.hpp:225-260: stages 1-2 write *k, stage 3 writes *2
 #pragma unroll
-            for (int t = 0; t < M; t++) o[t] = ta * qn1[t] + tb * qs_[v][t] + tc * k[v][t];
-            if constexpr (FILT) {
+            for (int v = 0; v < 5; v++) a[v] = ta * b[v] + tb * qs_[v][t] + tc * k[v][t];
+        }
+        if constexpr (FILT) {
 #pragma unroll
-                for (int t = 0; t < M; t++) fin[v][t] = o[t];
-            } else {
-                win.template store<true>(tvd_to_k ? F.qkw[v] : F.acc[v], o, c.len);
-            }
+            for (int v = 0; v < 5; v++) fin[v][t] = a[v];
+        } else {
+            if (st_ok) win.template store_quad_at<false>(out_a.m, t, a[0], a[1], a[2], a[3]);
+            eA[t] = a[4];
         }
     }
+    if constexpr (!FILT) win.template store<true>(out_a.e, eA, c.len);
+    if (store_q) win.template store<true>(F.qkw.e, eQ, c.len);
     if constexpr (FILT) {
         // filterConservedData's x sweep (CSolver_AWS.cpp:1682-1893) on the
         // new state in registers: out = f - A^-1 (A - B) f.  The halo (four
@@ -450,11 +468,10 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         }
         if (active) {
 #pragma unroll
-            for (int v = 0; v < 5; v++) {
+            for (int v = 0; v < 5; v++)
 #pragma unroll
                 for (int t = 0; t < M; t++) fin[v][t] -= r[v][t];
-                win.template store<true>(F.q[v], fin[v], c.len);
-            }
+            store_set_seg(F.q, fin);
         }
     }
 }

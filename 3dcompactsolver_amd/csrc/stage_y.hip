@@ -21,17 +21,13 @@
 namespace cs3d {
 
 template <typename real, int M, bool UNI, bool STALE>
-__global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
+__global__ __launch_bounds__((col_threads<real, 1, UNI, STALE>())) void k_ypass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     Fields<real> F, Phys<real> ph, int use_k, StageArgs sa)
 {
     constexpr int HW = 2, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), SWEEP_PLANES)
-    const real *q0 = use_k ? F.qk[0] : F.q[0];
-    const real *q1 = use_k ? F.qk[1] : F.q[1];
-    const real *q2 = use_k ? F.qk[2] : F.q[2];
-    const real *q3 = use_k ? F.qk[3] : F.q[3];
-    const real *q4 = use_k ? F.qk[4] : F.q[4];
+    const QSet<real> qs = use_k ? F.qk : F.q;
     real a[5][W];
     real(&a0)[W] = a[0];
     real(&a1)[W] = a[1];
@@ -41,11 +37,8 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
     // (staging the halos through LDS measured 3% slower than these direct
     // window loads: its barriers cost more than the halo reads)
-    win.load(q0, a0);
-    win.load(q1, a1);
-    win.load(q2, a2);
-    win.load(q3, a3);
-    win.load(q4, a4);
+    win.load_quad(qs.m, a0, a1, a2, a3);
+    win.load(qs.e, a4);
 #pragma unroll
     for (int t = 0; t < W; t++)
         prim(ph, a0[t], a1[t], a2[t], a3[t], a4[t], ir[t], U[t], V[t], Wv[t], pp[t], Tp[t]);
@@ -94,10 +87,9 @@ __global__ __launch_bounds__(sweep_threads<real>()) void k_ypass(
             us[t] = U[t];
             vs[t] = V[t];
             if (plane) {
-                real i_, u_, v_, w_, p_, t_;
-                prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
-                     ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
-                     ldo(F.qstale[4], win.at(t)), i_, u_, v_, w_, p_, t_);
+                real i_, u_, v_, w_, p_, t_, qo[5];
+                qset_load_at(F.qstale, win.at(t), qo);
+                prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], i_, u_, v_, w_, p_, t_);
                 us[t] = u_;
                 vs[t] = v_;
             }
@@ -156,6 +148,8 @@ hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const Laun
                         int use_k, const StageArgs &sa, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    const int cap = sa.stale ? col_threads<real, 1, UNI, true>() : col_threads<real, 1, UNI, false>();
+    if (lg.threads > cap) return hipErrorInvalidConfiguration;
     if (sa.stale)
         hipLaunchKernelGGL((k_ypass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,
                            lm, F, ph, use_k, sa);

@@ -21,30 +21,23 @@
 namespace cs3d {
 
 template <typename real, int M, bool UNI, bool STALE>
-__global__ __launch_bounds__((zpass_threads<real, UNI, STALE>())) void k_zpass(
+__global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     Fields<real> F, Phys<real> ph, int use_k, StageArgs sa)
 {
     constexpr int HW = 2, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), SWEEP_PLANES)
-    const real *q0 = use_k ? F.qk[0] : F.q[0];
-    const real *q1 = use_k ? F.qk[1] : F.q[1];
-    const real *q2 = use_k ? F.qk[2] : F.q[2];
-    const real *q3 = use_k ? F.qk[3] : F.q[3];
-    const real *q4 = use_k ? F.qk[4] : F.q[4];
+    const QSet<real> qs = use_k ? F.qk : F.q;
     // the conserved windows are read directly: staging them through LDS
     // (as the y sweep does) measured slower here
     real a0[W], a1[W], a2[W], a3[W], a4[W];
     real ir[W], U[W], V[W], Wv[W], pp[W], Tp[W];
-    win.load(q0, a0);
-    win.load(q1, a1);
-    win.load(q2, a2);
-    win.load(q3, a3);
-    win.load(q4, a4);
+    win.load_quad(qs.m, a0, a1, a2, a3);
+    win.load(qs.e, a4);
     // the own points of (Vy, Wy) for the mixed derivatives, issued now so
-    // they arrive during the first solve batch (not in the STALE variant,
-    // whose registers are full)
-    constexpr bool PRE = UNI && !STALE;
+    // they arrive during the first solve batch (not in the STALE variant or
+    // in FP64, whose registers are full)
+    constexpr bool PRE = UNI && !STALE && sizeof(real) == 4;
     real pv[2][M];
     if constexpr (PRE) {
 #pragma unroll
@@ -78,10 +71,9 @@ __global__ __launch_bounds__((zpass_threads<real, UNI, STALE>())) void k_zpass(
             us[t] = U[t];
             ws[t] = Wv[t];
             if (plane) {
-                real i_, u_, v_, w_, p_, t_;
-                prim(ph, ldo(F.qstale[0], win.at(t)), ldo(F.qstale[1], win.at(t)),
-                     ldo(F.qstale[2], win.at(t)), ldo(F.qstale[3], win.at(t)),
-                     ldo(F.qstale[4], win.at(t)), i_, u_, v_, w_, p_, t_);
+                real i_, u_, v_, w_, p_, t_, qo[5];
+                qset_load_at(F.qstale, win.at(t), qo);
+                prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], i_, u_, v_, w_, p_, t_);
                 us[t] = u_;
                 ws[t] = w_;
             }
@@ -200,7 +192,7 @@ hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const Laun
                         int use_k, const StageArgs &sa, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
-    const int cap = sa.stale ? zpass_threads<real, UNI, true>() : zpass_threads<real, UNI, false>();
+    const int cap = sa.stale ? col_threads<real, 2, UNI, true>() : col_threads<real, 2, UNI, false>();
     if (lg.threads > cap) return hipErrorInvalidConfiguration;
     if (sa.stale)
         hipLaunchKernelGGL((k_zpass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph
     const long ip = (long)ijk[2] * sxy + (long)ijk[1] * Nx + ijk[0];
     const long stride = w.dir == 0 ? 1 : w.dir == 1 ? (long)Nx : sxy;
     const long in = w.hi ? -stride : stride;
-    real *const *q = sa.rkStep > 1 ? F.qk : F.q;
+    const QSet<real> q = sa.rkStep > 1 ? F.qk : F.q;
     // temperatures (perturbation form T - T0s) of the six points inward;
     // the wall direction holds at least 7 points
     real Tn[6];
@@ -68,15 +68,15 @@ __global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph
         int nijk[3] = {ijk[0], ijk[1], ijk[2]};
         nijk[w.dir] += w.hi ? -(m + 1) : (m + 1);
         const bool st = sa.stale && sponge_plane(sa, nijk[0], nijk[1], nijk[2]);
-        real *const *src = st ? F.qstale : q;
-        real ir, U, V, Wv, pp;
-        prim(ph, src[0][o], src[1][o], src[2][o], src[3][o], src[4][o], ir, U, V, Wv, pp, Tn[m]);
+        real ir, U, V, Wv, pp, qo[5];
+        qset_load(st ? F.qstale : q, o, qo);
+        prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], ir, U, V, Wv, pp, Tn[m]);
     }
     // calcNeumann: its weights sum to one, so it acts on T - T0s alike
     const real Tw = (Tn[0] * real(360) - Tn[1] * real(450) + Tn[2] * real(400) -
                      Tn[3] * real(225) + Tn[4] * real(72) - Tn[5] * real(10)) /
                     real(147);
-    const real rp = q[0][ip];
+    const real rp = q.m[4 * ip];
     const real rho = ph.rho_ref + rp;
     const real u = (real)w.vel[0], v = (real)w.vel[1], ww = (real)w.vel[2];
     // p - p_ref = rho R T_w - rho_ref R T0s
@@ -84,17 +84,9 @@ __global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph
     // rhoE - rhoE_ref (p_ref = (gamma - 1) rhoE_ref)
     const real re = pp / ph.gm1 + real(0.5) * rho * (u * u + v * v + ww * ww);
     const real m1 = rho * u, m2 = rho * v, m3 = rho * ww;
-    q[1][ip] = m1;
-    q[2][ip] = m2;
-    q[3][ip] = m3;
-    q[4][ip] = re;
-    if (sa.stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) {
-        F.qstale[0][ip] = rp;
-        F.qstale[1][ip] = m1;
-        F.qstale[2][ip] = m2;
-        F.qstale[3][ip] = m3;
-        F.qstale[4][ip] = re;
-    }
+    const real qw[5] = {rp, m1, m2, m3, re};
+    qset_store(q, ip, qw);
+    if (sa.stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) qset_store(F.qstale, ip, qw);
 }
 
 template <typename real>
