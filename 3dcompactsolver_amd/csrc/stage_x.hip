@@ -237,10 +237,12 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     // only at a step's first stage (the launcher checks it), where the base
     // state is the stage state already in registers and RK4 has no
     // accumulator yet: it has nothing to issue early.  FP64, at one
-    // wavefront per SIMD with its registers full, loads them late: issued
-    // early they overflow into scratch.
-    constexpr bool HOIST = !STALE && sizeof(real) == 4;
-    constexpr bool HOIST_BASE = HOIST && !FILT;
+    // wavefront per SIMD with its registers full, issues only the
+    // accumulator early, and only where a stage needs it (not in the QS
+    // variants: RK4's first stage and TVDRK3, which keeps none); issuing
+    // the base state early as well overflows into scratch.
+    constexpr bool HOIST = !STALE && (sizeof(real) == 4 || !QS);
+    constexpr bool HOIST_BASE = HOIST && !FILT && sizeof(real) == 4;
     real qb[HOIST_BASE ? 5 : 1][M], ab[HOIST ? 5 : 1][M];
     if constexpr (HOIST) {
         if (sa.mode != 1) {
