@@ -89,6 +89,9 @@ def lib():
     L.cs3d_adjoint_release.argtypes = [_vp]
     L.cs3d_diag_shear_layer.argtypes = [_vp, _int, _dp]
     L.cs3d_diag_shear_layer_noalloc.argtypes = [_vp, _int, _dp]
+    L.cs3d_aero_optics.argtypes = [_vp, _int, _int, ctypes.c_double, _dp]
+    L.cs3d_aero_optics_field.argtypes = [_int, _int, _int] + [ctypes.c_double] * 3 + [
+        _dp, _int, ctypes.c_double, _dp]
     L.cs3d_get_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_set_run_state.argtypes = [_vp, ctypes.POINTER(RunState)]
     L.cs3d_stream_probe.argtypes = [_int, ctypes.c_size_t, _int, _dp, _dp]
@@ -307,6 +310,13 @@ class Solver:
         return {n: out[i * self.cfg.Ny:(i + 1) * self.cfg.Ny].copy()
                 for i, n in enumerate(self.SHEAR_FILES)}
 
+    def aero_optics(self, num_angles=5, max_angle=70.0, set_=1):
+        """optical-path integrals of the density (AeroOptics::computeAO),
+        shaped (num_angles, Nz, Nx)"""
+        out = np.zeros((num_angles, self.cfg.Nz, self.cfg.Nx))
+        check(lib().cs3d_aero_optics(self.h, set_, num_angles, max_angle, _dptr(out)))
+        return out
+
     def run_state(self):
         rs = RunState()
         check(lib().cs3d_get_run_state(self.h, ctypes.byref(rs)))
@@ -392,3 +402,13 @@ def stream_probe(kind="copy", nbytes=1 << 30, reps=5):
     check(lib().cs3d_stream_probe({"copy": 0, "triad": 1}[kind], nbytes, reps,
                                   ctypes.byref(best), ctypes.byref(mean)))
     return best.value, mean.value
+
+
+def aero_optics_field(rho, lengths, num_angles=5, max_angle=70.0):
+    """optical-path integrals of an FP64 density field shaped (Nz, Ny, Nx)"""
+    Nz, Ny, Nx = rho.shape
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    out = np.zeros((num_angles, Nz, Nx))
+    check(lib().cs3d_aero_optics_field(Nx, Ny, Nz, *lengths, _dptr(r), num_angles, max_angle,
+                                       _dptr(out)))
+    return out

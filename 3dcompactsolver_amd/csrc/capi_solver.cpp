@@ -243,6 +243,16 @@ int cs3d_diag_shear_layer_noalloc(cs3d_solver *s, int fix_q12, double *out)
     });
 }
 
+int cs3d_aero_optics(cs3d_solver *s, int set, int num_angles, double max_angle, double *opl)
+{
+    CS3D_NEED(s, "cs3d_aero_optics");
+    if (!opl) return set_error(CS3D_ERR_ARG, "cs3d_aero_optics: NULL");
+    return guard("cs3d_aero_optics", [&]() -> int {
+        s->impl->aero_optics(set, num_angles, max_angle, opl);
+        return CS3D_OK;
+    });
+}
+
 int cs3d_get_run_state(cs3d_solver *s, cs3d_run_state *rs)
 {
     CS3D_NEED(s, "cs3d_get_run_state");

@@ -20,6 +20,7 @@
 #include "plan_dev.hpp"
 #include "solver.hpp"
 #include "adjoint.hpp"
+#include "aero_optics.hpp"
 #include "solver_kernels.hpp"
 
 namespace cs3d {
@@ -766,6 +767,29 @@ struct SolverT : SolverBase {
                                         "must be at most " + std::to_string(SHEAR_MAX_ROWS) +
                                         " and Ny at most " + std::to_string(SHEAR_MAX_NY));
         hc(shear_profiles_static<real>(F, ph, Nx, Ny, Nz, fix_q12, out, st), "profiles");
+    }
+
+    // AeroOptics::computeAO (AeroOptics.cpp:45-93) on the density of set 1
+    // (the state) or set 4 (*2, the array the reference passes,
+    // CSolver_AWS.hpp:165)
+    void aero_optics(int set, int nA, double maxAngle, double *opl) override
+    {
+        need_ic();
+        if (set != 1 && set != 4) throw std::invalid_argument("aero optics: set must be 1 or 4");
+        const std::string bad = ao_check(Nx, Ny, Nz, cfg.Lx, cfg.Ly, cfg.Lz, nA, maxAngle);
+        if (!bad.empty()) throw std::invalid_argument("aero optics: " + bad);
+        AoDomain d{Nx, Ny, Nz, cfg.Lx, cfg.Ly, cfg.Lz};
+        AoAngles g;
+        ao_geometry(nA, maxAngle, cfg.Ly, std::fmin(dxyz[0], std::fmin(dxyz[1], dxyz[2])), g);
+        const size_t nout = (size_t)nA * Nx * Nz;
+        double *dout = nullptr;
+        hc(hipMalloc(&dout, nout * sizeof(double)), "hipMalloc(optical paths)");
+        hipError_t e = launch_opl_set<real>((set == 1 ? F.q : F.acc).m, rho_ref, d, g, dout, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(opl, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        (void)hipFree(dout);
+        hc(e, "optical paths");
     }
 
     void get_run_state(cs3d_run_state *rs) override

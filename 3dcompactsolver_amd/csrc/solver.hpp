@@ -28,6 +28,8 @@ struct SolverBase {
     virtual void shear_profiles(int fix_q12, double *out) = 0;
     // the same without allocating (device memory or host buffers)
     virtual void shear_profiles_noalloc(int fix_q12, double *out) = 0;
+    // optical-path integrals of the density of conserved set `set` (1 or 4)
+    virtual void aero_optics(int set, int num_angles, double max_angle, double *opl) = 0;
     virtual void set_run_state(const cs3d_run_state &rs) = 0;
     virtual void set_diag(int on) = 0;
     virtual int diag_tgv(double out[7]) = 0;

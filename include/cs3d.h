@@ -187,6 +187,23 @@ int cs3d_diag_shear_layer(cs3d_solver *s, int fix_q12, double *out);
  * refused with CS3D_ERR_ARG.  Calls that overlap on one GPU share that
  * array and race; calls on one solver handle never overlap. */
 int cs3d_diag_shear_layer_noalloc(cs3d_solver *s, int fix_q12, double *out);
+/* AeroOptics::computeAO (AeroOptics.cpp:45-93, the reference calls it
+ * from postStep, CSolver_AWS.cpp:2941-2942, with num_angles 5 and max_angle
+ * 70, CSolver_AWS.hpp:161-165): for each angle p/(num_angles-1) * max_angle
+ * (degrees) and every (x_i, z_k), the integral of the density along the
+ * ray from y = 0.2 Ly to 0.8 Ly leaning by that angle in x (x wrapped with
+ * fmod(., Lx)), by trilinear interpolation (Utils.cpp:261-332) and the
+ * trapezoid rule, FP64.  opl[p*Nx*Nz + k*Nx + i], num_angles*Nx*Nz doubles.
+ * set 1 integrates the state's density; set 4 the *2 array (the reference
+ * passes rho2, which after a filtered step holds the density filtered along
+ * two of the three directions).  2 <= num_angles <= 64, 0 <= max_angle <
+ * 90.  The reference also appends each angle's integrals to "<angle>.OPL"
+ * (writeOPLFiles, AeroOptics.cpp:27-43); cs3d.hpp does that. */
+int cs3d_aero_optics(cs3d_solver *s, int set, int num_angles, double max_angle, double *opl);
+/* the same integrals of any FP64 density field rho (XYZ) on the grid
+ * x_i = i/(Nx-1) Lx, ... (Domain.hpp:30-45), without a solver */
+int cs3d_aero_optics_field(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz,
+                           const double *rho, int num_angles, double max_angle, double *opl);
 /* volume-averaged kinetic energy of the current state, FP64 */
 int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
 

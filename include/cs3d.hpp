@@ -144,13 +144,19 @@ class Derivatives {
 
     Derivatives(Domain *dom, BC::BCType bcType, Direct currentDir)
         : Nx(dom->Nx), Ny(dom->Ny), Nz(dom->Nz), N(dom->N), dx(dom->dx), dy(dom->dy),
-          dz(dom->dz), currentDir(currentDir), bcType(bcType)
+          dz(dom->dz), currentDir(currentDir), bcType(bcType),
+          lineN(currentDir == DIRX ? dom->Nx : currentDir == DIRY ? dom->Ny : dom->Nz),
+          lineL(currentDir == DIRX ? dom->Lx : currentDir == DIRY ? dom->Ly : dom->Lz)
     {
         check(cs3d_ops_create(Nx, Ny, Nz, dom->Lx, dom->Ly, dom->Lz, (int)bcType,
                               (int)currentDir, 0.0, CS3D_FP64, &ops),
               "Derivatives");
     }
-    ~Derivatives() { cs3d_ops_destroy(ops); }
+    ~Derivatives()
+    {
+        cs3d_ops_destroy(ops);
+        cs3d_ops_destroy(lineOps);
+    }
     Derivatives(const Derivatives &) = delete;
     Derivatives &operator=(const Derivatives &) = delete;
 
@@ -179,8 +185,33 @@ class Derivatives {
     }
     void synchronize() { check(cs3d_ops_synchronize(ops), "Derivatives::synchronize"); }
 
+    // the per-line entry points on one contiguous line of the direction's
+    // length (Derivatives.hpp:170-171, 180-183; Derivatives.cpp:98-117,
+    // 250-284): the bc-specific forms take the same route, which follows
+    // bcType.  The stencil-only helpers multRHS* have no counterpart (they
+    // are internal to the reference's solves; the GPU path forms B f inside
+    // its line kernels).
+    void calc1stDeriv(double *phi, double *dphi) { line(CS3D_DERIV1, phi, dphi); }
+    void calc2ndDeriv(double *phi, double *dphi) { line(CS3D_DERIV2, phi, dphi); }
+    void Compact1stPeriodic(double *phi, double *dphidx) { line(CS3D_DERIV1, phi, dphidx); }
+    void Compact2ndPeriodic(double *phi, double *dphidx) { line(CS3D_DERIV2, phi, dphidx); }
+    void Compact1stDirichlet(double *phi, double *dphidx) { line(CS3D_DERIV1, phi, dphidx); }
+    void Compact2ndDirichlet(double *phi, double *dphidx) { line(CS3D_DERIV2, phi, dphidx); }
+
   private:
+    void line(int kind, const double *in, double *out)
+    {
+        if (!lineOps)
+            check(cs3d_ops_create(lineN, 1, 1, lineL, 1.0, 1.0, (int)bcType, CS3D_DIRX, 0.0,
+                                  CS3D_FP64, &lineOps),
+                  "Derivatives (line)");
+        check(cs3d_ops_apply(lineOps, kind, CS3D_LAYOUT_XYZ, in, out, CS3D_HOST),
+              "Derivatives (line)");
+    }
     cs3d_ops *ops = nullptr;
+    cs3d_ops *lineOps = nullptr;  // one line of the direction, made on first use
+    int lineN;
+    double lineL;
 };
 
 // Filter.hpp:11-115 over cs3d_ops (filterField, Filter.cpp:129-171)
@@ -193,13 +224,21 @@ class Filter {
 
     Filter(double alphaF, Domain *dom, BC::BCType bcType, Derivatives::Direct currentDir)
         : Nx(dom->Nx), Ny(dom->Ny), Nz(dom->Nz), N(dom->N), alphaF(alphaF),
-          currentDir(currentDir), bcType(bcType)
+          currentDir(currentDir), bcType(bcType),
+          lineN(currentDir == Derivatives::DIRX ? dom->Nx
+                : currentDir == Derivatives::DIRY ? dom->Ny : dom->Nz),
+          lineL(currentDir == Derivatives::DIRX ? dom->Lx
+                : currentDir == Derivatives::DIRY ? dom->Ly : dom->Lz)
     {
         check(cs3d_ops_create(Nx, Ny, Nz, dom->Lx, dom->Ly, dom->Lz, (int)bcType,
                               (int)currentDir, alphaF, CS3D_FP64, &ops),
               "Filter");
     }
-    ~Filter() { cs3d_ops_destroy(ops); }
+    ~Filter()
+    {
+        cs3d_ops_destroy(ops);
+        cs3d_ops_destroy(lineOps);
+    }
     Filter(const Filter &) = delete;
     Filter &operator=(const Filter &) = delete;
 
@@ -215,8 +254,100 @@ class Filter {
               "filterFieldDevice");
     }
 
+    // one contiguous line of the direction's length (Filter.hpp:111-114,
+    // Filter.cpp:98-127); the multRHS* stencil helpers have no counterpart
+    void compactFilter(double *phi, double *phiF) { line(phi, phiF); }
+    void FilterPeriodic(double *phi, double *phiF) { line(phi, phiF); }
+    void FilterDirichlet(double *phi, double *phiF) { line(phi, phiF); }
+
   private:
+    void line(const double *in, double *out)
+    {
+        if (!lineOps)
+            check(cs3d_ops_create(lineN, 1, 1, lineL, 1.0, 1.0, (int)bcType, CS3D_DIRX, alphaF,
+                                  CS3D_FP64, &lineOps),
+                  "Filter (line)");
+        check(cs3d_ops_apply(lineOps, CS3D_FILTER, CS3D_LAYOUT_XYZ, in, out, CS3D_HOST),
+              "Filter (line)");
+    }
     cs3d_ops *ops = nullptr;
+    cs3d_ops *lineOps = nullptr;
+    int lineN;
+    double lineL;
+};
+
+// AeroOptics.hpp:20-74 over cs3d_aero_optics_field / cs3d_aero_optics:
+// optical-path integrals of a density along slanted rays
+// (AeroOptics.cpp:45-93).  A standalone object integrates the host array it
+// was given (rho); the one CSolver_AWS owns integrates the solver's state on
+// the device.
+class AeroOptics {
+  public:
+    Domain *domain;
+    IdealGas *idealGas;
+    int Nx, Ny, Nz, N;
+    double dx, dy, dz, ds;
+    int Nline = 0;
+    double dxLine = 0.0, dyLine = 0.0, dsLine = 0.0;
+    double *rho, *p, *T;
+    int numAngles;
+    double maxAngle, lowerIBound, upperIBound;
+    double totalDX = 0.0, totalDY, Ldiag = 0.0;
+    double *OPL;
+    std::string outDir = ".";
+
+    AeroOptics(Domain *domain, IdealGas *idealGas, double *rho, double *p, double *T,
+               int numAngles, double maxAngle)
+        : domain(domain), idealGas(idealGas), Nx(domain->Nx), Ny(domain->Ny), Nz(domain->Nz),
+          N(domain->N), dx(domain->dx), dy(domain->dy), dz(domain->dz),
+          ds(std::fmin(dx, std::fmin(dy, dz))), rho(rho), p(p), T(T), numAngles(numAngles),
+          maxAngle(maxAngle), lowerIBound(0.2 * domain->Ly), upperIBound(0.8 * domain->Ly),
+          totalDY(upperIBound - lowerIBound),
+          OPL(new double[(size_t)numAngles * domain->Nx * domain->Nz])
+    {
+    }
+    ~AeroOptics() { delete[] OPL; }
+    AeroOptics(const AeroOptics &) = delete;
+    AeroOptics &operator=(const AeroOptics &) = delete;
+
+    // AeroOptics.cpp:3-24: the ray geometry of one angle (for callers that
+    // read Ldiag, Nline ...; the integrals compute their own)
+    void generateBaseBeamDiff(int angleNumber)
+    {
+        const double ang = ((double)angleNumber / ((double)(numAngles - 1))) * maxAngle * M_PI / 180.0;
+        Ldiag = totalDY / std::cos(ang);
+        totalDX = Ldiag * std::sin(ang);
+        Nline = (int)std::ceil(Ldiag / ds) + 1;
+        dsLine = Ldiag / (double)(Nline - 1);
+        dxLine = totalDX / (double)(Nline - 1);
+        dyLine = totalDY / (double)(Nline - 1);
+    }
+    // AeroOptics.cpp:27-43: appends each angle's Nx*Nz integrals to
+    // "<angle>.OPL" (binary FP64)
+    void writeOPLFiles()
+    {
+        for (int ap = 0; ap < numAngles; ap++) {
+            const double ang = ((double)ap / ((double)(numAngles - 1))) * maxAngle;
+            char buff[64];
+            std::snprintf(buff, sizeof(buff), "%4.2f", ang);
+            std::ofstream fp(outDir + "/" + buff + ".OPL", std::ios::binary | std::ios::app);
+            fp.write((const char *)&OPL[(size_t)ap * Nx * Nz], sizeof(double) * Nx * Nz);
+        }
+    }
+    // AeroOptics.cpp:45-93
+    void computeAO()
+    {
+        if (solver)
+            check(cs3d_aero_optics(solver, 1, numAngles, maxAngle, OPL), "computeAO");
+        else
+            check(cs3d_aero_optics_field(Nx, Ny, Nz, domain->Lx, domain->Ly, domain->Lz, rho,
+                                         numAngles, maxAngle, OPL),
+                  "computeAO");
+        writeOPLFiles();
+    }
+
+    // set by CSolver_AWS: integrate its state's density on the device
+    cs3d_solver *solver = nullptr;
 };
 
 // AbstractCSolver.hpp:9-43, with one extension: the RK driver tells the
@@ -341,6 +472,15 @@ class CSolver_AWS : public AbstractCSolver {
     // shearLayerInfoCalc's files every 25 steps, as the reference's postStep
     // writes them for every run; fixShearQ12 separates v'w' from u'w' (Q12)
     bool shearLayerOutput = true, fixShearQ12 = false;
+    // AeroOptics, as the reference builds it (CSolver_AWS.hpp:161-165: 5
+    // angles to 70 degrees) and runs it every aoWriteStep steps from
+    // postStep (CSolver_AWS.cpp:2941-2942), appending "<angle>.OPL" files
+    // in outDir; here on the state's density (the reference passes rho2,
+    // which after a filtered step holds the density filtered along two of
+    // the three directions).  aeroOpticsOutput = false skips it.
+    AeroOptics *ao = nullptr;
+    int aoWriteStep = 1;
+    bool aeroOpticsOutput = true;
 
     CSolver_AWS(Domain *dom, BC *bc, TimeStepping *ts, double alphaF, double mu_ref,
                 int blocksize, bool useTiming)
@@ -352,9 +492,11 @@ class CSolver_AWS : public AbstractCSolver {
         this->useTiming = useTiming;
         ig = new IdealGas(dom, mu_ref);
         initializeSolverData();
+        ao = new AeroOptics(dom, ig, rho2, nullptr, nullptr, 5, 70.0);
     }
     ~CSolver_AWS() override
     {
+        delete ao;
         if (h) cs3d_destroy(h);
         delete ig;
     }
@@ -411,6 +553,7 @@ class CSolver_AWS : public AbstractCSolver {
         check(cs3d_set_initial_conditions(h, rho0, U0, V0, W0, p0), "setInitialConditions");
         if (restartStep >= 0) applyRestart(restartStep);
         check(cs3d_set_diagnostics(h, tgvDiagnostics || turbDiagnostics), "set_diagnostics");
+        ao->solver = h;
         if (useTiming) check(cs3d_profile_enable(h, 1), "profile_enable");
         endFlag = false;
     }
@@ -439,8 +582,36 @@ class CSolver_AWS : public AbstractCSolver {
         if (turbDiagnostics) calcTurbulenceQuantities();
         checkSolution();
         dumpSolution();
+        if (aeroOpticsOutput && aoWriteStep > 0 && timeStep % aoWriteStep == 0) {
+            ao->outDir = outDir;
+            ao->computeAO();
+        }
         if (timeStep >= ts->maxTimeStep || time >= ts->maxTime) endFlag = true;
     }
+
+    // Gradients by the discrete adjoint (no reference counterpart: the
+    // reference has no adjoint).  keGradient advances nsteps RK4 steps as
+    // the time loop does and returns J, the final volume-averaged kinetic
+    // energy, with dJ/dU0, dJ/dV0, dJ/dW0 (host arrays of N doubles, XYZ;
+    // NULL skips one) at fixed initial rho and p, states recomputed from a
+    // checkpoint every `checkpoint` steps.  rhsVJP gives
+    // (d(dt*RHS)/dQ)^T lam at the state (set 1) or stage state (set 2).
+    // Periodic lines and no sponge or wall only (cs3d.h).
+    double keGradient(int nsteps, int checkpoint, double *gU, double *gV, double *gW)
+    {
+        double J = 0.0;
+        check(cs3d_adjoint_ke_gradient(h, nsteps, checkpoint, &J, gU, gV, gW), "keGradient");
+        check(cs3d_get_time(h, &time, &dt, &timeStep), "get_time");
+        return J;
+    }
+    void rhsVJP(int set, const double *const lam[5], double *const out[5])
+    {
+        check(cs3d_rhs_vjp(h, set, lam[0], lam[1], lam[2], lam[3], lam[4], out[0], out[1], out[2],
+                           out[3], out[4]),
+              "rhsVJP");
+    }
+    // frees the adjoint's device memory
+    void adjointRelease() { check(cs3d_adjoint_release(h), "adjointRelease"); }
 
     // calcTaylorGreenQuantities (CSolver_AWS.cpp:2175-2257): appends a line
     // to taylorgreen.out in the reference's format (:2240-2248)
@@ -662,6 +833,7 @@ class CSolver_AWS : public AbstractCSolver {
 
 #ifdef CS3D_DROPIN_GLOBAL
 using cs3d::AbstractCSolver;
+using cs3d::AeroOptics;
 using cs3d::AbstractRK;
 using cs3d::BC;
 using cs3d::CSolver_AWS;

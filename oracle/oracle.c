@@ -1236,3 +1236,81 @@ void or_diag_turb(or_solver *s, double out[11])
     out[9] = vp;
     out[10] = mKE;
 }
+
+/* ------------------------------------------------------------------ */
+/* Aero-optics                                                          */
+/* ------------------------------------------------------------------ */
+
+/* AeroOptics::computeAO (AeroOptics.cpp:45-93) with generateBaseBeamDiff
+ * (:3-24) and Utils.cpp's getBaseNodeIndex / linearInterpolation
+ * (:242-332): for angle p of nA (a_p = p/(nA-1) * maxAngle degrees) and
+ * every (x_i, z_k), the density interpolated trilinearly at Nline points of
+ * the ray from (x_i, 0.2 Ly, z_k) to 0.8 Ly leaning by a_p in x (x wrapped
+ * with fmod(., Lx)), integrated by the trapezoid rule.  out[p Nx Nz + k Nx
+ * + i].  Grid coordinates as Domain.hpp:30-45: x_i = i/(Nx-1) Lx.
+ * Last-plane rule: where the base index is the last point of an axis the
+ * reference reads one point past its arrays, with a weight that is zero
+ * (the query lies on that plane: z_k at k = Nz-1) or of rounding size; here
+ * the neighbour's coordinate is the formula's next point and its value the
+ * last point's own. */
+static double ao_coord(int i, int n, double L) { return (((double)i) / ((double)n - 1.0)) * L; }
+
+static double ao_interp(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz,
+                        const double *f, const double xp[3])
+{
+    const double dx = ao_coord(1, Nx, Lx), dy = ao_coord(1, Ny, Ly), dz = ao_coord(1, Nz, Lz);
+    int b[3] = {(int)(xp[0] / dx), (int)(xp[1] / dy), (int)(xp[2] / dz)};
+    const int n[3] = {Nx, Ny, Nz};
+    const double L[3] = {Lx, Ly, Lz};
+    double d[3];
+    int i0[3], i1[3];
+    for (int a = 0; a < 3; a++) {
+        const double c0 = ao_coord(b[a], n[a], L[a]), c1 = ao_coord(b[a] + 1, n[a], L[a]);
+        d[a] = (xp[a] - c0) / (c1 - c0);
+        i0[a] = b[a];
+        i1[a] = b[a] + 1 < n[a] ? b[a] + 1 : n[a] - 1;
+    }
+#define AO_F(ii, jj, kk) f[(long)(kk) * Nx * Ny + (long)(jj) * Nx + (ii)]
+    const double c000 = AO_F(i0[0], i0[1], i0[2]), c001 = AO_F(i0[0], i0[1], i1[2]);
+    const double c010 = AO_F(i0[0], i1[1], i0[2]), c011 = AO_F(i0[0], i1[1], i1[2]);
+    const double c100 = AO_F(i1[0], i0[1], i0[2]), c101 = AO_F(i1[0], i0[1], i1[2]);
+    const double c110 = AO_F(i1[0], i1[1], i0[2]), c111 = AO_F(i1[0], i1[1], i1[2]);
+#undef AO_F
+    const double c00 = c000 * (1 - d[0]) + c100 * d[0];
+    const double c01 = c001 * (1 - d[0]) + c101 * d[0];
+    const double c10 = c010 * (1 - d[0]) + c110 * d[0];
+    const double c11 = c011 * (1 - d[0]) + c111 * d[0];
+    const double c0 = c00 * (1 - d[1]) + c10 * d[1];
+    const double c1 = c01 * (1 - d[1]) + c11 * d[1];
+    return c0 * (1 - d[2]) + c1 * d[2];
+}
+
+void or_aero_optics(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, const double *rho,
+                    int nA, double maxAngle, double *out)
+{
+    const double dx = ao_coord(1, Nx, Lx), dy = ao_coord(1, Ny, Ly), dz = ao_coord(1, Nz, Lz);
+    const double ds = fmin(dx, fmin(dy, dz));
+    const double lower = 0.2 * Ly, upper = 0.8 * Ly, totalDY = upper - lower;
+    for (int ap = 0; ap < nA; ap++) {
+        const double ang = (((double)ap / ((double)(nA - 1))) * maxAngle) * M_PI / 180.0;
+        const double Ldiag = totalDY / cos(ang), totalDX = Ldiag * sin(ang);
+        const int Nline = (int)ceil(Ldiag / ds) + 1;
+        const double dsLine = Ldiag / (double)(Nline - 1);
+        const double dxLine = totalDX / (double)(Nline - 1), dyLine = totalDY / (double)(Nline - 1);
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < Nz; k++)
+            for (int i = 0; i < Nx; i++) {
+                /* the ray's points: sp = 0 .. Nline-1 */
+#define AO_AT(sp)                                                                          \
+    ao_interp(Nx, Ny, Nz, Lx, Ly, Lz, rho,                                                 \
+              (const double[3]){fmod(ao_coord(i, Nx, Lx) + (double)((sp) * dxLine), Lx),   \
+                                lower + (double)((sp) * dyLine), ao_coord(k, Nz, Lz)})
+                /* trapezoid rule in the reference's order: the two ends, then
+                 * the interior points in turn */
+                double acc = AO_AT(0) * dsLine / 2.0 + AO_AT(Nline - 1) * dsLine / 2.0;
+                for (int sp = 1; sp < Nline - 1; sp++) acc += AO_AT(sp) * dsLine;
+#undef AO_AT
+                out[(long)ap * Nx * Nz + (long)k * Nx + i] = acc;
+            }
+    }
+}

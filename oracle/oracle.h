@@ -142,6 +142,12 @@ double *or_sponge_avg(or_solver *s, int which);
 double or_calc_neumann(double f1, double f2, double f3, double f4, double f5,
                        double f6);
 
+/* AeroOptics::computeAO (AeroOptics.cpp:45-93): optical-path integrals of
+ * the density rho (XYZ, FP64) along nA rays per (x_i, z_k), the angles
+ * p/(nA-1) * maxAngle degrees; out[p*Nx*Nz + k*Nx + i] */
+void or_aero_optics(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, const double *rho,
+                    int nA, double maxAngle, double *out);
+
 #ifdef __cplusplus
 }
 #endif

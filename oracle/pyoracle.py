@@ -103,6 +103,8 @@ def _load(path):
                                                    ctypes.c_int, _dp, _dp]
     L.or_calc_neumann.restype = ctypes.c_double
     L.or_calc_neumann.argtypes = [ctypes.c_double] * 6
+    L.or_aero_optics.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [
+        _dp, ctypes.c_int, ctypes.c_double, _dp]
     return L
 
 
@@ -163,6 +165,16 @@ def rk_update(rk, rk_step, arrays):
     updated in place."""
     ptrs = (_dp * 20)(*[_ptr(a) for a in arrays])
     lib().or_rk_update_arrays(rk, rk_step, arrays[0].size, ptrs)
+
+
+def aero_optics(rho, lengths, num_angles=5, max_angle=70.0):
+    """AeroOptics::computeAO (AeroOptics.cpp:45-93) of the density rho
+    shaped (Nz, Ny, Nx): optical-path integrals shaped (num_angles, Nz, Nx)"""
+    Nz, Ny, Nx = rho.shape
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    out = np.zeros((num_angles, Nz, Nx))
+    lib().or_aero_optics(Nx, Ny, Nz, *lengths, _ptr(r), num_angles, max_angle, _ptr(out))
+    return out
 
 
 # --------------------------------------------------------------- solver
@@ -327,5 +339,7 @@ def ref_lib():
         L.ref_rk_update.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(_dp)]
         L.ref_calc_neumann.restype = ctypes.c_double
         L.ref_calc_neumann.argtypes = [ctypes.c_double] * 6
+        L.ref_aero_optics.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [
+            _dp, ctypes.c_int, ctypes.c_double, _dp]
         _ref = L
     return _ref

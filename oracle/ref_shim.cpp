@@ -13,6 +13,10 @@
 #include <iostream>
 #include <sstream>
 
+#include <cstring>
+#include <vector>
+
+#include "AeroOptics.hpp"
 #include "Domain.hpp"
 #include "BC.hpp"
 #include "Utils.hpp"
@@ -179,6 +183,43 @@ void ref_rk_update(int rk, int rkStep, int Nx, int Ny, int Nz,
 double ref_calc_neumann(double f1, double f2, double f3, double f4, double f5, double f6)
 {
     return calcNeumann(f1, f2, f3, f4, f5, f6);
+}
+
+
+// AeroOptics::computeAO (AeroOptics.cpp:45-93) of the reference on the
+// density rho (XYZ).  computeAO also appends its results to "<angle>.OPL"
+// files in the working directory (writeOPLFiles); callers run it in a
+// scratch directory.  Where a ray point lies on the last plane of an axis
+// the reference reads one point past its coordinate and field arrays (with
+// weight zero); to keep those reads defined the domain's coordinate arrays
+// are replaced by copies one point longer (the same formula,
+// Domain.hpp:30-45) and the density is padded with zeros by a plane and a
+// row.
+void ref_aero_optics(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, const double *rho,
+                     int numAngles, double maxAngle, double *out)
+{
+    QuietCout q;
+    Domain dom(Nx, Ny, Nz, Lx, Ly, Lz);
+    const int n[3] = {Nx, Ny, Nz};
+    const double L[3] = {Lx, Ly, Lz};
+    double **axes[3] = {&dom.x, &dom.y, &dom.z};
+    std::vector<std::vector<double>> ext(3);
+    for (int a = 0; a < 3; a++) {
+        ext[a].resize(n[a] + 1);
+        for (int i = 0; i <= n[a]; i++) ext[a][i] = (((double)i) / ((double)n[a] - 1.0)) * L[a];
+        delete[] *axes[a];
+        *axes[a] = ext[a].data();
+    }
+    const long N = (long)Nx * Ny * Nz;
+    std::vector<double> padded(N + (long)Nx * Ny + Nx + 1, 0.0);
+    std::memcpy(padded.data(), rho, N * sizeof(double));
+    IdealGas ig(&dom, 1.0);
+    AeroOptics ao(&dom, &ig, padded.data(), nullptr, nullptr, numAngles, maxAngle);
+    ao.computeAO();
+    std::memcpy(out, ao.OPL, (size_t)numAngles * Nx * Nz * sizeof(double));
+    delete[] ao.OPL;
+    // the coordinate arrays belong to `ext`; Domain has no destructor
+    for (int a = 0; a < 3; a++) *axes[a] = nullptr;
 }
 
 }  // extern "C"

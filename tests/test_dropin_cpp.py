@@ -95,3 +95,55 @@ def test_dropin_dump_and_restart(tmp_path):
         a = np.fromfile(full / f"{name}.out.6")
         b = np.fromfile(part / f"{name}.out.6")
         assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a)), name
+
+
+EXTRAS = os.path.join(ROOT, "tests", "cpp", "_build", "dropin_extras")
+
+
+@pytest.mark.gpu
+def test_dropin_lines_aero_optics_and_gradient(tmp_path):
+    """The drop-in's per-line Derivatives/Filter methods (Derivatives.hpp:
+    170-183, Filter.hpp:111-114) against the oracle's line operators, a
+    standalone AeroOptics (AeroOptics.cpp:27-93: the .OPL files) against the
+    oracle's integrals, and CSolver_AWS::keGradient against the C ABI's
+    gradient on the same run."""
+    exe = EXTRAS if os.access(EXTRAS, os.X_OK) else (_build() and EXTRAS)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    import pyoracle as po
+    dims, lengths = (16, 12, 10), (3.0, 2.0, 1.5)
+    lines = np.fromfile(tmp_path / "lines.bin")
+    pos = 0
+    for d in range(3):
+        n = dims[d]
+        dh = (1.0 / (n - 1.0)) * lengths[d]
+        for bc in (0, 1):
+            f, d1, d2, ff = (lines[pos + i * n:pos + (i + 1) * n] for i in range(4))
+            pos += 4 * n
+            for kind, got in ((1, d1), (2, d2), (0, ff)):
+                ref = po.line_op(kind, bc, f, dh=dh, alpha_f=0.49)
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                assert err < 1e-12, (d, bc, kind, err)
+    rho = np.fromfile(tmp_path / "rho.bin").reshape(dims[::-1])
+    ref = po.aero_optics(rho, lengths, 5, 70.0)
+    for ap, name in enumerate(("0.00", "17.50", "35.00", "52.50", "70.00")):
+        got = np.fromfile(tmp_path / f"{name}.OPL").reshape(dims[2], dims[0])
+        assert np.abs(got - ref[ap]).max() < 1e-12 * np.abs(ref[ap]).max(), name
+    # the same gradient through the C ABI
+    N = 16
+    L = 2 * np.pi - 2 * np.pi / N
+    x = np.arange(N) * (L / (N - 1))
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    p0 = 1 / (1.4 * 0.01)
+    p = p0 + (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0) / 16.0
+    g = cs3d.Solver(cs3d.config(N, N, N, Lx=L, Ly=L, Lz=L, alpha_f=0.49,
+                                mu_ref=(1 / 1600) / 100 ** 0.76, Pr=0.71, cfl=0.5,
+                                filter_step=1, rk=4, precision=cs3d.FP64))
+    g.set_initial_conditions(p / p0, np.sin(X) * np.cos(Y) * np.cos(Z),
+                             -np.cos(X) * np.sin(Y) * np.cos(Z), np.zeros_like(X), p)
+    J, gU, _, _ = g.ke_gradient(2, 1)
+    grad = np.fromfile(tmp_path / "grad.bin")
+    # the driver's initial field comes from libm, the Python side's from numpy
+    assert abs(grad[0] - J) <= 1e-12 * abs(J)
+    assert np.abs(grad[1:] - gU.ravel()).max() <= 1e-10 * np.abs(gU).max()
+    g.close()

@@ -125,3 +125,4 @@ def test_shear_layer_512x128x42():
     print(f"shear 500 steps: KE rel {rel:.2e}; fields " + " ".join(f"{e:.2e}" for e in errs),
           flush=True)
     assert rel <= 1e-5
+
