@@ -55,8 +55,8 @@ int check_config(const cs3d_config *c)
         return set_error(CS3D_ERR_ARG, "cs3d_create: rk must be 3 or 4");
     if (c->precision != CS3D_FP64 && c->precision != CS3D_FP32)
         return set_error(CS3D_ERR_ARG, "cs3d_create: bad precision");
-    if (c->compensated)
-        return set_error(CS3D_ERR_UNSUPPORTED, "cs3d_create: compensated mode not built");
+    if (c->compensated && c->precision != CS3D_FP32)
+        return set_error(CS3D_ERR_ARG, "cs3d_create: the compensated mode is an FP32 mode");
     if (c->filter_step < 1) return set_error(CS3D_ERR_ARG, "cs3d_create: filter_step < 1");
     if (!(c->gamma > 1.0) || !(c->Pr > 0.0) || !(c->mu_ref >= 0.0))
         return set_error(CS3D_ERR_ARG, "cs3d_create: bad gas constants");

@@ -166,8 +166,11 @@ struct SolverT : SolverBase {
                 if (sweepG_small[d].LX < 1)
                     throw std::invalid_argument("line too long for the fused sweeps");
             }
+            // (the compensated filter sweeps run at most 512 lanes)
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
-                                   filter_max_threads(d), 5, sizeof(real));
+                                   c.compensated ? std::min(filter_max_threads(d), 512)
+                                                 : filter_max_threads(d),
+                                   5, sizeof(real));
             if (filterG[d].LX < 1) throw std::invalid_argument("line too long for the filter");
             lm[d] = xyz_lines(d, Nx, Ny, Nz);
         }
@@ -177,7 +180,11 @@ struct SolverT : SolverBase {
         for (int f = 0; f < 6; f++) sponge |= c.bc_kind[f] == CS3D_BC_SPONGE;
 
         std::memset(&F, 0, sizeof(F));
-        int nf = 15 + 4 + 4 + 10 + 5 + 4 + (sponge ? 15 : 0);
+        // compensated mode (FP32): low parts of *1, *2 and, for TVDRK3, *k
+        const bool cmp = c.compensated != 0;
+        if (cmp && sizeof(real) != 4)
+            throw std::invalid_argument("the compensated mode is an FP32 mode");
+        int nf = 15 + 4 + 4 + 10 + 5 + 4 + (sponge ? 15 : 0) + (cmp ? (c.rk == 3 ? 15 : 10) : 0);
         // fields start on 512-byte boundaries (the vector and interleaved
         // accesses need 16/32-byte alignment whatever N is)
         const size_t Np = ((size_t)N + 63) / 64 * 64;
@@ -200,6 +207,11 @@ struct SolverT : SolverBase {
         F.q = take_set();
         F.qk = F.qkw = take_set();
         F.acc = take_set();
+        if (cmp) {
+            F.qlo = take_set();
+            F.acclo = take_set();
+            if (c.rk == 3) F.qklo = take_set();
+        }
         // interleaved intermediates: a pair field takes two fields' space,
         // a quad four
         F.dYut = take(2);
@@ -232,7 +244,7 @@ struct SolverT : SolverBase {
         base.Nx = Nx;
         base.Ny = Ny;
         base.Nz = Nz;
-        base.compensated = 0;
+        base.compensated = cmp ? 1 : 0;
         for (int f = 0; f < 6; f++) {
             base.sponge_face[f] = c.bc_kind[f] == CS3D_BC_SPONGE;
             base.wall_face[f] = is_wall(f);
@@ -526,12 +538,16 @@ struct SolverT : SolverBase {
     // workgroups of a filter sweep along dir
     int filter_blocks(int dir) const { return (lm[dir].nlines + filterG[dir].LX - 1) / filterG[dir].LX; }
 
-    void filter_sweep(int dir, const QSet<real> &in, const QSet<real> &out, double *part = nullptr)
+    // in_lo/out_lo: the sets' low parts in compensated mode (null m: none)
+    void filter_sweep(int dir, const QSet<real> &in, const QSet<real> &out, double *part = nullptr,
+                      const QSet<real> &in_lo = QSet<real>{}, const QSet<real> &out_lo = QSet<real>{})
     {
         FiltIO<real> io{};
         io.set = 1;
         io.qin = in;
         io.qout = out;
+        io.qin_lo = in_lo;
+        io.qout_lo = out_lo;
         filter_launch(dir, io, part);
     }
     // five separate fields (the adjoint's cotangents)
@@ -564,16 +580,17 @@ struct SolverT : SolverBase {
             filterTimeStep++;
             int o[3];
             filter_order(filterTimeStep % 3, o, fused);
-            if (!fused) filter_sweep(o[0], F.acc, F.q);
-            filter_sweep(o[1], F.q, F.acc);
+            if (!fused) filter_sweep(o[0], F.acc, F.q, nullptr, F.acclo, F.qlo);
+            filter_sweep(o[1], F.q, F.acc, nullptr, F.qlo, F.acclo);
             // the partials need 2 doubles per lane of the sweep's LDS
             const bool fuse = cfl && filterG[o[2]].shm >= 2 * sizeof(double) *
                                                            (size_t)filterG[o[2]].threads;
-            filter_sweep(o[2], F.acc, F.q, fuse ? dfpart : nullptr);
+            filter_sweep(o[2], F.acc, F.q, fuse ? dfpart : nullptr, F.acclo, F.qlo);
             return fuse ? filter_blocks(o[2]) : 0;
         }
         // the memcpy of *2 into *1 becomes a swap of the two buffers
         std::swap(F.q, F.acc);
+        std::swap(F.qlo, F.acclo);
         return 0;
     }
 
@@ -662,13 +679,30 @@ struct SolverT : SolverBase {
         std::vector<real> m(4 * (size_t)N), e((size_t)N);
         hc(hipMemcpy(m.data(), src.m, m.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
         hc(hipMemcpy(e.data(), src.e, e.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+        // compensated mode: hi + lo
+        std::vector<real> ml, el;
+        const QSet<real> *lo = lo_of(set);
+        if (lo) {
+            ml.resize(m.size());
+            el.resize(e.size());
+            hc(hipMemcpy(ml.data(), lo->m, ml.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+            hc(hipMemcpy(el.data(), lo->e, el.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+        }
         for (int v = 0; v < 4; v++) {
             if (!out[v]) continue;
             const double off = v == 0 ? rho_ref : 0.0;
-            for (long i = 0; i < N; i++) out[v][i] = (double)m[4 * i + v] + off;
+            for (long i = 0; i < N; i++)
+                out[v][i] = (double)m[4 * i + v] + (lo ? (double)ml[4 * i + v] : 0.0) + off;
         }
         if (out[4])
-            for (long i = 0; i < N; i++) out[4][i] = (double)e[i] + rhoE_ref;
+            for (long i = 0; i < N; i++) out[4][i] = (double)e[i] + (lo ? (double)el[i] : 0.0) + rhoE_ref;
+    }
+
+    // the low parts of set 1, 2 or 4 in compensated mode, else null
+    const QSet<real> *lo_of(int set) const
+    {
+        const QSet<real> *l = set == 1 ? &F.qlo : set == 2 ? &F.qklo : set == 4 ? &F.acclo : nullptr;
+        return (l && l->m) ? l : nullptr;
     }
 
     // the device fields of a conserved set (1, 2, 4, 5, 6; see cs3d.h)
@@ -703,16 +737,28 @@ struct SolverT : SolverBase {
             return;
         }
         const QSet<real> &dst = set_of(set);
-        std::vector<real> m(4 * (size_t)N), e((size_t)N);
+        const QSet<real> *lo = lo_of(set);
+        std::vector<real> m(4 * (size_t)N), e((size_t)N), ml, el;
+        if (lo) {
+            ml.resize(m.size());
+            el.resize(e.size());
+        }
+        // hi = the value rounded, lo = the rounding residual (compensated)
+        auto put = [&](double x, real &hi, real *lop) {
+            hi = (real)x;
+            if (lop) *lop = (real)(x - (double)hi);
+        };
         for (long i = 0; i < N; i++) {
-            m[4 * i + 0] = (real)(in[0][i] - rho_ref);
-            m[4 * i + 1] = (real)in[1][i];
-            m[4 * i + 2] = (real)in[2][i];
-            m[4 * i + 3] = (real)in[3][i];
-            e[i] = (real)(in[4][i] - rhoE_ref);
+            for (int v = 0; v < 4; v++)
+                put(in[v][i] - (v == 0 ? rho_ref : 0.0), m[4 * i + v], lo ? &ml[4 * i + v] : nullptr);
+            put(in[4][i] - rhoE_ref, e[i], lo ? &el[i] : nullptr);
         }
         hc(hipMemcpy(dst.m, m.data(), m.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
         hc(hipMemcpy(dst.e, e.data(), e.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
+        if (lo) {
+            hc(hipMemcpy(lo->m, ml.data(), ml.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
+            hc(hipMemcpy(lo->e, el.data(), el.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
+        }
     }
 
     // shearLayerInfoCalc (CSolver_AWS.cpp:2259-2456): x-z plane means and
@@ -968,6 +1014,7 @@ struct SolverT : SolverBase {
             if (cfg.bc_type[d] != CS3D_PERIODIC_SOLVE)
                 throw std::invalid_argument("the adjoint needs periodic lines in every direction");
         if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
+        if (cfg.compensated) throw std::invalid_argument("the adjoint does not cover the compensated mode");
         if (any_wall()) throw std::invalid_argument("the adjoint does not cover walls");
         if (!A[0]) {
             for (int i = 0; i < ADJ_FIELDS; i++) A[i] = adj_alloc_field();

@@ -71,6 +71,10 @@ struct Fields {
     QSet<real> qkw; // where the last sweep writes the next stage state: qk
                     // itself, or a stage-state store of the adjoint
     QSet<real> acc; // RK accumulator (*2)
+    // compensated FP32 mode: the rounding residuals (low parts) of the
+    // state, the RK4 accumulator and TVDRK3's stage state, so that each
+    // value is hi + lo to about twice the working precision; null when off
+    QSet<real> qlo, acclo, qklo;
     // intermediates of a stage, interleaved (pairs and quads, see Win):
     // column sweeps write 2 or 4 values per point with one store
     real *dYut;     // (Uy, Ty) pairs of the stage (first sweep)
@@ -180,8 +184,17 @@ template <typename real>
 struct FiltIO {
     int set;
     QSet<real> qin, qout;
+    QSet<real> qin_lo, qout_lo;  // compensated mode: the sets' low parts
     Ptr5<real> sep;
 };
+
+// hi + lo of a compensated value, and its split back into FP32 hi and lo
+__device__ __forceinline__ double two_part(float hi, float lo) { return (double)hi + (double)lo; }
+__device__ __forceinline__ void split_part(double v, float &hi, float &lo)
+{
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
+}
 
 // launchers (stage_*.hip)
 template <typename real>

@@ -13,8 +13,10 @@ namespace cs3d {
 // registers through the batched solve
 // FP32 column filters fit 1024 lanes (32 lines of 32 segments: 128-byte
 // rows); FP64 keeps 512 for its registers
-template <typename real>
-constexpr int filter_threads() { return sizeof(real) == 4 ? 1024 : 512; }
+// (the compensated variants, with the low parts and FP64 sums in
+// registers, need more than 128 VGPRs: 512 lanes)
+template <typename real, bool CMP = false>
+constexpr int filter_threads() { return (sizeof(real) == 4 && !CMP) ? 1024 : 512; }
 
 constexpr int RED_THREADS = 256;
 constexpr int RED_SLOTS = RED_SLOTS_HOST;
@@ -37,8 +39,10 @@ __device__ __forceinline__ double cfl_speed(const Phys<real> &ph, real rp, real 
 // RED: the last sweep of the filter also leaves, per workgroup, the CFL
 // maximum and the NaN count of its output (two doubles, for k_reduce_cfl)
 // SET: the five fields are a conserved set (quads + rhoE'), else separate
-template <typename real, int M, bool UNI, int MAP, bool RED, bool SET>
-__global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
+// CMP: compensated FP32 mode: the correction A^-1 (A - B) f is formed from
+// the high parts and subtracted from hi + lo in FP64
+template <typename real, int M, bool UNI, int MAP, bool RED, bool SET, bool CMP = false>
+__global__ __launch_bounds__((filter_threads<real, CMP>())) void k_filter5(const OpDev<real> *__restrict__ fo, LineMap lm,
                                                   FiltIO<real> io, Phys<real> ph,
                                                   double *__restrict__ part)
 {
@@ -71,6 +75,24 @@ __global__ __launch_bounds__(filter_threads<real>()) void k_filter5(const OpDev<
         }
     }
     solve_batch<5, kc_filter<real>(), real, M, UNI>(fo, c, r);
+    static_assert(!CMP || (SET && sizeof(real) == 4), "compensated sets are FP32 conserved sets");
+    if constexpr (CMP) {
+        if (active) {
+            // the correction folded into hi + lo here; r is spent
+            real lo[5][M];
+            win.load_seg_quad(io.qin_lo.m, lo[0], lo[1], lo[2], lo[3]);
+            win.load_seg(io.qin_lo.e, lo[4]);
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+#pragma unroll
+                for (int t = 0; t < M; t++) {
+                    split_part(two_part(fc[v][t], lo[v][t]) - (double)r[v][t], fc[v][t], lo[v][t]);
+                    r[v][t] = real(0);
+                }
+            win.store_quad(io.qout_lo.m, lo[0], lo[1], lo[2], lo[3], c.len);
+            win.store(io.qout_lo.e, lo[4], c.len);
+        }
+    }
     if (active) {
 #pragma unroll
         for (int v = 0; v < 5; v++)
@@ -320,6 +342,10 @@ __global__ void k_sponge(Fields<real> F, Phys<real> ph, StageArgs sa, const Step
         if (plane) {
             qset_store(F.qstale, i, qv);
             qset_store(F.q, i, a);
+            if (F.qlo.m) {
+                const real z[5] = {0, 0, 0, 0, 0};
+                qset_store(F.qlo, i, z);
+            }
         }
     }
 }
@@ -349,7 +375,7 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
                : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
 }
 
-template <typename real, bool RED, bool SET>
+template <typename real, bool RED, bool SET, bool CMP = false>
 static hipError_t launch_f5(const OpDev<real> *fo, int uni, int M, const LaunchGeom &lg, int map,
                             const LineMap &lm, const FiltIO<real> &io, const Phys<real> &ph,
                             double *part, hipStream_t st)
@@ -359,15 +385,15 @@ static hipError_t launch_f5(const OpDev<real> *fo, int uni, int M, const LaunchG
     if (uni) {
         if (M != SEG_FILTER) return hipErrorInvalidValue;
         if (map == MAP_ROW)
-            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_ROW, RED, SET, CMP>), grid, block, shm, st, fo, lm, io, ph, part);
         else
-            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_FILTER, true, MAP_COL, RED, SET, CMP>), grid, block, shm, st, fo, lm, io, ph, part);
     } else {
         if (M != SEG_TABLE) return hipErrorInvalidValue;
         if (map == MAP_ROW)
-            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_ROW, RED, SET, CMP>), grid, block, shm, st, fo, lm, io, ph, part);
         else
-            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL, RED, SET>), grid, block, shm, st, fo, lm, io, ph, part);
+            hipLaunchKernelGGL((k_filter5<real, SEG_TABLE, false, MAP_COL, RED, SET, CMP>), grid, block, shm, st, fo, lm, io, ph, part);
     }
     return hipGetLastError();
 }
@@ -382,6 +408,13 @@ hipError_t launch_filter5(const OpDev<real> *fo, int uni, int M, const LaunchGeo
     // the CFL partials come from the state: a conserved set only
     if (part && !io.set) return hipErrorInvalidValue;
     if (!io.set) return launch_f5<real, false, false>(fo, uni, M, lg, map, lm, io, ph, part, st);
+    if (io.qin_lo.m) {
+        if (lg.threads > filter_threads<real, true>()) return hipErrorInvalidConfiguration;
+        if constexpr (sizeof(real) == 4)
+            return part ? launch_f5<real, true, true, true>(fo, uni, M, lg, map, lm, io, ph, part, st)
+                        : launch_f5<real, false, true, true>(fo, uni, M, lg, map, lm, io, ph, part, st);
+        return hipErrorInvalidValue;
+    }
     return part ? launch_f5<real, true, true>(fo, uni, M, lg, map, lm, io, ph, part, st)
                 : launch_f5<real, false, true>(fo, uni, M, lg, map, lm, io, ph, part, st);
 }

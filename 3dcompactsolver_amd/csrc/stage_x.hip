@@ -34,7 +34,10 @@ namespace cs3d {
 // FILT: last stage of a step that filters, on uniform periodic x lines: the
 // new state is filtered along x in place of the x filter sweep and written
 // to *1 (fx: the x filter with this sweep's segmentation)
-template <typename real, int M, bool UNI, bool STALE, bool QS, bool FILT>
+// CMP: compensated FP32 mode: the state, the RK4 accumulator and TVDRK3's
+// stage state carry low parts (F.qlo, F.acclo, F.qklo); the update forms
+// hi + lo + c dt R in FP64 and splits the result back into hi and lo
+template <typename real, int M, bool UNI, bool STALE, bool QS, bool FILT, bool CMP = false>
 __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2,
     const OpDev<real> *__restrict__ fx, LineMap lm, Fields<real> F, Phys<real> ph, StageArgs sa,
@@ -388,6 +391,26 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         else
             return al[v][t];
     };
+    // compensated mode: the low parts of the base state, the accumulator and
+    // TVDRK3's stage state, and the update coefficients in FP64
+    static_assert(!CMP || sizeof(real) == 4, "the compensated mode is an FP32 mode");
+    real blo[CMP ? 5 : 1][M], alo[CMP ? 5 : 1][M], slo[CMP ? 5 : 1][M];
+    real finlo[CMP && FILT ? 5 : 1][M], eAL[CMP ? M : 1];
+    if constexpr (CMP) {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+            for (int t = 0; t < M; t++) blo[v][t] = alo[v][t] = slo[v][t] = real(0);
+        if (need_base) load_set_seg(F.qlo, blo);
+        if (sa.rk == 4 && !first) load_set_seg(F.acclo, alo);
+        if (sa.rk == 3 && sa.rkStep > 1) load_set_seg(F.qklo, slo);
+    }
+    const double rad = (sa.rkStep == 1 || sa.rkStep == 4) ? 1.0 / 6.0 : 1.0 / 3.0;
+    const double rbd = sa.rkStep == 3 ? 1.0 : 0.5;
+    const double tad = sa.rkStep == 1 ? 1.0 : sa.rkStep == 2 ? 0.75 : 1.0 / 3.0;
+    const double tbd = sa.rkStep == 1 ? 0.0 : sa.rkStep == 2 ? 0.25 : 2.0 / 3.0;
+    const double tcd = sa.rkStep == 1 ? 1.0 : sa.rkStep == 2 ? 0.25 : 2.0 / 3.0;
+    const QSet<real> out_lo = (sa.rk == 4 || !tvd_to_k) ? F.acclo : F.qklo;
     // point by point: each point's quad goes out as soon as it is formed;
     // the rhoE' values of the segment go out together at the end.  The quad
     // stores are ordinary ones: a wavefront's store of one point per lane
@@ -403,7 +426,37 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         real b[5], a[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) b[v] = base(v, t);
-        if (sa.rk == 4) {
+        if constexpr (CMP) {
+            real al_[5];
+            if (sa.rk == 4) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) {
+                    const double prev = first ? two_part(b[v], blo[v][t])
+                                              : two_part(accum(v, t), alo[v][t]);
+                    split_part(prev + rad * (double)k[v][t], a[v], al_[v]);
+                }
+                if (store_q) {
+#pragma unroll
+                    for (int v = 0; v < 5; v++)
+                        b[v] = (real)(two_part(b[v], blo[v][t]) + rbd * (double)k[v][t]);
+                    if (st_ok) win.template store_quad_at<false>(F.qkw.m, t, b[0], b[1], b[2], b[3]);
+                    eQ[t] = b[4];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                    split_part(tad * two_part(b[v], blo[v][t]) +
+                                   tbd * two_part(qs_[v][t], slo[v][t]) + tcd * (double)k[v][t],
+                               a[v], al_[v]);
+            }
+            if constexpr (FILT) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) finlo[v][t] = al_[v];
+            } else {
+                if (st_ok) win.template store_quad_at<false>(out_lo.m, t, al_[0], al_[1], al_[2], al_[3]);
+                eAL[t] = al_[4];
+            }
+        } else if (sa.rk == 4) {
             // RK4.hpp:45-111: *2 accumulates the stages, *k is the next
             // stage state
 #pragma unroll
@@ -428,6 +481,7 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
         }
     }
     if constexpr (!FILT) win.template store<true>(out_a.e, eA, c.len);
+    if constexpr (CMP && !FILT) win.template store<true>(out_lo.e, eAL, c.len);
     if (store_q) win.template store<true>(F.qkw.e, eQ, c.len);
     if constexpr (FILT) {
         // filterConservedData's x sweep (CSolver_AWS.cpp:1682-1893) on the
@@ -469,13 +523,40 @@ __global__ __launch_bounds__(X_SWEEP_THREADS) void k_xpass(
             solve_batch<5, kc_filter_x<real>(), real, M, true>(fx, c, r);
         }
         if (active) {
+            if constexpr (CMP) {
+                // the correction from the high parts, subtracted from hi + lo
 #pragma unroll
-            for (int v = 0; v < 5; v++)
+                for (int v = 0; v < 5; v++)
 #pragma unroll
-                for (int t = 0; t < M; t++) fin[v][t] -= r[v][t];
+                    for (int t = 0; t < M; t++)
+                        split_part(two_part(fin[v][t], finlo[v][t]) - (double)r[v][t], fin[v][t],
+                                   finlo[v][t]);
+                store_set_seg(F.qlo, finlo);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+#pragma unroll
+                    for (int t = 0; t < M; t++) fin[v][t] -= r[v][t];
+            }
             store_set_seg(F.q, fin);
         }
     }
+}
+
+template <typename real, bool UNI, bool STALE, bool FILT, bool CMP>
+static hipError_t launch_x5(const OpDev<real> *d1, const OpDev<real> *d2, const OpDev<real> *fx,
+                            const LaunchGeom &lg, const LineMap &lm, const Fields<real> &F,
+                            const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
+                            hipStream_t st)
+{
+    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
+    if (STALE || F.sigma || sa.rk == 3 || sa.rkStep == 1)
+        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true, FILT, CMP>), grid, block,
+                           lg.shm, st, d1, d2, fx, lm, F, ph, sa, sc);
+    else
+        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, false, FILT, CMP>), grid, block,
+                           lg.shm, st, d1, d2, fx, lm, F, ph, sa, sc);
+    return hipGetLastError();
 }
 
 template <typename real, bool UNI, bool STALE, bool FILT>
@@ -484,17 +565,16 @@ static hipError_t launch_x4(const OpDev<real> *d1, const OpDev<real> *d2, const 
                             const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                             hipStream_t st)
 {
-    dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
     // the stale primitives exist only at the first stage after a sponge
     // update, and the STALE kernel takes the first stage as given
     if (STALE && sa.rkStep != 1) return hipErrorInvalidValue;
-    if (STALE || F.sigma || sa.rk == 3 || sa.rkStep == 1)
-        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, true, FILT>), grid, block, lg.shm,
-                           st, d1, d2, fx, lm, F, ph, sa, sc);
-    else
-        hipLaunchKernelGGL((k_xpass<real, SEG_DERIV, UNI, STALE, false, FILT>), grid, block,
-                           lg.shm, st, d1, d2, fx, lm, F, ph, sa, sc);
-    return hipGetLastError();
+    // the compensated variants: FP32, whole updates only
+    if (sa.compensated && sa.mode == 0) {
+        if constexpr (sizeof(real) == 4)
+            return launch_x5<real, UNI, STALE, FILT, true>(d1, d2, fx, lg, lm, F, ph, sa, sc, st);
+        return hipErrorInvalidValue;
+    }
+    return launch_x5<real, UNI, STALE, FILT, false>(d1, d2, fx, lg, lm, F, ph, sa, sc, st);
 }
 
 template <typename real, bool UNI>

@@ -86,6 +86,12 @@ __global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph
     const real m1 = rho * u, m2 = rho * v, m3 = rho * ww;
     const real qw[5] = {rp, m1, m2, m3, re};
     qset_store(q, ip, qw);
+    // compensated mode: the wall values are set, not accumulated
+    const QSet<real> qlo = sa.rkStep > 1 ? F.qklo : F.qlo;
+    if (qlo.m) {
+        const real z[5] = {0, 0, 0, 0, 0};
+        qset_store(qlo, ip, z);
+    }
     if (sa.stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) qset_store(F.qstale, ip, qw);
 }
 

@@ -22,6 +22,8 @@ adjoint = cost of the kinetic-energy gradient through 40 RK4 steps of the
          the replay of the first segment from its checkpoint, and the
          reverse sweep, against 40 forward steps (reverse_over_forward =
          (replay + reverse) / forward).
+fp32_compensated = the same FP32 workload in the compensated mode (state,
+         accumulator and stage sums carried as FP32 hi + lo pairs).
 cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
          with the reference's flags) timed on this host's cores at 256^3
          (1 warm-up + 3 steps), with a 128^3 sample beside it.
@@ -117,9 +119,9 @@ def tgv_kw(N):
                 cfl=0.5, filter_step=1, rk=4), L
 
 
-def run_gpu(N, prec, steps, warmup, ic):
+def run_gpu(N, prec, steps, warmup, ic, **extra):
     kw, _ = tgv_kw(N)
-    g = cs3d.Solver(cs3d.config(N, N, N, precision=prec, **kw))
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=prec, **kw, **extra))
     g.set_initial_conditions(*ic)
     g.step(warmup)
     g.synchronize()
@@ -233,6 +235,8 @@ def main():
     ap.add_argument("--cpu-small-steps", type=int, default=6,
                     help="timed steps of the 128^3 CPU sample beside the main one (0: none)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
+    ap.add_argument("--no-compensated", action="store_true",
+                    help="skip the compensated FP32 leg")
     ap.add_argument("--adjoint-steps", type=int, default=40)
     ap.add_argument("--adjoint-checkpoint", type=int, default=20,
                     help="steps between adjoint checkpoints; 0: all steps (every stage state "
@@ -274,6 +278,14 @@ def main():
                 "roofline": roof64,
                 "kinetic_energy": ke64,
                 "fp32_ke_rel_err": abs(ke32 - ke64) / abs(ke64)}
+    # the compensated FP32 mode (off by default in the library): its cost
+    cmp = None
+    if not args.no_compensated:
+        msc, vc, _, kec = run_gpu(N, cs3d.FP32, args.steps, args.warmup, ic, compensated=1)
+        cmp = {"value": round(vc, 1), "ms_per_step": round(msc, 3),
+               "hbm_roof_frac_fixed_model": roof_fracs(vc, cs3d.FP32)[0]}
+        if fp64:
+            cmp["ke_rel_err_vs_fp64"] = abs(kec - fp64["kinetic_energy"]) / abs(fp64["kinetic_energy"])
     ck = args.adjoint_checkpoint if args.adjoint_checkpoint > 0 else args.adjoint_steps
     adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, ck, ms32)
     cpu = None
@@ -304,6 +316,7 @@ def main():
         "hbm_roof_frac": own32,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof32.items()},
         "fp64": fp64,
+        "fp32_compensated": cmp,
         "adjoint": adj,
         "roofline": roof,
         "hbm_stream_GBps": stream_bw,

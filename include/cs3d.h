@@ -78,7 +78,7 @@ typedef struct {
     double Pr;                /* IdealGas::Pr (reference default 0.7) */
     int rk;                   /* CS3D_RK4 or CS3D_TVDRK3 */
     int precision;            /* CS3D_FP64 or CS3D_FP32 */
-    int compensated;          /* FP32 only: compensated (Kahan) state and RK sums */
+    int compensated;          /* FP32 only: state and RK sums carried as FP32 hi + lo pairs */
     double sponge_frac[3];    /* sponge length / domain length (SpongeBC.hpp:57-59) */
     double sponge_strength;   /* 6.0 */
     double sponge_avgT;       /* 15.0 */

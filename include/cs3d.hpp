@@ -466,6 +466,7 @@ class CSolver_AWS : public AbstractCSolver {
     double Z0WallU = 0.0, Z0WallV = 0.0, Z1WallU = 0.0, Z1WallV = 0.0;
     // GPU-path options (no reference counterpart)
     int precision = CS3D_FP64;       // CS3D_FP32: FP32 bulk arithmetic
+    int compensated = 0;             // FP32 only: state and RK sums as hi + lo pairs
     bool tgvDiagnostics = false;     // append taylorgreen.out every step
     bool turbDiagnostics = false;    // append turbdata.out every step
     std::string outDir = ".";
@@ -541,6 +542,7 @@ class CSolver_AWS : public AbstractCSolver {
         c.Pr = ig->Pr;
         c.rk = rkOrder == 3 ? CS3D_TVDRK3 : CS3D_RK4;
         c.precision = precision;
+        c.compensated = compensated;
         const double wv[6][3] = {{0.0, X0WallV, X0WallW}, {0.0, X1WallV, X1WallW},
                                  {Y0WallU, 0.0, Y0WallW}, {Y1WallU, 0.0, Y1WallW},
                                  {Z0WallU, Z0WallV, 0.0}, {Z1WallU, Z1WallV, 0.0}};

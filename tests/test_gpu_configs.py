@@ -90,6 +90,8 @@ def test_hit128_tvdrk3():
     k32, q32, _ = run((N, N, N), kw, icf, cs3d.FP32, 500)
     rel = abs(k32 - k64) / abs(k64)
     errs = [relerr(a, b) for a, b in zip(q32, q64)]
+    # plain FP32 drifts ~2e-5 here; the compensated FP32 mode's test holds
+    # the same run to 1e-6
     print(f"HIT 500 steps: KE rel {rel:.2e}; fields " + " ".join(f"{e:.2e}" for e in errs),
           flush=True)
     assert rel <= 1e-4
