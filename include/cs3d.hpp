@@ -23,7 +23,11 @@
  *    stay valid until the solver is destroyed).
  *  - The RK stage update (AbstractRK::updateConservedData) is fused into the
  *    last sweep of each stage on the GPU; RK4/TVDRK3::updateConservedData
- *    are kept for the call sequence and do nothing.
+ *    are kept for the call sequence and do nothing.  A time integrator of
+ *    the caller's own (an AbstractRK subclass that updates rho1 ... rhoE2)
+ *    would act on host mirrors the GPU never reads back, so
+ *    CSolver_AWS::setInitialConditions aborts unless cs3d::RK4 or
+ *    cs3d::TVDRK3 has named the scheme through setRKOrder.
  *  - Errors: the reference returns void and prints; here a failed library
  *    call prints the library's message to std::cerr and aborts.
  */
@@ -522,6 +526,13 @@ class CSolver_AWS : public AbstractCSolver {
     // CSolver_AWS.cpp:199-504
     void setInitialConditions() override
     {
+        if (rkOrder != 3 && rkOrder != 4) {
+            std::cerr << " > CSolver_AWS advances its state on the GPU inside solveEqnSet; "
+                         "drive it with cs3d::RK4 or cs3d::TVDRK3 (they call setRKOrder), not "
+                         "an AbstractRK of your own"
+                      << std::endl;
+            std::abort();
+        }
         cs3d_config c;
         check(cs3d_config_default(&c, dom->Nx, dom->Ny, dom->Nz), "cs3d_config_default");
         c.Lx = dom->Lx;
@@ -827,7 +838,7 @@ class CSolver_AWS : public AbstractCSolver {
     }
 
     cs3d_solver *h = nullptr;
-    int rkOrder = 4;
+    int rkOrder = 0;  // set by RK4 / TVDRK3 through setRKOrder
     std::vector<double> store;
 };
 

@@ -34,6 +34,48 @@ def test_dropin_header_is_plain_cxx():
     assert r.returncode == 0, r.stderr.decode()
 
 
+CUSTOM_RK = r"""
+#define CS3D_DROPIN_GLOBAL
+#include "cs3d.hpp"
+// a time integrator of the caller's own: it would update the host mirrors
+class MyRK : public AbstractRK {
+  public:
+    explicit MyRK(AbstractCSolver *c) { cs = c; }
+    void updateConservedData() override {}
+    void executeSolverLoop() override { cs->setInitialConditions(); }
+};
+int main()
+{
+    Domain dom(8, 8, 8, 1.0, 1.0, 1.0);
+    BC bc(BC::PERIODIC_SOLVE, BC::PERIODIC, BC::PERIODIC, BC::PERIODIC_SOLVE, BC::PERIODIC,
+          BC::PERIODIC, BC::PERIODIC_SOLVE, BC::PERIODIC, BC::PERIODIC);
+    TimeStepping ts(TimeStepping::CONST_CFL, 0.5, 1, 1.0, 1, 1, 1);
+    CSolver_AWS cs(&dom, &bc, &ts, 0.49, 1e-3, 8, false);
+    MyRK rk(&cs);
+    rk.executeSolverLoop();
+    return 0;
+}
+"""
+
+
+def test_dropin_rejects_foreign_integrator(tmp_path):
+    """CSolver_AWS keeps its state on the GPU and fuses the RK update into its
+    sweeps, so an AbstractRK subclass of the caller's own must not run against
+    it silently: setInitialConditions aborts, naming RK4 / TVDRK3, before any
+    GPU call."""
+    _build()
+    src = tmp_path / "custom_rk.cpp"
+    src.write_text(CUSTOM_RK)
+    exe = tmp_path / "custom_rk"
+    lib = os.path.join(ROOT, "3dcompactsolver_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), str(src),
+                    "-L" + lib, "-l3dcompact_hip", "-Wl,-rpath," + lib, "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, timeout=60)
+    assert r.returncode != 0
+    assert b"cs3d::RK4 or cs3d::TVDRK3" in r.stderr
+
+
 @pytest.mark.gpu
 def test_dropin_driver_matches_c_abi(tmp_path):
     exe = BIN if os.access(BIN, os.X_OK) else _build()
