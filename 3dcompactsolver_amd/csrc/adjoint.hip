@@ -20,25 +20,6 @@
 
 namespace cs3d {
 
-template <typename real>
-__global__ void k_adj_prim(QSet<real> q, Phys<real> ph, AdjPrim<real> o, long N)
-{
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (long)gridDim.x * blockDim.x) {
-        // the forward's own primitive and viscosity evaluations
-        real ir, U, V, W, pp, Tp, mu, amu, q5[5];
-        qset_load(q, i, q5);
-        prim(ph, q5[0], q5[1], q5[2], q5[3], q5[4], ir, U, V, W, pp, Tp);
-        visc(ph, ph.T0s + Tp, mu, amu);
-        o.U[i] = U;
-        o.V[i] = V;
-        o.W[i] = W;
-        o.Tp[i] = Tp;
-        o.mu[i] = mu;
-        o.amu[i] = amu;
-    }
-}
-
 // One point of the cotangent seeds of the derivative fields, and of the
 // pointwise cotangents of U, V, W, T from the viscous and heat-flux terms.
 // First and second derivatives: index 3 * f + j with f = U, V, W, T and
@@ -118,18 +99,6 @@ __device__ __forceinline__ void seed_point(const Phys<real> &ph, const SeedPt<re
     o[SD_WB] = vE * (mu * L3 + a * A3);
 }
 
-// out = sum_k c[k] * in[k] (n terms)
-template <typename real>
-__global__ void k_adj_lincomb(AdjComb<real> a, long N)
-{
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (long)gridDim.x * blockDim.x) {
-        real s = a.base ? a.base[i] : real(0);
-        for (int k = 0; k < a.n; k++) s += a.c[k] * a.in[k][i];
-        a.out[i] = s;
-    }
-}
-
 // Chain the primitive cotangents (U, V, W, T) and the flux cotangents
 // Fb[3 * c + j] = D_j v_c of one point back to the conserved state (q in
 // the stored perturbation form).
@@ -188,338 +157,543 @@ __device__ __forceinline__ void chain_point(const Phys<real> &ph, const real (&q
     res[4] = Eb;
 }
 
-// N1 first-derivative and N2 second-derivative line solves of one direction
-// through one exchange (see LineTerms).
-template <int N1, int N2, bool ACC, typename real, int M, bool UNI, int MAP>
-__global__ __launch_bounds__(line_terms_threads<real>()) void k_line_terms(
-    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
-    LineTerms<real> a)
+// ------------------------------------------------------------ windows
+
+// The window of the stage state (perturbation form, a quad and the energy)
+// and the primitives U, V, W, T' at every window point, by the forward's
+// own primitive evaluation.
+template <typename real, int W, typename WinT>
+__device__ __forceinline__ void prim_windows(const WinT &win, const QSet<real> &q,
+                                             const Phys<real> &ph, real (&pw)[4][W],
+                                             real (&qw)[5][W])
 {
-    constexpr int HW = 2, W = M + 2 * HW, N = N1 + N2;
-    CS3D_LANE_SETUP(MAP, d1, sweep_kr(d1, d2), N)
-    (void)n;
-    (void)per;
-    real r[N][M];
+    win.load_quad(q.m, qw[0], qw[1], qw[2], qw[3]);
+    win.load(q.e, qw[4]);
 #pragma unroll
-    for (int t = 0; t < N; t++) {
-        real w[W];
-        win.load(a.in[t], w);
-        if (t < N1)
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[t]);
-        else
-            rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[t]);
-    }
-    solve_d12<N1, N2, real, M, UNI>(d1, d2, c, r);
-    if (ACC) {
-        real o[M];
-#pragma unroll
-        for (int t = 0; t < M; t++) o[t] = 0;
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-            if (a.base[b]) {
-                real bv[M];
-                win.load_seg(a.base[b], bv);
-#pragma unroll
-                for (int t = 0; t < M; t++) o[t] += a.bc[b] * bv[t];
-            }
-#pragma unroll
-        for (int k = 0; k < N; k++)
-#pragma unroll
-            for (int t = 0; t < M; t++) o[t] += a.c[k] * r[k][t];
-        if (active) win.store(a.out[0], o, c.len);
-    } else {
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-#pragma unroll
-            for (int t = 0; t < M; t++) r[k][t] *= a.c[k];
-            if (active) win.store(a.out[k], r[k], c.len);
-        }
+    for (int t = 0; t < W; t++) {
+        real ir, pp;
+        prim(ph, qw[0][t], qw[1][t], qw[2][t], qw[3][t], qw[4][t], ir, pw[0][t], pw[1][t],
+             pw[2][t], pp, pw[3][t]);
     }
 }
 
-template <int N1, int N2, bool ACC, typename real, bool UNI>
-static hipError_t launch_lt(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &g,
-                            int map, const LineMap &lm, const LineTerms<real> &a, hipStream_t st)
-{
-    constexpr int M = SEG_DERIV;
-    dim3 grid((lm.nlines + g.LX - 1) / g.LX), block(g.threads);
-    if (map == MAP_ROW)
-        hipLaunchKernelGGL((k_line_terms<N1, N2, ACC, real, M, UNI, MAP_ROW>), grid, block, g.shm,
-                           st, d1, d2, lm, a);
-    else
-        hipLaunchKernelGGL((k_line_terms<N1, N2, ACC, real, M, UNI, MAP_COL>), grid, block, g.shm,
-                           st, d1, d2, lm, a);
-    return hipGetLastError();
-}
-
-template <int N1, int N2, bool ACC, typename real>
-static hipError_t launch_lt2(const OpDev<real> *d1, const OpDev<real> *d2, int uni,
-                             const LaunchGeom &g, int map, const LineMap &lm,
-                             const LineTerms<real> &a, hipStream_t st)
-{
-    return uni ? launch_lt<N1, N2, ACC, real, true>(d1, d2, g, map, lm, a, st)
-               : launch_lt<N1, N2, ACC, real, false>(d1, d2, g, map, lm, a, st);
-}
-
+// Windows of values computed in registers on row (x) lines, through the LDS
+// staging area: each lane puts its own points at their positions along the
+// line (field f of line l at (f * LX + l) * n), and after a barrier reads
+// the window of a field, wrapping round the periodic line.  Any
+// segmentation (ragged lines included); the area needs NF * LX * n <=
+// NF * M * lanes elements.  The caller places the barriers: before the
+// first put (the area is the exchange ring's), after the last put, and
+// after the last read.
 template <typename real>
-hipError_t launch_line_terms(int n1, int n2, bool acc, const OpDev<real> *d1,
-                             const OpDev<real> *d2, int uni, int M, const LaunchGeom &g, int map,
-                             const LineMap &lm, const LineTerms<real> &a, hipStream_t st)
+__device__ __forceinline__ void stage_put(const Lane<real, MAP_ROW> &c, int n, int f, int t,
+                                          real v)
 {
-    if (M != SEG_DERIV) return hipErrorInvalidValue;
-    if (g.LX < 1 || g.threads < 1 || g.threads > line_terms_threads<real>())
-        return hipErrorInvalidConfiguration;
-#define CS3D_LT(A, B, C)                                                                   \
-    if (n1 == A && n2 == B && acc == C) return launch_lt2<A, B, C, real>(d1, d2, uni, g, map, lm, a, st);
-    CS3D_LT(4, 4, false)
-    CS3D_LT(2, 0, false)
-    CS3D_LT(4, 0, false)
-    CS3D_LT(3, 0, false)
-    CS3D_LT(5, 0, false)
-    CS3D_LT(1, 1, true)
-    CS3D_LT(2, 1, true)
-    CS3D_LT(1, 0, true)
-#undef CS3D_LT
-    return hipErrorInvalidValue;
+    if (t < c.len) c.xb[(f * c.LX + c.line) * n + c.s + t] = v;
+}
+template <typename real, int M, int HW>
+__device__ __forceinline__ void stage_get(const Lane<real, MAP_ROW> &c, int n, int f,
+                                          real (&w)[M + 2 * HW])
+{
+    const real *st = c.xb + (f * c.LX + c.line) * n;
+#pragma unroll
+    for (int t = 0; t < M + 2 * HW; t++) {
+        int p = c.s - HW + t;
+        p += p < 0 ? n : 0;
+        p -= p >= n ? n : 0;
+        w[t] = st[p];
+    }
 }
 
-// Gradient of J = (1/N) sum rho |u|^2 / 2 with respect to the conserved
-// state, and the map of a conserved cotangent to the initial primitive
-// velocities at fixed rho and p: m_i = rho u_i, E = p/(gamma-1) + rho|u|^2/2.
-// The x-direction derivatives of the stage (D1 of U, V, W, T and of Uy, Uz,
-// Vy, Wz for the mixed derivatives, D2 of U, V, W, T) through one batched
-// solve, then the cotangent seeds at every point from them and the stored
-// y and z derivatives.  The mixed derivatives are taken as D_x D_y U etc.:
-// periodic operators along different axes commute.
+// one point's values of a quad, pair or single field
+template <typename real, typename WinT>
+__device__ __forceinline__ void pt_quad(const WinT &win, const real *f, int t, real *o)
+{
+    typedef real vec4 __attribute__((ext_vector_type(4)));
+    const vec4 v = *(const vec4 *)WinT::quad_at(f, win.mid(t));
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
+    o[3] = v[3];
+}
+template <typename real, typename WinT>
+__device__ __forceinline__ void pt_pair(const WinT &win, const real *f, int t, real *o)
+{
+    typedef real vec2 __attribute__((ext_vector_type(2)));
+    const vec2 v = *(const vec2 *)WinT::pair_at(f, win.mid(t));
+    o[0] = v[0];
+    o[1] = v[1];
+}
+
+// ------------------------------------------------------------ recompute passes
+
+// y: D1 and D2 of U, V, W, T' along y -> (Uy, Ty), (Vy, Wy), (Uyy, Vyy, Wyy, Tyy)
 template <typename real, int M, bool UNI>
-__global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xseed(const OpDev<real> *__restrict__ d1,
-                                                             const OpDev<real> *__restrict__ d2,
-                                                             LineMap lm, AdjXSeed<real> in,
-                                                             AdjSeedOut<real> o, Phys<real> ph)
+__global__ __launch_bounds__(adj_rec_threads<real>()) void k_adj_yrec(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
+    AdjVjp<real> a, Phys<real> ph)
 {
     constexpr int HW = 2, W = M + 2 * HW;
-    CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 12)
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), ADJ_R_PLANES)
     (void)n;
     (void)per;
-    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx; own: the
-    // segment's own values of the eight window fields
-    real r[12][M], own[8][M];
+    real r[8][M];
     {
-        const real *src[8] = {in.U, in.V, in.W, in.Tp, in.dy1[0], in.dz1[0], in.dy1[1], in.dz1[2]};
+        real pw[4][W], qw[5][W];
+        prim_windows(win, a.q, ph, pw, qw);
 #pragma unroll
-        for (int f = 0; f < 8; f++) {
-            real w[W];
-            win.load(src[f], w);
-#pragma unroll
-            for (int t = 0; t < M; t++) own[f][t] = w[HW + t];
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[f]);
-            if (f < 4) rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[8 + f]);
+        for (int f = 0; f < 4; f++) {
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, pw[f], r[f]);
+            rhs<OP_D2, real, M, HW, UNI>(d2, c, pw[f], r[4 + f]);
         }
     }
-    solve_d12<8, 4, real, M, UNI>(d1, d2, c, r);
-    // the remaining per-point inputs, whole segments at a time
-    enum { P_MU, P_A, P_V1, P_V2, P_V3, P_VE, P_WY, P_TY, P_VZ, P_TZ, P_VYZ, P_WYZ, P_D2 };
-    real pt[P_D2 + 8][M];
+    solve_d12<4, 4, real, M, UNI>(d1, d2, c, r);
+    if (!active) return;
+    win.template store_pair<true>(a.rec.dYut, r[0], r[3], c.len);
+    win.template store_pair<true>(a.rec.dYvw, r[1], r[2], c.len);
+    win.template store_quad<true>(a.rec.dY2, r[4], r[5], r[6], r[7], c.len);
+}
+
+// z: D1 and D2 of U, V, W, T' and D1 of the stored Vy, Wy along z ->
+// (Uz, Vz, Wz, Tz), (Uzz, Vzz, Wzz, Tzz), (Vyz, Wyz)
+template <typename real, int M, bool UNI>
+__global__ __launch_bounds__(adj_rec_threads<real>()) void k_adj_zrec(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
+    AdjVjp<real> a, Phys<real> ph)
+{
+    constexpr int HW = 2, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), ADJ_R_PLANES)
+    (void)n;
+    (void)per;
+    real r[10][M];  // D1 of U V W T Vy Wy | D2 of U V W T
     {
-        const real *src[P_D2 + 8] = {in.mu, in.amu, in.v[1], in.v[2], in.v[3], in.v[4],
-                                     in.dy1[2], in.dy1[3], in.dz1[1], in.dz1[3], in.Vyz, in.Wyz,
-                                     in.dy2[0], in.dz2[0], in.dy2[1], in.dz2[1],
-                                     in.dy2[2], in.dz2[2], in.dy2[3], in.dz2[3]};
+        real pw[4][W], qw[5][W];
+        prim_windows(win, a.q, ph, pw, qw);
 #pragma unroll
-        for (int k = 0; k < P_D2 + 8; k++) win.load_seg(src[k], pt[k]);
+        for (int f = 0; f < 4; f++) {
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, pw[f], r[f]);
+            rhs<OP_D2, real, M, HW, UNI>(d2, c, pw[f], r[6 + f]);
+        }
     }
-    real out[SD_COUNT][M];
+    {
+        real mw[2][W];
+        win.load_pair(a.rec.dYvw, mw[0], mw[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[0], r[4]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[1], r[5]);
+    }
+    solve_d12<6, 4, real, M, UNI>(d1, d2, c, r);
+    if (!active) return;
+    win.template store_quad<true>(a.rec.dZ1, r[0], r[1], r[2], r[3], c.len);
+    win.template store_quad<true>(a.rec.dZ2, r[6], r[7], r[8], r[9], c.len);
+    win.template store_pair<true>(a.rec.mYZ, r[4], r[5], c.len);
+}
+
+// ------------------------------------------------------------ x pass
+
+// The x derivatives of the stage (D1 of U, V, W, T and of the stored Uy, Uz,
+// Vy, Wz for the mixed derivatives Uxy, Uxz, Vxy, Wxz; D2 of U, V, W, T)
+// in one batch; the cotangent seeds at every point from them and the
+// stored y and z derivatives; then every x-direction transposed term in a
+// second batch over windows of the seeds staged through LDS: D_x of the
+// first-derivative seeds g1UX, g1XY, g1XZ, g1TX and of the flux cotangents
+// v_c, D2_x of s0, s1, s2, gq, and Xs_i = D_x s_i for the mixed transposes
+// the column passes finish.  The pointwise cotangents (ub, vb, wb, Tb) and
+// these x terms are chained to the conserved state (resx).  The mixed
+// derivatives are taken as D_x D_y U etc.: periodic operators along
+// different axes commute.  Values needed after a solve batch are read again
+// per point rather than held across it (they come from the caches).
+template <typename real, int M, bool UNI>
+__global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__restrict__ d1,
+                                                         const OpDev<real> *__restrict__ d2,
+                                                         LineMap lm, AdjVjp<real> a,
+                                                         Phys<real> ph)
+{
+    constexpr int HW = 2, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), ADJ_X_PLANES)
+    (void)per;
+    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx
+    real r[12][M];
+    {
+        real pw[4][W], qw[5][W];
+        prim_windows(win, a.q, ph, pw, qw);
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            rhs<OP_D1, real, M, HW, UNI>(d1, c, pw[f], r[f]);
+            rhs<OP_D2, real, M, HW, UNI>(d2, c, pw[f], r[8 + f]);
+        }
+    }
+    {
+        real w0[W], w1[W], w2[W], w3[W];
+        win.load_pair(a.rec.dYut, w0, w1);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[4]);
+        win.load_pair(a.rec.dYvw, w0, w1);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[6]);
+        win.load_quad(a.rec.dZ1, w0, w1, w2, w3);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[5]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w2, r[7]);
+    }
+    solve_d12<8, 4, real, M, UNI>(d1, d2, c, r);
+    // seeds, point by point; the eight that the x solves need go to the
+    // staging area, the ones the column passes need are kept or stored
+    constexpr int NS = ADJ_X_STAGE;
+    const int stg[NS] = {SD_G1 + G1_UX, SD_G1 + G1_XY, SD_G1 + G1_XZ, SD_G1 + G1_TX,
+                         SD_S,          SD_S + 1,      SD_S + 2,      SD_GQ};
+    enum { K_UB, K_VB, K_WB, K_TB, K_XZ, K_WZ, K_TZ, K_XY, K_VY, K_TY, K_YZ, K_COUNT };
+    const int kept[K_COUNT] = {SD_UB,         SD_VB,         SD_WB,         SD_TB,
+                               SD_G1 + G1_XZ, SD_G1 + G1_WZ, SD_G1 + G1_TZ, SD_G1 + G1_XY,
+                               SD_G1 + G1_VY, SD_G1 + G1_TY, SD_G1 + G1_YZ};
+    // the kept values wait in LDS past the ring and the window staging (the
+    // host sizes the area: adj_x_lds), at this lane's own slots
+    real *kp = xb + (2 * c.cap > NS * M * c.LX * c.P ? 2 * c.cap : NS * M * c.LX * c.P);
+    const int T = c.LX * c.P, me = c.line * c.P + c.g;
+    auto kslot = [&](int k, int t) -> real & { return kp[(k * M + t) * T + me]; };
+    __syncthreads();
 #pragma unroll
     for (int t = 0; t < M; t++) {
+        real q[5], ut[2], vw[2], z1[4], y2[4], z2[4], myz[2], v4[4];
+        pt_quad(win, a.q.m, t, q);
+        q[4] = ldo(a.q.e, win.mid(t));
+        pt_pair(win, a.rec.dYut, t, ut);
+        pt_pair(win, a.rec.dYvw, t, vw);
+        pt_quad(win, a.rec.dZ1, t, z1);
+        pt_quad(win, a.rec.dY2, t, y2);
+        pt_quad(win, a.rec.dZ2, t, z2);
+        pt_pair(win, a.rec.mYZ, t, myz);
+        pt_quad(win, a.v.m, t, v4);
+        const real vE = ldo(a.v.e, win.mid(t));
         SeedPt<real> p;
-        p.U = own[0][t];
-        p.V = own[1][t];
-        p.W = own[2][t];
-        p.Tp = own[3][t];
-        p.mu = pt[P_MU][t];
-        p.a = pt[P_A][t];
-        p.v1 = pt[P_V1][t];
-        p.v2 = pt[P_V2][t];
-        p.v3 = pt[P_V3][t];
-        p.vE = pt[P_VE][t];
+        real ir, pp;
+        prim(ph, q[0], q[1], q[2], q[3], q[4], ir, p.U, p.V, p.W, pp, p.Tp);
+        visc(ph, ph.T0s + p.Tp, p.mu, p.a);
+        p.v1 = v4[1];
+        p.v2 = v4[2];
+        p.v3 = v4[3];
+        p.vE = vE;
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             p.d1[3 * f] = r[f][t];
             p.d2[3 * f] = r[8 + f][t];
-            p.d2[3 * f + 1] = pt[P_D2 + 2 * f][t];
-            p.d2[3 * f + 2] = pt[P_D2 + 2 * f + 1][t];
+            p.d2[3 * f + 1] = y2[f];
+            p.d2[3 * f + 2] = z2[f];
         }
-        p.d1[1] = own[4][t];  // Uy
-        p.d1[2] = own[5][t];  // Uz
-        p.d1[4] = own[6][t];  // Vy
-        p.d1[5] = pt[P_VZ][t];
-        p.d1[7] = pt[P_WY][t];
-        p.d1[8] = own[7][t];  // Wz
-        p.d1[10] = pt[P_TY][t];
-        p.d1[11] = pt[P_TZ][t];
+        p.d1[1] = ut[0];   // Uy
+        p.d1[2] = z1[0];   // Uz
+        p.d1[4] = vw[0];   // Vy
+        p.d1[5] = z1[1];   // Vz
+        p.d1[7] = vw[1];   // Wy
+        p.d1[8] = z1[2];   // Wz
+        p.d1[10] = ut[1];  // Ty
+        p.d1[11] = z1[3];  // Tz
         p.mx[0] = r[4][t];  // Uxy
         p.mx[1] = r[5][t];  // Uxz
         p.mx[2] = r[6][t];  // Vxy
-        p.mx[3] = pt[P_VYZ][t];
+        p.mx[3] = myz[0];   // Vyz
         p.mx[4] = r[7][t];  // Wxz
-        p.mx[5] = pt[P_WYZ][t];
-        real sd[SD_COUNT];
-        seed_point(ph, p, sd);
+        p.mx[5] = myz[1];   // Wyz
+        real o[SD_COUNT];
+        seed_point(ph, p, o);
 #pragma unroll
-        for (int k = 0; k < SD_COUNT; k++) out[k][t] = sd[k];
+        for (int f = 0; f < NS; f++) stage_put(c, n, f, t, o[stg[f]]);
+#pragma unroll
+        for (int k = 0; k < K_COUNT; k++) kslot(k, t) = o[kept[k]];
+        if (active && (UNI || t < c.len))
+            win.store_quad_at(a.sd.sq, t, o[SD_S], o[SD_S + 1], o[SD_S + 2], o[SD_GQ]);
     }
-    if (!active) return;
-    real *dst[SD_COUNT];
+    __syncthreads();
+    // x: D1 of g1UX g1XY g1XZ g1TX s0 s1 s2, D1 of v0..v4 at XV, D2 of s0 s1
+    // s2 gq at X2.  FP32 solves them in one batch; FP64 in two (7 + 4, then
+    // 5), whose carry reads (11 terms each) would not fit the registers at
+    // once.
+    constexpr bool ONE = sizeof(real) == 4;
+    constexpr int XV = ONE ? 7 : 11, X2 = ONE ? 12 : 7;
+    real x[16][M];
 #pragma unroll
-    for (int k = 0; k < 3; k++) dst[SD_S + k] = o.s[k];
-    dst[SD_GQ] = o.gq;
-#pragma unroll
-    for (int k = 0; k < G1_COUNT; k++) dst[SD_G1 + k] = o.g1[k];
-    dst[SD_UB] = o.ub;
-    dst[SD_VB] = o.vb;
-    dst[SD_WB] = o.wb;
-    dst[SD_TB] = o.Tb;
-#pragma unroll
-    for (int k = 0; k < SD_COUNT; k++) win.store(dst[k], out[k], c.len);
-}
-
-// The x-direction transposed terms of the primitive cotangents and the x
-// flux cotangents through one batched solve, then the chain back to the
-// conserved state at every point, with its epilogue (AdjFinal).
-template <typename real, int M, bool UNI>
-__global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_xfinal(const OpDev<real> *__restrict__ d1,
-                                                              const OpDev<real> *__restrict__ d2,
-                                                              LineMap lm, AdjFinal<real> f,
-                                                              Phys<real> ph)
-{
-    constexpr int HW = 2, W = M + 2 * HW;
-    CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), 14)
-    (void)n;
-    (void)per;
-    // r: D_x of cU cV m1 gxz gTx v0..v4 | D2_x of s1 s2 s3 gq
-    real r[14][M];
+    for (int f = 0; f < NS; f++) {
+        real w[W];
+        stage_get<real, M, HW>(c, n, f, w);
+        if (f < 7) rhs<OP_D1, real, M, HW, UNI>(d1, c, w, x[f]);
+        if (f >= 4) rhs<OP_D2, real, M, HW, UNI>(d2, c, w, x[X2 + f - 4]);
+    }
+    __syncthreads();
     {
-        const real *in1[10] = {f.cU, f.cV, f.m1, f.gxz, f.gTx, f.v[0], f.v[1], f.v[2], f.v[3], f.v[4]};
+        real vw[5][W];
+        win.load_quad(a.v.m, vw[0], vw[1], vw[2], vw[3]);
+        win.load(a.v.e, vw[4]);
 #pragma unroll
-        for (int k = 0; k < 10; k++) {
-            real w[W];
-            win.load(in1[k], w);
-            rhs<OP_D1, real, M, HW, UNI>(d1, c, w, r[k]);
-        }
-        const real *in2[4] = {f.s[0], f.s[1], f.s[2], f.gq};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            real w[W];
-            win.load(in2[k], w);
-            rhs<OP_D2, real, M, HW, UNI>(d2, c, w, r[10 + k]);
-        }
+        for (int k = 0; k < 5; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, vw[k], x[XV + k]);
     }
-    solve_d12<10, 4, real, M, UNI>(d1, d2, c, r);
-    const real c13 = real(1) / real(3), c43 = real(4) / real(3);
-    // the per-point inputs, whole segments at a time: tot + totY, D_y v,
-    // D_z v, q
-    real pt[19][M];
-    {
-        const real *src[14] = {f.tot[0], f.tot[1], f.tot[2], f.tot[3], f.Fy[0], f.Fy[1], f.Fy[2],
-                               f.Fy[3], f.Fy[4], f.Fz[0], f.Fz[1], f.Fz[2], f.Fz[3], f.Fz[4]};
-#pragma unroll
-        for (int k = 0; k < 14; k++) win.load_seg(src[k], pt[k]);
-        win.load_seg_quad(f.q.m, pt[14], pt[15], pt[16], pt[17]);
-        win.load_seg(f.q.e, pt[18]);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            real ty[M];
-            win.load_seg(f.totY[k], ty);
-#pragma unroll
-            for (int t = 0; t < M; t++) pt[k][t] += ty[t];
-        }
+    if constexpr (ONE) {
+        solve_d12<12, 4, real, M, UNI>(d1, d2, c, x);
+    } else {
+        solve_d12<7, 4, real, M, UNI>(d1, d2, c, *reinterpret_cast<real(*)[11][M]>(&x[0]));
+        solve_d12<5, 0, real, M, UNI>(d1, d2, c, *reinterpret_cast<real(*)[5][M]>(&x[11]));
     }
+    const real c43 = real(4) / real(3);
     real res[5][M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
         real q[5], Fb[15], ub[3];
+        pt_quad(win, a.q.m, t, q);
+        q[4] = ldo(a.q.e, win.mid(t));
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            q[k] = pt[14 + k][t];
-            Fb[3 * k] = r[5 + k][t];
-            Fb[3 * k + 1] = pt[4 + k][t];
-            Fb[3 * k + 2] = pt[9 + k][t];
+            Fb[3 * k] = x[XV + k][t];
+            Fb[3 * k + 1] = 0;
+            Fb[3 * k + 2] = 0;
         }
-        ub[0] = pt[0][t] + r[0][t] + c43 * r[10][t];
-        ub[1] = pt[1][t] + r[1][t] + r[11][t];
-        ub[2] = pt[2][t] + c13 * r[2][t] - r[3][t] + r[12][t];
-        const real Tb = pt[3][t] - r[4][t] + r[13][t];
+        ub[0] = kslot(K_UB, t) - x[0][t] + c43 * x[X2][t];
+        ub[1] = kslot(K_VB, t) - x[1][t] + x[X2 + 1][t];
+        ub[2] = kslot(K_WB, t) - x[2][t] + x[X2 + 2][t];
+        const real Tb = kslot(K_TB, t) - x[3][t] + x[X2 + 3][t];
         real rp[5];
         chain_point(ph, q, Fb, ub, Tb, rp);
 #pragma unroll
         for (int k = 0; k < 5; k++) res[k][t] = rp[k];
     }
     if (!active) return;
+    win.store_quad(a.resx.m, res[0], res[1], res[2], res[3], c.len);
+    win.store(a.resx.e, res[4], c.len);
+    real g[K_COUNT][M];
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-        if (f.out[k]) win.store(f.out[k], res[k], c.len);
-        if (f.acc[k]) {
-            real b[M];
-            if (f.acc_base[k])
-                win.load_seg(f.acc_base[k], b);
-            else
+    for (int k = K_XZ; k < K_COUNT; k++)
 #pragma unroll
-                for (int t = 0; t < M; t++) b[t] = 0;
+        for (int t = 0; t < M; t++) g[k][t] = kslot(k, t);
+    win.store_quad(a.sd.gz, g[K_XZ], g[K_WZ], g[K_TZ], x[6], c.len);
+    win.store_quad(a.sd.gy, g[K_XY], g[K_VY], g[K_TY], x[5], c.len);
+    win.store_pair(a.sd.gs, g[K_YZ], x[4], c.len);
+}
+
+// ------------------------------------------------------------ column passes
+
+// z: the z-direction transposed terms, D1 of v_c (flux), D2 of s0..gq, D1
+// of g1XZ, g1WZ, g1TZ, Xs2, g1YZ, Xs0 and of s1, s2 (mzz, for the y pass);
+// chained and added to resx -> resxz.
+template <typename real, int M, bool UNI>
+__global__ __launch_bounds__(adj_col_threads<real>()) void k_adj_z(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
+    AdjVjp<real> a, Phys<real> ph)
+{
+    constexpr int HW = 2, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), ADJ_T_PLANES)
+    (void)n;
+    (void)per;
+    real f1[9][M];  // D1 of v0..v4 | D2 of s0 s1 s2 gq
+    real f2[8][M];  // D1 of g1XZ g1WZ g1TZ Xs2 g1YZ Xs0 s1 s2
+    {
+        real w[4][W];
+        win.load_quad(a.sd.sq, w[0], w[1], w[2], w[3]);
 #pragma unroll
-            for (int t = 0; t < M; t++) b[t] += res[k][t];
-            win.store(f.acc[k], b, c.len);
+        for (int k = 0; k < 4; k++) rhs<OP_D2, real, M, HW, UNI>(d2, c, w[k], f1[5 + k]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[1], f2[6]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[2], f2[7]);
+        win.load_quad(a.sd.gz, w[0], w[1], w[2], w[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, w[k], f2[k]);
+        win.load_pair(a.sd.gs, w[0], w[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f2[4]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[1], f2[5]);
+        win.load_quad(a.v.m, w[0], w[1], w[2], w[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, w[k], f1[k]);
+        win.load(a.v.e, w[0]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f1[4]);
+    }
+    solve_d12<5, 4, real, M, UNI>(d1, d2, c, f1);
+    solve_d12<8, 0, real, M, UNI>(d1, d2, c, f2);
+    real qv[5][M], rx[5][M];
+    win.load_seg_quad(a.q.m, qv[0], qv[1], qv[2], qv[3]);
+    win.load_seg(a.q.e, qv[4]);
+    win.load_seg_quad(a.resx.m, rx[0], rx[1], rx[2], rx[3]);
+    win.load_seg(a.resx.e, rx[4]);
+    const real c13 = real(1) / real(3), c43 = real(4) / real(3);
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        real q[5], Fb[15], ub[3];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            q[k] = qv[k][t];
+            Fb[3 * k] = 0;
+            Fb[3 * k + 1] = 0;
+            Fb[3 * k + 2] = f1[k][t];
         }
-        if (f.vn[k]) {
-            real l[M];
-            win.load_seg(f.la[k], l);
+        ub[0] = f1[5][t] - f2[0][t] + c13 * f2[3][t];
+        ub[1] = f1[6][t] - f2[4][t];
+        ub[2] = c43 * f1[7][t] - f2[1][t] + c13 * f2[5][t];
+        const real Tb = f1[8][t] - f2[2][t];
+        real rp[5];
+        chain_point(ph, q, Fb, ub, Tb, rp);
 #pragma unroll
-            for (int t = 0; t < M; t++) l[t] = f.ca * l[t] + f.cb * res[k][t];
-            win.store(f.vn[k], l, c.len);
+        for (int k = 0; k < 5; k++) rx[k][t] += rp[k];
+    }
+    if (!active) return;
+    win.template store_quad<true>(a.resxz.m, rx[0], rx[1], rx[2], rx[3], c.len);
+    win.template store<true>(a.resxz.e, rx[4], c.len);
+    win.template store_pair<true>(a.sd.mzz, f2[6], f2[7], c.len);
+}
+
+// y: the y-direction transposed terms, D1 of v_c, D2 of s0..gq, D1 of
+// g1XY, g1VY, g1TY, Xs1, g1YZ, Xs0 and of D_z s1, D_z s2; chained and added
+// to resxz -> res, then the epilogue (AdjVjp).
+template <typename real, int M, bool UNI>
+__global__ __launch_bounds__(adj_col_threads<real>()) void k_adj_y(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
+    AdjVjp<real> a, Phys<real> ph)
+{
+    constexpr int HW = 2, W = M + 2 * HW;
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), ADJ_T_PLANES)
+    (void)n;
+    (void)per;
+    real f1[9][M];  // D1 of v0..v4 | D2 of s0 s1 s2 gq
+    real f2[8][M];  // D1 of g1XY g1VY g1TY Xs1 g1YZ Xs0 Dz s1 Dz s2
+    {
+        real w[4][W];
+        win.load_quad(a.sd.sq, w[0], w[1], w[2], w[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) rhs<OP_D2, real, M, HW, UNI>(d2, c, w[k], f1[5 + k]);
+        win.load_quad(a.sd.gy, w[0], w[1], w[2], w[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, w[k], f2[k]);
+        win.load_pair(a.sd.gs, w[0], w[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f2[4]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[1], f2[5]);
+        win.load_pair(a.sd.mzz, w[0], w[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f2[6]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[1], f2[7]);
+        win.load_quad(a.v.m, w[0], w[1], w[2], w[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, w[k], f1[k]);
+        win.load(a.v.e, w[0]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f1[4]);
+    }
+    solve_d12<5, 4, real, M, UNI>(d1, d2, c, f1);
+    solve_d12<8, 0, real, M, UNI>(d1, d2, c, f2);
+    real qv[5][M], rx[5][M];
+    win.load_seg_quad(a.q.m, qv[0], qv[1], qv[2], qv[3]);
+    win.load_seg(a.q.e, qv[4]);
+    win.load_seg_quad(a.resxz.m, rx[0], rx[1], rx[2], rx[3]);
+    win.load_seg(a.resxz.e, rx[4]);
+    const real c13 = real(1) / real(3), c43 = real(4) / real(3);
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        real q[5], Fb[15], ub[3];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            q[k] = qv[k][t];
+            Fb[3 * k] = 0;
+            Fb[3 * k + 1] = f1[k][t];
+            Fb[3 * k + 2] = 0;
         }
+        ub[0] = f1[5][t] - f2[0][t] + c13 * f2[3][t];
+        ub[1] = c43 * f1[6][t] - f2[1][t] + c13 * (f2[5][t] + f2[7][t]);
+        ub[2] = f1[7][t] - f2[4][t] + c13 * f2[6][t];
+        const real Tb = f1[8][t] - f2[2][t];
+        real rp[5];
+        chain_point(ph, q, Fb, ub, Tb, rp);
+#pragma unroll
+        for (int k = 0; k < 5; k++) rx[k][t] += rp[k];
+    }
+    if (!active) return;
+    if (a.out.m) {
+        win.store_quad(a.out.m, rx[0], rx[1], rx[2], rx[3], c.len);
+        win.store(a.out.e, rx[4], c.len);
+    }
+    if (a.acc.m) {
+        real b[5][M];
+        if (a.acc_base.m) {
+            win.load_seg_quad(a.acc_base.m, b[0], b[1], b[2], b[3]);
+            win.load_seg(a.acc_base.e, b[4]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+#pragma unroll
+                for (int t = 0; t < M; t++) b[k][t] = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+#pragma unroll
+            for (int t = 0; t < M; t++) b[k][t] += rx[k][t];
+        win.store_quad(a.acc.m, b[0], b[1], b[2], b[3], c.len);
+        win.store(a.acc.e, b[4], c.len);
+    }
+    if (a.vn.m) {
+        real l[5][M];
+        win.load_seg_quad(a.la.m, l[0], l[1], l[2], l[3]);
+        win.load_seg(a.la.e, l[4]);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+#pragma unroll
+            for (int t = 0; t < M; t++) l[k][t] = a.ca * l[k][t] + a.cb * rx[k][t];
+        win.store_quad(a.vn.m, l[0], l[1], l[2], l[3], c.len);
+        win.store(a.vn.e, l[4], c.len);
     }
 }
 
 template <typename real, bool UNI>
-static hipError_t launch_x(int which, const OpDev<real> *d1, const OpDev<real> *d2,
-                           const LaunchGeom &g, const LineMap &lm, const AdjXSeed<real> *in,
-                           const AdjSeedOut<real> *so, const AdjFinal<real> *fin,
-                           const Phys<real> &ph, hipStream_t st)
+static hipError_t launch_pass(int pass, const OpDev<real> *d1, const OpDev<real> *d2,
+                              const LaunchGeom &g, const LineMap &lm, const AdjVjp<real> &a,
+                              const Phys<real> &ph, hipStream_t st)
 {
     constexpr int M = SEG_DERIV;
     dim3 grid((lm.nlines + g.LX - 1) / g.LX), block(g.threads);
-    if (which == 0)
-        hipLaunchKernelGGL((k_adj_xseed<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, *in,
-                           *so, ph);
-    else
-        hipLaunchKernelGGL((k_adj_xfinal<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, *fin,
-                           ph);
+    switch (pass) {
+    case ADJ_YREC:
+        hipLaunchKernelGGL((k_adj_yrec<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, a, ph);
+        break;
+    case ADJ_ZREC:
+        hipLaunchKernelGGL((k_adj_zrec<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, a, ph);
+        break;
+    case ADJ_X:
+        hipLaunchKernelGGL((k_adj_x<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, a, ph);
+        break;
+    case ADJ_Z:
+        hipLaunchKernelGGL((k_adj_z<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, a, ph);
+        break;
+    default:
+        hipLaunchKernelGGL((k_adj_y<real, M, UNI>), grid, block, g.shm, st, d1, d2, lm, a, ph);
+    }
     return hipGetLastError();
 }
 
+// the lanes each pass's launch bound allows
 template <typename real>
-hipError_t launch_adj_xseed(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
-                            const LaunchGeom &g, const LineMap &lm, const AdjXSeed<real> &in,
-                            const AdjSeedOut<real> &o, const Phys<real> &ph, hipStream_t st)
+static int pass_threads(int pass)
 {
-    if (M != SEG_DERIV) return hipErrorInvalidValue;
-    if (g.LX < 1 || g.threads < 1 || g.threads > ADJ_X_THREADS) return hipErrorInvalidConfiguration;
-    return uni ? launch_x<real, true>(0, d1, d2, g, lm, &in, &o, nullptr, ph, st)
-               : launch_x<real, false>(0, d1, d2, g, lm, &in, &o, nullptr, ph, st);
+    if (pass == ADJ_X) return ADJ_X_THREADS;
+    if (pass == ADJ_YREC || pass == ADJ_ZREC) return adj_rec_threads<real>();
+    return adj_col_threads<real>();
 }
 
 template <typename real>
-hipError_t launch_adj_xfinal(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
-                             const LaunchGeom &g, const LineMap &lm, const AdjFinal<real> &f,
-                             const Phys<real> &ph, hipStream_t st)
+hipError_t launch_adj_pass(int pass, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                           const LaunchGeom &g, const LineMap &lm, const AdjVjp<real> &a,
+                           const Phys<real> &ph, hipStream_t st)
 {
-    if (M != SEG_DERIV) return hipErrorInvalidValue;
-    if (g.LX < 1 || g.threads < 1 || g.threads > ADJ_X_THREADS) return hipErrorInvalidConfiguration;
-    return uni ? launch_x<real, true>(1, d1, d2, g, lm, nullptr, nullptr, &f, ph, st)
-               : launch_x<real, false>(1, d1, d2, g, lm, nullptr, nullptr, &f, ph, st);
+    if (M != SEG_DERIV || pass < ADJ_YREC || pass > ADJ_Y) return hipErrorInvalidValue;
+    if (g.LX < 1 || g.threads < 1 || g.threads > pass_threads<real>(pass))
+        return hipErrorInvalidConfiguration;
+    return uni ? launch_pass<real, true>(pass, d1, d2, g, lm, a, ph, st)
+               : launch_pass<real, false>(pass, d1, d2, g, lm, a, ph, st);
 }
 
+// ------------------------------------------------------------ pointwise
+
 template <typename real>
-__global__ void k_adj_ke_seed(QSet<real> q, Phys<real> ph, Ptr5<real> out, long N)
+__global__ void k_adj_scale(QSet<real> out, QSet<real> in, real c, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < 5 * N;
+         i += (long)gridDim.x * blockDim.x) {
+        if (i < 4 * N)
+            out.m[i] = c * in.m[i];
+        else
+            out.e[i - 4 * N] = c * in.e[i - 4 * N];
+    }
+}
+
+// Gradient of J = (1/N) sum rho |u|^2 / 2 with respect to the conserved
+// state, and the map of a conserved cotangent to the initial primitive
+// velocities at fixed rho and p: m_i = rho u_i, E = p/(gamma-1) + rho|u|^2/2.
+template <typename real>
+__global__ void k_adj_ke_seed(QSet<real> q, Phys<real> ph, QSet<real> out, long N)
 {
     const real iN = real(1) / real(N);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -528,27 +702,24 @@ __global__ void k_adj_ke_seed(QSet<real> q, Phys<real> ph, Ptr5<real> out, long 
         qset_load(q, i, q5);
         const real rho = ph.rho_ref + q5[0];
         const real u = q5[1] / rho, v = q5[2] / rho, w = q5[3] / rho;
-        out.out[0][i] = -real(0.5) * (u * u + v * v + w * w) * iN;
-        out.out[1][i] = u * iN;
-        out.out[2][i] = v * iN;
-        out.out[3][i] = w * iN;
-        out.out[4][i] = 0;
+        const real o[5] = {-real(0.5) * (u * u + v * v + w * w) * iN, u * iN, v * iN, w * iN, 0};
+        qset_store(out, i, o);
     }
 }
 
 template <typename real>
-__global__ void k_adj_to_velocity(QSet<real> q, Ptr5<real> qb, Phys<real> ph, real *gU, real *gV,
+__global__ void k_adj_to_velocity(QSet<real> q, QSet<real> qb, Phys<real> ph, real *gU, real *gV,
                                   real *gW, long N)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long)gridDim.x * blockDim.x) {
-        real q5[5];
+        real q5[5], b[5];
         qset_load(q, i, q5);
+        qset_load(qb, i, b);
         const real rho = ph.rho_ref + q5[0];
-        const real Eb = qb.in[4][i];
-        gU[i] = rho * (qb.in[1][i] + (q5[1] / rho) * Eb);
-        gV[i] = rho * (qb.in[2][i] + (q5[2] / rho) * Eb);
-        gW[i] = rho * (qb.in[3][i] + (q5[3] / rho) * Eb);
+        gU[i] = rho * (b[1] + (q5[1] / rho) * b[4]);
+        gV[i] = rho * (b[2] + (q5[2] / rho) * b[4]);
+        gW[i] = rho * (b[3] + (q5[3] / rho) * b[4]);
     }
 }
 
@@ -560,27 +731,21 @@ static dim3 grid_for(long N)
 }
 
 template <typename real>
-hipError_t launch_adj_prim(const QSet<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
-                           long N, hipStream_t st)
+hipError_t launch_adj_scale(const QSet<real> &out, const QSet<real> &in, real c, long N,
+                            hipStream_t st)
 {
-    hipLaunchKernelGGL((k_adj_prim<real>), grid_for(N), dim3(256), 0, st, q, ph, o, N);
+    hipLaunchKernelGGL((k_adj_scale<real>), grid_for(5 * N), dim3(256), 0, st, out, in, c, N);
     return hipGetLastError();
 }
 template <typename real>
-hipError_t launch_adj_lincomb(const AdjComb<real> &a, long N, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_adj_lincomb<real>), grid_for(N), dim3(256), 0, st, a, N);
-    return hipGetLastError();
-}
-template <typename real>
-hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
+hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,
                               long N, hipStream_t st)
 {
     hipLaunchKernelGGL((k_adj_ke_seed<real>), grid_for(N), dim3(256), 0, st, q, ph, out, N);
     return hipGetLastError();
 }
 template <typename real>
-hipError_t launch_adj_to_velocity(const QSet<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
+hipError_t launch_adj_to_velocity(const QSet<real> &q, const QSet<real> &qb, const Phys<real> &ph,
                                   real *gU, real *gV, real *gW, long N, hipStream_t st)
 {
     hipLaunchKernelGGL((k_adj_to_velocity<real>), grid_for(N), dim3(256), 0, st, q, qb, ph, gU, gV,
@@ -589,24 +754,15 @@ hipError_t launch_adj_to_velocity(const QSet<real> &q, const Ptr5<real> &qb, con
 }
 
 #define CS3D_INST_ADJ(real)                                                                       \
-    template hipError_t launch_adj_prim<real>(const QSet<real> &, const Phys<real> &,              \
-                                              const AdjPrim<real> &, long, hipStream_t);         \
-    template hipError_t launch_adj_xseed<real>(const OpDev<real> *, const OpDev<real> *, int, int, \
-                                               const LaunchGeom &, const LineMap &,              \
-                                               const AdjXSeed<real> &, const AdjSeedOut<real> &, \
-                                               const Phys<real> &, hipStream_t);                 \
-    template hipError_t launch_adj_lincomb<real>(const AdjComb<real> &, long, hipStream_t);        \
-    template hipError_t launch_line_terms<real>(int, int, bool, const OpDev<real> *,               \
-                                                const OpDev<real> *, int, int, const LaunchGeom &, \
-                                                int, const LineMap &, const LineTerms<real> &,     \
-                                                hipStream_t);                                      \
-    template hipError_t launch_adj_xfinal<real>(const OpDev<real> *, const OpDev<real> *, int,     \
-                                                int, const LaunchGeom &, const LineMap &,        \
-                                                const AdjFinal<real> &, const Phys<real> &,      \
-                                                hipStream_t);                                    \
+    template hipError_t launch_adj_pass<real>(int, const OpDev<real> *, const OpDev<real> *, int,  \
+                                              int, const LaunchGeom &, const LineMap &,          \
+                                              const AdjVjp<real> &, const Phys<real> &,          \
+                                              hipStream_t);                                      \
+    template hipError_t launch_adj_scale<real>(const QSet<real> &, const QSet<real> &, real, long, \
+                                               hipStream_t);                                     \
     template hipError_t launch_adj_ke_seed<real>(const QSet<real> &, const Phys<real> &,           \
-                                                 const Ptr5<real> &, long, hipStream_t);         \
-    template hipError_t launch_adj_to_velocity<real>(const QSet<real> &, const Ptr5<real> &,       \
+                                                 const QSet<real> &, long, hipStream_t);         \
+    template hipError_t launch_adj_to_velocity<real>(const QSet<real> &, const QSet<real> &,       \
                                                      const Phys<real> &, real *, real *, real *, \
                                                      long, hipStream_t);
 CS3D_INST_ADJ(float)

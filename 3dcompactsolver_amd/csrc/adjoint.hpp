@@ -1,27 +1,27 @@
 // adjoint.hpp -- argument blocks and launchers of the adjoint kernels
-// (adjoint.hip).  Field indices: first and second derivatives 3 * f + j with
-// f = U, V, W, T and j = x, y, z; mixed derivatives Uxy, Uxz, Vxy, Vyz, Wxz,
-// Wyz; flux cotangents 3 * c + j with c = rho, rhoU, rhoV, rhoW, rhoE.
+// (adjoint.hip).  One vector-Jacobian product of the stage right-hand side
+// is five line passes: the y and z derivative fields of the stage state
+// recomputed (y, z), the x derivatives, the cotangent seeds and every
+// x-direction transposed term (x), then the z and the y transposed terms,
+// each pass chaining its share of the primitive and flux cotangents back to
+// the conserved state (the chain is linear in them) and adding it to the
+// previous pass's share; the y pass also forms the RK epilogue.  Cotangents
+// of the conserved state are stored like the state: a quad (rho, rhoU, rhoV,
+// rhoW) per point and the energy on its own (QSet).
 #pragma once
 
 #include "stage_common.hpp"
 
 namespace cs3d {
 
+// The derivative fields of the stage state the x pass reads: from the y
+// pass (Uy, Ty), (Vy, Wy) pairs and the quad (Uyy, Vyy, Wyy, Tyy); from the
+// z pass the quads (Uz, Vz, Wz, Tz), (Uzz, Vzz, Wzz, Tzz) and the pair
+// (Vyz, Wyz) = D_z of (Vy, Wy).
 template <typename real>
-struct AdjPrim {
-    real *U, *V, *W, *Tp, *mu, *amu;
-};
-
-// Inputs of the fused x seed kernel: the primitives, the stored y and z
-// derivatives (U, V, W, T order), the two mixed derivatives that are not
-// taken along x, and the stage cotangent v = dt * lam.
-template <typename real>
-struct AdjXSeed {
-    const real *U, *V, *W, *Tp, *mu, *amu;
-    const real *dy1[4], *dz1[4], *dy2[4], *dz2[4];
-    const real *Vyz, *Wyz;
-    const real *v[5];
+struct AdjRec {
+    real *dYut, *dYvw, *dY2;
+    real *dZ1, *dZ2, *mYZ;
 };
 
 // The cotangent seeds, deduplicated: every second-derivative and mixed
@@ -30,91 +30,80 @@ struct AdjXSeed {
 // (Uy, Vx), (Uz, Wx), (Vz, Wy) coincide.
 enum { G1_UX, G1_VY, G1_WZ, G1_XY, G1_XZ, G1_YZ, G1_TX, G1_TY, G1_TZ, G1_COUNT };
 
+// What the x pass leaves for the column passes (Xs_i = D_x s_i, the x half
+// of the mixed-derivative transposes): the quads sq = (s0, s1, s2, gq),
+// gz = (g1XZ, g1WZ, g1TZ, Xs2), gy = (g1XY, g1VY, g1TY, Xs1), the pair
+// gs = (g1YZ, Xs0); and the pair mzz = (D_z s1, D_z s2) the z pass leaves
+// for the y pass.
 template <typename real>
-struct AdjSeedOut {
-    real *s[3];
-    real *gq;
-    real *g1[G1_COUNT];
-    real *ub, *vb, *wb, *Tb;
+struct AdjSeeds {
+    real *sq, *gz, *gy, *gs, *mzz;
 };
 
-constexpr int ADJ_COMB_MAX = 8;
-
+// One product res = (d(dt R)/dQ)^T lam at the stage state q, with the stage
+// cotangent v = dt * lam given; the passes add their shares through resx
+// (x) and resxz (x + z).  Epilogue of the y pass, each part skipped when
+// its set is null (m == nullptr): out = res, acc = acc_base + res (acc may
+// alias acc_base; a null acc_base counts as zero), vn = ca la + cb res (vn
+// must not alias v).
 template <typename real>
-struct AdjComb {
-    real *out;
-    const real *base;  // may be null
-    int n;
-    real c[ADJ_COMB_MAX];
-    const real *in[ADJ_COMB_MAX];
-};
-
-// Inputs of the fused x chain kernel and what is made of its result res,
-// the conserved cotangent of the vector-Jacobian product: the x solves
-// (D_x of cU, cV, m1/3, -gxz, -gTx and of v_c; D2_x of 4/3 s1, s2, s3, gq),
-// the z and y parts of the primitive cotangents (tot, totY), the y and z flux
-// cotangents D_y v_c, D_z v_c, and the state q.  Epilogue: out = res,
-// acc = acc_base + res, vn = ca la + cb res (each skipped when its pointer
-// is null; acc may alias acc_base; vn must not alias v).
-template <typename real>
-struct AdjFinal {
-    const real *cU, *cV, *m1, *gxz, *gTx, *s[3], *gq, *v[5];
-    const real *tot[4], *totY[4], *Fy[5], *Fz[5];
-    QSet<real> q;
-    real *out[5];
-    real *acc[5];
-    const real *acc_base[5];
-    real *vn[5];
-    const real *la[5];
+struct AdjVjp {
+    QSet<real> q, v;
+    AdjRec<real> rec;
+    AdjSeeds<real> sd;
+    QSet<real> resx, resxz;
+    QSet<real> out, acc, acc_base, vn, la;
     real ca, cb;
 };
 
+// most lanes per workgroup: the x pass takes whole lines as the x sweep
+// does; the transposed column passes hold 17 solves per lane, which 512
+// lanes (FP32) and 256 (FP64) give the registers for; the recompute passes
+// hold 8 and 10
 constexpr int ADJ_X_THREADS = 256;
-
-// Batched compact line operators along one direction, N1 first-derivative
-// terms then N2 second-derivative terms, term t acting on in[t] with weight
-// c[t].  Separate form: out[t] = c[t] OP_t(in[t]).  Accumulated form:
-// out[0] = sum_b bc[b] base[b] + sum_t c[t] OP_t(in[t]) (null bases skipped).
-constexpr int LINE_TERMS_MAX = 8;
-// most lanes per block of the batched line kernels (FP64 needs the larger
-// register budget of half-size blocks)
 template <typename real>
-constexpr int line_terms_threads()
+constexpr int adj_col_threads()
+{
+    return sizeof(real) == 4 ? 512 : 256;
+}
+template <typename real>
+constexpr int adj_rec_threads()
 {
     return sizeof(real) == 4 ? 1024 : 512;
 }
+// ring planes (values per exchange position) and LDS staging of each pass
+constexpr int ADJ_X_PLANES = 16, ADJ_X_STAGE = 8;
+constexpr int ADJ_T_PLANES = 9;   // transposed column passes: batches of 9 and 8
+constexpr int ADJ_R_PLANES = 10;  // recompute passes: 8 (y) and 10 (z) solves
 
-template <typename real>
-struct LineTerms {
-    const real *in[LINE_TERMS_MAX];
-    real *out[LINE_TERMS_MAX];
-    real c[LINE_TERMS_MAX];
-    const real *base[2];
-    real bc[2];
-};
+enum { ADJ_YREC, ADJ_ZREC, ADJ_X, ADJ_Z, ADJ_Y };
 
+// values per point the x pass keeps in LDS across its second solve batch
+constexpr int ADJ_X_KEEP = 11;
+// LDS bytes of an x-pass workgroup of LX lines of P segments of M points:
+// the exchange ring or the window staging, whichever is larger, then the
+// kept values
+inline size_t adj_x_lds(int P, int kr, int LX, int M, int esize)
+{
+    const size_t ring = ring_bytes(P, kr, LX, ADJ_X_PLANES, esize);
+    const size_t stage = (size_t)ADJ_X_STAGE * M * LX * P * esize;
+    return (ring > stage ? ring : stage) + (size_t)ADJ_X_KEEP * M * LX * P * esize;
+}
+
+// pass: ADJ_*; g from line_geom with the planes and staging above
 template <typename real>
-hipError_t launch_adj_prim(const QSet<real> &q, const Phys<real> &ph, const AdjPrim<real> &o,
-                           long N, hipStream_t st);
+hipError_t launch_adj_pass(int pass, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
+                           const LaunchGeom &g, const LineMap &lm, const AdjVjp<real> &a,
+                           const Phys<real> &ph, hipStream_t st);
+// out = c * in over a conserved set (in may alias out)
 template <typename real>
-hipError_t launch_adj_xseed(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
-                            const LaunchGeom &g, const LineMap &lm, const AdjXSeed<real> &in,
-                            const AdjSeedOut<real> &o, const Phys<real> &ph, hipStream_t st);
+hipError_t launch_adj_scale(const QSet<real> &out, const QSet<real> &in, real c, long N,
+                            hipStream_t st);
 template <typename real>
-hipError_t launch_adj_lincomb(const AdjComb<real> &a, long N, hipStream_t st);
-template <typename real>
-hipError_t launch_adj_xfinal(const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
-                             const LaunchGeom &g, const LineMap &lm, const AdjFinal<real> &f,
-                             const Phys<real> &ph, hipStream_t st);
-template <typename real>
-hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const Ptr5<real> &out,
+hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,
                               long N, hipStream_t st);
 template <typename real>
-hipError_t launch_line_terms(int n1, int n2, bool acc, const OpDev<real> *d1,
-                             const OpDev<real> *d2, int uni, int M, const LaunchGeom &g, int map,
-                             const LineMap &lm, const LineTerms<real> &a, hipStream_t st);
-template <typename real>
-hipError_t launch_adj_to_velocity(const QSet<real> &q, const Ptr5<real> &qb, const Phys<real> &ph,
+hipError_t launch_adj_to_velocity(const QSet<real> &q, const QSet<real> &qb, const Phys<real> &ph,
                                   real *gU, real *gV, real *gW, long N, hipStream_t st);
 
 }  // namespace cs3d

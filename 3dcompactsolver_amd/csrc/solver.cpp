@@ -71,11 +71,6 @@ struct SolverT : SolverBase {
     {
         return env_cap("CS3D_ADJ_X_THREADS", ADJ_X_THREADS, std::min(ADJ_X_THREADS, std::max(64, P)));
     }
-    static int line_terms_max_threads(int dir, int P)
-    {
-        if (dir == 0) return adj_x_threads(P);
-        return env_cap("CS3D_ADJ_THREADS", line_terms_threads<real>());
-    }
 
     cs3d_config cfg;
     int Nx, Ny, Nz;
@@ -984,19 +979,24 @@ struct SolverT : SolverBase {
 
     // ------------------------------------------------------------------
     // Discrete adjoint.  The transpose of the discrete stage map on periodic
-    // lines without sponges: the forward derivative fields are recomputed
-    // with batched compact line solves, the cotangent seeds formed pointwise
-    // (adjoint.hip), the transposed operators applied (periodic D1 is
-    // skew-adjoint, D2 and the filter self-adjoint) with the terms of each
-    // direction summed inside one batched solve, and the result chained back
-    // to the conserved state.  RK4 is reversed stage by stage with the
-    // forward dt sequence frozen; the stage states of each checkpoint
-    // segment are replayed with the fused forward stages and kept.
+    // lines without sponges, as five line passes per vector-Jacobian
+    // product (adjoint.hpp): the y and z derivative fields of the stage
+    // state recomputed, the x derivatives with the cotangent seeds and all
+    // x-direction transposed terms, then the z and y transposed terms
+    // (periodic D1 is skew-adjoint, D2 and the filter self-adjoint), each
+    // pass chaining its share back to the conserved state.  RK4 is reversed
+    // stage by stage with the forward dt sequence frozen; the stage states of
+    // each checkpoint segment are replayed with the fused forward stages and
+    // kept.  Cotangents are conserved sets (a quad and the energy).
 
-    static constexpr int ADJ_FIELDS = 45;
     std::vector<void *> adjalloc;
-    real *A[ADJ_FIELDS] = {};  // vjp work fields
-    real *lamb[5] = {}, *vb5[5] = {}, *vb5b[5] = {}, *qb5[5] = {}, *la5[5] = {}, *sc5[5] = {};
+    bool adj_ready = false;
+    AdjRec<real> arec{};
+    AdjSeeds<real> aseed{};
+    QSet<real> aresx{}, aresxz{};
+    QSet<real> lamb{}, la5{}, sc5{}, vb5{}, vb5b{}, qb5{};
+    real *gvel[3] = {};
+    LaunchGeom adjG[5];
     std::vector<QSet<real>> adjpool;  // gradient states kept across calls
 
     real *adj_alloc_field(int width = 1)
@@ -1005,6 +1005,18 @@ struct SolverT : SolverBase {
         hc(hipMalloc(&p, (size_t)width * N * sizeof(real)), "hipMalloc(adjoint)");
         adjalloc.push_back(p);
         return (real *)p;
+    }
+    QSet<real> adj_alloc_set()
+    {
+        QSet<real> a;
+        a.m = adj_alloc_field(4);
+        a.e = adj_alloc_field();
+        return a;
+    }
+
+    static int adj_pass_dir(int pass)
+    {
+        return pass == ADJ_X ? 0 : (pass == ADJ_ZREC || pass == ADJ_Z) ? 2 : 1;
     }
 
     void adj_require()
@@ -1016,179 +1028,65 @@ struct SolverT : SolverBase {
         if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
         if (cfg.compensated) throw std::invalid_argument("the adjoint does not cover the compensated mode");
         if (any_wall()) throw std::invalid_argument("the adjoint does not cover walls");
-        if (!A[0]) {
-            for (int i = 0; i < ADJ_FIELDS; i++) A[i] = adj_alloc_field();
-            real **sets[6] = {lamb, vb5, vb5b, qb5, la5, sc5};
-            for (auto set : sets)
-                for (int c = 0; c < 5; c++) set[c] = adj_alloc_field();
+        if (adj_ready) return;
+        for (int pass = ADJ_YREC; pass <= ADJ_Y; pass++) {
+            const int dir = adj_pass_dir(pass);
+            const OpHost &h1 = D1[dir].host, &h2 = D2[dir].host;
+            if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
+                throw std::logic_error("derivative plans of one direction differ in segmentation");
+            const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
+            LaunchGeom &g = adjG[pass];
+            if (pass == ADJ_X) {
+                g = line_geom(h1.P, kr, adj_x_threads(h1.P), ADJ_X_PLANES, sizeof(real));
+                while (g.LX > 0 && adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) > LDS_BYTES) g.LX--;
+                g.threads = g.LX * h1.P;
+                g.shm = g.LX > 0 ? adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) : 0;
+            }
+            else if (pass == ADJ_YREC || pass == ADJ_ZREC)
+                g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_rec_threads<real>()),
+                              ADJ_R_PLANES, sizeof(real));
+            else
+                g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_col_threads<real>()),
+                              ADJ_T_PLANES, sizeof(real));
+            if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
         }
-    }
-
-    typedef std::pair<real, const real *> Term;
-
-    void line_terms(int dir, int n1, int n2, bool acc, const LineTerms<real> &a)
-    {
-        const OpHost &h1 = D1[dir].host, &h2 = D2[dir].host;
-        if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
-            throw std::logic_error("derivative plans of one direction differ in segmentation");
-        const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
-        LaunchGeom g = line_geom(h1.P, kr, line_terms_max_threads(dir, h1.P), n1 + n2, sizeof(real));
-        if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
-        hc(launch_line_terms<real>(n1, n2, acc, D1[dir].dev, D2[dir].dev, h1.uniform, h1.M, g,
-                                   dir == 0 ? MAP_ROW : MAP_COL, lm[dir], a, st),
-           "adjoint line solves");
-    }
-
-    // out[t] = OP_t(in[t]): n1 first derivatives, then n2 second derivatives
-    void sep(int dir, int n1, int n2, std::initializer_list<const real *> in,
-             std::initializer_list<real *> out)
-    {
-        LineTerms<real> a{};
-        int t = 0;
-        for (auto p : in) a.in[t++] = p;
-        t = 0;
-        for (auto p : out) {
-            a.out[t] = p;
-            a.c[t++] = (real)1;
-        }
-        line_terms(dir, n1, n2, false, a);
-    }
-
-    // out = b0 + b1 + sum c D1(in) + sum c D2(in)
-    void acc(int dir, real *out, Term b0, Term b1, std::initializer_list<Term> t1,
-             std::initializer_list<Term> t2)
-    {
-        LineTerms<real> a{};
-        a.out[0] = out;
-        a.base[0] = b0.second;
-        a.bc[0] = b0.first;
-        a.base[1] = b1.second;
-        a.bc[1] = b1.first;
-        int t = 0;
-        for (auto &x : t1) {
-            a.c[t] = x.first;
-            a.in[t++] = x.second;
-        }
-        for (auto &x : t2) {
-            a.c[t] = x.first;
-            a.in[t++] = x.second;
-        }
-        line_terms(dir, (int)t1.size(), (int)t2.size(), true, a);
-    }
-
-    void comb(real *out, const real *base, std::initializer_list<Term> t)
-    {
-        AdjComb<real> a;
-        a.out = out;
-        a.base = base;
-        a.n = 0;
-        for (auto &x : t) {
-            a.c[a.n] = x.first;
-            a.in[a.n] = x.second;
-            a.n++;
-        }
-        hc(launch_adj_lincomb<real>(a, N, st), "adjoint combination");
-    }
-
-    void x_geom(int planes, LaunchGeom &g)
-    {
-        const OpHost &h1 = D1[0].host, &h2 = D2[0].host;
-        if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
-            throw std::logic_error("derivative plans of one direction differ in segmentation");
-        g = line_geom(h1.P, std::max(std::max(h1.K, h2.K), kc_deriv<real>()), adj_x_threads(h1.P),
-                      planes, sizeof(real));
-        if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
+        arec.dYut = adj_alloc_field(2);
+        arec.dYvw = adj_alloc_field(2);
+        arec.dY2 = adj_alloc_field(4);
+        arec.dZ1 = adj_alloc_field(4);
+        arec.dZ2 = adj_alloc_field(4);
+        arec.mYZ = adj_alloc_field(2);
+        aseed.sq = adj_alloc_field(4);
+        aseed.gz = adj_alloc_field(4);
+        aseed.gy = adj_alloc_field(4);
+        aseed.gs = adj_alloc_field(2);
+        aseed.mzz = adj_alloc_field(2);
+        aresx = adj_alloc_set();
+        aresxz = adj_alloc_set();
+        QSet<real> *sets[6] = {&lamb, &la5, &sc5, &vb5, &vb5b, &qb5};
+        for (auto p : sets) *p = adj_alloc_set();
+        for (auto &p : gvel) p = adj_alloc_field();
+        adj_ready = true;
     }
 
     // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
-    // with v = dt * lam given; fin says where res goes (AdjFinal).
-    void vjp(const QSet<real> &Q, real *const *v, AdjFinal<real> fin)
+    // with v = dt * lam given; a holds the epilogue (AdjVjp).
+    void vjp(const QSet<real> &Q, const QSet<real> &v, AdjVjp<real> a)
     {
-        real *U = A[0], *V = A[1], *W = A[2], *Tp = A[3];
-        AdjPrim<real> pr{U, V, W, Tp, A[4], A[5]};
-        hc(launch_adj_prim<real>(Q, ph, pr, N, st), "adjoint primitives");
-        // y and z derivative fields (U, V, W, T order) and the mixed D_z Vy, D_z Wy
-        real **dy1 = A + 6, **dy2 = A + 10, **dz1 = A + 14, **dz2 = A + 18;
-        real *Vyz = A[22], *Wyz = A[23];
-        sep(1, 4, 4, {U, V, W, Tp, U, V, W, Tp},
-            {dy1[0], dy1[1], dy1[2], dy1[3], dy2[0], dy2[1], dy2[2], dy2[3]});
-        sep(2, 4, 4, {U, V, W, Tp, U, V, W, Tp},
-            {dz1[0], dz1[1], dz1[2], dz1[3], dz2[0], dz2[1], dz2[2], dz2[3]});
-        sep(2, 2, 0, {dy1[1], dy1[2]}, {Vyz, Wyz});
-        // the x derivatives and the seeds in one pass
-        real **s = A + 24, *gq = A[27], **g1 = A + 28;
-        real *ub = A[37], *vb = A[38], *wb = A[39], *Tb = A[40];
-        {
-            AdjXSeed<real> xi;
-            xi.U = U; xi.V = V; xi.W = W; xi.Tp = Tp; xi.mu = A[4]; xi.amu = A[5];
-            for (int f = 0; f < 4; f++) {
-                xi.dy1[f] = dy1[f];
-                xi.dy2[f] = dy2[f];
-                xi.dz1[f] = dz1[f];
-                xi.dz2[f] = dz2[f];
-            }
-            xi.Vyz = Vyz;
-            xi.Wyz = Wyz;
-            for (int c = 0; c < 5; c++) xi.v[c] = v[c];
-            AdjSeedOut<real> so;
-            for (int i = 0; i < 3; i++) so.s[i] = s[i];
-            so.gq = gq;
-            for (int i = 0; i < G1_COUNT; i++) so.g1[i] = g1[i];
-            so.ub = ub; so.vb = vb; so.wb = wb; so.Tb = Tb;
-            LaunchGeom g;
-            x_geom(12, g);
-            hc(launch_adj_xseed<real>(D1[0].dev, D2[0].dev, D1[0].host.uniform, D1[0].host.M, g,
-                                      lm[0], xi, so, ph, st),
-               "adjoint seeds");
+        if (a.vn.m && (a.vn.m == v.m || a.vn.e == v.e))
+            throw std::logic_error("adjoint: next cotangent aliases v");
+        a.q = Q;
+        a.v = v;
+        a.rec = arec;
+        a.sd = aseed;
+        a.resx = aresx;
+        a.resxz = aresxz;
+        for (int pass = ADJ_YREC; pass <= ADJ_Y; pass++) {
+            const int dir = adj_pass_dir(pass);
+            hc(launch_adj_pass<real>(pass, D1[dir].dev, D2[dir].dev, D1[dir].host.uniform,
+                                     D1[dir].host.M, adjG[pass], lm[dir], a, ph, st),
+               "adjoint pass");
         }
-        // transposed operators, z then y; the primitive and derivative
-        // fields are dead and their storage holds the partial sums
-        real *m1 = A[0], *m2 = A[1], *m3 = A[2], *cU = A[3], *cV = A[4];
-        real **tot = A + 6, **Fy = A + 10, **Fz = A + 15;
-        const real one = 1, m = -1, c13 = (real)(1.0 / 3.0), c43 = (real)(4.0 / 3.0);
-        const Term none{0, nullptr};
-        sep(2, 3, 0, {s[0], s[1], s[2]}, {m1, m2, m3});
-        acc(2, tot[0], {one, ub}, none, {{m, g1[G1_XZ]}}, {{one, s[0]}});
-        acc(2, tot[1], {one, vb}, none, {{m, g1[G1_YZ]}}, {{one, s[1]}});
-        acc(2, tot[2], {one, wb}, none, {{m, g1[G1_WZ]}}, {{c43, s[2]}});
-        acc(2, tot[3], {one, Tb}, none, {{m, g1[G1_TZ]}}, {{one, gq}});
-        // y parts into fields of their own (no read-modify-write on the
-        // column path; the x chain adds them)
-        real **totY = A + 41;
-        acc(1, totY[0], none, none, {{m, g1[G1_XY]}}, {{one, s[0]}});
-        acc(1, totY[1], none, none, {{c13, m3}, {m, g1[G1_VY]}}, {{c43, s[1]}});
-        acc(1, totY[2], none, none, {{c13, m2}, {m, g1[G1_YZ]}}, {{one, s[2]}});
-        acc(1, totY[3], none, none, {{m, g1[G1_TY]}}, {{one, gq}});
-        acc(1, cU, {c13, m3}, {m, g1[G1_UX]}, {{c13, s[1]}}, {});
-        acc(1, cV, {m, g1[G1_XY]}, none, {{c13, s[0]}}, {});
-        // flux cotangents D_y v_c, D_z v_c
-        sep(1, 5, 0, {v[0], v[1], v[2], v[3], v[4]}, {Fy[0], Fy[1], Fy[2], Fy[3], Fy[4]});
-        sep(2, 5, 0, {v[0], v[1], v[2], v[3], v[4]}, {Fz[0], Fz[1], Fz[2], Fz[3], Fz[4]});
-        // x: the remaining terms and the chain to the conserved state
-        fin.cU = cU;
-        fin.cV = cV;
-        fin.m1 = m1;
-        fin.gxz = g1[G1_XZ];
-        fin.gTx = g1[G1_TX];
-        for (int i = 0; i < 3; i++) fin.s[i] = s[i];
-        fin.gq = gq;
-        for (int c = 0; c < 5; c++) {
-            fin.v[c] = v[c];
-            fin.Fy[c] = Fy[c];
-            fin.Fz[c] = Fz[c];
-        }
-        fin.q = Q;
-        for (int i = 0; i < 4; i++) {
-            fin.tot[i] = tot[i];
-            fin.totY[i] = totY[i];
-        }
-        for (int c = 0; c < 5; c++)
-            if (fin.vn[c] == v[c]) throw std::logic_error("adjoint: next cotangent aliases v");
-        LaunchGeom g;
-        x_geom(14, g);
-        hc(launch_adj_xfinal<real>(D1[0].dev, D2[0].dev, D1[0].host.uniform, D1[0].host.M, g,
-                                   lm[0], fin, ph, st),
-           "adjoint chain");
     }
 
     // the direction order filter_conserved uses for filter count m
@@ -1213,7 +1111,8 @@ struct SolverT : SolverBase {
 
     // the transposed filter of filter count m: the three sweeps in reverse
     // order (each periodic line filter is symmetric)
-    void filter_transposed(real *const *in, real *const *out, real *const *scratch, int m)
+    void filter_transposed(const QSet<real> &in, const QSet<real> &out, const QSet<real> &scratch,
+                           int m)
     {
         int o[3];
         filter_order(m, o, fx_ok);
@@ -1264,59 +1163,66 @@ struct SolverT : SolverBase {
 
     // lam (cotangent of q_{n+1}) -> cotangent of q_n for the step whose
     // stage states are stg
-    void adjoint_step(const QSet<real> *stg, const StepRec &r, real *const *lam)
+    void adjoint_step(const QSet<real> *stg, const StepRec &r, const QSet<real> &lam)
     {
         const real rdt = (real)r.dt;
-        real *const *la = lam;
+        QSet<real> la = lam;
         if (r.order >= 0) {
             filter_transposed(lam, la5, sc5, r.order);
             la = la5;
         }
         // acc = q + k1/6 + k2/3 + k3/3 + k4/6 (filtered into q_{n+1});
         // Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3; k_s = dt R(Q_s)
-        for (int c = 0; c < 5; c++) comb(vb5[c], nullptr, {{rdt / 6, la[c]}});
-        // stage cotangents alternate between two buffers: the x chain kernel
-        // reads v's windows while it writes the next one
-        real *const *vin = vb5, *const *vout = vb5b;
-        AdjFinal<real> f{};
-        for (int c = 0; c < 5; c++) {
-            f.acc[c] = qb5[c];
-            f.acc_base[c] = la[c];
-            f.vn[c] = vout[c];
-            f.la[c] = la[c];
-        }
+        hc(launch_adj_scale<real>(vb5, la, rdt / 6, N, st), "adjoint scale");
+        // stage cotangents alternate between two sets: the y pass reads v's
+        // windows while it writes the next one
+        QSet<real> vin = vb5, vout = vb5b;
+        AdjVjp<real> f{};
+        f.acc = qb5;
+        f.acc_base = la;
+        f.vn = vout;
+        f.la = la;
         f.ca = rdt / 3;
         f.cb = rdt;
         vjp(stg[3], vin, f);
         std::swap(vin, vout);
-        for (int c = 0; c < 5; c++) {
-            f.acc_base[c] = qb5[c];
-            f.vn[c] = vout[c];
-        }
+        f.acc_base = qb5;
+        f.vn = vout;
         f.ca = rdt / 3;
         f.cb = rdt / 2;
         vjp(stg[2], vin, f);
         std::swap(vin, vout);
-        for (int c = 0; c < 5; c++) f.vn[c] = vout[c];
+        f.vn = vout;
         f.ca = rdt / 6;
         f.cb = rdt / 2;
         vjp(stg[1], vin, f);
         std::swap(vin, vout);
-        AdjFinal<real> l{};
-        for (int c = 0; c < 5; c++) {
-            l.acc[c] = lam[c];
-            l.acc_base[c] = qb5[c];
-        }
+        AdjVjp<real> l{};
+        l.acc = lam;
+        l.acc_base = qb5;
         vjp(stg[0], vin, l);
     }
 
-    void upload5(const double *const *in, real *const *dst)
+    // five host fields (FP64) <-> a device set, no reference offsets
+    void upload_set(const double *const *in, const QSet<real> &dst)
     {
-        std::vector<real> buf((size_t)N);
-        for (int v = 0; v < 5; v++) {
-            for (long i = 0; i < N; i++) buf[i] = (real)in[v][i];
-            hc(hipMemcpy(dst[v], buf.data(), (size_t)N * sizeof(real), hipMemcpyHostToDevice), "upload");
+        std::vector<real> m(4 * (size_t)N), e((size_t)N);
+        for (long i = 0; i < N; i++) {
+            for (int v = 0; v < 4; v++) m[4 * i + v] = (real)in[v][i];
+            e[i] = (real)in[4][i];
         }
+        hc(hipMemcpy(dst.m, m.data(), m.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
+        hc(hipMemcpy(dst.e, e.data(), e.size() * sizeof(real), hipMemcpyHostToDevice), "upload");
+    }
+    void download_set(const QSet<real> &src, double *const *out)
+    {
+        std::vector<real> m(4 * (size_t)N), e((size_t)N);
+        hc(hipStreamSynchronize(st), "sync");
+        hc(hipMemcpy(m.data(), src.m, m.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+        hc(hipMemcpy(e.data(), src.e, e.size() * sizeof(real), hipMemcpyDeviceToHost), "download");
+        for (int v = 0; v < 5; v++)
+            if (out[v])
+                for (long i = 0; i < N; i++) out[v][i] = (double)(v < 4 ? m[4 * i + v] : e[i]);
     }
 
     void download(const real *src, double *dst)
@@ -1330,16 +1236,17 @@ struct SolverT : SolverBase {
     void rhs_vjp(int set, const double *const *lam, double *const *out) override
     {
         adj_require();
+        for (int c = 0; c < 5; c++)
+            if (!lam[c]) throw std::invalid_argument("rhs_vjp needs all five cotangent fields");
         StepScalars h;
         hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
-        upload5(lam, la5);
-        for (int c = 0; c < 5; c++) comb(vb5[c], nullptr, {{(real)h.dt, la5[c]}});
-        AdjFinal<real> f{};
-        for (int c = 0; c < 5; c++) f.out[c] = qb5[c];
+        upload_set(lam, la5);
+        hc(launch_adj_scale<real>(vb5, la5, (real)h.dt, N, st), "adjoint scale");
+        AdjVjp<real> f{};
+        f.out = qb5;
         vjp(set == 1 ? F.q : F.qk, vb5, f);
-        for (int c = 0; c < 5; c++)
-            if (out[c]) download(qb5[c], out[c]);
+        download_set(qb5, out);
     }
 
     void ke_gradient(int nsteps, int K, double *J, double *gU, double *gV, double *gW) override
@@ -1352,12 +1259,7 @@ struct SolverT : SolverBase {
         // come from a pool that persists across calls
         size_t used = 0;
         auto state = [&]() {
-            if (used == adjpool.size()) {
-                QSet<real> a;
-                a.m = adj_alloc_field(4);
-                a.e = adj_alloc_field();
-                adjpool.push_back(a);
-            }
+            if (used == adjpool.size()) adjpool.push_back(adj_alloc_set());
             return adjpool[used++];
         };
         std::vector<QSet<real>> ck(nseg), stg(4 * (size_t)K);
@@ -1379,9 +1281,7 @@ struct SolverT : SolverBase {
         if (J) hc(kinetic_energy(J) == CS3D_OK ? hipSuccess : hipErrorUnknown, "kinetic energy");
         copy_set(fin, F.q);
         // cotangent of the final state
-        Ptr5<real> lo;
-        for (int v = 0; v < 5; v++) lo.out[v] = lamb[v];
-        hc(launch_adj_ke_seed<real>(F.q, ph, lo, N, st), "adjoint seed");
+        hc(launch_adj_ke_seed<real>(F.q, ph, lamb, N, st), "adjoint seed");
         // reverse, segment by segment
         for (int sgi = nseg - 1; sgi >= 0; sgi--) {
             const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
@@ -1392,13 +1292,11 @@ struct SolverT : SolverBase {
             for (int n = n1 - 1; n >= n0; n--) adjoint_step(stages(n - n0), rec[n], lamb);
         }
         // to the initial primitive velocities at fixed rho and p
-        Ptr5<real> lb;
-        for (int v = 0; v < 5; v++) lb.in[v] = lamb[v];
-        hc(launch_adj_to_velocity<real>(ck[0], lb, ph, qb5[0], qb5[1], qb5[2], N, st),
+        hc(launch_adj_to_velocity<real>(ck[0], lamb, ph, gvel[0], gvel[1], gvel[2], N, st),
            "velocity gradient");
-        if (gU) download(qb5[0], gU);
-        if (gV) download(qb5[1], gV);
-        if (gW) download(qb5[2], gW);
+        if (gU) download(gvel[0], gU);
+        if (gV) download(gvel[1], gV);
+        if (gW) download(gvel[2], gW);
         // back to the final state of the forward pass
         copy_set(F.q, fin);
         timeStep = ts_end;
@@ -1414,7 +1312,7 @@ struct SolverT : SolverBase {
         for (void *p : adjalloc) (void)hipFree(p);
         adjalloc.clear();
         adjpool.clear();
-        for (auto &p : A) p = nullptr;
+        adj_ready = false;
     }
 
     void synchronize() override { hc(hipStreamSynchronize(st), "sync"); }
