@@ -229,7 +229,7 @@ __device__ __forceinline__ void pt_pair(const WinT &win, const real *f, int t, r
 
 // y: D1 and D2 of U, V, W, T' along y -> (Uy, Ty), (Vy, Wy), (Uyy, Vyy, Wyy, Tyy)
 template <typename real, int M, bool UNI>
-__global__ __launch_bounds__(adj_rec_threads<real>()) void k_adj_yrec(
+__global__ __launch_bounds__(adj_rec_threads<real>(UNI)) void k_adj_yrec(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     AdjVjp<real> a, Phys<real> ph)
 {
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(adj_rec_threads<real>()) void k_adj_yrec(
 // z: D1 and D2 of U, V, W, T' and D1 of the stored Vy, Wy along z ->
 // (Uz, Vz, Wz, Tz), (Uzz, Vzz, Wzz, Tzz), (Vyz, Wyz)
 template <typename real, int M, bool UNI>
-__global__ __launch_bounds__(adj_rec_threads<real>()) void k_adj_zrec(
+__global__ __launch_bounds__(adj_rec_threads<real>(UNI)) void k_adj_zrec(
     const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
     AdjVjp<real> a, Phys<real> ph)
 {
@@ -311,11 +311,16 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
     constexpr int HW = 2, W = M + 2 * HW;
     CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), ADJ_X_PLANES)
     (void)per;
-    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx
-    real r[12][M];
+    // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx; own values of the
+    // state and of Uy Ty Vy Wy Uz Vz Wz Tz
+    real r[12][M], qo[5][M], yz[8][M];
     {
         real pw[4][W], qw[5][W];
         prim_windows(win, a.q, ph, pw, qw);
+#pragma unroll
+        for (int f = 0; f < 5; f++)
+#pragma unroll
+            for (int t = 0; t < M; t++) qo[f][t] = qw[f][HW + t];
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             rhs<OP_D1, real, M, HW, UNI>(d1, c, pw[f], r[f]);
@@ -325,50 +330,84 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
     {
         real w0[W], w1[W], w2[W], w3[W];
         win.load_pair(a.rec.dYut, w0, w1);
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            yz[0][t] = w0[HW + t];
+            yz[1][t] = w1[HW + t];
+        }
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[4]);
         win.load_pair(a.rec.dYvw, w0, w1);
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            yz[2][t] = w0[HW + t];
+            yz[3][t] = w1[HW + t];
+        }
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[6]);
         win.load_quad(a.rec.dZ1, w0, w1, w2, w3);
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            yz[4][t] = w0[HW + t];
+            yz[5][t] = w1[HW + t];
+            yz[6][t] = w2[HW + t];
+            yz[7][t] = w3[HW + t];
+        }
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[5]);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w2, r[7]);
     }
     solve_d12<8, 4, real, M, UNI>(d1, d2, c, r);
-    // seeds, point by point; the eight that the x solves need go to the
-    // staging area, the ones the column passes need are kept or stored
+    // x: D1 of g1UX g1XY g1XZ g1TX s0 s1 s2, D1 of v0..v4 at XV, D2 of s0 s1
+    // s2 gq at X2.  FP32 solves them in one batch; FP64 in two (7 + 4, then
+    // 5), whose carry reads (11 terms each) would not fit the registers at
+    // once.  The v windows come first: their centres are the seeds' v.
+    constexpr bool ONE = sizeof(real) == 4;
+    constexpr int XV = ONE ? 7 : 11, X2 = ONE ? 12 : 7;
+    real x[16][M], vo[4][M];
+    {
+        real vw[5][W];
+        win.load_quad(a.v.m, vw[0], vw[1], vw[2], vw[3]);
+        win.load(a.v.e, vw[4]);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+#pragma unroll
+            for (int t = 0; t < W; t++) vw[k][t] *= a.cv;
+#pragma unroll
+        for (int k = 0; k < 5; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, vw[k], x[XV + k]);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int t = 0; t < M; t++) vo[k][t] = vw[k + 1][HW + t];
+    }
+    // seeds, point by point: the eight the x solves need go to the window
+    // staging, the ones needed after the solve wait in LDS past the ring and
+    // the staging (the host sizes the area: adj_x_lds) at this lane's slots
     constexpr int NS = ADJ_X_STAGE;
     const int stg[NS] = {SD_G1 + G1_UX, SD_G1 + G1_XY, SD_G1 + G1_XZ, SD_G1 + G1_TX,
                          SD_S,          SD_S + 1,      SD_S + 2,      SD_GQ};
-    enum { K_UB, K_VB, K_WB, K_TB, K_XZ, K_WZ, K_TZ, K_XY, K_VY, K_TY, K_YZ, K_COUNT };
+    enum { K_UB, K_VB, K_WB, K_TB, K_XZ, K_WZ, K_TZ, K_XY, K_VY, K_TY, K_YZ,
+           K_S0, K_S1, K_S2, K_GQ, K_COUNT };
+    static_assert(K_COUNT == ADJ_X_KEEP, "kept values and their LDS area differ");
     const int kept[K_COUNT] = {SD_UB,         SD_VB,         SD_WB,         SD_TB,
                                SD_G1 + G1_XZ, SD_G1 + G1_WZ, SD_G1 + G1_TZ, SD_G1 + G1_XY,
-                               SD_G1 + G1_VY, SD_G1 + G1_TY, SD_G1 + G1_YZ};
-    // the kept values wait in LDS past the ring and the window staging (the
-    // host sizes the area: adj_x_lds), at this lane's own slots
+                               SD_G1 + G1_VY, SD_G1 + G1_TY, SD_G1 + G1_YZ, SD_S,
+                               SD_S + 1,      SD_S + 2,      SD_GQ};
     real *kp = xb + (2 * c.cap > NS * M * c.LX * c.P ? 2 * c.cap : NS * M * c.LX * c.P);
     const int T = c.LX * c.P, me = c.line * c.P + c.g;
     auto kslot = [&](int k, int t) -> real & { return kp[(k * M + t) * T + me]; };
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        real q[5], ut[2], vw[2], z1[4], y2[4], z2[4], myz[2], v4[4];
-        pt_quad(win, a.q.m, t, q);
-        q[4] = ldo(a.q.e, win.mid(t));
-        pt_pair(win, a.rec.dYut, t, ut);
-        pt_pair(win, a.rec.dYvw, t, vw);
-        pt_quad(win, a.rec.dZ1, t, z1);
+        real y2[4], z2[4], myz[2];
         pt_quad(win, a.rec.dY2, t, y2);
         pt_quad(win, a.rec.dZ2, t, z2);
         pt_pair(win, a.rec.mYZ, t, myz);
-        pt_quad(win, a.v.m, t, v4);
-        const real vE = ldo(a.v.e, win.mid(t));
         SeedPt<real> p;
         real ir, pp;
-        prim(ph, q[0], q[1], q[2], q[3], q[4], ir, p.U, p.V, p.W, pp, p.Tp);
+        prim(ph, qo[0][t], qo[1][t], qo[2][t], qo[3][t], qo[4][t], ir, p.U, p.V, p.W, pp, p.Tp);
         visc(ph, ph.T0s + p.Tp, p.mu, p.a);
-        p.v1 = v4[1];
-        p.v2 = v4[2];
-        p.v3 = v4[3];
-        p.vE = vE;
+        p.v1 = vo[0][t];
+        p.v2 = vo[1][t];
+        p.v3 = vo[2][t];
+        p.vE = vo[3][t];
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             p.d1[3 * f] = r[f][t];
@@ -376,37 +415,28 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
             p.d2[3 * f + 1] = y2[f];
             p.d2[3 * f + 2] = z2[f];
         }
-        p.d1[1] = ut[0];   // Uy
-        p.d1[2] = z1[0];   // Uz
-        p.d1[4] = vw[0];   // Vy
-        p.d1[5] = z1[1];   // Vz
-        p.d1[7] = vw[1];   // Wy
-        p.d1[8] = z1[2];   // Wz
-        p.d1[10] = ut[1];  // Ty
-        p.d1[11] = z1[3];  // Tz
-        p.mx[0] = r[4][t];  // Uxy
-        p.mx[1] = r[5][t];  // Uxz
-        p.mx[2] = r[6][t];  // Vxy
-        p.mx[3] = myz[0];   // Vyz
-        p.mx[4] = r[7][t];  // Wxz
-        p.mx[5] = myz[1];   // Wyz
+        p.d1[1] = yz[0][t];   // Uy
+        p.d1[2] = yz[4][t];   // Uz
+        p.d1[4] = yz[2][t];   // Vy
+        p.d1[5] = yz[5][t];   // Vz
+        p.d1[7] = yz[3][t];   // Wy
+        p.d1[8] = yz[6][t];   // Wz
+        p.d1[10] = yz[1][t];  // Ty
+        p.d1[11] = yz[7][t];  // Tz
+        p.mx[0] = r[4][t];    // Uxy
+        p.mx[1] = r[5][t];    // Uxz
+        p.mx[2] = r[6][t];    // Vxy
+        p.mx[3] = myz[0];     // Vyz
+        p.mx[4] = r[7][t];    // Wxz
+        p.mx[5] = myz[1];     // Wyz
         real o[SD_COUNT];
         seed_point(ph, p, o);
 #pragma unroll
         for (int f = 0; f < NS; f++) stage_put(c, n, f, t, o[stg[f]]);
 #pragma unroll
         for (int k = 0; k < K_COUNT; k++) kslot(k, t) = o[kept[k]];
-        if (active && (UNI || t < c.len))
-            win.store_quad_at(a.sd.sq, t, o[SD_S], o[SD_S + 1], o[SD_S + 2], o[SD_GQ]);
     }
     __syncthreads();
-    // x: D1 of g1UX g1XY g1XZ g1TX s0 s1 s2, D1 of v0..v4 at XV, D2 of s0 s1
-    // s2 gq at X2.  FP32 solves them in one batch; FP64 in two (7 + 4, then
-    // 5), whose carry reads (11 terms each) would not fit the registers at
-    // once.
-    constexpr bool ONE = sizeof(real) == 4;
-    constexpr int XV = ONE ? 7 : 11, X2 = ONE ? 12 : 7;
-    real x[16][M];
 #pragma unroll
     for (int f = 0; f < NS; f++) {
         real w[W];
@@ -415,13 +445,6 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
         if (f >= 4) rhs<OP_D2, real, M, HW, UNI>(d2, c, w, x[X2 + f - 4]);
     }
     __syncthreads();
-    {
-        real vw[5][W];
-        win.load_quad(a.v.m, vw[0], vw[1], vw[2], vw[3]);
-        win.load(a.v.e, vw[4]);
-#pragma unroll
-        for (int k = 0; k < 5; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, vw[k], x[XV + k]);
-    }
     if constexpr (ONE) {
         solve_d12<12, 4, real, M, UNI>(d1, d2, c, x);
     } else {
@@ -433,10 +456,9 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
 #pragma unroll
     for (int t = 0; t < M; t++) {
         real q[5], Fb[15], ub[3];
-        pt_quad(win, a.q.m, t, q);
-        q[4] = ldo(a.q.e, win.mid(t));
 #pragma unroll
         for (int k = 0; k < 5; k++) {
+            q[k] = qo[k][t];
             Fb[3 * k] = x[XV + k][t];
             Fb[3 * k + 1] = 0;
             Fb[3 * k + 2] = 0;
@@ -458,6 +480,7 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
     for (int k = K_XZ; k < K_COUNT; k++)
 #pragma unroll
         for (int t = 0; t < M; t++) g[k][t] = kslot(k, t);
+    win.store_quad(a.sd.sq, g[K_S0], g[K_S1], g[K_S2], g[K_GQ], c.len);
     win.store_quad(a.sd.gz, g[K_XZ], g[K_WZ], g[K_TZ], x[6], c.len);
     win.store_quad(a.sd.gy, g[K_XY], g[K_VY], g[K_TY], x[5], c.len);
     win.store_pair(a.sd.gs, g[K_YZ], x[4], c.len);
@@ -497,6 +520,10 @@ __global__ __launch_bounds__(adj_col_threads<real>()) void k_adj_z(
         for (int k = 0; k < 4; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, w[k], f1[k]);
         win.load(a.v.e, w[0]);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f1[4]);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+#pragma unroll
+            for (int t = 0; t < M; t++) f1[k][t] *= a.cv;
     }
     solve_d12<5, 4, real, M, UNI>(d1, d2, c, f1);
     solve_d12<8, 0, real, M, UNI>(d1, d2, c, f2);
@@ -564,6 +591,10 @@ __global__ __launch_bounds__(adj_col_threads<real>()) void k_adj_y(
         for (int k = 0; k < 4; k++) rhs<OP_D1, real, M, HW, UNI>(d1, c, w[k], f1[k]);
         win.load(a.v.e, w[0]);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w[0], f1[4]);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+#pragma unroll
+            for (int t = 0; t < M; t++) f1[k][t] *= a.cv;
     }
     solve_d12<5, 4, real, M, UNI>(d1, d2, c, f1);
     solve_d12<8, 0, real, M, UNI>(d1, d2, c, f2);
@@ -656,10 +687,10 @@ static hipError_t launch_pass(int pass, const OpDev<real> *d1, const OpDev<real>
 
 // the lanes each pass's launch bound allows
 template <typename real>
-static int pass_threads(int pass)
+static int pass_threads(int pass, bool uni)
 {
     if (pass == ADJ_X) return ADJ_X_THREADS;
-    if (pass == ADJ_YREC || pass == ADJ_ZREC) return adj_rec_threads<real>();
+    if (pass == ADJ_YREC || pass == ADJ_ZREC) return adj_rec_threads<real>(uni);
     return adj_col_threads<real>();
 }
 
@@ -669,25 +700,13 @@ hipError_t launch_adj_pass(int pass, const OpDev<real> *d1, const OpDev<real> *d
                            const Phys<real> &ph, hipStream_t st)
 {
     if (M != SEG_DERIV || pass < ADJ_YREC || pass > ADJ_Y) return hipErrorInvalidValue;
-    if (g.LX < 1 || g.threads < 1 || g.threads > pass_threads<real>(pass))
+    if (g.LX < 1 || g.threads < 1 || g.threads > pass_threads<real>(pass, uni != 0))
         return hipErrorInvalidConfiguration;
     return uni ? launch_pass<real, true>(pass, d1, d2, g, lm, a, ph, st)
                : launch_pass<real, false>(pass, d1, d2, g, lm, a, ph, st);
 }
 
 // ------------------------------------------------------------ pointwise
-
-template <typename real>
-__global__ void k_adj_scale(QSet<real> out, QSet<real> in, real c, long N)
-{
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < 5 * N;
-         i += (long)gridDim.x * blockDim.x) {
-        if (i < 4 * N)
-            out.m[i] = c * in.m[i];
-        else
-            out.e[i - 4 * N] = c * in.e[i - 4 * N];
-    }
-}
 
 // Gradient of J = (1/N) sum rho |u|^2 / 2 with respect to the conserved
 // state, and the map of a conserved cotangent to the initial primitive
@@ -731,13 +750,6 @@ static dim3 grid_for(long N)
 }
 
 template <typename real>
-hipError_t launch_adj_scale(const QSet<real> &out, const QSet<real> &in, real c, long N,
-                            hipStream_t st)
-{
-    hipLaunchKernelGGL((k_adj_scale<real>), grid_for(5 * N), dim3(256), 0, st, out, in, c, N);
-    return hipGetLastError();
-}
-template <typename real>
 hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,
                               long N, hipStream_t st)
 {
@@ -758,8 +770,6 @@ hipError_t launch_adj_to_velocity(const QSet<real> &q, const QSet<real> &qb, con
                                               int, const LaunchGeom &, const LineMap &,          \
                                               const AdjVjp<real> &, const Phys<real> &,          \
                                               hipStream_t);                                      \
-    template hipError_t launch_adj_scale<real>(const QSet<real> &, const QSet<real> &, real, long, \
-                                               hipStream_t);                                     \
     template hipError_t launch_adj_ke_seed<real>(const QSet<real> &, const Phys<real> &,           \
                                                  const QSet<real> &, long, hipStream_t);         \
     template hipError_t launch_adj_to_velocity<real>(const QSet<real> &, const QSet<real> &,       \

@@ -41,7 +41,8 @@ struct AdjSeeds {
 };
 
 // One product res = (d(dt R)/dQ)^T lam at the stage state q, with the stage
-// cotangent v = dt * lam given; the passes add their shares through resx
+// cotangent dt * lam = cv * v given (the passes scale v as they read it);
+// the passes add their shares through resx
 // (x) and resxz (x + z).  Epilogue of the y pass, each part skipped when
 // its set is null (m == nullptr): out = res, acc = acc_base + res (acc may
 // alias acc_base; a null acc_base counts as zero), vn = ca la + cb res (vn
@@ -53,13 +54,13 @@ struct AdjVjp {
     AdjSeeds<real> sd;
     QSet<real> resx, resxz;
     QSet<real> out, acc, acc_base, vn, la;
-    real ca, cb;
+    real cv, ca, cb;
 };
 
 // most lanes per workgroup: the x pass takes whole lines as the x sweep
 // does; the transposed column passes hold 17 solves per lane, which 512
 // lanes (FP32) and 256 (FP64) give the registers for; the recompute passes
-// hold 8 and 10
+// hold 8 and 10 (1024 lanes on uniform FP32 lines, else 512)
 constexpr int ADJ_X_THREADS = 256;
 template <typename real>
 constexpr int adj_col_threads()
@@ -67,9 +68,9 @@ constexpr int adj_col_threads()
     return sizeof(real) == 4 ? 512 : 256;
 }
 template <typename real>
-constexpr int adj_rec_threads()
+constexpr int adj_rec_threads(bool uni)
 {
-    return sizeof(real) == 4 ? 1024 : 512;
+    return sizeof(real) == 4 && uni ? 1024 : 512;
 }
 // ring planes (values per exchange position) and LDS staging of each pass
 constexpr int ADJ_X_PLANES = 16, ADJ_X_STAGE = 8;
@@ -79,7 +80,7 @@ constexpr int ADJ_R_PLANES = 10;  // recompute passes: 8 (y) and 10 (z) solves
 enum { ADJ_YREC, ADJ_ZREC, ADJ_X, ADJ_Z, ADJ_Y };
 
 // values per point the x pass keeps in LDS across its second solve batch
-constexpr int ADJ_X_KEEP = 11;
+constexpr int ADJ_X_KEEP = 15;
 // LDS bytes of an x-pass workgroup of LX lines of P segments of M points:
 // the exchange ring or the window staging, whichever is larger, then the
 // kept values
@@ -95,10 +96,6 @@ template <typename real>
 hipError_t launch_adj_pass(int pass, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
                            const LaunchGeom &g, const LineMap &lm, const AdjVjp<real> &a,
                            const Phys<real> &ph, hipStream_t st);
-// out = c * in over a conserved set (in may alias out)
-template <typename real>
-hipError_t launch_adj_scale(const QSet<real> &out, const QSet<real> &in, real c, long N,
-                            hipStream_t st);
 template <typename real>
 hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,
                               long N, hipStream_t st);

@@ -68,9 +68,11 @@ __host__ __device__ constexpr int inv_mod32(int s)
 // row * S = 32 / LX (mod 32).  Lanes of a half-wavefront then hit distinct
 // LDS banks: in MAP_COL they are 32 / LX consecutive segments of LX lines, at
 // offsets line * row * S + g * S; in MAP_ROW consecutive segments of one
-// line, at offsets g * S.
+// line, at offsets g * S.  A workgroup of one line needs no padding (its
+// lanes are consecutive segments at offsets g * S either way).
 __host__ __device__ inline int ring_row(int P, int KR, int LX, int S)
 {
+    if (LX == 1) return P + KR;
     const int want = LX >= 32 ? 1 : 32 / LX;
     const int r = P + KR;
     const int target = (want * inv_mod32(S)) & 31;

@@ -1043,7 +1043,8 @@ struct SolverT : SolverBase {
                 g.shm = g.LX > 0 ? adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) : 0;
             }
             else if (pass == ADJ_YREC || pass == ADJ_ZREC)
-                g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_rec_threads<real>()),
+                g = line_geom(h1.P, kr,
+                              env_cap("CS3D_ADJ_THREADS", adj_rec_threads<real>(h1.uniform != 0)),
                               ADJ_R_PLANES, sizeof(real));
             else
                 g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_col_threads<real>()),
@@ -1172,35 +1173,35 @@ struct SolverT : SolverBase {
             la = la5;
         }
         // acc = q + k1/6 + k2/3 + k3/3 + k4/6 (filtered into q_{n+1});
-        // Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3; k_s = dt R(Q_s)
-        hc(launch_adj_scale<real>(vb5, la, rdt / 6, N, st), "adjoint scale");
-        // stage cotangents alternate between two sets: the y pass reads v's
-        // windows while it writes the next one
-        QSet<real> vin = vb5, vout = vb5b;
+        // Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3; k_s = dt R(Q_s).  The
+        // first product reads dt/6 la as la scaled on load; the stage
+        // cotangents after it alternate between the sets A and B (the y pass
+        // reads v's windows while it writes the next one)
+        const QSet<real> A = vb5, B = vb5b;
         AdjVjp<real> f{};
         f.acc = qb5;
-        f.acc_base = la;
-        f.vn = vout;
         f.la = la;
+        f.cv = rdt / 6;  // stage 4: v = la, next A
+        f.acc_base = la;
+        f.vn = A;
         f.ca = rdt / 3;
         f.cb = rdt;
-        vjp(stg[3], vin, f);
-        std::swap(vin, vout);
+        vjp(stg[3], la, f);
+        f.cv = 1;  // stage 3: v = A, next B
         f.acc_base = qb5;
-        f.vn = vout;
+        f.vn = B;
         f.ca = rdt / 3;
         f.cb = rdt / 2;
-        vjp(stg[2], vin, f);
-        std::swap(vin, vout);
-        f.vn = vout;
+        vjp(stg[2], A, f);
+        f.vn = A;  // stage 2: v = B, next A
         f.ca = rdt / 6;
         f.cb = rdt / 2;
-        vjp(stg[1], vin, f);
-        std::swap(vin, vout);
-        AdjVjp<real> l{};
+        vjp(stg[1], B, f);
+        AdjVjp<real> l{};  // stage 1: v = A; lam = acc + res
+        l.cv = 1;
         l.acc = lam;
         l.acc_base = qb5;
-        vjp(stg[0], vin, l);
+        vjp(stg[0], A, l);
     }
 
     // five host fields (FP64) <-> a device set, no reference offsets
@@ -1242,10 +1243,10 @@ struct SolverT : SolverBase {
         hc(hipMemcpyAsync(&h, dsc, sizeof(h), hipMemcpyDeviceToHost, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
         upload_set(lam, la5);
-        hc(launch_adj_scale<real>(vb5, la5, (real)h.dt, N, st), "adjoint scale");
         AdjVjp<real> f{};
+        f.cv = (real)h.dt;
         f.out = qb5;
-        vjp(set == 1 ? F.q : F.qk, vb5, f);
+        vjp(set == 1 ? F.q : F.qk, la5, f);
         download_set(qb5, out);
     }
 
