@@ -984,8 +984,8 @@ struct SolverT : SolverBase {
     // state recomputed, the x derivatives with the cotangent seeds and all
     // x-direction transposed terms, then the z and y transposed terms
     // (periodic D1 is skew-adjoint, D2 and the filter self-adjoint), each
-    // pass chaining its share back to the conserved state.  RK4 is reversed
-    // stage by stage with the forward dt sequence frozen; the stage states of
+    // pass chaining its share back to the conserved state.  RK4 and TVDRK3
+    // are reversed stage by stage with the forward dt sequence frozen; the stage states of
     // each checkpoint segment are replayed with the fused forward stages and
     // kept.  Cotangents are conserved sets (a quad and the energy).
 
@@ -1122,9 +1122,12 @@ struct SolverT : SolverBase {
         filter_sweep(o[0], scratch, out);
     }
 
+    // stages per step of the configured scheme
+    int nstages() const { return cfg.rk == CS3D_TVDRK3 ? 3 : 4; }
+
     // one step on the fused forward path from F.q; stg (when given) receives
-    // the stage states q, Q2, Q3, Q4.  Replays (dt > 0) restore the step
-    // counters the filter round robin reads.
+    // the stage states q, Q2, Q3 (and Q4 for RK4).  Replays (dt > 0)
+    // restore the step counters the filter round robin reads.
     struct StepRec {
         double dt;
         int timeStep, filterTimeStep, order;  // counters before the step; order -1: no filter
@@ -1149,13 +1152,14 @@ struct SolverT : SolverBase {
             r.order = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
         }
         if (stg) copy_set(stg[0], F.q);
-        // the last sweep of stage s < 4 writes stage state s + 1 straight
+        // the last sweep of stage s < ns writes stage state s + 1 straight
         // into the store, and stage s + 1 reads it there
         const QSet<real> own = F.qk;
-        for (int s = 1; s <= 4; s++) {
-            if (stg && s < 4) F.qkw = stg[s];
-            sub_step(s, s == 4);
-            if (stg && s < 4) F.qk = stg[s];
+        const int ns = nstages();
+        for (int s = 1; s <= ns; s++) {
+            if (stg && s < ns) F.qkw = stg[s];
+            sub_step(s, s == ns);
+            if (stg && s < ns) F.qk = stg[s];
         }
         F.qk = F.qkw = own;
         post_step();
@@ -1181,6 +1185,30 @@ struct SolverT : SolverBase {
         AdjVjp<real> f{};
         f.acc = qb5;
         f.la = la;
+        if (cfg.rk == CS3D_TVDRK3) {
+            // Q2 = q + k1, Q3 = 3/4 q + 1/4 (Q2 + k2), acc = 1/3 q + 2/3 (Q3 +
+            // k3): dt k3b = 2 dt/3 la; dt k2b = dt/4 Q3b = dt/6 la + dt/4 res3;
+            // dt k1b = dt Q2b = dt k2b + dt res2; qb = la + res3 + res2 + res1
+            f.cv = 2 * rdt / 3;  // stage 3: v = la, next A
+            f.acc_base = la;
+            f.vn = A;
+            f.ca = rdt / 6;
+            f.cb = rdt / 4;
+            vjp(stg[2], la, f);
+            f.cv = 1;  // stage 2: v = A, next B = A + dt res
+            f.acc_base = qb5;
+            f.la = A;
+            f.vn = B;
+            f.ca = 1;
+            f.cb = rdt;
+            vjp(stg[1], A, f);
+            AdjVjp<real> l{};  // stage 1: v = B; lam = acc + res
+            l.cv = 1;
+            l.acc = lam;
+            l.acc_base = qb5;
+            vjp(stg[0], B, l);
+            return;
+        }
         f.cv = rdt / 6;  // stage 4: v = la, next A
         f.acc_base = la;
         f.vn = A;
@@ -1253,7 +1281,6 @@ struct SolverT : SolverBase {
     void ke_gradient(int nsteps, int K, double *J, double *gU, double *gV, double *gW) override
     {
         if (nsteps < 1 || K < 1) throw std::invalid_argument("nsteps and checkpoint must be >= 1");
-        if (cfg.rk != CS3D_RK4) throw std::invalid_argument("the adjoint reverses RK4 only");
         adj_require();
         const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
         // checkpoints, the stage states of one segment and the final state
@@ -1263,11 +1290,12 @@ struct SolverT : SolverBase {
             if (used == adjpool.size()) adjpool.push_back(adj_alloc_set());
             return adjpool[used++];
         };
-        std::vector<QSet<real>> ck(nseg), stg(4 * (size_t)K);
+        const int ns = nstages();
+        std::vector<QSet<real>> ck(nseg), stg(ns * (size_t)K);
         for (auto &c : stg) c = state();
         const QSet<real> fin = state();
         for (auto &c : ck) c = state();
-        auto stages = [&](int i) { return (const QSet<real> *)&stg[4 * (size_t)i]; };
+        auto stages = [&](int i) { return (const QSet<real> *)&stg[ns * (size_t)i]; };
         std::vector<StepRec> rec(nsteps);
         // forward pass on the fused path: checkpoints, and the stage states
         // of the last segment

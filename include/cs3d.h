@@ -219,14 +219,15 @@ int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *l
                  const double *lam_rhoV, const double *lam_rhoW, const double *lam_rhoE,
                  double *out_rho, double *out_rhoU, double *out_rhoV, double *out_rhoW,
                  double *out_rhoE);
-/* Advance nsteps RK4 steps from the current state (as cs3d_step does) and
- * return J, the final volume-averaged kinetic energy, and dJ/dU0, dJ/dV0,
- * dJ/dW0: its gradient with respect to the initial velocity fields at fixed
- * initial density and pressure, by the discrete adjoint of the steps taken
- * (the dt sequence of the forward pass held fixed, the filter included).
- * States are recomputed from a checkpoint every `checkpoint` steps; the
- * reverse pass holds ceil(nsteps/checkpoint) + 4*checkpoint + 1 states
- * (5 fields each) and 71 work fields, kept from call to call until
+/* Advance nsteps steps of the configured scheme (RK4 or TVDRK3) from the
+ * current state (as cs3d_step does) and return J, the final volume-averaged
+ * kinetic energy, and dJ/dU0, dJ/dV0, dJ/dW0: its gradient with respect to
+ * the initial velocity fields at fixed initial density and pressure, by the
+ * discrete adjoint of the steps taken (the dt sequence of the forward pass
+ * held fixed, the filter included).  States are recomputed from a
+ * checkpoint every `checkpoint` steps; the reverse pass holds
+ * ceil(nsteps/checkpoint) + S*checkpoint + 1 states (S = 4 for RK4, 3 for
+ * TVDRK3; 5 fields each) and 77 work fields, kept from call to call until
  * cs3d_adjoint_release or cs3d_destroy.  The solver is left at the final
  * state.  NULL outputs are skipped. */
 int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double *J, double *gU,

@@ -6,8 +6,8 @@ adjoint.  The product path never imports this module.
 The reference has no adjoint.  This module is the reverse of the oracle's
 own stage map (oracle.c pre_step_derivatives / solve_eqn_set, which restate
 CSolver_AWS.cpp:829-1504 on periodic lines without sponges) and of its RK4
-step with the filter (RK4.hpp:35-118, CSolver_AWS.cpp:1682-1893), written
-independently of the GPU kernels:
+and TVDRK3 steps with the filter (RK4.hpp:35-118, TVDRK3.hpp:38-96,
+CSolver_AWS.cpp:1682-1893), written independently of the GPU kernels:
 
 - every compact operator is the dense matrix of the oracle's own line
   operator (or_line_op, column by column), and its adjoint is that matrix
@@ -19,10 +19,11 @@ independently of the GPU kernels:
   reverse sweep differentiates every monomial factor by factor;
 - nothing is taped: the reverse pass recomputes the forward intermediates.
 
-Pinned by tests/test_oracle_adjoint.py: the numpy forward stage and RK4
-step against the C oracle (1e-12), the dot-product identity against
-central differences of the forward stage, and the kinetic-energy gradient
-against central differences of whole C-oracle runs with the dt held fixed.
+Pinned by tests/test_oracle_adjoint.py: the numpy forward stage and the RK4
+and TVDRK3 steps against the C oracle (1e-12), the dot-product identity
+against central differences of the forward stage, and the kinetic-energy
+gradient through either scheme against central differences of whole
+C-oracle runs with the dt held fixed.
 """
 import numpy as np
 
@@ -111,8 +112,9 @@ DIR = {"x": 0, "y": 1, "z": 2, "X": 0, "Y": 1, "Z": 2}
 
 
 class Adjoint:
-    """Dense-operator forward and reverse of the oracle's stage map and RK4
-    step on a periodic (Nz, Ny, Nx) grid.  `cfg` is a pyoracle config."""
+    """Dense-operator forward and reverse of the oracle's stage map and of
+    its RK4 or TVDRK3 step (cfg.rk) on a periodic (Nz, Ny, Nx) grid.
+    `cfg` is a pyoracle config."""
 
     def __init__(self, cfg):
         self.dims = (cfg.Nx, cfg.Ny, cfg.Nz)
@@ -121,6 +123,7 @@ class Adjoint:
             if cfg.bc_type[d] != po.PERIODIC_SOLVE:
                 raise ValueError("the numpy adjoint covers periodic lines only")
         lengths = (cfg.Lx, cfg.Ly, cfg.Lz)
+        self.rk = cfg.rk
         self.gamma = cfg.gamma
         self.R = 1.0 / cfg.gamma
         self.cpPr = self.R * cfg.gamma / (cfg.gamma - 1.0) / cfg.Pr
@@ -271,14 +274,35 @@ class Adjoint:
                for a, k1_, k2_, k3_, k4_ in zip(q, k1, k2, k3, k4)]
         return self.filt(acc, order) if order is not None else acc
 
+    def tvdrk3_step(self, q, dt, order):
+        """one TVDRK3 step (TVDRK3.hpp:38-96: Q2 = q + k1, Q3 = 3/4 q +
+        1/4 (Q2 + k2), q' = 1/3 q + 2/3 (Q3 + k3)) followed by the filter in
+        `order` (None: no filter)"""
+        k1 = self.rhs(q, dt)
+        Q2 = [a + k for a, k in zip(q, k1)]
+        k2 = self.rhs(Q2, dt)
+        Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)]
+        k3 = self.rhs(Q3, dt)
+        acc = [a / 3 + 2.0 / 3.0 * b + 2.0 / 3.0 * k for a, b, k in zip(q, Q3, k3)]
+        return self.filt(acc, order) if order is not None else acc
+
+    def step(self, q, dt, order):
+        return self.rk4_step(q, dt, order) if self.rk == 4 else self.tvdrk3_step(q, dt, order)
+
+    def step_vjp(self, q, dt, order, lam):
+        if self.rk == 4:
+            return self.rk4_step_vjp(q, dt, order, lam)
+        return self.tvdrk3_step_vjp(q, dt, order, lam)
+
     def conserved_from_ic(self, rho, U, V, W, p):
         g = self.gamma
         return [rho, rho * U, rho * V, rho * W, p / (g - 1) + 0.5 * rho * (U * U + V * V + W * W)]
 
     def ke_gradient(self, ic, nsteps, dt, filter_step=1):
-        """J = <rho |u|^2 / 2> after nsteps RK4 steps with fixed dt and the
-        filter every filter_step steps; returns J, dJ/dU0, dJ/dV0, dJ/dW0 at
-        fixed initial density and pressure"""
+        """J = <rho |u|^2 / 2> after nsteps steps of the configured scheme
+        (RK4 or TVDRK3) with fixed dt and the filter every filter_step
+        steps; returns J, dJ/dU0, dJ/dV0, dJ/dW0 at fixed initial density
+        and pressure"""
         rho0, U0, V0, W0, p0 = [np.asarray(a, dtype=np.float64) for a in ic]
         q = self.conserved_from_ic(rho0, U0, V0, W0, p0)
         states, orders = [], []
@@ -290,7 +314,7 @@ class Adjoint:
                 fcount += 1
                 order = self.filter_order(fcount)
             orders.append(order)
-            q = self.rk4_step(q, dt, order)
+            q = self.step(q, dt, order)
         N = q[0].size
         rho = q[0]
         u = [q[i] / rho for i in (1, 2, 3)]
@@ -298,7 +322,7 @@ class Adjoint:
         lam = [-0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2) / N, u[0] / N, u[1] / N, u[2] / N,
                np.zeros(self.shape)]
         for n in range(nsteps - 1, -1, -1):
-            lam = self.rk4_step_vjp(states[n], dt, orders[n], lam)
+            lam = self.step_vjp(states[n], dt, orders[n], lam)
         # m_i = rho U_i, E = p/(g-1) + rho |U|^2 / 2 at fixed rho and p
         grads = [rho0 * (lam[i + 1] + (U0, V0, W0)[i] * lam[4]) for i in range(3)]
         return J, grads[0], grads[1], grads[2]
@@ -323,3 +347,20 @@ class Adjoint:
         qb = [a + c for a, c in zip(qb, Q2b)]
         Q1b = self.rhs_vjp(q, [x / 6 + 0.5 * c for x, c in zip(la, Q2b)], dt)
         return [a + c for a, c in zip(qb, Q1b)]
+
+    def tvdrk3_step_vjp(self, q, dt, order, lam):
+        """cotangent of the step's start state from that of its end state
+        for tvdrk3_step: with la the cotangent of the unfiltered result,
+        Q3b = 2/3 la + J3^T (2/3 la), Q2b = 1/4 Q3b + J2^T (1/4 Q3b),
+        qb = 1/3 la + 3/4 Q3b + Q2b + J1^T Q2b (J_s^T: rhs_vjp at Q_s)"""
+        k1 = self.rhs(q, dt)
+        Q2 = [a + k for a, k in zip(q, k1)]
+        k2 = self.rhs(Q2, dt)
+        Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)]
+        la = self.filt(lam, order, transpose=True) if order is not None else list(lam)
+        t3 = [2.0 / 3.0 * x for x in la]
+        Q3b = [a + c for a, c in zip(t3, self.rhs_vjp(Q3, t3, dt))]
+        t2 = [0.25 * x for x in Q3b]
+        Q2b = [a + c for a, c in zip(t2, self.rhs_vjp(Q2, t2, dt))]
+        J1 = self.rhs_vjp(q, Q2b, dt)
+        return [x / 3 + 0.75 * a + b + c for x, a, b, c in zip(la, Q3b, Q2b, J1)]

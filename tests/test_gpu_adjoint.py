@@ -16,7 +16,7 @@ import pytest
 import adjoint as oa
 import cs3d
 import pyoracle as po
-from helpers import random_ic, relerr, smooth, tgv_ic
+from helpers import random_ic, relerr, shear_ic, smooth, tgv_ic
 
 pytestmark = pytest.mark.gpu
 
@@ -82,13 +82,14 @@ def run_ke(dims, ic, kw, nsteps, prec=cs3d.FP64):
     return g.kinetic_energy()
 
 
-@pytest.mark.parametrize("nsteps,checkpoint", [(6, 4), (5, 1)])
-def test_ke_gradient_matches_finite_differences_fp64(nsteps, checkpoint):
+@pytest.mark.parametrize("nsteps,checkpoint,rk", [(6, 4, 4), (5, 1, 4), (6, 4, 3), (5, 2, 3)])
+def test_ke_gradient_matches_finite_differences_fp64(nsteps, checkpoint, rk):
     """dJ/dU0 . dU (and V, W) against central differences of J(U0 + e dU)
-    through nsteps RK4 steps with the filter every step; 1e-4 relative"""
+    through nsteps RK4 or TVDRK3 steps with the filter every step; 1e-4
+    relative"""
     dims = (16, 16, 16)
     N = 16
-    kw = adj_kw(dims, dt_fixed=0.02)
+    kw = adj_kw(dims, dt_fixed=0.02, rk=rk)
     L = kw["Lx"]
     rho, U, V, W, p = tgv_ic(N, L, p0=1.0 / 1.4 * 10)
     rng = np.random.default_rng(3)
@@ -134,15 +135,28 @@ def test_ke_gradient_fp32_tracks_fp64():
 
 
 def test_adjoint_rejects_unsupported_configurations():
+    """walls, sponges, non-periodic lines and the compensated mode are not
+    reversed; neither are step counts below one"""
     N = 16
     kw = adj_kw((N, N, N))
-    kw["rk"] = 3
     g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
     g.set_initial_conditions(*tgv_ic(N, kw["Lx"]))
     with pytest.raises(cs3d.CS3DError):
-        g.ke_gradient(2, 1)
-    with pytest.raises(cs3d.CS3DError):
         g.ke_gradient(0, 1)
+    g.close()
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP32, compensated=1, **kw))
+    g.set_initial_conditions(*tgv_ic(N, kw["Lx"]))
+    with pytest.raises(cs3d.CS3DError):
+        g.ke_gradient(2, 1)
+    g.close()
+    # sponge layers on non-periodic y lines
+    kw = dict(Lx=20.0, Ly=16.0, Lz=8.0, alpha_f=0.49, mu_ref=1e-2, Pr=0.7, cfl=0.5,
+              filter_step=1, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 1, 1, 0, 0),
+              sponge_frac=(0.25, 0.2, 0.25))
+    g = cs3d.Solver(cs3d.config(16, 16, 8, precision=cs3d.FP64, **kw))
+    g.set_initial_conditions(*shear_ic(16, 16, 8, 16.0, seed=1))
+    with pytest.raises(cs3d.CS3DError):
+        g.ke_gradient(2, 1)
 
 
 def oracle_cfg(dims, kw):
@@ -170,12 +184,12 @@ def test_rhs_vjp_matches_oracle_adjoint_fp64():
     assert max(errs) <= 1e-11, errs
 
 
-@pytest.mark.parametrize("filter_step", [1, 2])
-def test_ke_gradient_matches_oracle_adjoint_fp64(filter_step):
+@pytest.mark.parametrize("filter_step,rk", [(1, 4), (2, 4), (1, 3), (2, 3)])
+def test_ke_gradient_matches_oracle_adjoint_fp64(filter_step, rk):
     dims = (16, 12, 20)
     Nx, Ny, Nz = dims
     shape = (Nz, Ny, Nx)
-    kw = adj_kw(dims, dt_fixed=0.02, filter_step=filter_step)
+    kw = adj_kw(dims, dt_fixed=0.02, filter_step=filter_step, rk=rk)
     ic = random_ic(shape, seed=23, amp=0.5)
     g = cs3d.Solver(cs3d.config(Nx, Ny, Nz, precision=cs3d.FP64, **kw))
     g.set_initial_conditions(*ic)
