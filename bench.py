@@ -190,8 +190,9 @@ def cpu_model():
 
 
 def stream_bandwidth():
-    """the library's copy and triad kernels over 1 GiB arrays (5 timed
-    after 2 warm-up launches): best and mean GB/s"""
+    """the library's copy and triad kernels over 1 GiB arrays (the best of
+    six launch shapes, 5 timed after 2 warm-up launches each): best and
+    mean GB/s"""
     out = {}
     for kind in ("copy", "triad"):
         best, mean = cs3d.stream_probe(kind, 1 << 30, 5)
@@ -248,9 +249,10 @@ def main():
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.copy_probe:
-        # 2 untimed + 3 timed copies of 256 MiB by the library's copy kernel
+        # 256 MiB copies by the library's copy kernel: six launch shapes, 2
+        # untimed + 3 timed launches each (every dispatch moves the same bytes)
         best, _ = cs3d.stream_probe("copy", 256 << 20, 3)
-        print(f"copy probe: 5 x 256 MiB, best {best:.0f} GB/s", flush=True)
+        print(f"copy probe: 30 x 256 MiB, best {best:.0f} GB/s", flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "

@@ -252,10 +252,12 @@ int cs3d_profile_report(cs3d_solver *s, double ms[6], int counts[6]);
  * 16-byte-vector kernels (no reference counterpart: a measured ceiling for
  * the sweeps' roofline fractions, and a copy of known size to calibrate
  * HBM traffic counters).  kind 0: copy b = a (2 arrays), 1: triad
- * a = b + s c (3 arrays) of `bytes` each; 2 untimed then `reps` timed
- * launches on the null stream, HIP events around each.  Bytes read plus
- * written per second (1e9), the best launch and the mean.  Allocates and
- * frees its arrays. */
+ * a = b + s c (3 arrays) of `bytes` each, in six launch shapes (grid-
+ * strided or one pass, 4 to 16 vectors in flight per lane); per shape 2
+ * untimed then `reps` timed launches on the null stream, HIP events around
+ * each.  Bytes read plus written per second (1e9): the best launch of any
+ * shape, and the mean of that shape's launches.  Allocates and frees its
+ * arrays. */
 int cs3d_stream_probe(int kind, size_t bytes, int reps, double *best_gbps, double *mean_gbps);
 
 /*
