@@ -603,8 +603,9 @@ class CSolver_AWS : public AbstractCSolver {
     }
 
     // Gradients by the discrete adjoint (no reference counterpart: the
-    // reference has no adjoint).  keGradient advances nsteps RK4 steps as
-    // the time loop does and returns J, the final volume-averaged kinetic
+    // reference has no adjoint).  keGradient advances nsteps steps of the
+    // solver's scheme (RK4 or TVDRK3) as the time loop does and returns J,
+    // the final volume-averaged kinetic
     // energy, with dJ/dU0, dJ/dV0, dJ/dW0 (host arrays of N doubles, XYZ;
     // NULL skips one) at fixed initial rho and p, states recomputed from a
     // checkpoint every `checkpoint` steps.  rhsVJP gives
