@@ -21,6 +21,7 @@ PERIODIC_SOLVE, DIRICHLET_SOLVE = 0, 1
 BC_PERIODIC, BC_SPONGE, BC_ADIABATIC_WALL, BC_MOVING_ADIABATIC_WALL, BC_INLET = range(5)
 DIRX, DIRY, DIRZ = 0, 1, 2
 FILTER, DERIV1, DERIV2 = 0, 1, 2
+FILTER_T, DERIV1_T, DERIV2_T = 3, 4, 5   # transposed operators (the adjoint's)
 FP64, FP32 = 0, 1
 TVDRK3, RK4 = 3, 4
 HOST, DEVICE = 0, 1
@@ -172,6 +173,13 @@ class Derivatives(_Ops):
     def calc2ndDerivField(self, data, layout=LAYOUT_XYZ):
         return self.apply(DERIV2, data, layout)
 
+    # transposed operators: <D f, g> = <f, D^T g> over the field
+    def calc1stDerivFieldTransposed(self, data, layout=LAYOUT_XYZ):
+        return self.apply(DERIV1_T, data, layout)
+
+    def calc2ndDerivFieldTransposed(self, data, layout=LAYOUT_XYZ):
+        return self.apply(DERIV2_T, data, layout)
+
 
 class Filter(_Ops):
     """Filter(alphaF, dom, bcType, currentDir) (Filter.hpp:31): 8th-order
@@ -182,6 +190,9 @@ class Filter(_Ops):
 
     def filterField(self, data, layout=LAYOUT_XYZ):
         return self.apply(FILTER, data, layout)
+
+    def filterFieldTransposed(self, data, layout=LAYOUT_XYZ):
+        return self.apply(FILTER_T, data, layout)
 
 
 # ------------------------------------------------------------ solver

@@ -15,9 +15,11 @@ using namespace cs3d;
 
 struct cs3d_ops {
     int Nx, Ny, Nz, dir, precision;
+    int periodic = 1;
+    double dh = 1.0, alpha_f = 0.0;
     hipStream_t stream = nullptr;
-    PlanDev<float> f[3];   // indexed by kind: filter, D1, D2
-    PlanDev<double> d[3];
+    PlanDev<float> f[6];   // indexed by kind: filter, D1, D2, then their
+    PlanDev<double> d[6];  // transposes (built on first use)
     void *dbuf_in = nullptr, *dbuf_out = nullptr;
     size_t nbytes = 0;
 };
@@ -44,6 +46,9 @@ int cs3d_ops_create(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int
     // Domain.hpp:30-45: x_i = i*L/(N-1), dx = x_1 - x_0
     double dh = (1.0 / ((double)n - 1.0)) * L;
     int periodic = bc_type == CS3D_PERIODIC_SOLVE;
+    o->periodic = periodic;
+    o->dh = dh;
+    o->alpha_f = alpha_f;
     try {
         for (int kind = 0; kind < 3; kind++) {
             // filter plans only need alpha_f; skip them when it is unusable
@@ -69,8 +74,27 @@ int cs3d_ops_create(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int
 int cs3d_ops_apply(cs3d_ops *o, int kind, int layout, const void *in, void *out, int ptr_kind)
 {
     if (!o || !in || !out) return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: NULL argument");
-    if (kind < 0 || kind > 2) return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: bad kind");
+    if (kind < 0 || kind > 5) return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: bad kind");
     bool fp32 = o->precision == CS3D_FP32;
+    const bool transposed = kind >= CS3D_FILTER_T;
+    if (transposed && in == out)
+        return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: a transposed operator needs out != in");
+    if (transposed && ((fp32 && !o->f[kind].dev) || (!fp32 && !o->d[kind].dev))) {
+        const int base = kind - CS3D_FILTER_T;
+        const int n = o->dir == 0 ? o->Nx : o->dir == 1 ? o->Ny : o->Nz;
+        if (base == OP_FILTER && !(o->alpha_f > -0.5 && o->alpha_f < 0.5))
+            return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: operator not available (alpha_f?)");
+        try {
+            if (fp32)
+                make_plan(o->f[kind], base, o->periodic, n, o->dh, o->alpha_f, 0, true);
+            else
+                make_plan(o->d[kind], base, o->periodic, n, o->dh, o->alpha_f, 0, true);
+        } catch (const std::invalid_argument &ex) {
+            return set_error(CS3D_ERR_ARG, std::string("cs3d_ops_apply: ") + ex.what());
+        } catch (const std::exception &ex) {
+            return set_error(CS3D_ERR_HIP, std::string("cs3d_ops_apply: ") + ex.what());
+        }
+    }
     if ((fp32 && !o->f[kind].dev) || (!fp32 && !o->d[kind].dev))
         return set_error(CS3D_ERR_ARG, "cs3d_ops_apply: operator not available (alpha_f?)");
     LineMap lm = layout == CS3D_LAYOUT_REFERENCE ? ref_lines(o->dir, o->Nx, o->Ny, o->Nz)
@@ -109,7 +133,13 @@ int cs3d_ops_apply(cs3d_ops *o, int kind, int layout, const void *in, void *out,
     e = hipGetLastError();
     if (e != hipSuccess)
         return set_error(CS3D_ERR_HIP, hip_msg(e, "cs3d_ops_apply: earlier HIP failure"));
-    if (fp32)
+    if (transposed && fp32)
+        e = launch_field_opT<float>(o->f[kind], map, lm, (const float *)din, (float *)dout, 1.0f,
+                                    0, o->stream);
+    else if (transposed)
+        e = launch_field_opT<double>(o->d[kind], map, lm, (const double *)din, (double *)dout,
+                                     1.0, 0, o->stream);
+    else if (fp32)
         e = launch_field_op<float>(o->f[kind], map, lm, (const float *)din, (float *)dout, o->stream);
     else
         e = launch_field_op<double>(o->d[kind], map, lm, (const double *)din, (double *)dout,
@@ -179,7 +209,7 @@ int cs3d_plan_info(int kind, int bc_type, int n, double dh, double alpha_f, int 
 void cs3d_ops_destroy(cs3d_ops *o)
 {
     if (!o) return;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 6; k++) {
         o->f[k].release();
         o->d[k].release();
     }

@@ -26,11 +26,14 @@ void PlanDev<real>::release()
 {
     if (dev) (void)hipFree(dev);
     if (tables) (void)hipFree(tables);
+    if (tdev) (void)hipFree(tdev);
     dev = nullptr;
     tables = nullptr;
+    tdev = nullptr;
 }
 
-OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32, int seg)
+OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32, int seg,
+                bool transpose)
 {
     // cross-segment terms below this weight are dropped: a sixth (FP32) and
     // a tenth (FP64) of the unit roundoff, relative to the carried value
@@ -38,15 +41,17 @@ OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool f
     // uniform kernels only for periodic lines that split into equal
     // segments of the uniform length; everything else takes the table kernels
     const int MU = seg > 0 ? seg : kind == OP_FILTER ? SEG_FILTER : SEG_DERIV;
-    if (periodic && n % MU == 0) return build_plan(kind, periodic, n, dh, alpha_f, MU, tol, true);
-    return build_plan(kind, periodic, n, dh, alpha_f, SEG_TABLE, tol, false);
+    if (periodic && n % MU == 0)
+        return build_plan(kind, periodic, n, dh, alpha_f, MU, tol, true, transpose);
+    return build_plan(kind, periodic, n, dh, alpha_f, SEG_TABLE, tol, false, transpose);
 }
 
 template <typename real>
-void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f, int seg)
+void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f, int seg,
+               bool transpose)
 {
     pl.release();
-    OpHost h = plan_for(kind, periodic, n, dh, alpha_f, sizeof(real) == 4, seg);
+    OpHost h = plan_for(kind, periodic, n, dh, alpha_f, sizeof(real) == 4, seg, transpose);
     pl.host = h;
     OpDev<real> d;
     std::memset(&d, 0, sizeof(d));
@@ -135,6 +140,32 @@ void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, doub
     const int kc = kind == OP_FILTER ? kc_filter<real>() : kc_deriv<real>();
     LaunchGeom g = line_geom(h.P, std::max(h.K, kc), 1024, 1, sizeof(real));
     if (g.LX < 1) throw std::invalid_argument("too many segments per line");
+    if (transpose) {
+        // the solved lines (n + 1 values each) go to LDS after the ring
+        auto bytes = [&](int LX) {
+            return ring_bytes(h.P, std::max(h.K, kc), LX, 1, sizeof(real)) +
+                   (size_t)LX * (h.n + 1) * sizeof(real);
+        };
+        while (g.LX > 0 && bytes(g.LX) > LDS_BYTES) g.LX--;
+        if (g.LX < 1) throw std::invalid_argument("line too long for the transposed operator");
+        g.threads = g.LX * h.P;
+        g.shm = bytes(g.LX);
+        pl.thost = trans_table(h);
+        OpTDev<real> t;
+        std::memset(&t, 0, sizeof(t));
+        t.n = pl.thost.n;
+        t.periodic = pl.thost.periodic;
+        t.kind = pl.thost.kind;
+        t.JB = pl.thost.JB;
+        t.UB = pl.thost.UB;
+        for (int j = 0; j < TJB; j++)
+            for (int u = 0; u < TUB; u++) {
+                t.BTl[j][u] = (real)pl.thost.BTl[j][u];
+                t.BTr[j][u] = (real)pl.thost.BTr[j][u];
+            }
+        hip_check(hipMalloc(&pl.tdev, sizeof(t)), "hipMalloc(transposed plan)");
+        hip_check(hipMemcpy(pl.tdev, &t, sizeof(t), hipMemcpyHostToDevice), "hipMemcpy(plan)");
+    }
     pl.LX = g.LX;
     pl.threads = g.threads;
     pl.shm = g.shm;
@@ -197,6 +228,128 @@ __global__ __launch_bounds__(1024) void k_field_op(const OpDev<real> *__restrict
     if (active) win.store(out, r[0], c.len);
 }
 
+// The transposed operator (plan.hpp): the line's own values are solved with
+// A^T (op: the transposed plan), the solved line is staged in LDS after the
+// exchange ring, and every lane applies B^T (or (A - B)^T) to its points:
+// the interior stencil (negated for D1) from a window of the staged line,
+// the dense boundary columns of tt on non-periodic lines.  The filter's
+// transpose is g - (A - B)^T A^-T g.
+template <int KIND, typename real, int M, bool UNI, int MAP>
+__global__ __launch_bounds__(1024) void k_field_opT(const OpDev<real> *__restrict__ op,
+                                                    const OpTDev<real> *__restrict__ tt,
+                                                    LineMap lm, const real *__restrict__ in,
+                                                    real *__restrict__ out, real coef,
+                                                    int accumulate)
+{
+    extern __shared__ char smem_raw[];
+    real *xb = (real *)smem_raw;
+    constexpr int HW = KIND == OP_FILTER ? 4 : 2, W = M + 2 * HW;
+    constexpr int KC = KIND == OP_FILTER ? kc_filter<real>()
+                       : KIND == OP_D2   ? kc_d2<real>()
+                                         : kc_deriv<real>();
+    const int P = op->P, n = op->n;
+    const int LX = blockDim.x / P;
+    Lane<real, MAP> c = make_lane<real, MAP>(op, P, LX, op->K > KC ? op->K : KC, 1, xb);
+    const int L = blockIdx.x * LX + c.line;
+    const bool active = L < lm.nlines;
+    Win<real, M, 0, false> win;
+    win.init((unsigned)lm.base(active ? L : lm.nlines - 1), (unsigned)lm.sp, c.s, n, op->cyclic);
+    real g[M], r[1][M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        g[t] = t < c.len ? ldo(in, win.mid(t)) : real(0);
+        r[0][t] = g[t];
+    }
+    solve_batch<1, KC, real, M, UNI>(op, c, r);
+    // the solved line, n + 1 values per line (odd stride across the lines of
+    // a column workgroup), after both parities of the ring
+    real *st = xb + 2 * c.cap + c.line * (n + 1);
+#pragma unroll
+    for (int t = 0; t < M; t++)
+        if (t < c.len) st[c.s + t] = r[0][t];
+    __syncthreads();
+    real w[W], res[M];
+#pragma unroll
+    for (int t = 0; t < W; t++) {
+        int p = c.s - HW + t;
+        if (op->cyclic) {
+            p += p < 0 ? n : 0;
+            p -= p >= n ? n : 0;
+        } else {
+            p = p < 0 ? 0 : p >= n ? n - 1 : p;
+        }
+        w[t] = st[p];
+    }
+    stencil<KIND, real, M, HW>(op, w, res);
+    if (!op->cyclic) {
+        const int JB = tt->JB, UB = tt->UB;
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            const int j = c.s + t;
+            if (t < c.len && j < JB) {
+                real a = 0;
+                for (int u = 0; u < UB; u++) a += tt->BTl[j][u] * st[u];
+                res[t] = KIND == OP_D1 ? -a : a;
+            } else if (t < c.len && j >= n - JB) {
+                real a = 0;
+                for (int u = 0; u < UB; u++) a += tt->BTr[j - (n - JB)][u] * st[n - UB + u];
+                res[t] = KIND == OP_D1 ? -a : a;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        // the D1 stencil is antisymmetric: its transpose is its negative
+        real v = KIND == OP_D1 ? -res[t] : KIND == OP_FILTER ? g[t] - res[t] : res[t];
+        v *= coef;
+        if (active && t < c.len) {
+            real *o = (real *)((char *)out + win.mid(t));
+            *o = accumulate ? *o + v : v;
+        }
+    }
+}
+
+template <int KIND, typename real, int M, bool UNI>
+static hipError_t launch_kmuT(const PlanDev<real> &pl, int map, const LineMap &lm,
+                              const real *in, real *out, real coef, int acc, hipStream_t st)
+{
+    if (pl.LX < 1 || pl.threads < 1 || pl.threads > 1024 || !pl.tdev)
+        return hipErrorInvalidConfiguration;
+    dim3 grid((lm.nlines + pl.LX - 1) / pl.LX), block(pl.threads);
+    if (map == MAP_ROW)
+        hipLaunchKernelGGL((k_field_opT<KIND, real, M, UNI, MAP_ROW>), grid, block, pl.shm, st,
+                           pl.dev, pl.tdev, lm, in, out, coef, acc);
+    else
+        hipLaunchKernelGGL((k_field_opT<KIND, real, M, UNI, MAP_COL>), grid, block, pl.shm, st,
+                           pl.dev, pl.tdev, lm, in, out, coef, acc);
+    return hipGetLastError();
+}
+
+template <int KIND, typename real>
+static hipError_t launch_kT(const PlanDev<real> &pl, int map, const LineMap &lm, const real *in,
+                            real *out, real coef, int acc, hipStream_t st)
+{
+    constexpr int MU = KIND == OP_FILTER ? SEG_FILTER : SEG_DERIV;
+    if (pl.host.uniform) {
+        if (pl.host.M != MU) return hipErrorInvalidConfiguration;
+        return launch_kmuT<KIND, real, MU, true>(pl, map, lm, in, out, coef, acc, st);
+    }
+    if (pl.host.M != SEG_TABLE) return hipErrorInvalidConfiguration;
+    return launch_kmuT<KIND, real, SEG_TABLE, false>(pl, map, lm, in, out, coef, acc, st);
+}
+
+template <typename real>
+hipError_t launch_field_opT(const PlanDev<real> &pl, int map, const LineMap &lm, const real *in,
+                            real *out, real coef, int accumulate, hipStream_t st)
+{
+    if (lm.nlines <= 0) return hipSuccess;
+    switch (pl.host.kind) {
+    case OP_D1: return launch_kT<OP_D1, real>(pl, map, lm, in, out, coef, accumulate, st);
+    case OP_D2: return launch_kT<OP_D2, real>(pl, map, lm, in, out, coef, accumulate, st);
+    default: return launch_kT<OP_FILTER, real>(pl, map, lm, in, out, coef, accumulate, st);
+    }
+}
+
 template <int KIND, typename real, int M, bool UNI>
 static hipError_t launch_kmu(const PlanDev<real> &pl, int map, const LineMap &lm,
                              const real *in, real *out, hipStream_t st)
@@ -239,8 +392,12 @@ hipError_t launch_field_op(const PlanDev<real> &pl, int map, const LineMap &lm,
 
 template struct PlanDev<float>;
 template struct PlanDev<double>;
-template void make_plan<float>(PlanDev<float> &, int, int, int, double, double, int);
-template void make_plan<double>(PlanDev<double> &, int, int, int, double, double, int);
+template void make_plan<float>(PlanDev<float> &, int, int, int, double, double, int, bool);
+template void make_plan<double>(PlanDev<double> &, int, int, int, double, double, int, bool);
+template hipError_t launch_field_opT<float>(const PlanDev<float> &, int, const LineMap &,
+                                            const float *, float *, float, int, hipStream_t);
+template hipError_t launch_field_opT<double>(const PlanDev<double> &, int, const LineMap &,
+                                             const double *, double *, double, int, hipStream_t);
 template hipError_t launch_field_op<float>(const PlanDev<float> &, int, const LineMap &,
                                            const float *, float *, hipStream_t);
 template hipError_t launch_field_op<double>(const PlanDev<double> &, int, const LineMap &,

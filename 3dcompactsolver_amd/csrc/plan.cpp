@@ -61,7 +61,7 @@ static void segment(int n, int M, OpHost &h, bool want_uniform)
 }
 
 OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f,
-                  int M, double tol, bool allow_uniform)
+                  int M, double tol, bool allow_uniform, bool transpose)
 {
     OpHost h;
     h.n = n;
@@ -106,6 +106,16 @@ OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f,
     }
     h.a[0] = periodic ? h.a[0] : 0.0;          // no sub-diagonal in row 0
     h.c[n - 1] = periodic ? h.c[n - 1] : 0.0;  // no super-diagonal in row n-1
+    if (transpose && !periodic) {
+        // the solve of the transposed operator runs on A^T: sub-diagonal
+        // a'_i = c_{i-1}, super-diagonal c'_i = a_{i+1} (the periodic A is a
+        // symmetric circulant: A^T = A)
+        std::vector<double> at(n, 0.0), ct(n, 0.0);
+        for (int i = 1; i < n; i++) at[i] = h.c[i - 1];
+        for (int i = 0; i + 1 < n; i++) ct[i] = h.a[i + 1];
+        h.a = at;
+        h.c = ct;
+    }
 
     // ---- explicit stencil B (Derivatives.cpp:119-247, Filter.cpp:5-96)
     if (kind == OP_D1) {
@@ -320,6 +330,48 @@ OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f,
             h.psi[(size_t)g * K + k] = psi[(size_t)g * P + k];
         }
     return h;
+}
+
+// Entry (u, j) of the explicit matrix the kernels apply before the solve:
+// B for the derivatives, A - B for the filter (rows as plan.hpp describes:
+// closure rows bcl/bcr at the ends of non-periodic lines, the interior
+// stencil elsewhere).
+static double b_entry(const OpHost &h, int u, int j)
+{
+    const int n = h.n;
+    if (u < h.NB) return j < 7 ? h.bcl[u][j] : 0.0;
+    if (u >= n - h.NB) {
+        const int jj = j - (n - 7);
+        return jj >= 0 ? h.bcr[u - (n - h.NB)][jj] : 0.0;
+    }
+    const int e = j - u;
+    if (e < -h.H || e > h.H) return 0.0;
+    if (h.kind == OP_D1) return e > 0 ? h.tap[e] : e < 0 ? -h.tap[-e] : 0.0;
+    if (h.kind == OP_D2) return h.tap[e < 0 ? -e : e];
+    // kappa * D8: the 8th central difference (1, -8, 28, -56, 70, ...)
+    static const double d8[9] = {1, -8, 28, -56, 70, -56, 28, -8, 1};
+    return h.tap[0] * d8[e + 4];
+}
+
+OpTHost trans_table(const OpHost &h)
+{
+    OpTHost t;
+    t.n = h.n;
+    t.periodic = h.cyclic;
+    t.kind = h.kind;
+    if (h.cyclic) return t;  // no boundary columns
+    t.JB = std::max(7, h.NB + h.H);
+    t.UB = t.JB + h.H;
+    if (t.JB > TJB || t.UB > TUB) fail("transposed boundary block too large");
+    if (h.n < 2 * t.UB)
+        fail("transposed operator needs at least " + std::to_string(2 * t.UB) +
+             " points on a non-periodic line");
+    for (int j = 0; j < t.JB; j++)
+        for (int u = 0; u < t.UB; u++) {
+            t.BTl[j][u] = b_entry(h, u, j);
+            t.BTr[j][u] = b_entry(h, h.n - t.UB + u, h.n - t.JB + j);
+        }
+    return t;
 }
 
 }  // namespace cs3d

@@ -75,6 +75,31 @@ struct OpDev {
     const int *sstart, *slen;           // [P]
 };
 
+// The transposed operator (the adjoint's): D^T g = B^T (A^-T g) and, for the
+// filter, F^T g = g - (A - B)^T (A^-T g).  The solve with A^T runs on a plan
+// built with its LHS transposed; B^T (or (A - B)^T) then acts on the solved
+// line.  In the interior B is a Toeplitz stencil, so B^T is the same stencil
+// (D1 antisymmetric: negated; D2 and the filter's difference symmetric).
+// Near the ends of a non-periodic line the boundary rows of B make its
+// columns irregular: columns j < JB (and j >= n - JB) are dense sums over
+// the first (last) UB points, JB = max(7, NB + H) (the 7-point closure
+// rows, and the interior rows reaching past them), UB = JB + H.
+constexpr int TJB = 8;
+constexpr int TUB = 12;
+
+struct OpTHost {
+    int n = 0, periodic = 0, kind = 0, JB = 0, UB = 0;
+    double BTl[TJB][TUB] = {};  // column j < JB:     sum_u BTl[j][u] z[u]
+    double BTr[TJB][TUB] = {};  // column n - JB + j: sum_u BTr[j][u] z[n - UB + u]
+};
+
+template <typename real>
+struct OpTDev {
+    int n, periodic, kind, JB, UB;
+    real BTl[TJB][TUB];
+    real BTr[TJB][TUB];
+};
+
 // Host-side FP64 description of a plan, before upload.
 struct OpHost {
     int n = 0, M = 0, P = 0, uniform = 0, cyclic = 0, kind = 0, H = 0, NB = 0, K = 0;
@@ -89,5 +114,9 @@ struct OpHost {
     std::vector<double> a, b, c;
     double corner = 0.0;  // periodic corner value (alpha)
 };
+
+// the boundary columns of B^T (or (A - B)^T) of a plan (plan.cpp); non-
+// periodic lines need n >= 2 UB
+OpTHost trans_table(const OpHost &h);
 
 }  // namespace cs3d

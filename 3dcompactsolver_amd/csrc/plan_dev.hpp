@@ -13,13 +13,15 @@ namespace cs3d {
 
 // allow_uniform = false forces the per-point table path even when the
 // segments come out equal (the caller picked M for the table kernels).
+// transpose: the solve runs on A^T (the transposed operator's first step)
 OpHost build_plan(int kind, int periodic, int n, double dh, double alpha_f, int M, double tol,
-                  bool allow_uniform);
+                  bool allow_uniform, bool transpose = false);
 // the plan the library uses for an operator in a precision (segment length
 // and truncation chosen as the kernels expect)
 // seg > 0: the uniform segment length (default: SEG_FILTER for the filter,
 // SEG_DERIV for the derivatives)
-OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32, int seg = 0);
+OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool fp32, int seg = 0,
+                bool transpose = false);
 
 // Points per lane.  Derivatives keep 4 points per lane in both precisions
 // (the fused sweeps hold many windows at once); the periodic filter keeps 8;
@@ -53,6 +55,10 @@ struct PlanDev {
     int LX = 0;                 // lines per workgroup (standalone operator)
     int threads = 0;            // workgroup size = P * LX
     size_t shm = 0;             // LDS bytes of the exchange ring
+    // transposed plans (make_plan with transpose): the boundary columns of
+    // B^T, and LDS for the solved lines after the ring (shm includes it)
+    OpTDev<real> *tdev = nullptr;
+    OpTHost thost;
 
     void release();
 };
@@ -60,14 +66,23 @@ struct PlanDev {
 // Builds (FP64 host factorisation) and uploads a plan.  Throws
 // std::invalid_argument for unsupported sizes, std::runtime_error on HIP
 // failures.
+// transpose: the plan of the transposed operator (the solve on A^T, and the
+// boundary columns of B^T; launch it with launch_field_opT)
 template <typename real>
 void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, double alpha_f,
-               int seg = 0);
+               int seg = 0, bool transpose = false);
 
 // out = op(in) on every line of lm; map is MAP_ROW for contiguous lines.
 template <typename real>
 hipError_t launch_field_op(const PlanDev<real> &pl, int map, const LineMap &lm,
                            const real *in, real *out, hipStream_t st);
+
+// out = coef * op^T(in), or out += coef * op^T(in) with accumulate, on every
+// line of lm (pl: a transposed plan).  in and out must not overlap.
+template <typename real>
+hipError_t launch_field_opT(const PlanDev<real> &pl, int map, const LineMap &lm,
+                            const real *in, real *out, real coef, int accumulate,
+                            hipStream_t st);
 
 // line maps of an Nx*Ny*Nz field in XYZ order (Macros.hpp:8), and of the
 // reference's per-direction layouts YZX / ZXY (Macros.hpp:9-10)

@@ -46,8 +46,12 @@ enum {
 };
 /* Derivatives.hpp:48 Direct */
 enum { CS3D_DIRX = 0, CS3D_DIRY = 1, CS3D_DIRZ = 2 };
-/* operator kinds */
-enum { CS3D_FILTER = 0, CS3D_DERIV1 = 1, CS3D_DERIV2 = 2 };
+/* operator kinds; the _T kinds are the transposed operators F^T, D1^T,
+ * D2^T (the adjoint's; no reference counterpart) */
+enum {
+    CS3D_FILTER = 0, CS3D_DERIV1 = 1, CS3D_DERIV2 = 2,
+    CS3D_FILTER_T = 3, CS3D_DERIV1_T = 4, CS3D_DERIV2_T = 5
+};
 /* arithmetic of the bulk path */
 enum { CS3D_FP64 = 0, CS3D_FP32 = 1 };
 /* time integrator (RK4.hpp, TVDRK3.hpp) */
@@ -270,6 +274,11 @@ int cs3d_stream_probe(int kind, size_t bytes, int reps, double *best_gbps, doubl
  * precision CS3D_FP64: in/out are double*; CS3D_FP32: float*.
  * ptr_kind CS3D_HOST stages through device buffers; CS3D_DEVICE runs in
  * place on device memory, ordered on the ops stream.
+ * Kinds CS3D_FILTER_T, CS3D_DERIV1_T, CS3D_DERIV2_T apply the transposed
+ * operator (out = M^T in for the operator's matrix M along each line: the
+ * product the discrete adjoint needs, e.g. <D f, g> = <f, D^T g>; no
+ * reference counterpart).  They need in != out and, on non-periodic lines,
+ * at least 18 points (24 for the filter).
  */
 int cs3d_ops_create(int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int bc_type,
                     int dir, double alpha_f, int precision, cs3d_ops **out);
