@@ -4,10 +4,13 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the GPU
 adjoint.  The product path never imports this module.
 
 The reference has no adjoint.  This module is the reverse of the oracle's
-own stage map (oracle.c pre_step_derivatives / solve_eqn_set, which restate
-CSolver_AWS.cpp:829-1504 on periodic lines without sponges) and of its RK4
-and TVDRK3 steps with the filter (RK4.hpp:35-118, TVDRK3.hpp:38-96,
-CSolver_AWS.cpp:1682-1893), written independently of the GPU kernels:
+own stage map (oracle.c pre_step_derivatives / solve_eqn_set /
+post_step_bc, which restate CSolver_AWS.cpp:829-1670), of its RK4 and
+TVDRK3 steps with the filter (RK4.hpp:35-118, TVDRK3.hpp:38-96,
+CSolver_AWS.cpp:1682-1893) and of the sponge update (updateSponge,
+CSolver_AWS.cpp:1937-2031), on periodic or non-periodic lines, with or
+without sponge faces (walls are not covered), written independently of the
+GPU kernels:
 
 - every compact operator is the dense matrix of the oracle's own line
   operator (or_line_op, column by column), and its adjoint is that matrix
@@ -17,13 +20,23 @@ CSolver_AWS.cpp:1682-1893), written independently of the GPU kernels:
 - the pointwise right-hand side is held as lists of monomials in named
   intermediates (each reference statement expanded term by term), and the
   reverse sweep differentiates every monomial factor by factor;
-- nothing is taped: the reverse pass recomputes the forward intermediates.
+- nothing is taped: the reverse pass recomputes the forward intermediates;
+- sponges: the source sigma (A - q) (twice in the energy equation, Q3),
+  the zeroed right-hand side on the sponge face planes, the running
+  average A with its energy blend and the reset of the face planes to A
+  after each step, and the stale primitives: at the first stage after a
+  sponge update the x derivatives, x fluxes and mu of the reference come
+  from the primitive arrays of the filtered state before the reset (the
+  "X set" below), so their cotangents flow to that state, not to the
+  stage state.
 
 Pinned by tests/test_oracle_adjoint.py: the numpy forward stage and the RK4
-and TVDRK3 steps against the C oracle (1e-12), the dot-product identity
-against central differences of the forward stage, and the kinetic-energy
-gradient through either scheme against central differences of whole
-C-oracle runs with the dt held fixed.
+and TVDRK3 steps against the C oracle (1e-12; with sponges and non-periodic
+lines over several steps, averages included), the dot-product identity
+against central differences of the forward stage (in the stage state, the
+stale state and the averages), and the kinetic-energy gradient through
+either scheme against central differences of whole C-oracle runs with the
+dt held fixed, periodic and with sponges.
 """
 import numpy as np
 
@@ -111,17 +124,24 @@ MIXED = {"Uxy": ("Ux", 1), "Vxy": ("Vx", 1), "Uxz": ("Ux", 2), "Vyz": ("Vy", 2),
 DIR = {"x": 0, "y": 1, "z": 2, "X": 0, "Y": 1, "Z": 2}
 
 
+# the X set: the primitives the reference's x block reads (its primitive
+# arrays as they stand: those of the stage state, or at the first stage
+# after a sponge update those of the filtered state before the reset)
+XSET = {"U": "U_X", "V": "V_X", "W": "W_X", "p": "p_X", "T": "T_X"}
+
+
 class Adjoint:
     """Dense-operator forward and reverse of the oracle's stage map and of
-    its RK4 or TVDRK3 step (cfg.rk) on a periodic (Nz, Ny, Nx) grid.
-    `cfg` is a pyoracle config."""
+    its RK4 or TVDRK3 step (cfg.rk) on an (Nz, Ny, Nx) grid, periodic or
+    non-periodic per direction, with or without sponge faces.  `cfg` is a
+    pyoracle config."""
 
     def __init__(self, cfg):
         self.dims = (cfg.Nx, cfg.Ny, cfg.Nz)
         self.shape = (cfg.Nz, cfg.Ny, cfg.Nx)
-        for d in range(3):
-            if cfg.bc_type[d] != po.PERIODIC_SOLVE:
-                raise ValueError("the numpy adjoint covers periodic lines only")
+        kinds = [cfg.bc_kind[f] for f in range(6)]
+        if any(k in (po.ADIABATIC_WALL, po.MOVING_ADIABATIC_WALL) for k in kinds):
+            raise ValueError("the numpy adjoint does not cover walls")
         lengths = (cfg.Lx, cfg.Ly, cfg.Lz)
         self.rk = cfg.rk
         self.gamma = cfg.gamma
@@ -133,9 +153,29 @@ class Adjoint:
             n = self.dims[d]
             dh = lengths[d] / (n - 1.0)
             for kind in (0, 1, 2):  # filter, D1, D2
-                cols = [po.line_op(kind, po.PERIODIC_SOLVE, e, dh, cfg.alpha_f)
+                cols = [po.line_op(kind, cfg.bc_type[d], e, dh, cfg.alpha_f)
                         for e in np.eye(n)]
                 self.M[kind, d] = np.array(cols).T
+        # sponges (CSolver_AWS.cpp:1937-2031, SpongeBC.hpp): sigma from the
+        # oracle, the face planes, the averaging constants
+        self.sponge = any(k == po.SPONGE for k in kinds)
+        self.sigma = None
+        self.onplane = np.zeros(self.shape, dtype=bool)
+        if self.sponge:
+            o = po.Solver(cfg)
+            self.sigma = o.sponge_sigma().copy()
+            o.close()
+            for f in range(6):
+                if kinds[f] != po.SPONGE:
+                    continue
+                d, hi = f // 2, f % 2
+                idx = [slice(None)] * 3
+                idx[2 - d] = -1 if hi else 0
+                self.onplane[tuple(idx)] = True
+        self.avgT = cfg.sponge_avgT
+        self.epsP = cfg.sponge_epsP
+        self.spP = 1.0 / cfg.gamma
+        self.zmask = np.where(self.onplane, 0.0, 1.0)
 
     # ---------------------------------------------------------------- operators
     def op(self, kind, d, f, transpose=False):
@@ -153,65 +193,110 @@ class Adjoint:
         return out
 
     # ---------------------------------------------------------------- forward
-    def intermediates(self, q):
+    def prims(self, q):
         g = self.gamma
         rho, rU, rV, rW, rE = q
+        U, V, W = rU / rho, rV / rho, rW / rho
+        p = (g - 1) * (rE - 0.5 * rho * (U ** 2 + V ** 2 + W ** 2))
+        return {"U": U, "V": V, "W": W, "p": p, "T": p / (rho * self.R)}
+
+    def intermediates(self, q, qs=None):
+        """qs: the state of the X set (None: the stage state itself)"""
+        rho, rU, rV, rW, rE = q
         v = {"rho": rho, "rU": rU, "rV": rV, "rW": rW, "rE": rE}
-        v["U"], v["V"], v["W"] = rU / rho, rV / rho, rW / rho
-        v["p"] = (g - 1) * (rE - 0.5 * rho * (v["U"] ** 2 + v["V"] ** 2 + v["W"] ** 2))
-        v["T"] = v["p"] / (rho * self.R)
-        v["mu"] = self.mu_ref * v["T"] ** 0.76
-        v["amu"] = 0.76 * self.mu_ref * v["T"] ** (0.76 - 1.0)
+        v.update(self.prims(q))
+        for k, val in self.prims(q if qs is None else qs).items():
+            v[XSET[k]] = val
+        v["mu"] = self.mu_ref * v["T_X"] ** 0.76
+        v["amu"] = 0.76 * self.mu_ref * v["T_X"] ** (0.76 - 1.0)
         for f in FIRST:
-            v[f] = self.op(1, DIR[f[1]], v[f[0]])
+            v[f] = self.op(1, DIR[f[1]], v[self.src(f)])
         for f in SECOND:
-            v[f] = self.op(2, DIR[f[1]], v[f[0]])
+            v[f] = self.op(2, DIR[f[1]], v[self.src(f)])
         for m, (inner, d) in MIXED.items():
             v[m] = self.op(1, d, v[inner])
         for j in "xyz":
             v["Mu" + j.upper()] = v["amu"] * v["T" + j]
         for d, u in (("X", "U"), ("Y", "V"), ("Z", "W")):
-            vel = v[u]
+            vel, pr = self.flux_prims(v, d, u)
             v["cont" + d] = self.op(1, DIR[d], {"X": rU, "Y": rV, "Z": rW}[d])
-            fl = {"mx": rU * vel, "my": rV * vel, "mz": rW * vel, "e": rE * vel + vel * v["p"]}
-            fl[{"X": "mx", "Y": "my", "Z": "mz"}[d]] += v["p"]
+            fl = {"mx": rU * v[vel], "my": rV * v[vel], "mz": rW * v[vel],
+                  "e": rE * v[vel] + v[vel] * v[pr]}
+            fl[{"X": "mx", "Y": "my", "Z": "mz"}[d]] += v[pr]
             for k, t in fl.items():
                 v[k + d] = self.op(1, DIR[d], t)
         return v
 
+    @staticmethod
+    def src(f):
+        """the field a first or second derivative name differentiates: x
+        derivatives read the X set"""
+        return XSET[f[0]] if f[1] == "x" else f[0]
+
+    @staticmethod
+    def flux_prims(v, d, u):
+        """names of the velocity and pressure in the fluxes along d"""
+        return (XSET[u], "p_X") if d == "X" else (u, "p")
+
     def _coef(self, c):
         return self.cpPr if c == "cpPr" else c
 
-    def rhs(self, q, dt):
-        """dt * RHS(q), the oracle's k2 for stage state q"""
-        v = self.intermediates(q)
+    def rhs(self, q, dt, A=None, qs=None):
+        """dt * RHS(q) with the sponge source toward the averages A, zeroed
+        on the sponge face planes: the oracle's k2 for stage state q (qs: the
+        X set's state at the first stage after a sponge update)"""
+        v = self.intermediates(q, qs)
         out = []
-        for terms in RHS_TERMS:
+        for c, terms in enumerate(RHS_TERMS):
             acc = np.zeros(self.shape)
-            for c, fac in terms:
-                t = self._coef(c)
+            for coef, fac in terms:
+                t = self._coef(coef)
                 for f in fac:
                     t = t * v[f]
                 acc = acc + t
-            out.append(dt * acc)
+            if self.sponge:
+                spg = self.sigma * (A[c] - q[c])
+                acc = acc + (2.0 * spg if c == 4 else spg)
+            out.append(dt * acc * self.zmask)
         return out
 
     # ---------------------------------------------------------------- reverse
-    def rhs_vjp(self, q, lam, dt):
-        """(d(dt R)/dq)^T lam"""
+    def _chain(self, b, v, names, rho, out):
+        """cotangents of the primitives `names` (U V W p T) of a state with
+        density rho and velocities v[names U..W] into its conserved
+        cotangents out (dict)"""
         g = self.gamma
-        v = self.intermediates(q)
+        zero = np.zeros(self.shape)
+        nU, nV, nW, np_, nT = (names[k] for k in ("U", "V", "W", "p", "T"))
+
+        def add(k, val):
+            out[k] = out.get(k, 0.0) + val
+
+        Tb = b.get(nT, zero)
+        pb = b.get(np_, zero) + Tb / (rho * self.R)
+        add("rho", -Tb * v[nT] / rho)
+        add("rE", (g - 1) * pb)
+        add("rho", -(g - 1) * 0.5 * (v[nU] ** 2 + v[nV] ** 2 + v[nW] ** 2) * pb)
+        for u, m in ((nU, "rU"), (nV, "rV"), (nW, "rW")):
+            ub = b.get(u, zero) - (g - 1) * rho * v[u] * pb
+            add(m, ub / rho)
+            add("rho", -ub * v[u] / rho)
+
+    def rhs_vjp_full(self, q, lam, dt, A=None, qs=None):
+        """(d(dt R)/dq)^T lam, (d(dt R)/dqs)^T lam (None without qs) and
+        (d(dt R)/dA)^T lam (None without sponges)"""
+        v = self.intermediates(q, qs)
         b = {}
 
         def add(name, val):
             b[name] = b.get(name, 0.0) + val
 
+        w = [dt * self.zmask * x for x in lam]
         for c, terms in enumerate(RHS_TERMS):
-            w = dt * lam[c]
             for coef, fac in terms:
                 cf = self._coef(coef)
                 for k, f in enumerate(fac):
-                    t = cf * w
+                    t = cf * w[c]
                     for kk, ff in enumerate(fac):
                         if kk != k:
                             t = t * v[ff]
@@ -219,18 +304,18 @@ class Adjoint:
         zero = np.zeros(self.shape)
         for d, u in (("X", "U"), ("Y", "V"), ("Z", "W")):
             dd = DIR[d]
+            vel, pr = self.flux_prims(v, d, u)
             add({"X": "rU", "Y": "rV", "Z": "rW"}[d], self.op(1, dd, b.get("cont" + d, zero), True))
-            vel = v[u]
             for k, m in (("mx", "rU"), ("my", "rV"), ("mz", "rW")):
                 tb = self.op(1, dd, b.get(k + d, zero), True)
-                add(m, tb * vel)
-                add(u, tb * v[m])
+                add(m, tb * v[vel])
+                add(vel, tb * v[m])
                 if k == {"X": "mx", "Y": "my", "Z": "mz"}[d]:
-                    add("p", tb)
+                    add(pr, tb)
             tb = self.op(1, dd, b.get("e" + d, zero), True)
-            add("rE", tb * vel)
-            add(u, tb * (v["rE"] + v["p"]))
-            add("p", tb * vel)
+            add("rE", tb * v[vel])
+            add(vel, tb * (v["rE"] + v[pr]))
+            add(pr, tb * v[vel])
         for j in "xyz":
             mb = b.get("Mu" + j.upper(), zero)
             add("amu", mb * v["T" + j])
@@ -238,24 +323,67 @@ class Adjoint:
         for m, (inner, d) in MIXED.items():
             add(inner, self.op(1, d, b.get(m, zero), True))
         for f in SECOND:
-            add(f[0], self.op(2, DIR[f[1]], b.get(f, zero), True))
+            add(self.src(f), self.op(2, DIR[f[1]], b.get(f, zero), True))
         for f in FIRST:
-            add(f[0], self.op(1, DIR[f[1]], b.get(f, zero), True))
-        T = v["T"]
-        add("T", b.get("mu", zero) * 0.76 * v["mu"] / T)
-        add("T", b.get("amu", zero) * (0.76 - 1.0) * v["amu"] / T)
-        rho = v["rho"]
-        Tb = b.get("T", zero)
-        add("p", Tb / (rho * self.R))
-        add("rho", -Tb * T / rho)
-        pb = b.get("p", zero)
-        add("rE", (g - 1) * pb)
-        add("rho", -(g - 1) * 0.5 * (v["U"] ** 2 + v["V"] ** 2 + v["W"] ** 2) * pb)
-        for u, m in (("U", "rU"), ("V", "rV"), ("W", "rW")):
-            ub = b.get(u, zero) - (g - 1) * rho * v[u] * pb
-            add(m, ub / rho)
-            add("rho", -ub * v[u] / rho)
-        return [b.get(k, zero) for k in ("rho", "rU", "rV", "rW", "rE")]
+            add(self.src(f), self.op(1, DIR[f[1]], b.get(f, zero), True))
+        TX = v["T_X"]
+        add("T_X", b.get("mu", zero) * 0.76 * v["mu"] / TX)
+        add("T_X", b.get("amu", zero) * (0.76 - 1.0) * v["amu"] / TX)
+        # conserved terms, then the primitives of the stage state and of the
+        # X set's state
+        qb = {k: b.get(k, zero) for k in ("rho", "rU", "rV", "rW", "rE")}
+        self._chain(b, v, {k: k for k in XSET}, q[0], qb)
+        if qs is None:
+            self._chain(b, v, XSET, q[0], qb)
+            qsb = None
+        else:
+            d = {}
+            self._chain(b, v, XSET, qs[0], d)
+            qsb = [d.get(k, zero) for k in ("rho", "rU", "rV", "rW", "rE")]
+        out = [qb[k] for k in ("rho", "rU", "rV", "rW", "rE")]
+        Ab = None
+        if self.sponge:
+            Ab = []
+            for c in range(5):
+                s = (2.0 if c == 4 else 1.0) * self.sigma * w[c]
+                out[c] = out[c] - s
+                Ab.append(s)
+        return out, qsb, Ab
+
+    def rhs_vjp(self, q, lam, dt, A=None, qs=None):
+        """(d(dt R)/dq)^T lam"""
+        return self.rhs_vjp_full(q, lam, dt, A, qs)[0]
+
+    # ---------------------------------------------------------------- sponge
+    def sponge_update(self, qf, A, dt):
+        """updateSponge (CSolver_AWS.cpp:1937-2031) after the filtered state
+        qf: the averages relax toward qf with eps = 1/(avgT/dt + 1), the
+        energy average blends toward the sponge pressure 1/gamma at the
+        averages' density and momenta, and the face planes of the state are
+        reset to the averages.  Returns (new state, new averages)."""
+        eps = 1.0 / (self.avgT / dt + 1.0)
+        a = [x + eps * (y - x) for x, y in zip(A, qf)]
+        a[4] = self.epsP * a[4] + (1.0 - self.epsP) * (
+            self.spP / (self.gamma - 1.0) + 0.5 * (a[1] ** 2 + a[2] ** 2 + a[3] ** 2) / a[0])
+        qn = [np.where(self.onplane, x, y) for x, y in zip(a, qf)]
+        return qn, a
+
+    def sponge_update_vjp(self, qf, A, dt, lq, lA):
+        """cotangents of (qf, A) from those of the new state lq and the new
+        averages lA"""
+        eps = 1.0 / (self.avgT / dt + 1.0)
+        a = [x + eps * (y - x) for x, y in zip(A, qf)]
+        lqf = [np.where(self.onplane, 0.0, x) for x in lq]
+        la = [y + np.where(self.onplane, x, 0.0) for x, y in zip(lq, lA)]
+        l4 = la[4]
+        c = (1.0 - self.epsP) * l4
+        la[4] = self.epsP * l4
+        for m in (1, 2, 3):
+            la[m] = la[m] + c * a[m] / a[0]
+        la[0] = la[0] - c * 0.5 * (a[1] ** 2 + a[2] ** 2 + a[3] ** 2) / a[0] ** 2
+        lAo = [(1.0 - eps) * x for x in la]
+        lqf = [x + eps * y for x, y in zip(lqf, la)]
+        return lqf, lAo
 
     # ---------------------------------------------------------------- RK4
     @staticmethod
@@ -263,40 +391,69 @@ class Adjoint:
         m = count % 3
         return (0, 1, 2) if m == 1 else (1, 2, 0) if m == 2 else (2, 0, 1)
 
-    def rk4_step(self, q, dt, order):
+    def rk4_step(self, q, dt, order, A=None, qs=None):
         """one RK4 step (RK4.hpp:35-118) followed by the filter in `order`
-        (None: no filter)"""
-        k1 = self.rhs(q, dt)
-        k2 = self.rhs([a + 0.5 * k for a, k in zip(q, k1)], dt)
-        k3 = self.rhs([a + 0.5 * k for a, k in zip(q, k2)], dt)
-        k4 = self.rhs([a + k for a, k in zip(q, k3)], dt)
+        (None: no filter); the filtered state before any sponge update"""
+        k1 = self.rhs(q, dt, A, qs)
+        k2 = self.rhs([a + 0.5 * k for a, k in zip(q, k1)], dt, A)
+        k3 = self.rhs([a + 0.5 * k for a, k in zip(q, k2)], dt, A)
+        k4 = self.rhs([a + k for a, k in zip(q, k3)], dt, A)
         acc = [a + k1_ / 6 + k2_ / 3 + k3_ / 3 + k4_ / 6
                for a, k1_, k2_, k3_, k4_ in zip(q, k1, k2, k3, k4)]
         return self.filt(acc, order) if order is not None else acc
 
-    def tvdrk3_step(self, q, dt, order):
+    def tvdrk3_step(self, q, dt, order, A=None, qs=None):
         """one TVDRK3 step (TVDRK3.hpp:38-96: Q2 = q + k1, Q3 = 3/4 q +
         1/4 (Q2 + k2), q' = 1/3 q + 2/3 (Q3 + k3)) followed by the filter in
         `order` (None: no filter)"""
-        k1 = self.rhs(q, dt)
+        k1 = self.rhs(q, dt, A, qs)
         Q2 = [a + k for a, k in zip(q, k1)]
-        k2 = self.rhs(Q2, dt)
+        k2 = self.rhs(Q2, dt, A)
         Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)]
-        k3 = self.rhs(Q3, dt)
+        k3 = self.rhs(Q3, dt, A)
         acc = [a / 3 + 2.0 / 3.0 * b + 2.0 / 3.0 * k for a, b, k in zip(q, Q3, k3)]
         return self.filt(acc, order) if order is not None else acc
 
-    def step(self, q, dt, order):
-        return self.rk4_step(q, dt, order) if self.rk == 4 else self.tvdrk3_step(q, dt, order)
+    def step(self, q, dt, order, A=None, qs=None):
+        if self.rk == 4:
+            return self.rk4_step(q, dt, order, A, qs)
+        return self.tvdrk3_step(q, dt, order, A, qs)
+
+    def step_vjp_full(self, q, dt, order, lam, A=None, qs=None):
+        """cotangents of (q, qs, A) of a step (before its sponge update)
+        from that of its filtered result"""
+        f = self.rk4_step_vjp if self.rk == 4 else self.tvdrk3_step_vjp
+        return f(q, dt, order, lam, A, qs)
 
     def step_vjp(self, q, dt, order, lam):
-        if self.rk == 4:
-            return self.rk4_step_vjp(q, dt, order, lam)
-        return self.tvdrk3_step_vjp(q, dt, order, lam)
+        return self.step_vjp_full(q, dt, order, lam)[0]
 
     def conserved_from_ic(self, rho, U, V, W, p):
         g = self.gamma
         return [rho, rho * U, rho * V, rho * W, p / (g - 1) + 0.5 * rho * (U * U + V * V + W * W)]
+
+    def run(self, q, nsteps, dt, filter_step=1):
+        """the forward map of nsteps steps from state q (the averages start
+        from q, as set_initial_conditions sets them); returns the final
+        state, the final averages and the per-step records (state,
+        averages, X-set state, filter order, filtered state)"""
+        A = [x.copy() for x in q] if self.sponge else None
+        qs = None
+        recs = []
+        fcount = 0
+        for n in range(1, nsteps + 1):
+            order = None
+            if n % filter_step == 0:
+                fcount += 1
+                order = self.filter_order(fcount)
+            qf = self.step(q, dt, order, A, qs)
+            recs.append((q, A, qs, order, qf))
+            if self.sponge:
+                q, A = self.sponge_update(qf, A, dt)
+                qs = qf
+            else:
+                q = qf
+        return q, A, recs
 
     def ke_gradient(self, ic, nsteps, dt, filter_step=1):
         """J = <rho |u|^2 / 2> after nsteps steps of the configured scheme
@@ -304,63 +461,80 @@ class Adjoint:
         steps; returns J, dJ/dU0, dJ/dV0, dJ/dW0 at fixed initial density
         and pressure"""
         rho0, U0, V0, W0, p0 = [np.asarray(a, dtype=np.float64) for a in ic]
-        q = self.conserved_from_ic(rho0, U0, V0, W0, p0)
-        states, orders = [], []
-        fcount = 0
-        for n in range(1, nsteps + 1):
-            states.append(q)
-            order = None
-            if n % filter_step == 0:
-                fcount += 1
-                order = self.filter_order(fcount)
-            orders.append(order)
-            q = self.step(q, dt, order)
+        q0 = self.conserved_from_ic(rho0, U0, V0, W0, p0)
+        q, A, recs = self.run(q0, nsteps, dt, filter_step)
         N = q[0].size
         rho = q[0]
         u = [q[i] / rho for i in (1, 2, 3)]
         J = float(np.sum(0.5 * rho * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)) / N)
+        zero = np.zeros(self.shape)
         lam = [-0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2) / N, u[0] / N, u[1] / N, u[2] / N,
-               np.zeros(self.shape)]
+               zero]
+        lA = [zero] * 5 if self.sponge else None
+        lqs = None  # cotangent of a step's filtered state from the next step's X set
         for n in range(nsteps - 1, -1, -1):
-            lam = self.step_vjp(states[n], dt, orders[n], lam)
+            qn, An, qsn, order, qf = recs[n]
+            if self.sponge:
+                lam, lA = self.sponge_update_vjp(qf, An, dt, lam, lA)
+                if lqs is not None:
+                    lam = [a + b for a, b in zip(lam, lqs)]
+            lam, lqs, lAs = self.step_vjp_full(qn, dt, order, lam, An, qsn)
+            if lAs is not None:
+                lA = [a + b for a, b in zip(lA, lAs)]
+        if self.sponge:
+            lam = [a + b for a, b in zip(lam, lA)]  # the averages start as q0
         # m_i = rho U_i, E = p/(g-1) + rho |U|^2 / 2 at fixed rho and p
         grads = [rho0 * (lam[i + 1] + (U0, V0, W0)[i] * lam[4]) for i in range(3)]
         return J, grads[0], grads[1], grads[2]
 
-    def rk4_step_vjp(self, q, dt, order, lam):
-        """cotangent of the step's start state from that of its end state;
-        with Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3 and the filtered
-        acc = q + k1/6 + k2/3 + k3/3 + k4/6"""
-        k1 = self.rhs(q, dt)
+    @staticmethod
+    def _acc(x, y):
+        if y is None:
+            return x
+        return y if x is None else [a + b for a, b in zip(x, y)]
+
+    def rk4_step_vjp(self, q, dt, order, lam, A=None, qs=None):
+        """cotangents of the step's start state (and of qs, A) from that of
+        its filtered result; with Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3
+        and the filtered acc = q + k1/6 + k2/3 + k3/3 + k4/6"""
+        k1 = self.rhs(q, dt, A, qs)
         Q2 = [a + 0.5 * k for a, k in zip(q, k1)]
-        k2 = self.rhs(Q2, dt)
+        k2 = self.rhs(Q2, dt, A)
         Q3 = [a + 0.5 * k for a, k in zip(q, k2)]
-        k3 = self.rhs(Q3, dt)
+        k3 = self.rhs(Q3, dt, A)
         Q4 = [a + k for a, k in zip(q, k3)]
         la = self.filt(lam, order, transpose=True) if order is not None else list(lam)
         qb = list(la)
-        Q4b = self.rhs_vjp(Q4, [x / 6 for x in la], dt)
+        Q4b, _, Ab = self.rhs_vjp_full(Q4, [x / 6 for x in la], dt, A)
         qb = [a + c for a, c in zip(qb, Q4b)]
-        Q3b = self.rhs_vjp(Q3, [x / 3 + c for x, c in zip(la, Q4b)], dt)
+        Q3b, _, Ab3 = self.rhs_vjp_full(Q3, [x / 3 + c for x, c in zip(la, Q4b)], dt, A)
         qb = [a + c for a, c in zip(qb, Q3b)]
-        Q2b = self.rhs_vjp(Q2, [x / 3 + 0.5 * c for x, c in zip(la, Q3b)], dt)
+        Q2b, _, Ab2 = self.rhs_vjp_full(Q2, [x / 3 + 0.5 * c for x, c in zip(la, Q3b)], dt, A)
         qb = [a + c for a, c in zip(qb, Q2b)]
-        Q1b = self.rhs_vjp(q, [x / 6 + 0.5 * c for x, c in zip(la, Q2b)], dt)
-        return [a + c for a, c in zip(qb, Q1b)]
+        Q1b, qsb, Ab1 = self.rhs_vjp_full(q, [x / 6 + 0.5 * c for x, c in zip(la, Q2b)], dt, A,
+                                          qs)
+        for x in (Ab3, Ab2, Ab1):
+            Ab = self._acc(Ab, x)
+        return [a + c for a, c in zip(qb, Q1b)], qsb, Ab
 
-    def tvdrk3_step_vjp(self, q, dt, order, lam):
-        """cotangent of the step's start state from that of its end state
-        for tvdrk3_step: with la the cotangent of the unfiltered result,
-        Q3b = 2/3 la + J3^T (2/3 la), Q2b = 1/4 Q3b + J2^T (1/4 Q3b),
-        qb = 1/3 la + 3/4 Q3b + Q2b + J1^T Q2b (J_s^T: rhs_vjp at Q_s)"""
-        k1 = self.rhs(q, dt)
+    def tvdrk3_step_vjp(self, q, dt, order, lam, A=None, qs=None):
+        """cotangents of the step's start state (and of qs, A) from that of
+        its filtered result for tvdrk3_step: with la the cotangent of the
+        unfiltered result, Q3b = 2/3 la + J3^T (2/3 la), Q2b = 1/4 Q3b +
+        J2^T (1/4 Q3b), qb = 1/3 la + 3/4 Q3b + Q2b + J1^T Q2b (J_s^T:
+        rhs_vjp at Q_s)"""
+        k1 = self.rhs(q, dt, A, qs)
         Q2 = [a + k for a, k in zip(q, k1)]
-        k2 = self.rhs(Q2, dt)
+        k2 = self.rhs(Q2, dt, A)
         Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)]
         la = self.filt(lam, order, transpose=True) if order is not None else list(lam)
         t3 = [2.0 / 3.0 * x for x in la]
-        Q3b = [a + c for a, c in zip(t3, self.rhs_vjp(Q3, t3, dt))]
+        J3, _, Ab = self.rhs_vjp_full(Q3, t3, dt, A)
+        Q3b = [a + c for a, c in zip(t3, J3)]
         t2 = [0.25 * x for x in Q3b]
-        Q2b = [a + c for a, c in zip(t2, self.rhs_vjp(Q2, t2, dt))]
-        J1 = self.rhs_vjp(q, Q2b, dt)
-        return [x / 3 + 0.75 * a + b + c for x, a, b, c in zip(la, Q3b, Q2b, J1)]
+        J2, _, Ab2 = self.rhs_vjp_full(Q2, t2, dt, A)
+        Q2b = [a + c for a, c in zip(t2, J2)]
+        J1, qsb, Ab1 = self.rhs_vjp_full(q, Q2b, dt, A, qs)
+        for x in (Ab2, Ab1):
+            Ab = self._acc(Ab, x)
+        return [x / 3 + 0.75 * a + b + c for x, a, b, c in zip(la, Q3b, Q2b, J1)], qsb, Ab
