@@ -765,6 +765,316 @@ hipError_t launch_adj_to_velocity(const QSet<real> &q, const QSet<real> &qb, con
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ general path
+//
+// Stage products on any boundary set the forward covers without walls
+// (non-periodic lines with their closures, sponge faces): the derivative
+// fields of the stage state are recomputed with the standalone operators,
+// the seeds formed point by point, every transposed term applied with the
+// transposed operators (adjoint.hpp: GenAdj), and the chain to the
+// conserved state taken point by point with the sponge terms and the RK
+// epilogue.
+
+// whether flat point i lies on a sponge face plane
+__device__ __forceinline__ bool on_sponge_plane(const StageArgs &sa, long i)
+{
+    const int ii = (int)(i % sa.Nx), jj = (int)((i / sa.Nx) % sa.Ny),
+              kk = (int)(i / ((long)sa.Nx * sa.Ny));
+    return (sa.sponge_face[0] && ii == 0) || (sa.sponge_face[1] && ii == sa.Nx - 1) ||
+           (sa.sponge_face[2] && jj == 0) || (sa.sponge_face[3] && jj == sa.Ny - 1) ||
+           (sa.sponge_face[4] && kk == 0) || (sa.sponge_face[5] && kk == sa.Nz - 1);
+}
+
+// the state the x block's primitives come from at point i: the stage state,
+// or at the first stage after a sponge update the pre-reset values on the
+// sponge planes
+template <typename real>
+__device__ __forceinline__ void xset_state(const GenAdj<real> &g, long i, const real (&q)[5],
+                                           real (&qx)[5], bool &split)
+{
+    split = g.stale && on_sponge_plane(g.sa, i);
+#pragma unroll
+    for (int k = 0; k < 5; k++) qx[k] = q[k];
+    if (split) qset_load(g.qstale, i, qx);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_gadj_prims(GenAdj<real> g, Phys<real> ph, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real q[5], qx[5], ir, pp;
+        bool split;
+        qset_load(g.q, i, q);
+        xset_state(g, i, q, qx, split);
+        prim(ph, q[0], q[1], q[2], q[3], q[4], ir, g.pY[0][i], g.pY[1][i], g.pY[2][i], pp,
+             g.pY[3][i]);
+        if (g.pX[0] != g.pY[0])
+            prim(ph, qx[0], qx[1], qx[2], qx[3], qx[4], ir, g.pX[0][i], g.pX[1][i], g.pX[2][i],
+                 pp, g.pX[3][i]);
+    }
+}
+
+// Seeds of every derivative field (seed_point), the scaled and masked stage
+// cotangent, and the starting values of the primitive cotangents (the
+// pointwise terms; the transposed operators add the rest).
+template <typename real>
+__global__ __launch_bounds__(256) void k_gadj_seeds(GenAdj<real> g, Phys<real> ph, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real q[5], v[5];
+        qset_load(g.q, i, q);
+        qset_load(g.v, i, v);
+        const real mask = on_sponge_plane(g.sa, i) ? real(0) : g.cv;
+        SeedPt<real> p;
+        real ir, pp, Tp;
+        prim(ph, q[0], q[1], q[2], q[3], q[4], ir, p.U, p.V, p.W, pp, Tp);
+        p.Tp = g.pX[3][i];
+        visc(ph, ph.T0s + p.Tp, p.mu, p.a);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            v[k] *= mask;
+            g.vs[k][i] = v[k];
+        }
+        p.v1 = v[1];
+        p.v2 = v[2];
+        p.v3 = v[3];
+        p.vE = v[4];
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            p.d1[k] = g.d1[k][i];
+            p.d2[k] = g.d2[k][i];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) p.mx[k] = g.mx[k][i];
+        real o[SD_COUNT];
+        seed_point(ph, p, o);
+#pragma unroll
+        for (int k = 0; k < 3; k++) g.s[k][i] = o[SD_S + k];
+        g.s[3][i] = o[SD_GQ];
+#pragma unroll
+        for (int k = 0; k < G1_COUNT; k++) g.g1[k][i] = o[SD_G1 + k];
+        g.bY[0][i] = o[SD_UB];
+        g.bY[1][i] = o[SD_VB];
+        g.bY[2][i] = o[SD_WB];
+        g.bY[3][i] = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) g.bX[k][i] = 0;
+        g.bX[3][i] = o[SD_TB];
+    }
+}
+
+// Primitive cotangents (U, V, W, T) of one set into its conserved
+// cotangents: T = p / (rho R), p = (gamma - 1)(E - |m|^2 / (2 rho)) as a
+// function of the conserved values, u = m / rho.  ub and pb carry the flux
+// parts already.
+template <typename real>
+__device__ __forceinline__ void chain_prims(const Phys<real> &ph, const real (&q)[5],
+                                            real (&ub)[3], real pb, real Tb, real (&res)[5])
+{
+    const real rho = ph.rho_ref + q[0];
+    const real ir = real(1) / rho;
+    const real u[3] = {q[1] * ir, q[2] * ir, q[3] * ir};
+    const real pp = ph.gm1 * (q[4] - real(0.5) * (q[1] * u[0] + q[2] * u[1] + q[3] * u[2]));
+    const real T = ph.T0s + (pp - q[0] * ph.RT0s) * ir * ph.iR;
+    pb += Tb * ir * ph.iR;
+    res[0] -= Tb * T * ir;
+    res[4] += ph.gm1 * pb;
+    res[0] += ph.gm1 * real(0.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * pb;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        res[1 + k] += ub[k] * ir - ph.gm1 * u[k] * pb;
+        res[0] -= ub[k] * u[k] * ir;
+    }
+}
+
+// The chain to the conserved state: the flux cotangents F_d[c] of direction
+// d (velocity and pressure of the x block's set along x, of the stage state
+// along y and z; the conserved factors always of the stage state), the
+// primitive cotangents of both sets, the sponge terms; then the epilogue.
+template <typename real>
+__global__ __launch_bounds__(256) void k_gadj_chain(GenAdj<real> g, AdjVjp<real> a, Phys<real> ph, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real q[5], qx[5];
+        bool split;
+        qset_load(g.q, i, q);
+        xset_state(g, i, q, qx, split);
+        real res[5] = {0, 0, 0, 0, 0}, rsx[5] = {0, 0, 0, 0, 0};
+        real ub[3] = {g.bY[0][i], g.bY[1][i], g.bY[2][i]}, pb = 0;
+        real ubx[3] = {g.bX[0][i], g.bX[1][i], g.bX[2][i]}, pbx = 0;
+        const real rho = ph.rho_ref + q[0], ir = real(1) / rho;
+        const real E = ph.rhoE_ref + q[4];
+        const real m[3] = {q[1], q[2], q[3]};
+        const real rhox = ph.rho_ref + qx[0], irx = real(1) / rhox;
+        const real pY = ph.p_ref + ph.gm1 * (q[4] - real(0.5) * (m[0] * m[0] + m[1] * m[1] +
+                                                                m[2] * m[2]) * ir);
+        const real pX = ph.p_ref + ph.gm1 * (qx[4] - real(0.5) * (qx[1] * qx[1] + qx[2] * qx[2] +
+                                                                 qx[3] * qx[3]) * irx);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            real Fb[5];
+#pragma unroll
+            for (int c = 0; c < 5; c++) Fb[c] = g.F[d][c][i];
+            const real ud = d == 0 ? qx[1] * irx : m[d] * ir;
+            const real p = d == 0 ? pX : pY;
+            real &ubd = d == 0 ? ubx[0] : ub[d];
+            real &pbd = d == 0 ? pbx : pb;
+            res[1 + d] += Fb[0];  // continuity flux m_d
+#pragma unroll
+            for (int k = 0; k < 3; k++) {  // momentum fluxes m_k u_d + p d_kd
+                res[1 + k] += Fb[1 + k] * ud;
+                ubd += Fb[1 + k] * m[k];
+            }
+            pbd += Fb[1 + d];
+            res[4] += Fb[4] * ud;  // energy flux (E + p) u_d
+            pbd += Fb[4] * ud;
+            ubd += Fb[4] * (E + p);
+        }
+        chain_prims(ph, q, ub, pb, g.bY[3][i], res);
+        chain_prims(ph, qx, ubx, pbx, g.bX[3][i], rsx);
+        if (!split) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                res[k] += rsx[k];
+                rsx[k] = 0;
+            }
+        }
+        if (g.stale) qset_store(g.res_stale, i, rsx);
+        if (g.sigma) {
+            real w[5], ra[5];
+            qset_load(g.v, i, w);
+            const real mask = on_sponge_plane(g.sa, i) ? real(0) : g.cv;
+            qset_load(g.resA, i, ra);
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const real sp = (k == 4 ? real(2) : real(1)) * g.sigma[i] * mask * w[k];
+                res[k] -= sp;
+                ra[k] += sp;
+            }
+            qset_store(g.resA, i, ra);
+        }
+        if (a.out.m) qset_store(a.out, i, res);
+        if (a.acc.m) {
+            real b[5] = {0, 0, 0, 0, 0};
+            if (a.acc_base.m) qset_load(a.acc_base, i, b);
+#pragma unroll
+            for (int k = 0; k < 5; k++) b[k] += res[k];
+            qset_store(a.acc, i, b);
+        }
+        if (a.vn.m) {
+            real l[5];
+            qset_load(a.la, i, l);
+#pragma unroll
+            for (int k = 0; k < 5; k++) l[k] = a.ca * l[k] + a.cb * res[k];
+            qset_store(a.vn, i, l);
+        }
+    }
+}
+
+// Reverse of updateSponge (CSolver_AWS.cpp:1937-2031) after a step whose
+// filtered state was qf and whose averages were A: lq (cotangent of the
+// new state, in place) becomes that of qf, lA (in place) that of A.  The
+// planes' values came from the new averages; a = A + eps (qf - A) before
+// the energy blend a4' = epsP a4 + (1 - epsP)(pS/(gamma - 1) + |a_m|^2 /
+// (2 a_rho)).  ls (null: none): the stale cotangent of qf from the next
+// step's first stage.
+template <typename real>
+__global__ __launch_bounds__(256) void k_gadj_sponge_post(QSet<real> qf, QSet<real> A, QSet<real> lq, QSet<real> lA,
+                                   QSet<real> ls, StageArgs sa, Phys<real> ph, real eps,
+                                   real epsP, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real f[5], av[5], l[5], la[5];
+        qset_load(qf, i, f);
+        qset_load(A, i, av);
+        qset_load(lq, i, l);
+        qset_load(lA, i, la);
+        const bool plane = on_sponge_plane(sa, i);
+        real a[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            a[k] = av[k] + eps * (f[k] - av[k]);
+            if (plane) {
+                la[k] += l[k];
+                l[k] = 0;
+            }
+        }
+        const real rho = ph.rho_ref + a[0];
+        const real c = (real(1) - epsP) * la[4];
+        la[4] = epsP * la[4];
+#pragma unroll
+        for (int k = 1; k < 4; k++) la[k] += c * a[k] / rho;
+        la[0] -= c * real(0.5) * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / (rho * rho);
+        if (ls.m) {
+            real s[5];
+            qset_load(ls, i, s);
+#pragma unroll
+            for (int k = 0; k < 5; k++) l[k] += s[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            l[k] += eps * la[k];
+            la[k] *= real(1) - eps;
+        }
+        qset_store(lq, i, l);
+        qset_store(lA, i, la);
+    }
+}
+
+// b += a over a conserved set
+template <typename real>
+__global__ __launch_bounds__(256) void k_gadj_add(QSet<real> b, QSet<real> a, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real x[5], y[5];
+        qset_load(b, i, x);
+        qset_load(a, i, y);
+#pragma unroll
+        for (int k = 0; k < 5; k++) x[k] += y[k];
+        qset_store(b, i, x);
+    }
+}
+
+template <typename real>
+hipError_t launch_gadj(int which, const GenAdj<real> &g, const AdjVjp<real> &a,
+                       const Phys<real> &ph, long N, hipStream_t st)
+{
+    const dim3 grid = grid_for(N), block(256);
+    switch (which) {
+    case GADJ_PRIMS: hipLaunchKernelGGL((k_gadj_prims<real>), grid, block, 0, st, g, ph, N); break;
+    case GADJ_SEEDS: hipLaunchKernelGGL((k_gadj_seeds<real>), grid, block, 0, st, g, ph, N); break;
+    case GADJ_CHAIN:
+        hipLaunchKernelGGL((k_gadj_chain<real>), grid, block, 0, st, g, a, ph, N);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_gadj_sponge_post(const QSet<real> &qf, const QSet<real> &A,
+                                   const QSet<real> &lq, const QSet<real> &lA,
+                                   const QSet<real> &ls, const StageArgs &sa,
+                                   const Phys<real> &ph, double eps, double epsP, long N,
+                                   hipStream_t st)
+{
+    hipLaunchKernelGGL((k_gadj_sponge_post<real>), grid_for(N), dim3(256), 0, st, qf, A, lq, lA,
+                       ls, sa, ph, (real)eps, (real)epsP, N);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_gadj_add(const QSet<real> &b, const QSet<real> &a, long N, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_gadj_add<real>), grid_for(N), dim3(256), 0, st, b, a, N);
+    return hipGetLastError();
+}
+
 #define CS3D_INST_ADJ(real)                                                                       \
     template hipError_t launch_adj_pass<real>(int, const OpDev<real> *, const OpDev<real> *, int,  \
                                               int, const LaunchGeom &, const LineMap &,          \
@@ -774,7 +1084,15 @@ hipError_t launch_adj_to_velocity(const QSet<real> &q, const QSet<real> &qb, con
                                                  const QSet<real> &, long, hipStream_t);         \
     template hipError_t launch_adj_to_velocity<real>(const QSet<real> &, const QSet<real> &,       \
                                                      const Phys<real> &, real *, real *, real *, \
-                                                     long, hipStream_t);
+                                                     long, hipStream_t);                         \
+    template hipError_t launch_gadj<real>(int, const GenAdj<real> &, const AdjVjp<real> &,         \
+                                          const Phys<real> &, long, hipStream_t);                \
+    template hipError_t launch_gadj_sponge_post<real>(                                            \
+        const QSet<real> &, const QSet<real> &, const QSet<real> &, const QSet<real> &,          \
+        const QSet<real> &, const StageArgs &, const Phys<real> &, double, double, long,         \
+        hipStream_t);                                                                            \
+    template hipError_t launch_gadj_add<real>(const QSet<real> &, const QSet<real> &, long,        \
+                                              hipStream_t);
 CS3D_INST_ADJ(float)
 CS3D_INST_ADJ(double)
 

@@ -96,6 +96,46 @@ template <typename real>
 hipError_t launch_adj_pass(int pass, const OpDev<real> *d1, const OpDev<real> *d2, int uni, int M,
                            const LaunchGeom &g, const LineMap &lm, const AdjVjp<real> &a,
                            const Phys<real> &ph, hipStream_t st);
+// ---- general path (any boundary set without walls: non-periodic lines,
+// sponge faces).  One product: the primitives U, V, W, T' of the stage
+// state (pY) and of the x block's set (pX: the stage state, or at the first
+// stage after a sponge update the pre-reset values on the sponge planes;
+// pX may alias pY), their derivative fields with the standalone operators
+// (d1[3 f + j], d2[3 f + j] for f = U V W T and j = x y z, x derivatives of
+// the pX set; mx = Uxy Uxz Vxy Vyz Wxz Wyz), the seeds (s: s0 s1 s2 gq; g1
+// as G1_*), the masked and scaled stage cotangent vs, the primitive
+// cotangents of both sets (bY, bX: U V W T) and the flux cotangents F[d][c]
+// after the transposed operators, then the chain (res_stale: the x block's
+// share on the sponge planes at a stale stage; resA += the cotangent of the
+// sponge averages).
+enum { GADJ_PRIMS, GADJ_SEEDS, GADJ_CHAIN };
+template <typename real>
+struct GenAdj {
+    QSet<real> q, qstale, v;
+    StageArgs sa;
+    int stale;
+    real cv;
+    real *pY[4], *pX[4];
+    real *d1[12], *d2[12], *mx[6];
+    real *s[4], *g1[G1_COUNT], *vs[5];
+    real *bY[4], *bX[4];
+    real *F[3][5];
+    real *sigma;            // null without sponges
+    QSet<real> res_stale, resA;
+};
+template <typename real>
+hipError_t launch_gadj(int which, const GenAdj<real> &g, const AdjVjp<real> &a,
+                       const Phys<real> &ph, long N, hipStream_t st);
+// reverse of the sponge update of one step (adjoint.hip)
+template <typename real>
+hipError_t launch_gadj_sponge_post(const QSet<real> &qf, const QSet<real> &A,
+                                   const QSet<real> &lq, const QSet<real> &lA,
+                                   const QSet<real> &ls, const StageArgs &sa,
+                                   const Phys<real> &ph, double eps, double epsP, long N,
+                                   hipStream_t st);
+template <typename real>
+hipError_t launch_gadj_add(const QSet<real> &b, const QSet<real> &a, long N, hipStream_t st);
+
 template <typename real>
 hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,
                               long N, hipStream_t st);

@@ -261,6 +261,11 @@ struct SolverT : SolverBase {
             FL[d].release();
         }
         FLX.release();
+        for (int d = 0; d < 3; d++) {
+            D1T[d].release();
+            D2T[d].release();
+            FLT[d].release();
+        }
         for (void *p : adjalloc) (void)hipFree(p);
         if (mem) (void)hipFree(mem);
         if (dsc) (void)hipFree(dsc);
@@ -1019,63 +1024,219 @@ struct SolverT : SolverBase {
         return pass == ADJ_X ? 0 : (pass == ADJ_ZREC || pass == ADJ_Z) ? 2 : 1;
     }
 
+    // General path (non-periodic lines or sponge faces; CS3D_ADJ_GENERAL in
+    // the environment takes it on periodic grids too): the transposed
+    // operators' plans, the work fields of one product (GenAdj), the
+    // cotangents of the sponge averages and of the stale planes, and per
+    // step of a segment the state on the stale planes, the averages and the
+    // filtered state before the sponge update.
+    bool adj_general = false;
+    PlanDev<real> D1T[3], D2T[3], FLT[3];
+    GenAdj<real> gadj{};
+    QSet<real> gl_avg{}, gl_stale{};
+    const QSet<real> *gcur_S = nullptr;
+
     void adj_require()
     {
         need_ic();
-        for (int d = 0; d < 3; d++)
-            if (cfg.bc_type[d] != CS3D_PERIODIC_SOLVE)
-                throw std::invalid_argument("the adjoint needs periodic lines in every direction");
-        if (sponge) throw std::invalid_argument("the adjoint does not cover sponge zones");
+        bool general = sponge || std::getenv("CS3D_ADJ_GENERAL") != nullptr;
+        for (int d = 0; d < 3; d++) general |= cfg.bc_type[d] != CS3D_PERIODIC_SOLVE;
         if (cfg.compensated) throw std::invalid_argument("the adjoint does not cover the compensated mode");
         if (any_wall()) throw std::invalid_argument("the adjoint does not cover walls");
-        if (adj_ready) return;
-        for (int pass = ADJ_YREC; pass <= ADJ_Y; pass++) {
-            const int dir = adj_pass_dir(pass);
-            const OpHost &h1 = D1[dir].host, &h2 = D2[dir].host;
-            if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
-                throw std::logic_error("derivative plans of one direction differ in segmentation");
-            const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
-            LaunchGeom &g = adjG[pass];
-            if (pass == ADJ_X) {
-                g = line_geom(h1.P, kr, adj_x_threads(h1.P), ADJ_X_PLANES, sizeof(real));
-                while (g.LX > 0 && adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) > LDS_BYTES) g.LX--;
-                g.threads = g.LX * h1.P;
-                g.shm = g.LX > 0 ? adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) : 0;
-            }
-            else if (pass == ADJ_YREC || pass == ADJ_ZREC)
-                g = line_geom(h1.P, kr,
-                              env_cap("CS3D_ADJ_THREADS", adj_rec_threads<real>(h1.uniform != 0)),
-                              ADJ_R_PLANES, sizeof(real));
-            else
-                g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_col_threads<real>()),
-                              ADJ_T_PLANES, sizeof(real));
-            if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
+        if (adj_ready) {
+            if (general != adj_general) throw std::logic_error("adjoint path changed");
+            return;
         }
-        arec.dYut = adj_alloc_field(2);
-        arec.dYvw = adj_alloc_field(2);
-        arec.dY2 = adj_alloc_field(4);
-        arec.dZ1 = adj_alloc_field(4);
-        arec.dZ2 = adj_alloc_field(4);
-        arec.mYZ = adj_alloc_field(2);
-        aseed.sq = adj_alloc_field(4);
-        aseed.gz = adj_alloc_field(4);
-        aseed.gy = adj_alloc_field(4);
-        aseed.gs = adj_alloc_field(2);
-        aseed.mzz = adj_alloc_field(2);
-        aresx = adj_alloc_set();
-        aresxz = adj_alloc_set();
+        adj_general = general;
+        if (general) {
+            // the transposed operators address quad components with 32-bit
+            // byte offsets
+            if ((unsigned long long)N * 4 * sizeof(real) > 0xFFFFFFFFull)
+                throw std::invalid_argument("grid too large for the general adjoint path");
+            const int dims[3] = {Nx, Ny, Nz};
+            for (int d = 0; d < 3; d++) {
+                const int per = cfg.bc_type[d] == CS3D_PERIODIC_SOLVE;
+                make_plan(D1T[d], OP_D1, per, dims[d], dxyz[d], cfg.alpha_f, 0, true);
+                make_plan(D2T[d], OP_D2, per, dims[d], dxyz[d], cfg.alpha_f, 0, true);
+                make_plan(FLT[d], OP_FILTER, per, dims[d], dxyz[d], cfg.alpha_f, 0, true);
+            }
+            GenAdj<real> &g = gadj;
+            for (int k = 0; k < 4; k++) g.pY[k] = adj_alloc_field();
+            for (int k = 0; k < 4; k++) g.pX[k] = sponge ? adj_alloc_field() : g.pY[k];
+            for (int k = 0; k < 12; k++) g.d1[k] = adj_alloc_field();
+            for (int k = 0; k < 12; k++) g.d2[k] = adj_alloc_field();
+            for (int k = 0; k < 6; k++) g.mx[k] = adj_alloc_field();
+            for (int k = 0; k < 4; k++) g.s[k] = adj_alloc_field();
+            for (int k = 0; k < G1_COUNT; k++) g.g1[k] = adj_alloc_field();
+            for (int k = 0; k < 5; k++) g.vs[k] = adj_alloc_field();
+            for (int k = 0; k < 4; k++) g.bY[k] = adj_alloc_field();
+            for (int k = 0; k < 4; k++) g.bX[k] = adj_alloc_field();
+            for (int d = 0; d < 3; d++)
+                for (int c = 0; c < 5; c++) g.F[d][c] = adj_alloc_field();
+            g.sa = base;
+            g.sigma = sponge ? F.sigma : nullptr;
+            if (sponge) {
+                gl_avg = adj_alloc_set();
+                gl_stale = adj_alloc_set();
+            }
+        } else {
+            for (int pass = ADJ_YREC; pass <= ADJ_Y; pass++) {
+                const int dir = adj_pass_dir(pass);
+                const OpHost &h1 = D1[dir].host, &h2 = D2[dir].host;
+                if (h1.P != h2.P || h1.M != h2.M || h1.uniform != h2.uniform)
+                    throw std::logic_error("derivative plans of one direction differ in segmentation");
+                const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
+                LaunchGeom &g = adjG[pass];
+                if (pass == ADJ_X) {
+                    g = line_geom(h1.P, kr, adj_x_threads(h1.P), ADJ_X_PLANES, sizeof(real));
+                    while (g.LX > 0 && adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) > LDS_BYTES) g.LX--;
+                    g.threads = g.LX * h1.P;
+                    g.shm = g.LX > 0 ? adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) : 0;
+                }
+                else if (pass == ADJ_YREC || pass == ADJ_ZREC)
+                    g = line_geom(h1.P, kr,
+                                  env_cap("CS3D_ADJ_THREADS", adj_rec_threads<real>(h1.uniform != 0)),
+                                  ADJ_R_PLANES, sizeof(real));
+                else
+                    g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_col_threads<real>()),
+                                  ADJ_T_PLANES, sizeof(real));
+                if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
+            }
+            arec.dYut = adj_alloc_field(2);
+            arec.dYvw = adj_alloc_field(2);
+            arec.dY2 = adj_alloc_field(4);
+            arec.dZ1 = adj_alloc_field(4);
+            arec.dZ2 = adj_alloc_field(4);
+            arec.mYZ = adj_alloc_field(2);
+            aseed.sq = adj_alloc_field(4);
+            aseed.gz = adj_alloc_field(4);
+            aseed.gy = adj_alloc_field(4);
+            aseed.gs = adj_alloc_field(2);
+            aseed.mzz = adj_alloc_field(2);
+            aresx = adj_alloc_set();
+            aresxz = adj_alloc_set();
+        }
         QSet<real> *sets[6] = {&lamb, &la5, &sc5, &vb5, &vb5b, &qb5};
         for (auto p : sets) *p = adj_alloc_set();
         for (auto &p : gvel) p = adj_alloc_field();
         adj_ready = true;
     }
 
+    int map_of(int d) const { return d == 0 ? MAP_ROW : MAP_COL; }
+
+    // out (+)= coef op^T(in) along direction d (op: a transposed plan)
+    void opT(PlanDev<real> &pl, int d, const real *in, real *out, real coef, int acc)
+    {
+        hc(launch_field_opT<real>(pl, map_of(d), lm[d], in, out, coef, acc, st), "transposed operator");
+    }
+    void opF(PlanDev<real> &pl, int d, const real *in, real *out)
+    {
+        hc(launch_field_op<real>(pl, map_of(d), lm[d], in, out, st), "operator");
+    }
+
+    // The general-path product (adjoint.hpp: GenAdj): stale marks the first
+    // stage after a sponge update, whose x block read the stale planes
+    // gcur_S.  (The product needs no averages: the sponge source is linear
+    // in the state, with slope -sigma.)
+    void vjp_general(const QSet<real> &Q, const QSet<real> &v, const AdjVjp<real> &a, bool stale)
+    {
+        GenAdj<real> g = gadj;
+        g.q = Q;
+        g.v = v;
+        g.cv = a.cv;
+        g.stale = stale ? 1 : 0;
+        if (stale) {
+            if (!gcur_S) throw std::logic_error("adjoint: stale stage without its planes");
+            g.qstale = *gcur_S;
+            g.res_stale = gl_stale;
+        }
+        if (sponge) g.resA = gl_avg;
+        hc(launch_gadj<real>(GADJ_PRIMS, g, a, ph, N, st), "adjoint prims");
+        // derivative fields: x derivatives of the x block's set
+        for (int f = 0; f < 4; f++) {
+            opF(D1[0], 0, g.pX[f], g.d1[3 * f]);
+            opF(D2[0], 0, g.pX[f], g.d2[3 * f]);
+            for (int d = 1; d < 3; d++) {
+                opF(D1[d], d, g.pY[f], g.d1[3 * f + d]);
+                opF(D2[d], d, g.pY[f], g.d2[3 * f + d]);
+            }
+        }
+        // mixed: Uxy Uxz Vxy Vyz Wxz Wyz = D_y Ux, D_z Ux, D_y Vx, D_z Vy, D_z Wx, D_z Wy
+        opF(D1[1], 1, g.d1[0], g.mx[0]);
+        opF(D1[2], 2, g.d1[0], g.mx[1]);
+        opF(D1[1], 1, g.d1[3], g.mx[2]);
+        opF(D1[2], 2, g.d1[4], g.mx[3]);
+        opF(D1[2], 2, g.d1[6], g.mx[4]);
+        opF(D1[2], 2, g.d1[7], g.mx[5]);
+        hc(launch_gadj<real>(GADJ_SEEDS, g, a, ph, N, st), "adjoint seeds");
+        // transposed terms: z, then y, then x.  A shared first-derivative
+        // seed (g1XY for Uy and Vx, g1XZ for Uz and Wx, g1YZ for Vz and Wy)
+        // is used along its first direction before the mixed terms of its
+        // other use are added to it.
+        const real c13 = real(1) / real(3), c43 = real(4) / real(3);
+        real *const *G = g.g1;
+        real *s0 = g.s[0], *s1 = g.s[1], *s2 = g.s[2], *gq = g.s[3];
+        opT(D1T[2], 2, G[G1_XZ], g.bY[0], 1, 1);
+        opT(D1T[2], 2, G[G1_YZ], g.bY[1], 1, 1);
+        opT(D1T[2], 2, G[G1_WZ], g.bY[2], 1, 1);
+        opT(D1T[2], 2, G[G1_TZ], g.bY[3], 1, 1);
+        opT(D2T[2], 2, s0, g.bY[0], 1, 1);
+        opT(D2T[2], 2, s1, g.bY[1], 1, 1);
+        opT(D2T[2], 2, s2, g.bY[2], c43, 1);
+        opT(D2T[2], 2, gq, g.bY[3], 1, 1);
+        opT(D1T[2], 2, s2, G[G1_VY], c13, 1);  // Vyz
+        opT(D1T[2], 2, s2, G[G1_UX], c13, 1);  // Uxz
+        opT(D1T[2], 2, s0, G[G1_XZ], c13, 1);  // Wxz
+        opT(D1T[2], 2, s1, G[G1_YZ], c13, 1);  // Wyz
+        for (int c = 0; c < 5; c++) opT(D1T[2], 2, g.vs[c], g.F[2][c], -1, 0);
+        opT(D1T[1], 1, G[G1_XY], g.bY[0], 1, 1);
+        opT(D1T[1], 1, G[G1_VY], g.bY[1], 1, 1);
+        opT(D1T[1], 1, G[G1_YZ], g.bY[2], 1, 1);
+        opT(D1T[1], 1, G[G1_TY], g.bY[3], 1, 1);
+        opT(D2T[1], 1, s0, g.bY[0], 1, 1);
+        opT(D2T[1], 1, s1, g.bY[1], c43, 1);
+        opT(D2T[1], 1, s2, g.bY[2], 1, 1);
+        opT(D2T[1], 1, gq, g.bY[3], 1, 1);
+        opT(D1T[1], 1, s1, G[G1_UX], c13, 1);  // Uxy
+        opT(D1T[1], 1, s0, G[G1_XY], c13, 1);  // Vxy
+        for (int c = 0; c < 5; c++) opT(D1T[1], 1, g.vs[c], g.F[1][c], -1, 0);
+        opT(D1T[0], 0, G[G1_UX], g.bX[0], 1, 1);
+        opT(D1T[0], 0, G[G1_XY], g.bX[1], 1, 1);
+        opT(D1T[0], 0, G[G1_XZ], g.bX[2], 1, 1);
+        opT(D1T[0], 0, G[G1_TX], g.bX[3], 1, 1);
+        opT(D2T[0], 0, s0, g.bX[0], c43, 1);
+        opT(D2T[0], 0, s1, g.bX[1], 1, 1);
+        opT(D2T[0], 0, s2, g.bX[2], 1, 1);
+        opT(D2T[0], 0, gq, g.bX[3], 1, 1);
+        for (int c = 0; c < 5; c++) opT(D1T[0], 0, g.vs[c], g.F[0][c], -1, 0);
+        hc(launch_gadj<real>(GADJ_CHAIN, g, a, ph, N, st), "adjoint chain");
+    }
+
+    // F^T along d of every field of a set: the quads' components as strided
+    // lines (in != out)
+    void filter_set_T(int d, const QSet<real> &in, const QSet<real> &out)
+    {
+        LineMap l4 = lm[d];
+        l4.s1 *= 4;
+        l4.s2 *= 4;
+        l4.sp *= 4;
+        for (int k = 0; k < 4; k++)
+            hc(launch_field_opT<real>(FLT[d], map_of(d), l4, in.m + k, out.m + k, 1, 0, st),
+               "transposed filter");
+        opT(FLT[d], d, in.e, out.e, 1, 0);
+    }
+
     // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
     // with v = dt * lam given; a holds the epilogue (AdjVjp).
-    void vjp(const QSet<real> &Q, const QSet<real> &v, AdjVjp<real> a)
+    void vjp(const QSet<real> &Q, const QSet<real> &v, AdjVjp<real> a, bool stale = false)
     {
         if (a.vn.m && (a.vn.m == v.m || a.vn.e == v.e))
             throw std::logic_error("adjoint: next cotangent aliases v");
+        if (adj_general) {
+            vjp_general(Q, v, a, stale);
+            return;
+        }
+        if (stale) throw std::logic_error("adjoint: a stale stage on the periodic path");
         a.q = Q;
         a.v = v;
         a.rec = arec;
@@ -1117,6 +1278,12 @@ struct SolverT : SolverBase {
     {
         int o[3];
         filter_order(m, o, fx_ok);
+        if (adj_general) {
+            filter_set_T(o[2], in, out);
+            filter_set_T(o[1], out, scratch);
+            filter_set_T(o[0], scratch, out);
+            return;
+        }
         filter_sweep(o[2], in, out);
         filter_sweep(o[1], out, scratch);
         filter_sweep(o[0], scratch, out);
@@ -1131,19 +1298,25 @@ struct SolverT : SolverBase {
     struct StepRec {
         double dt;
         int timeStep, filterTimeStep, order;  // counters before the step; order -1: no filter
+        int stale;  // the first stage read the stale sponge planes
     };
 
-    StepRec fused_step(const StepRec *replay, const QSet<real> *stg)
+    // srec (sponges): receives the stale planes and the averages at the
+    // step's start and the filtered state before its sponge update
+    StepRec fused_step(const StepRec *replay, const QSet<real> *stg,
+                       const QSet<real> *srec = nullptr)
     {
         StepRec r;
         if (replay) {
             r = *replay;
             timeStep = r.timeStep + 1;
             filterTimeStep = r.filterTimeStep;
+            stale_pending = r.stale != 0;
             hc(launch_dt(dsc, cfg.cfl, r.dt, st), "dt");
         } else {
             r.timeStep = timeStep;
             r.filterTimeStep = filterTimeStep;
+            r.stale = stale_pending ? 1 : 0;
             pre_step();
             StepScalars hs;
             hc(hipMemcpyAsync(&hs, dsc, sizeof(hs), hipMemcpyDeviceToHost, st), "scalars");
@@ -1152,6 +1325,10 @@ struct SolverT : SolverBase {
             r.order = timeStep % cfg.filter_step == 0 ? (filterTimeStep + 1) % 3 : -1;
         }
         if (stg) copy_set(stg[0], F.q);
+        if (srec) {
+            copy_set(srec[0], F.qstale);
+            copy_set(srec[1], F.avg);
+        }
         // the last sweep of stage s < ns writes stage state s + 1 straight
         // into the store, and stage s + 1 reads it there
         const QSet<real> own = F.qk;
@@ -1162,6 +1339,7 @@ struct SolverT : SolverBase {
             if (stg && s < ns) F.qk = stg[s];
         }
         F.qk = F.qkw = own;
+        if (srec) copy_set(srec[2], F.q);
         post_step();
         return r;
     }
@@ -1206,7 +1384,7 @@ struct SolverT : SolverBase {
             l.cv = 1;
             l.acc = lam;
             l.acc_base = qb5;
-            vjp(stg[0], B, l);
+            vjp(stg[0], B, l, r.stale != 0);
             return;
         }
         f.cv = rdt / 6;  // stage 4: v = la, next A
@@ -1229,7 +1407,7 @@ struct SolverT : SolverBase {
         l.cv = 1;
         l.acc = lam;
         l.acc_base = qb5;
-        vjp(stg[0], A, l);
+        vjp(stg[0], A, l, r.stale != 0);
     }
 
     // five host fields (FP64) <-> a device set, no reference offsets
@@ -1274,14 +1452,29 @@ struct SolverT : SolverBase {
         AdjVjp<real> f{};
         f.cv = (real)h.dt;
         f.out = qb5;
-        vjp(set == 1 ? F.q : F.qk, la5, f);
+        // with sponges: the averages and the stale planes held fixed
+        const bool stale = sponge && set == 1 && stale_pending;
+        gcur_S = stale ? &F.qstale : nullptr;
+        if (sponge) zero_set(gl_avg);
+        vjp(set == 1 ? F.q : F.qk, la5, f, stale);
         download_set(qb5, out);
+    }
+
+    void zero_set(const QSet<real> &a)
+    {
+        hc(hipMemsetAsync(a.m, 0, 4 * (size_t)N * sizeof(real), st), "zero");
+        hc(hipMemsetAsync(a.e, 0, (size_t)N * sizeof(real), st), "zero");
     }
 
     void ke_gradient(int nsteps, int K, double *J, double *gU, double *gV, double *gW) override
     {
         if (nsteps < 1 || K < 1) throw std::invalid_argument("nsteps and checkpoint must be >= 1");
         adj_require();
+        // with sponges the averages at the start are the initial state
+        // itself (set_initial_conditions), so the gradient counts their
+        // cotangent as the state's
+        if (sponge && timeStep != 0)
+            throw std::invalid_argument("with sponges the gradient starts from set_initial_conditions");
         const int nseg = (nsteps + K - 1) / K, last0 = (nseg - 1) * K;
         // checkpoints, the stage states of one segment and the final state
         // come from a pool that persists across calls
@@ -1296,30 +1489,83 @@ struct SolverT : SolverBase {
         const QSet<real> fin = state();
         for (auto &c : ck) c = state();
         auto stages = [&](int i) { return (const QSet<real> *)&stg[ns * (size_t)i]; };
+        // sponges: per checkpoint the averages and the stale planes, per step
+        // of a segment the planes, the averages and the filtered state
+        // (fused_step), and the end state's averages and planes
+        std::vector<QSet<real>> ckA, ckS, srec;
+        QSet<real> finA{}, finS{};
+        if (sponge) {
+            ckA.resize(nseg);
+            ckS.resize(nseg);
+            srec.resize(3 * (size_t)K);
+            for (auto &c : ckA) c = state();
+            for (auto &c : ckS) c = state();
+            for (auto &c : srec) c = state();
+            finA = state();
+            finS = state();
+        }
+        auto sponge_rec = [&](int i) { return sponge ? (const QSet<real> *)&srec[3 * (size_t)i] : nullptr; };
+        auto save_ck = [&](int c) {
+            copy_set(ck[c], F.q);
+            if (sponge) {
+                copy_set(ckA[c], F.avg);
+                copy_set(ckS[c], F.qstale);
+            }
+        };
         std::vector<StepRec> rec(nsteps);
         // forward pass on the fused path: checkpoints, and the stage states
         // of the last segment
         for (int n = 0; n < nsteps; n++) {
-            if (n % K == 0) copy_set(ck[n / K], F.q);
-            rec[n] = fused_step(nullptr, n >= last0 ? stages(n - last0) : nullptr);
+            if (n % K == 0) save_ck(n / K);
+            const bool keep = n >= last0;
+            rec[n] = fused_step(nullptr, keep ? stages(n - last0) : nullptr,
+                                keep ? sponge_rec(n - last0) : nullptr);
         }
         StepScalars after;
         hc(hipMemcpyAsync(&after, dsc, sizeof(after), hipMemcpyDeviceToHost, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
         const int ts_end = timeStep, fts_end = filterTimeStep;
+        const bool stale_end = stale_pending;
         if (J) hc(kinetic_energy(J) == CS3D_OK ? hipSuccess : hipErrorUnknown, "kinetic energy");
         copy_set(fin, F.q);
+        if (sponge) {
+            copy_set(finA, F.avg);
+            copy_set(finS, F.qstale);
+            zero_set(gl_avg);
+        }
         // cotangent of the final state
         hc(launch_adj_ke_seed<real>(F.q, ph, lamb, N, st), "adjoint seed");
-        // reverse, segment by segment
+        // reverse, segment by segment; with sponges each step first reverses
+        // its sponge update (taking in the stale cotangent the next step's
+        // first stage left), then its stages
+        bool have_stale = false;
         for (int sgi = nseg - 1; sgi >= 0; sgi--) {
             const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
             if (sgi != nseg - 1) {
                 copy_set(F.q, ck[sgi]);
-                for (int n = n0; n < n1; n++) fused_step(&rec[n], stages(n - n0));
+                if (sponge) {
+                    copy_set(F.avg, ckA[sgi]);
+                    copy_set(F.qstale, ckS[sgi]);
+                }
+                for (int n = n0; n < n1; n++) fused_step(&rec[n], stages(n - n0), sponge_rec(n - n0));
             }
-            for (int n = n1 - 1; n >= n0; n--) adjoint_step(stages(n - n0), rec[n], lamb);
+            for (int n = n1 - 1; n >= n0; n--) {
+                if (sponge) {
+                    const QSet<real> *sr = sponge_rec(n - n0);
+                    const double eps = 1.0 / (cfg.sponge_avgT / rec[n].dt + 1.0);
+                    hc(launch_gadj_sponge_post<real>(sr[2], sr[1], lamb, gl_avg,
+                                                     have_stale ? gl_stale : QSet<real>{}, base,
+                                                     ph, eps, cfg.sponge_epsP, N, st),
+                       "sponge update reverse");
+                    gcur_S = &sr[0];
+                }
+                adjoint_step(stages(n - n0), rec[n], lamb);
+                have_stale = rec[n].stale != 0;
+            }
         }
+        if (have_stale) throw std::logic_error("adjoint: stale first stage at the start");
+        if (sponge) hc(launch_gadj_add<real>(lamb, gl_avg, N, st), "adjoint averages");
+        gcur_S = nullptr;
         // to the initial primitive velocities at fixed rho and p
         hc(launch_adj_to_velocity<real>(ck[0], lamb, ph, gvel[0], gvel[1], gvel[2], N, st),
            "velocity gradient");
@@ -1328,8 +1574,13 @@ struct SolverT : SolverBase {
         if (gW) download(gvel[2], gW);
         // back to the final state of the forward pass
         copy_set(F.q, fin);
+        if (sponge) {
+            copy_set(F.avg, finA);
+            copy_set(F.qstale, finS);
+        }
         timeStep = ts_end;
         filterTimeStep = fts_end;
+        stale_pending = stale_end;
         hc(hipMemcpyAsync(dsc, &after, sizeof(after), hipMemcpyHostToDevice, st), "scalars");
         hc(hipStreamSynchronize(st), "sync");
         diag_valid = false;  // the replays overwrote the step diagnostics
@@ -1341,6 +1592,11 @@ struct SolverT : SolverBase {
         for (void *p : adjalloc) (void)hipFree(p);
         adjalloc.clear();
         adjpool.clear();
+        for (int d = 0; d < 3; d++) {
+            D1T[d].release();
+            D2T[d].release();
+            FLT[d].release();
+        }
         adj_ready = false;
     }
 
