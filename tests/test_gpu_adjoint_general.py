@@ -198,3 +198,63 @@ def test_sponge_gradient_needs_the_initial_averages():
     g.step(1)
     with pytest.raises(cs3d.CS3DError):
         g.ke_gradient(2, 1)
+
+
+def test_shear_layer_512x128x42_fp32_gradient():
+    """the shear-layer configuration at its full size (512 x 128 x 42: y
+    non-periodic between sponge layers of 15% of the height, z periodic on
+    42 points, alpha_f 0.495, the filter every 5 steps): the FP32 adjoint
+    through 20 RK4 steps with a checkpoint every 5 steps against FP64
+    central differences of whole forward runs (eps 1e-3) along 4 seeded
+    Gaussian directions; the dt frozen at the first step's CFL dt.  1e-4
+    relative.  Also prints the cost against the forward steps."""
+    import time
+    import ic as icmod
+    dims = (512, 128, 42)
+    kw = dict(Lx=172.0, Ly=129.0, Lz=28.2, alpha_f=0.495, mu_ref=0.00375, Pr=0.7, cfl=0.8,
+              filter_step=5, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 1, 1, 0, 0),
+              sponge_frac=(0.25, 0.15, 0.25))
+    icf = icmod.shear_layer_ic(*dims, Ly=kw["Ly"], delta_u=1.2, delta0=1.0, seed=0)
+    probe = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP64, **kw))
+    probe.set_initial_conditions(*icf)
+    probe.pre_step()
+    kw["dt_fixed"] = probe.time()[1]
+    probe.close()
+    nsteps, K = 20, 5
+    f = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP32, **kw))
+    f.set_initial_conditions(*icf)
+    f.step(2)
+    f.synchronize()
+    t0 = time.perf_counter()
+    f.step(nsteps)
+    f.synchronize()
+    fwd = time.perf_counter() - t0
+    f.close()
+    g = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP32, **kw))
+    g.set_initial_conditions(*icf)
+    t0 = time.perf_counter()
+    J, gU, gV, gW = g.ke_gradient(nsteps, K)
+    tot = time.perf_counter() - t0
+    g.close()
+    print(f"shear adjoint: J {J:.10e}; forward {fwd * 1e3:.1f} ms, gradient call "
+          f"{tot * 1e3:.1f} ms (first call, allocation included): {tot / fwd:.2f}x", flush=True)
+    rho, U, V, W, p = icf
+    eps = 1e-3
+    worst = 0.0
+    for seed in range(4):
+        rng = np.random.default_rng(seed)
+        d = [rng.standard_normal(U.shape) for _ in range(3)]
+        Js = []
+        for sgn in (1, -1):
+            h = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP64, **kw))
+            h.set_initial_conditions(rho, U + sgn * eps * d[0], V + sgn * eps * d[1],
+                                     W + sgn * eps * d[2], p)
+            h.step(nsteps)
+            Js.append(h.kinetic_energy())
+            h.close()
+        fd = (Js[0] - Js[1]) / (2 * eps)
+        ad = float(np.vdot(gU, d[0]) + np.vdot(gV, d[1]) + np.vdot(gW, d[2]))
+        rel = abs(ad - fd) / abs(fd)
+        worst = max(worst, rel)
+        print(f"shear direction {seed}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}", flush=True)
+    assert worst <= 1e-4
