@@ -636,7 +636,9 @@ __device__ __forceinline__ void rhs(const OpDev<real> *__restrict__ op, const La
 // ------------------------------------------------------------ recurrences
 
 // forward recurrence y_t = r_t - m_t y_{t-1} with zero carry; returns the
-// segment's last value
+// segment's last value.  Table plans read the lane's constants at its slot
+// g * M (plan.hpp) and carry the last value through the points past a
+// short segment's end with selects, not branches.
 template <typename real, int M, bool UNI, int MAP>
 __device__ __forceinline__ real fwd_local(const OpDev<real> *__restrict__ op,
                                           const Lane<real, MAP> &c, real (&r)[M])
@@ -647,16 +649,13 @@ __device__ __forceinline__ real fwd_local(const OpDev<real> *__restrict__ op,
         for (int t = 1; t < M; t++) r[t] = r[t] - m * r[t - 1];
         return r[M - 1];
     } else {
-        const real *__restrict__ tm = op->m + c.s;
-        real e = r[0];
+        const real *__restrict__ tm = op->m + c.g * M;
 #pragma unroll
         for (int t = 1; t < M; t++) {
-            if (t < c.len) {
-                r[t] = r[t] - tm[t] * r[t - 1];
-                e = r[t];
-            }
+            const real y = r[t] - tm[t] * r[t - 1];
+            r[t] = t < c.len ? y : r[t - 1];
         }
-        return e;
+        return r[M - 1];
     }
 }
 
@@ -675,23 +674,23 @@ __device__ __forceinline__ real bwd_local(const OpDev<real> *__restrict__ op,
         for (int t = M - 2; t >= 0; t--) r[t] = r[t] + mu * r[t + 1];
         return r[0];
     } else {
-        const real *__restrict__ tiw = op->iw + c.s;
-        const real *__restrict__ tmu = op->mu + c.s;
-        const real *__restrict__ tpf = op->pf + c.s;
+        const real *__restrict__ tiw = op->iw + c.g * M;
+        const real *__restrict__ tmu = op->mu + c.g * M;
+        const real *__restrict__ tpf = op->pf + c.g * M;
         real carry = 0;
 #pragma unroll
         for (int t = M - 1; t >= 0; t--) {
-            if (t < c.len) {
-                const real y = r[t] + tpf[t] * C;
-                carry = tiw[t] * y + tmu[t] * carry;
-                r[t] = carry;
-            }
+            const real y = r[t] + tpf[t] * C;
+            const real x = tiw[t] * y + tmu[t] * carry;
+            carry = t < c.len ? x : real(0);
+            r[t] = carry;
         }
         return r[0];
     }
 }
 
-// backward fix-up and output scale: out_t = scale (x_t + pb_t D)
+// backward fix-up and output scale: out_t = scale (x_t + pb_t D) (points
+// past a short segment's end are computed and never stored)
 template <typename real, int M, bool UNI, int MAP>
 __device__ __forceinline__ void finish(const OpDev<real> *__restrict__ op,
                                        const Lane<real, MAP> &c, real (&r)[M], real D)
@@ -701,23 +700,22 @@ __device__ __forceinline__ void finish(const OpDev<real> *__restrict__ op,
 #pragma unroll
         for (int t = 0; t < M; t++) r[t] = sc * (r[t] + op->upb[t] * D);
     } else {
-        const real *__restrict__ tpb = op->pb + c.s;
+        const real *__restrict__ tpb = op->pb + c.g * M;
 #pragma unroll
-        for (int t = 0; t < M; t++)
-            if (t < c.len) r[t] = sc * (r[t] + tpb[t] * D);
+        for (int t = 0; t < M; t++) r[t] = sc * (r[t] + tpb[t] * D);
     }
 }
 
 // ------------------------------------------------------------ exchange
 
 // Carry weights of one operator for this lane: uniform plans share one row,
-// table plans hold one row of K per segment.
+// table plans hold one row of Ks per segment.
 template <typename real, bool UNI, bool FWD, int MAP>
 __device__ __forceinline__ const real *carry_weights(const OpDev<real> *__restrict__ op,
                                                      const Lane<real, MAP> &c)
 {
     if (UNI) return FWD ? op->uphi : op->upsi;
-    return (FWD ? op->phi : op->psi) + (long)c.g * op->K;
+    return (FWD ? op->phi : op->psi) + (long)c.g * op->Ks;
 }
 
 // Publish v[0..N1+N2) for this lane's segment; after the barrier,
@@ -725,9 +723,9 @@ __device__ __forceinline__ const real *carry_weights(const OpDev<real> *__restri
 // operator 1 for i < N1 and of operator 2 after that.  The values of one
 // ring position sit side by side at an odd stride S (entry (q, i) at
 // q * S + i), so every read of a lane is at a constant offset from one
-// address.  Uniform plans store zero weights past K, so on lines of at least
-// KR >= KC segments their first KC terms run unrolled without branches;
-// terms past KC, shorter lines and table plans run in a plain loop.
+// address.  Both plan kinds store zero weights past K (at least KC of them),
+// so on lines of at least KR >= KC segments the first KC terms run unrolled
+// without branches; terms past KC and shorter lines run in a plain loop.
 template <int N1, int N2, bool FWD, int KC1, int KC2, bool UNI, typename real, int MAP>
 __device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 + N2],
                                          real (&out)[N1 + N2], const real *__restrict__ w1,
@@ -752,7 +750,7 @@ __device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 
     const real *p = b + prim;
     // the unrolled reads reach KC1 (KC2) positions back/forward; every one
     // of them was written only when the line has at least KR >= KC segments
-    const bool unrolled = UNI && c.P >= c.KR;
+    const bool unrolled = c.P >= c.KR;
     const int k1 = unrolled ? KC1 : 0, k2 = unrolled ? KC2 : 0;
     if (unrolled) {
 #pragma unroll

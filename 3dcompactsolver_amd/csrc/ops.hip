@@ -103,21 +103,37 @@ void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, doub
             }
         }
     }
-    // tables: m iw mu pf pb (n each), phi psi (P*K each), sstart slen (P)
-    size_t nreal = 5 * (size_t)h.n + 2 * (size_t)h.P * h.K;
+    // tables, laid out per lane slot so that a lane reads its segment's
+    // M constants at g * M + t whatever its length (zeros past the end of a
+    // short segment: the kernels need no bounds test): m iw mu pf pb (P * M
+    // each); phi psi (P rows of Ks >= K, zero past K, so that the kernels'
+    // unrolled carry terms read zero weights); sstart slen (P)
+    const int kcu = kind == OP_FILTER ? kc_filter<real>() : kc_deriv<real>();
+    const int Ks = std::max(h.K, kcu);
+    d.Ks = Ks;
+    const size_t slots = (size_t)h.P * h.M;
+    size_t nreal = 5 * slots + 2 * (size_t)h.P * Ks;
     size_t bytes = nreal * sizeof(real) + 2 * (size_t)h.P * sizeof(int);
-    std::vector<char> buf(bytes);
+    std::vector<char> buf(bytes, 0);
     real *rp = (real *)buf.data();
-    auto put = [&](const std::vector<double> &v) {
-        for (double x : v) *rp++ = (real)x;
+    auto put_slots = [&](const std::vector<double> &v) {
+        for (int g = 0; g < h.P; g++)
+            for (int t = 0; t < h.M; t++)
+                rp[(size_t)g * h.M + t] = t < h.slen[g] ? (real)v[h.sstart[g] + t] : real(0);
+        rp += slots;
     };
-    put(h.m);
-    put(h.iw);
-    put(h.mu);
-    put(h.pf);
-    put(h.pb);
-    put(h.phi);
-    put(h.psi);
+    auto put_rows = [&](const std::vector<double> &v) {
+        for (int g = 0; g < h.P; g++)
+            for (int k = 0; k < h.K; k++) rp[(size_t)g * Ks + k] = (real)v[(size_t)g * h.K + k];
+        rp += (size_t)h.P * Ks;
+    };
+    put_slots(h.m);
+    put_slots(h.iw);
+    put_slots(h.mu);
+    put_slots(h.pf);
+    put_slots(h.pb);
+    put_rows(h.phi);
+    put_rows(h.psi);
     int *ip = (int *)rp;
     for (int g = 0; g < h.P; g++) ip[g] = h.sstart[g];
     for (int g = 0; g < h.P; g++) ip[h.P + g] = h.slen[g];
@@ -125,13 +141,13 @@ void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, doub
     hip_check(hipMemcpy(pl.tables, buf.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(plan)");
     real *dp = (real *)pl.tables;
     d.m = dp;
-    d.iw = dp + h.n;
-    d.mu = dp + 2 * h.n;
-    d.pf = dp + 3 * h.n;
-    d.pb = dp + 4 * h.n;
-    d.phi = dp + 5 * h.n;
-    d.psi = d.phi + (size_t)h.P * h.K;
-    d.sstart = (const int *)(d.psi + (size_t)h.P * h.K);
+    d.iw = dp + slots;
+    d.mu = dp + 2 * slots;
+    d.pf = dp + 3 * slots;
+    d.pb = dp + 4 * slots;
+    d.phi = dp + 5 * slots;
+    d.psi = d.phi + (size_t)h.P * Ks;
+    d.sstart = (const int *)(d.psi + (size_t)h.P * Ks);
     d.slen = d.sstart + h.P;
     pl.hd = d;
     hip_check(hipMalloc(&pl.dev, sizeof(OpDev<real>)), "hipMalloc(plan)");

@@ -47,7 +47,7 @@ struct OpDev {
     int H;        // interior stencil half width (2 or 4)
     int NB;       // boundary rows with their own stencil at each end
     int K;        // cross-segment terms kept (forward and backward)
-    int pad_;
+    int Ks;       // table plans: row stride of phi and psi (>= K, zero past K)
     real scale;   // output scale (1 + rho^2 for periodic, 1 otherwise)
     real tap[5];  // interior stencil: D1 t1,t2 (odd), D2 t0,t1,t2, already
                   // divided by dh or dh^2; filter: tap[0] = (1 - 2 alpha_f)/256,
@@ -70,8 +70,9 @@ struct OpDev {
     real wlam[2][4];
     real wrow[2][65];
     // table path (device pointers into the plan's own allocation)
-    const real *m, *iw, *mu, *pf, *pb;  // per point, length n
-    const real *phi, *psi;              // [P][K]
+    const real *m, *iw, *mu, *pf, *pb;  // per lane slot g * M + t (P * M each);
+                                        // zero past a short segment's end
+    const real *phi, *psi;              // [P][Ks]
     const int *sstart, *slen;           // [P]
 };
 

@@ -155,7 +155,8 @@ struct SolverT : SolverBase {
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             sweepG_small[d] = sweepG[d];
             if (d > 0) {
-                sweepG_small[d] = line_geom(P, kr, std::min(sweep_max_threads(d, P), 512),
+                sweepG_small[d] = line_geom(P, kr,
+                                            std::min(sweep_max_threads(d, P), sweep_threads<real>() / 2),
                                             SWEEP_PLANES, sizeof(real),
                                             SWEEP_STAGE_FIELDS * D1[d].host.M);
                 if (sweepG_small[d].LX < 1)

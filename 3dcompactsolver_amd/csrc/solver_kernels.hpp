@@ -156,20 +156,23 @@ constexpr int SWEEP_STAGE_FIELDS = 5;
 template <typename real>
 constexpr int sweep_threads() { return sizeof(real) == 4 ? 1024 : 512; }
 
-// column sweep variants that carry more live values than 128 VGPRs run at
-// most 512 lanes (256 VGPRs): in FP32, the z sweep for table plans (ragged
-// or non-periodic z lines) and for the stale stage, the y sweep for table
-// plans at the stale stage.  At 1024 lanes they spilled 12-80 bytes per
-// lane to scratch.  The host launches them with a matching geometry.
+// Column sweep variants that carry more live values than the registers of a
+// full workgroup allow run half-size workgroups (twice the registers per
+// lane): in FP32 (128 VGPRs at 1024 lanes) the z sweep for table plans
+// (ragged or non-periodic z lines) and for the stale stage, the y sweep for
+// table plans; in FP64 (256 VGPRs at 512 lanes) the z sweep for table
+// plans and the y sweep's stale stage on them.  At full size they spilled
+// 12-80 bytes per lane to scratch.  The host launches them with a matching
+// geometry.
 template <typename real>
 constexpr bool col_small(int dir, bool uni, bool stale)
 {
-    return sizeof(real) == 4 && (dir == 2 ? (!uni || stale) : (!uni && stale));
+    return sizeof(real) == 4 ? (dir == 2 ? (!uni || stale) : !uni) : (!uni && (stale || dir == 2));
 }
 template <typename real, int DIR, bool UNI, bool STALE>
 constexpr int col_threads()
 {
-    return col_small<real>(DIR, UNI, STALE) ? 512 : sweep_threads<real>();
+    return col_small<real>(DIR, UNI, STALE) ? sweep_threads<real>() / 2 : sweep_threads<real>();
 }
 
 template <typename real>

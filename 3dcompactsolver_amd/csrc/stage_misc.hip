@@ -74,7 +74,9 @@ __global__ __launch_bounds__((filter_threads<real, CMP>())) void k_filter5(const
             one(v, w);
         }
     }
-    solve_batch<5, kc_filter<real>(), real, M, UNI>(fo, c, r);
+    // table plans unroll fewer carry terms (all of the filter's would not
+    // fit the registers of a 1024-lane workgroup) and loop over the rest
+    solve_batch<5, UNI ? kc_filter<real>() : kc_deriv<real>(), real, M, UNI>(fo, c, r);
     static_assert(!CMP || (SET && sizeof(real) == 4), "compensated sets are FP32 conserved sets");
     if constexpr (CMP) {
         if (active) {
