@@ -408,9 +408,10 @@ class Solver:
 
 def stream_probe(kind="copy", nbytes=1 << 30, reps=5):
     """(best, mean) HBM rate in GB/s (bytes read + written) of the library's
-    16-byte-vector copy or triad kernel over arrays of nbytes each"""
+    16-byte-vector copy, triad or column-sweep mix ("mix": 5 arrays read and
+    9 written) kernel over arrays of nbytes each"""
     best, mean = ctypes.c_double(), ctypes.c_double()
-    check(lib().cs3d_stream_probe({"copy": 0, "triad": 1}[kind], nbytes, reps,
+    check(lib().cs3d_stream_probe({"copy": 0, "triad": 1, "mix": 2}[kind], nbytes, reps,
                                   ctypes.byref(best), ctypes.byref(mean)))
     return best.value, mean.value
 

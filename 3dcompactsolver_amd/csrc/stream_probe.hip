@@ -1,6 +1,7 @@
 // stream_probe.hip -- the HBM rate this GPU sustains on plain streams,
-// measured with hand-written kernels: a copy (b = a) and a triad
-// (a = b + s c) over 16-byte vectors.  No reference counterpart: it gives
+// measured with hand-written kernels: a copy (b = a), a triad
+// (a = b + s c) and the column sweeps' mix (5 arrays read, 9 written per
+// element) over 16-byte vectors.  No reference counterpart: it gives
 // the sweeps' roofline fractions a measured ceiling beside the 8 TB/s peak,
 // and a copy of known size calibrates the HBM traffic counters.
 #include <hip/hip_runtime.h>
@@ -67,6 +68,39 @@ __global__ __launch_bounds__(256) void k_triad(f4 *__restrict__ a, const f4 *__r
     for (; i < n4; i += step) a[i] = b[i] + s * c[i];
 }
 
+// the mix of the column sweeps: every element reads 5 arrays and writes 9
+constexpr int MIX_R = 5, MIX_W = 9;
+struct MixPtrs {
+    const f4 *in[MIX_R];
+    f4 *out[MIX_W];
+};
+
+template <int U, bool GS>
+__global__ __launch_bounds__(256) void k_mix(MixPtrs p, long n4)
+{
+    long step;
+    long i = first_index<U, GS>(step);
+    for (; i + (U - 1) * step < n4; i += U * step) {
+        f4 x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) x[u] = p.in[0][i + u * step];
+#pragma unroll
+        for (int k = 1; k < MIX_R; k++)
+#pragma unroll
+            for (int u = 0; u < U; u++) x[u] += p.in[k][i + u * step];
+#pragma unroll
+        for (int k = 0; k < MIX_W; k++)
+#pragma unroll
+            for (int u = 0; u < U; u++) __builtin_nontemporal_store(x[u] * (float)(k + 1), &p.out[k][i + u * step]);
+        if (!GS) return;
+    }
+    for (; i < n4; i += step) {
+        f4 x = p.in[0][i];
+        for (int k = 1; k < MIX_R; k++) x += p.in[k][i];
+        for (int k = 0; k < MIX_W; k++) p.out[k][i] = x * (float)(k + 1);
+    }
+}
+
 __global__ void k_fill(f4 *__restrict__ a, long n4, float v)
 {
     const long stride = (long)gridDim.x * blockDim.x;
@@ -87,11 +121,17 @@ static const ProbeShape PROBE_SHAPES[] = {{4, true, 8}, {8, true, 8}, {4, true, 
 template <int U, bool GS>
 static void launch_probe(int kind, dim3 grid, f4 *const *buf, long n4)
 {
-    if (kind == 0)
+    if (kind == 0) {
         hipLaunchKernelGGL((k_copy<U, GS>), grid, dim3(256), 0, nullptr, buf[0], buf[1], n4);
-    else
+    } else if (kind == 1) {
         hipLaunchKernelGGL((k_triad<U, GS>), grid, dim3(256), 0, nullptr, buf[0], buf[1], buf[2],
                            0.5f, n4);
+    } else {
+        MixPtrs p;
+        for (int k = 0; k < MIX_R; k++) p.in[k] = buf[k];
+        for (int k = 0; k < MIX_W; k++) p.out[k] = buf[MIX_R + k];
+        hipLaunchKernelGGL((k_mix<U, GS>), grid, dim3(256), 0, nullptr, p, n4);
+    }
 }
 
 static void launch_shape(const ProbeShape &p, int kind, int cus, f4 *const *buf, long n4)
@@ -114,14 +154,14 @@ using namespace cs3d;
 extern "C" int cs3d_stream_probe(int kind, size_t bytes, int reps, double *best_gbps,
                                  double *mean_gbps)
 {
-    if ((kind != 0 && kind != 1) || bytes < (1u << 20) || bytes % 16 || reps < 1)
-        return set_error(CS3D_ERR_ARG, "cs3d_stream_probe: kind 0/1, bytes >= 1 MiB and a "
+    if (kind < 0 || kind > 2 || bytes < (1u << 20) || bytes % 16 || reps < 1)
+        return set_error(CS3D_ERR_ARG, "cs3d_stream_probe: kind 0/1/2, bytes >= 1 MiB and a "
                                        "multiple of 16, reps >= 1");
     if (bytes / 16 / 256 > 0x7fffffffL)
         return set_error(CS3D_ERR_ARG, "cs3d_stream_probe: array too large");
     const long n4 = (long)(bytes / 16);
-    const int narr = kind == 0 ? 2 : 3;
-    f4 *buf[3] = {nullptr, nullptr, nullptr};
+    const int narr = kind == 0 ? 2 : kind == 1 ? 3 : MIX_R + MIX_W;
+    f4 *buf[MIX_R + MIX_W] = {};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::string err;
     auto ok = [&](hipError_t e, const char *what) {

@@ -28,8 +28,10 @@ cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
          with the reference's flags) timed on this host's cores at 256^3
          (1 warm-up + 3 steps), with a 128^3 sample beside it.
 hbm_stream_GBps = the library's hand-written 16-byte-vector copy and triad
-         kernels over 1 GiB arrays, bytes read + written per second: the
-         HBM rate plain streams reach on this GPU, beside the 8 TB/s peak.
+         kernels over 1 GiB arrays, and the column sweeps' mix (5 arrays
+         read, 9 written per element) over 512 MiB arrays, bytes read +
+         written per second: the HBM rate plain streams reach on this GPU,
+         beside the 8 TB/s peak.
 
 Only one GPU: --gpus N with N > 1 exits with a message.
 """
@@ -197,6 +199,10 @@ def stream_bandwidth():
     for kind in ("copy", "triad"):
         best, mean = cs3d.stream_probe(kind, 1 << 30, 5)
         out[kind] = {"best": round(best, 1), "mean": round(mean, 1)}
+    # the column sweeps' read/write mix (5 read, 9 written; 14 arrays of
+    # 512 MiB)
+    best, mean = cs3d.stream_probe("mix", 1 << 29, 5)
+    out["mix_5r9w"] = {"best": round(best, 1), "mean": round(mean, 1)}
     return out
 
 

@@ -265,7 +265,8 @@ int cs3d_profile_report(cs3d_solver *s, double ms[6], int counts[6]);
  * 16-byte-vector kernels (no reference counterpart: a measured ceiling for
  * the sweeps' roofline fractions, and a copy of known size to calibrate
  * HBM traffic counters).  kind 0: copy b = a (2 arrays), 1: triad
- * a = b + s c (3 arrays) of `bytes` each, in six launch shapes (grid-
+ * a = b + s c (3 arrays), 2: the column sweeps' mix (5 arrays read and 9
+ * written per element: 14 arrays) of `bytes` each, in six launch shapes (grid-
  * strided or one pass, 4 to 16 vectors in flight per lane); per shape 2
  * untimed then `reps` timed launches on the null stream, HIP events around
  * each.  Bytes read plus written per second (1e9): the best launch of any
