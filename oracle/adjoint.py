@@ -8,9 +8,10 @@ own stage map (oracle.c pre_step_derivatives / solve_eqn_set /
 post_step_bc, which restate CSolver_AWS.cpp:829-1670), of its RK4 and
 TVDRK3 steps with the filter (RK4.hpp:35-118, TVDRK3.hpp:38-96,
 CSolver_AWS.cpp:1682-1893) and of the sponge update (updateSponge,
-CSolver_AWS.cpp:1937-2031), on periodic or non-periodic lines, with or
-without sponge faces (walls are not covered), written independently of the
-GPU kernels:
+CSolver_AWS.cpp:1937-2031) and of the wall handling (preStepBCHandling,
+:553-826; the wall rows of postStepBCHandling, :1506-1605), on periodic or
+non-periodic lines, with sponge faces or with walls (not both), written
+independently of the GPU kernels:
 
 - every compact operator is the dense matrix of the oracle's own line
   operator (or_line_op, column by column), and its adjoint is that matrix
@@ -28,7 +29,13 @@ GPU kernels:
   sponge update the x derivatives, x fluxes and mu of the reference come
   from the primitive arrays of the filtered state before the reset (the
   "X set" below), so their cotangents flow to that state, not to the
-  stage state.
+  stage state;
+- walls: before each stage the wall faces (adiabatic X0..Z1, then moving
+  X0..Z1) overwrite the momenta with rho times the wall velocity and the
+  energy with that of p = rho R T_w, T_w = calcNeumann of the six inward
+  temperatures, in place (at the first stage the state the RK update
+  builds on); the right-hand side on a wall plane keeps only the normal
+  continuity flux term of the last wall face through the point.
 
 Pinned by tests/test_oracle_adjoint.py: the numpy forward stage and the RK4
 and TVDRK3 steps against the C oracle (1e-12; with sponges and non-periodic
@@ -140,8 +147,9 @@ class Adjoint:
         self.dims = (cfg.Nx, cfg.Ny, cfg.Nz)
         self.shape = (cfg.Nz, cfg.Ny, cfg.Nx)
         kinds = [cfg.bc_kind[f] for f in range(6)]
-        if any(k in (po.ADIABATIC_WALL, po.MOVING_ADIABATIC_WALL) for k in kinds):
-            raise ValueError("the numpy adjoint does not cover walls")
+        is_wall = [k in (po.ADIABATIC_WALL, po.MOVING_ADIABATIC_WALL) for k in kinds]
+        if any(is_wall) and any(k == po.SPONGE for k in kinds):
+            raise ValueError("the numpy adjoint does not cover walls beside sponges")
         lengths = (cfg.Lx, cfg.Ly, cfg.Lz)
         self.rk = cfg.rk
         self.gamma = cfg.gamma
@@ -176,6 +184,81 @@ class Adjoint:
         self.epsP = cfg.sponge_epsP
         self.spP = 1.0 / cfg.gamma
         self.zmask = np.where(self.onplane, 0.0, 1.0)
+        # walls: the faces in the reference's order (adiabatic, then moving)
+        # with their velocities, and for the right-hand side the normal of
+        # the last wall face through each point (-1: none)
+        self.walls = []
+        for kind in (po.ADIABATIC_WALL, po.MOVING_ADIABATIC_WALL):
+            for f in range(6):
+                if kinds[f] != kind:
+                    continue
+                d = f // 2
+                uw = [cfg.wall_vel[f][m] if (kind == po.MOVING_ADIABATIC_WALL and m != d) else 0.0
+                      for m in range(3)]
+                self.walls.append((d, f % 2, uw))
+        self.rowdir = np.full(self.shape, -1, dtype=int)
+        for f in range(6):
+            if is_wall[f]:
+                self.rowdir[self.plane(f // 2, f % 2)] = f // 2
+
+    @staticmethod
+    def plane(d, hi, m=0):
+        """index of the plane m points inward from face (d, hi)"""
+        idx = [slice(None)] * 3
+        idx[2 - d] = -1 - m if hi else m
+        return tuple(idx)
+
+    # ---------------------------------------------------------------- walls
+    CN = np.array([360.0, -450.0, 400.0, -225.0, 72.0, -10.0]) / 147.0  # calcNeumann
+
+    def wall_pre(self, q, keep=None):
+        """preStepBCHandling on a state (a new list); keep (a list) receives
+        the state before each face"""
+        q = [x.copy() for x in q]
+        g = self.gamma
+        for d, hi, uw in self.walls:
+            if keep is not None:
+                keep.append([x.copy() for x in q])
+            T = self.prims(q)["T"]
+            Tw = sum(c * T[self.plane(d, hi, m + 1)] for m, c in enumerate(self.CN))
+            fc = self.plane(d, hi)
+            rho = q[0][fc]
+            for k in range(3):
+                q[1 + k][fc] = rho * uw[k]
+            q[4][fc] = rho * self.R * Tw / (g - 1) + 0.5 * rho * sum(u * u for u in uw)
+        return q
+
+    def wall_pre_vjp(self, q, lam):
+        """the cotangent of the state before preStepBCHandling from that of
+        the state after it"""
+        if not self.walls:
+            return list(lam)
+        keep = []
+        self.wall_pre(q, keep)
+        lam = [x.copy() for x in lam]
+        g = self.gamma
+        for (d, hi, uw), st in zip(self.walls[::-1], keep[::-1]):
+            v = self.prims(st)
+            fc = self.plane(d, hi)
+            rho = st[0][fc]
+            Tw = sum(c * v["T"][self.plane(d, hi, m + 1)] for m, c in enumerate(self.CN))
+            lE = lam[4][fc].copy()
+            lam[0][fc] += sum(lam[1 + k][fc] * uw[k] for k in range(3)) + lE * (
+                self.R * Tw / (g - 1) + 0.5 * sum(u * u for u in uw))
+            for k in range(1, 5):
+                lam[k][fc] = 0.0
+            gT = lE * rho * self.R / (g - 1)
+            for m, c in enumerate(self.CN):
+                ip = self.plane(d, hi, m + 1)
+                r = st[0][ip]
+                u = [st[k][ip] / r for k in (1, 2, 3)]
+                Tb = gT * c
+                lam[4][ip] += Tb * (g - 1) / (r * self.R)
+                for k in range(3):
+                    lam[1 + k][ip] -= Tb * (g - 1) * u[k] / (r * self.R)
+                lam[0][ip] += Tb * ((g - 1) * 0.5 * sum(x * x for x in u) / (r * self.R)
+                                    - v["T"][ip] / r)
+        return lam
 
     # ---------------------------------------------------------------- operators
     def op(self, kind, d, f, transpose=False):
@@ -258,6 +341,12 @@ class Adjoint:
                 spg = self.sigma * (A[c] - q[c])
                 acc = acc + (2.0 * spg if c == 4 else spg)
             out.append(dt * acc * self.zmask)
+        if self.walls:
+            # postStepBCHandling: a wall plane keeps -dt D_n(rho u_n)
+            for c in range(5):
+                out[c] = np.where(self.rowdir >= 0, 0.0, out[c])
+            for d, D in enumerate("XYZ"):
+                out[0] = np.where(self.rowdir == d, -dt * v["cont" + D], out[0])
         return out
 
     # ---------------------------------------------------------------- reverse
@@ -291,7 +380,10 @@ class Adjoint:
         def add(name, val):
             b[name] = b.get(name, 0.0) + val
 
-        w = [dt * self.zmask * x for x in lam]
+        w = [dt * self.zmask * np.where(self.rowdir >= 0, 0.0, x) for x in lam]
+        for d, D in enumerate("XYZ"):
+            if (self.rowdir == d).any():
+                b["cont" + D] = np.where(self.rowdir == d, -dt * lam[0], 0.0)
         for c, terms in enumerate(RHS_TERMS):
             for coef, fac in terms:
                 cf = self._coef(coef)
@@ -393,11 +485,15 @@ class Adjoint:
 
     def rk4_step(self, q, dt, order, A=None, qs=None):
         """one RK4 step (RK4.hpp:35-118) followed by the filter in `order`
-        (None: no filter); the filtered state before any sponge update"""
+        (None: no filter); the filtered state before any sponge update.
+        With walls every stage state is wall-corrected first (the first in
+        place: the steps build on it)"""
+        W = self.wall_pre if self.walls else (lambda x: x)
+        q = W(q)
         k1 = self.rhs(q, dt, A, qs)
-        k2 = self.rhs([a + 0.5 * k for a, k in zip(q, k1)], dt, A)
-        k3 = self.rhs([a + 0.5 * k for a, k in zip(q, k2)], dt, A)
-        k4 = self.rhs([a + k for a, k in zip(q, k3)], dt, A)
+        k2 = self.rhs(W([a + 0.5 * k for a, k in zip(q, k1)]), dt, A)
+        k3 = self.rhs(W([a + 0.5 * k for a, k in zip(q, k2)]), dt, A)
+        k4 = self.rhs(W([a + k for a, k in zip(q, k3)]), dt, A)
         acc = [a + k1_ / 6 + k2_ / 3 + k3_ / 3 + k4_ / 6
                for a, k1_, k2_, k3_, k4_ in zip(q, k1, k2, k3, k4)]
         return self.filt(acc, order) if order is not None else acc
@@ -405,11 +501,13 @@ class Adjoint:
     def tvdrk3_step(self, q, dt, order, A=None, qs=None):
         """one TVDRK3 step (TVDRK3.hpp:38-96: Q2 = q + k1, Q3 = 3/4 q +
         1/4 (Q2 + k2), q' = 1/3 q + 2/3 (Q3 + k3)) followed by the filter in
-        `order` (None: no filter)"""
+        `order` (None: no filter); with walls q, Q2 and Q3 wall-corrected"""
+        W = self.wall_pre if self.walls else (lambda x: x)
+        q = W(q)
         k1 = self.rhs(q, dt, A, qs)
-        Q2 = [a + k for a, k in zip(q, k1)]
+        Q2 = W([a + k for a, k in zip(q, k1)])
         k2 = self.rhs(Q2, dt, A)
-        Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)]
+        Q3 = W([0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)])
         k3 = self.rhs(Q3, dt, A)
         acc = [a / 3 + 2.0 / 3.0 * b + 2.0 / 3.0 * k for a, b, k in zip(q, Q3, k3)]
         return self.filt(acc, order) if order is not None else acc
@@ -455,13 +553,15 @@ class Adjoint:
                 q = qf
         return q, A, recs
 
-    def ke_gradient(self, ic, nsteps, dt, filter_step=1):
+    def ke_gradient(self, ic, nsteps, dt, filter_step=1, q0=None):
         """J = <rho |u|^2 / 2> after nsteps steps of the configured scheme
         (RK4 or TVDRK3) with fixed dt and the filter every filter_step
         steps; returns J, dJ/dU0, dJ/dV0, dJ/dW0 at fixed initial density
-        and pressure"""
-        rho0, U0, V0, W0, p0 = [np.asarray(a, dtype=np.float64) for a in ic]
-        q0 = self.conserved_from_ic(rho0, U0, V0, W0, p0)
+        and pressure.  q0: the initial conserved state (default: formed from
+        ic; with walls pass the oracle's, whose set_initial_conditions also
+        corrects the wall faces)"""
+        if q0 is None:
+            q0 = self.conserved_from_ic(*[np.asarray(a, dtype=np.float64) for a in ic])
         q, A, recs = self.run(q0, nsteps, dt, filter_step)
         N = q[0].size
         rho = q[0]
@@ -483,8 +583,10 @@ class Adjoint:
                 lA = [a + b for a, b in zip(lA, lAs)]
         if self.sponge:
             lam = [a + b for a, b in zip(lam, lA)]  # the averages start as q0
-        # m_i = rho U_i, E = p/(g-1) + rho |U|^2 / 2 at fixed rho and p
-        grads = [rho0 * (lam[i + 1] + (U0, V0, W0)[i] * lam[4]) for i in range(3)]
+        # m_i = rho u_i, E = p/(g-1) + rho |u|^2 / 2 at fixed rho and p, with
+        # rho and u of the initial state
+        r0 = q0[0]
+        grads = [r0 * (lam[i + 1] + (q0[i + 1] / r0) * lam[4]) for i in range(3)]
         return J, grads[0], grads[1], grads[2]
 
     @staticmethod
@@ -495,46 +597,61 @@ class Adjoint:
 
     def rk4_step_vjp(self, q, dt, order, lam, A=None, qs=None):
         """cotangents of the step's start state (and of qs, A) from that of
-        its filtered result; with Q2 = q + k1/2, Q3 = q + k2/2, Q4 = q + k3
-        and the filtered acc = q + k1/6 + k2/3 + k3/3 + k4/6"""
-        k1 = self.rhs(q, dt, A, qs)
-        Q2 = [a + 0.5 * k for a, k in zip(q, k1)]
-        k2 = self.rhs(Q2, dt, A)
-        Q3 = [a + 0.5 * k for a, k in zip(q, k2)]
-        k3 = self.rhs(Q3, dt, A)
-        Q4 = [a + k for a, k in zip(q, k3)]
+        its filtered result; with q1 = W(q), Q2 = q1 + k1/2, Q3 = q1 + k2/2,
+        Q4 = q1 + k3, k_s = dt R(W(Q_s)) (W: the wall correction, the
+        identity without walls) and the filtered acc = q1 + k1/6 + k2/3 +
+        k3/3 + k4/6"""
+        W = self.wall_pre if self.walls else (lambda x: x)
+        WT = self.wall_pre_vjp
+        q1 = W(q)
+        k1 = self.rhs(q1, dt, A, qs)
+        Q2 = [a + 0.5 * k for a, k in zip(q1, k1)]
+        k2 = self.rhs(W(Q2), dt, A)
+        Q3 = [a + 0.5 * k for a, k in zip(q1, k2)]
+        k3 = self.rhs(W(Q3), dt, A)
+        Q4 = [a + k for a, k in zip(q1, k3)]
         la = self.filt(lam, order, transpose=True) if order is not None else list(lam)
         qb = list(la)
-        Q4b, _, Ab = self.rhs_vjp_full(Q4, [x / 6 for x in la], dt, A)
+        Q4b, _, Ab = self.rhs_vjp_full(W(Q4), [x / 6 for x in la], dt, A)
+        Q4b = WT(Q4, Q4b)
         qb = [a + c for a, c in zip(qb, Q4b)]
-        Q3b, _, Ab3 = self.rhs_vjp_full(Q3, [x / 3 + c for x, c in zip(la, Q4b)], dt, A)
+        Q3b, _, Ab3 = self.rhs_vjp_full(W(Q3), [x / 3 + c for x, c in zip(la, Q4b)], dt, A)
+        Q3b = WT(Q3, Q3b)
         qb = [a + c for a, c in zip(qb, Q3b)]
-        Q2b, _, Ab2 = self.rhs_vjp_full(Q2, [x / 3 + 0.5 * c for x, c in zip(la, Q3b)], dt, A)
+        Q2b, _, Ab2 = self.rhs_vjp_full(W(Q2), [x / 3 + 0.5 * c for x, c in zip(la, Q3b)], dt, A)
+        Q2b = WT(Q2, Q2b)
         qb = [a + c for a, c in zip(qb, Q2b)]
-        Q1b, qsb, Ab1 = self.rhs_vjp_full(q, [x / 6 + 0.5 * c for x, c in zip(la, Q2b)], dt, A,
+        Q1b, qsb, Ab1 = self.rhs_vjp_full(q1, [x / 6 + 0.5 * c for x, c in zip(la, Q2b)], dt, A,
                                           qs)
         for x in (Ab3, Ab2, Ab1):
             Ab = self._acc(Ab, x)
-        return [a + c for a, c in zip(qb, Q1b)], qsb, Ab
+        return WT(q, [a + c for a, c in zip(qb, Q1b)]), qsb, Ab
 
     def tvdrk3_step_vjp(self, q, dt, order, lam, A=None, qs=None):
         """cotangents of the step's start state (and of qs, A) from that of
         its filtered result for tvdrk3_step: with la the cotangent of the
         unfiltered result, Q3b = 2/3 la + J3^T (2/3 la), Q2b = 1/4 Q3b +
         J2^T (1/4 Q3b), qb = 1/3 la + 3/4 Q3b + Q2b + J1^T Q2b (J_s^T:
-        rhs_vjp at Q_s)"""
-        k1 = self.rhs(q, dt, A, qs)
-        Q2 = [a + k for a, k in zip(q, k1)]
-        k2 = self.rhs(Q2, dt, A)
-        Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q, Q2, k2)]
+        rhs_vjp at Q_s); with walls each Q_s b taken back through the wall
+        correction of its stage"""
+        W = self.wall_pre if self.walls else (lambda x: x)
+        WT = self.wall_pre_vjp
+        q1 = W(q)
+        k1 = self.rhs(q1, dt, A, qs)
+        Q2 = [a + k for a, k in zip(q1, k1)]
+        Q2w = W(Q2)
+        k2 = self.rhs(Q2w, dt, A)
+        Q3 = [0.75 * a + 0.25 * b + 0.25 * k for a, b, k in zip(q1, Q2w, k2)]
+        Q3w = W(Q3)
         la = self.filt(lam, order, transpose=True) if order is not None else list(lam)
         t3 = [2.0 / 3.0 * x for x in la]
-        J3, _, Ab = self.rhs_vjp_full(Q3, t3, dt, A)
-        Q3b = [a + c for a, c in zip(t3, J3)]
+        J3, _, Ab = self.rhs_vjp_full(Q3w, t3, dt, A)
+        Q3b = WT(Q3, [a + c for a, c in zip(t3, J3)])
         t2 = [0.25 * x for x in Q3b]
-        J2, _, Ab2 = self.rhs_vjp_full(Q2, t2, dt, A)
-        Q2b = [a + c for a, c in zip(t2, J2)]
-        J1, qsb, Ab1 = self.rhs_vjp_full(q, Q2b, dt, A, qs)
+        J2, _, Ab2 = self.rhs_vjp_full(Q2w, t2, dt, A)
+        Q2b = WT(Q2, [a + c for a, c in zip(t2, J2)])
+        J1, qsb, Ab1 = self.rhs_vjp_full(q1, Q2b, dt, A, qs)
         for x in (Ab2, Ab1):
             Ab = self._acc(Ab, x)
-        return [x / 3 + 0.75 * a + b + c for x, a, b, c in zip(la, Q3b, Q2b, J1)], qsb, Ab
+        out = [x / 3 + 0.75 * a + b + c for x, a, b, c in zip(la, Q3b, Q2b, J1)]
+        return WT(q, out), qsb, Ab
