@@ -785,6 +785,18 @@ __device__ __forceinline__ bool on_sponge_plane(const StageArgs &sa, long i)
            (sa.sponge_face[4] && kk == 0) || (sa.sponge_face[5] && kk == sa.Nz - 1);
 }
 
+// the normal direction of the last wall face through flat point i (faces
+// X0..Z1 in turn, as postStepBCHandling visits them), -1 off the walls
+__device__ __forceinline__ int wall_row_dir(const StageArgs &sa, long i)
+{
+    const int ii = (int)(i % sa.Nx), jj = (int)((i / sa.Nx) % sa.Ny),
+              kk = (int)(i / ((long)sa.Nx * sa.Ny));
+    if ((sa.wall_face[4] && kk == 0) || (sa.wall_face[5] && kk == sa.Nz - 1)) return 2;
+    if ((sa.wall_face[2] && jj == 0) || (sa.wall_face[3] && jj == sa.Ny - 1)) return 1;
+    if ((sa.wall_face[0] && ii == 0) || (sa.wall_face[1] && ii == sa.Nx - 1)) return 0;
+    return -1;
+}
+
 // the state the x block's primitives come from at point i: the stage state,
 // or at the first stage after a sponge update the pre-reset values on the
 // sponge planes
@@ -827,6 +839,16 @@ __global__ __launch_bounds__(256) void k_gadj_seeds(GenAdj<real> g, Phys<real> p
         qset_load(g.q, i, q);
         qset_load(g.v, i, v);
         const real mask = on_sponge_plane(g.sa, i) ? real(0) : g.cv;
+        // postStepBCHandling: a wall plane keeps only -dt D_n(rho u_n) of
+        // its last wall face (X0..Z1); every other term and seed is zero
+        const int wd = wall_row_dir(g.sa, i);
+        if (g.vc[0] != g.vs[0]) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) g.vc[d][i] = (wd < 0 || wd == d) ? v[0] * mask : real(0);
+        }
+        if (wd >= 0)
+#pragma unroll
+            for (int k = 0; k < 5; k++) v[k] = 0;
         SeedPt<real> p;
         real ir, pp, Tp;
         prim(ph, q[0], q[1], q[2], q[3], q[4], ir, p.U, p.V, p.W, pp, Tp);
@@ -1041,6 +1063,29 @@ __global__ __launch_bounds__(256) void k_gadj_add(QSet<real> b, QSet<real> a, lo
 }
 
 template <typename real>
+__global__ __launch_bounds__(256) void k_gadj_lincomb(QSet<real> out, real ca, QSet<real> a,
+                                                      real cb, QSet<real> b, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        real x[5], y[5] = {0, 0, 0, 0, 0};
+        qset_load(a, i, x);
+        if (b.m) qset_load(b, i, y);
+#pragma unroll
+        for (int k = 0; k < 5; k++) x[k] = ca * x[k] + (b.m ? cb * y[k] : real(0));
+        qset_store(out, i, x);
+    }
+}
+
+template <typename real>
+hipError_t launch_gadj_lincomb(const QSet<real> &out, real ca, const QSet<real> &a, real cb,
+                               const QSet<real> &b, long N, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_gadj_lincomb<real>), grid_for(N), dim3(256), 0, st, out, ca, a, cb, b, N);
+    return hipGetLastError();
+}
+
+template <typename real>
 hipError_t launch_gadj(int which, const GenAdj<real> &g, const AdjVjp<real> &a,
                        const Phys<real> &ph, long N, hipStream_t st)
 {
@@ -1092,7 +1137,9 @@ hipError_t launch_gadj_add(const QSet<real> &b, const QSet<real> &a, long N, hip
         const QSet<real> &, const StageArgs &, const Phys<real> &, double, double, long,         \
         hipStream_t);                                                                            \
     template hipError_t launch_gadj_add<real>(const QSet<real> &, const QSet<real> &, long,        \
-                                              hipStream_t);
+                                              hipStream_t);                                      \
+    template hipError_t launch_gadj_lincomb<real>(const QSet<real> &, real, const QSet<real> &,    \
+                                                  real, const QSet<real> &, long, hipStream_t);
 CS3D_INST_ADJ(float)
 CS3D_INST_ADJ(double)
 

@@ -118,6 +118,10 @@ struct GenAdj {
     real *pY[4], *pX[4];
     real *d1[12], *d2[12], *mx[6];
     real *s[4], *g1[G1_COUNT], *vs[5];
+    // the continuity cotangent of each direction's flux term: vs[0], except
+    // on wall planes, which keep only the normal term of their last wall
+    // face (vc[d] alias vs[0] without walls)
+    real *vc[3];
     real *bY[4], *bX[4];
     real *F[3][5];
     real *sigma;            // null without sponges
@@ -135,6 +139,17 @@ hipError_t launch_gadj_sponge_post(const QSet<real> &qf, const QSet<real> &A,
                                    hipStream_t st);
 template <typename real>
 hipError_t launch_gadj_add(const QSet<real> &b, const QSet<real> &a, long N, hipStream_t st);
+// out = ca a + cb b (b may be null: cb ignored; out may alias a or b)
+template <typename real>
+hipError_t launch_gadj_lincomb(const QSet<real> &out, real ca, const QSet<real> &a, real cb,
+                               const QSet<real> &b, long N, hipStream_t st);
+// Reverse of one wall face's correction (walls.hip: k_wall_face): lam, the
+// cotangent of the state after the face, becomes that of the state before
+// it, st (in place); the face's momenta and energy were overwritten, its
+// density kept, and its energy depends on the six inward temperatures.
+template <typename real>
+hipError_t launch_wall_face_vjp(const QSet<real> &st, const QSet<real> &lam, const Phys<real> &ph,
+                                const StageArgs &sa, const WallFace &w, hipStream_t s);
 
 template <typename real>
 hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,

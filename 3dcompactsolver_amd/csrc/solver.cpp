@@ -1036,14 +1036,17 @@ struct SolverT : SolverBase {
     GenAdj<real> gadj{};
     QSet<real> gl_avg{}, gl_stale{};
     const QSet<real> *gcur_S = nullptr;
+    QSet<real> gw_q{};
+    std::vector<QSet<real>> gw_keep;
 
     void adj_require()
     {
         need_ic();
-        bool general = sponge || std::getenv("CS3D_ADJ_GENERAL") != nullptr;
+        bool general = sponge || any_wall() || std::getenv("CS3D_ADJ_GENERAL") != nullptr;
         for (int d = 0; d < 3; d++) general |= cfg.bc_type[d] != CS3D_PERIODIC_SOLVE;
         if (cfg.compensated) throw std::invalid_argument("the adjoint does not cover the compensated mode");
-        if (any_wall()) throw std::invalid_argument("the adjoint does not cover walls");
+        if (any_wall() && sponge)
+            throw std::invalid_argument("the adjoint does not cover walls beside sponges");
         if (adj_ready) {
             if (general != adj_general) throw std::logic_error("adjoint path changed");
             return;
@@ -1070,6 +1073,7 @@ struct SolverT : SolverBase {
             for (int k = 0; k < 4; k++) g.s[k] = adj_alloc_field();
             for (int k = 0; k < G1_COUNT; k++) g.g1[k] = adj_alloc_field();
             for (int k = 0; k < 5; k++) g.vs[k] = adj_alloc_field();
+            for (int d = 0; d < 3; d++) g.vc[d] = any_wall() ? adj_alloc_field() : g.vs[0];
             for (int k = 0; k < 4; k++) g.bY[k] = adj_alloc_field();
             for (int k = 0; k < 4; k++) g.bX[k] = adj_alloc_field();
             for (int d = 0; d < 3; d++)
@@ -1079,6 +1083,12 @@ struct SolverT : SolverBase {
             if (sponge) {
                 gl_avg = adj_alloc_set();
                 gl_stale = adj_alloc_set();
+            }
+            // walls: the corrected stage state and the state before each face
+            if (any_wall()) {
+                gw_q = adj_alloc_set();
+                gw_keep.clear();
+                for (size_t f = 0; f < wall_order().size(); f++) gw_keep.push_back(adj_alloc_set());
             }
         } else {
             for (int pass = ADJ_YREC; pass <= ADJ_Y; pass++) {
@@ -1189,7 +1199,7 @@ struct SolverT : SolverBase {
         opT(D1T[2], 2, s2, G[G1_UX], c13, 1);  // Uxz
         opT(D1T[2], 2, s0, G[G1_XZ], c13, 1);  // Wxz
         opT(D1T[2], 2, s1, G[G1_YZ], c13, 1);  // Wyz
-        for (int c = 0; c < 5; c++) opT(D1T[2], 2, g.vs[c], g.F[2][c], -1, 0);
+        for (int c = 0; c < 5; c++) opT(D1T[2], 2, c ? g.vs[c] : g.vc[2], g.F[2][c], -1, 0);
         opT(D1T[1], 1, G[G1_XY], g.bY[0], 1, 1);
         opT(D1T[1], 1, G[G1_VY], g.bY[1], 1, 1);
         opT(D1T[1], 1, G[G1_YZ], g.bY[2], 1, 1);
@@ -1200,7 +1210,7 @@ struct SolverT : SolverBase {
         opT(D2T[1], 1, gq, g.bY[3], 1, 1);
         opT(D1T[1], 1, s1, G[G1_UX], c13, 1);  // Uxy
         opT(D1T[1], 1, s0, G[G1_XY], c13, 1);  // Vxy
-        for (int c = 0; c < 5; c++) opT(D1T[1], 1, g.vs[c], g.F[1][c], -1, 0);
+        for (int c = 0; c < 5; c++) opT(D1T[1], 1, c ? g.vs[c] : g.vc[1], g.F[1][c], -1, 0);
         opT(D1T[0], 0, G[G1_UX], g.bX[0], 1, 1);
         opT(D1T[0], 0, G[G1_XY], g.bX[1], 1, 1);
         opT(D1T[0], 0, G[G1_XZ], g.bX[2], 1, 1);
@@ -1209,7 +1219,7 @@ struct SolverT : SolverBase {
         opT(D2T[0], 0, s1, g.bX[1], 1, 1);
         opT(D2T[0], 0, s2, g.bX[2], 1, 1);
         opT(D2T[0], 0, gq, g.bX[3], 1, 1);
-        for (int c = 0; c < 5; c++) opT(D1T[0], 0, g.vs[c], g.F[0][c], -1, 0);
+        for (int c = 0; c < 5; c++) opT(D1T[0], 0, c ? g.vs[c] : g.vc[0], g.F[0][c], -1, 0);
         hc(launch_gadj<real>(GADJ_CHAIN, g, a, ph, N, st), "adjoint chain");
     }
 
@@ -1229,6 +1239,54 @@ struct SolverT : SolverBase {
 
     // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
     // with v = dt * lam given; a holds the epilogue (AdjVjp).
+    // preStepBCHandling on a copy: dst = W(src), the state before each face
+    // kept in gw_keep for wall_vjp
+    void wall_apply(const QSet<real> &src, const QSet<real> &dst)
+    {
+        copy_set(dst, src);
+        Fields<real> Fw = F;
+        Fw.q = dst;
+        Fw.qlo = QSet<real>{};
+        StageArgs sa = base;
+        sa.rkStep = 1;
+        sa.stale = 0;
+        const std::vector<int> order = wall_order();
+        for (size_t i = 0; i < order.size(); i++) {
+            copy_set(gw_keep[i], dst);
+            hc(launch_wall_face<real>(Fw, ph, sa, wall_face_of(order[i]), st), "walls");
+        }
+    }
+    // lam (in place): from the cotangent of W(src) to that of src, for the
+    // src of the last wall_apply
+    void wall_vjp(const QSet<real> &lam)
+    {
+        const std::vector<int> order = wall_order();
+        for (size_t i = order.size(); i-- > 0;)
+            hc(launch_wall_face_vjp<real>(gw_keep[i], lam, ph, base, wall_face_of(order[i]), st),
+               "walls reverse");
+    }
+    WallFace wall_face_of(int f) const
+    {
+        WallFace w;
+        w.dir = f / 2;
+        w.hi = f % 2;
+        wall_velocity(f, w.vel);
+        w.R = R_gas;
+        return w;
+    }
+
+    // The product J^T(W(Q)) (cv v) into out, then taken back through the
+    // wall correction: (d(dt R(W(Q)))/dQ)^T v
+    void vjp_walls(const QSet<real> &Q, const QSet<real> &v, real cv, const QSet<real> &out)
+    {
+        wall_apply(Q, gw_q);
+        AdjVjp<real> f{};
+        f.cv = cv;
+        f.out = out;
+        vjp(gw_q, v, f);
+        wall_vjp(out);
+    }
+
     void vjp(const QSet<real> &Q, const QSet<real> &v, AdjVjp<real> a, bool stale = false)
     {
         if (a.vn.m && (a.vn.m == v.m || a.vn.e == v.e))
@@ -1345,10 +1403,71 @@ struct SolverT : SolverBase {
         return r;
     }
 
+    // One step back with walls (the general path): every stage state was
+    // wall-corrected before its right-hand side, the first one in place
+    // (the RK combination builds on it).  RK4: with la the cotangent of
+    // the unfiltered result, r4 = W4^T J4^T (la/6), r3 = W3^T J3^T (la/3 +
+    // r4), r2 = W2^T J2^T (la/3 + r3/2), and lam = W1^T (la + r4 + r3 + r2 +
+    // J1^T (la/6 + r2/2)).  TVDRK3: Q3b = W3^T (2/3 la + J3^T (2/3 la)),
+    // Q2b = W2^T (Q3b/4 + J2^T (Q3b/4)), lam = W1^T (la/3 + 3/4 Q3b + Q2b +
+    // J1^T Q2b).  (J_s: the derivative of dt R at the corrected state.)
+    void adjoint_step_walls(const QSet<real> *stg, const StepRec &r, const QSet<real> &lam)
+    {
+        const real dt = (real)r.dt;
+        QSet<real> la = lam;
+        if (r.order >= 0) {
+            filter_transposed(lam, la5, sc5, r.order);
+            la = la5;
+        }
+        const QSet<real> X = vb5, Y = vb5b, acc = qb5;
+        const QSet<real> none{};
+        const long n = N;
+        auto lc = [&](const QSet<real> &o, real ca, const QSet<real> &a, real cb, const QSet<real> &b) {
+            hc(launch_gadj_lincomb<real>(o, ca, a, cb, b, n, st), "adjoint combination");
+        };
+        if (cfg.rk == CS3D_TVDRK3) {
+            lc(X, real(2) / 3, la, 0, none);                  // 2/3 la
+            wall_apply(stg[2], gw_q);
+            { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp(gw_q, X, f); }
+            lc(Y, 1, X, 1, Y);                                // 2/3 la + J3^T
+            wall_vjp(Y);                                      // Q3b
+            lc(acc, real(1) / 3, la, real(3) / 4, Y);         // la/3 + 3/4 Q3b
+            lc(X, real(1) / 4, Y, 0, none);                   // Q3b/4
+            wall_apply(stg[1], gw_q);
+            { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp(gw_q, X, f); }
+            lc(Y, 1, X, 1, Y);
+            wall_vjp(Y);                                      // Q2b
+            lc(acc, 1, acc, 1, Y);
+            wall_apply(stg[0], gw_q);
+            { AdjVjp<real> f{}; f.cv = dt; f.out = X; vjp(gw_q, Y, f); }
+            lc(lam, 1, acc, 1, X);
+            wall_vjp(lam);
+            return;
+        }
+        copy_set(acc, la);
+        vjp_walls(stg[3], la, dt / 6, Y);                     // r4
+        lc(acc, 1, acc, 1, Y);
+        lc(X, real(1) / 3, la, 1, Y);                         // la/3 + r4
+        vjp_walls(stg[2], X, dt, Y);                          // r3
+        lc(acc, 1, acc, 1, Y);
+        lc(X, real(1) / 3, la, real(1) / 2, Y);               // la/3 + r3/2
+        vjp_walls(stg[1], X, dt, Y);                          // r2
+        lc(acc, 1, acc, 1, Y);
+        lc(X, real(1) / 6, la, real(1) / 2, Y);               // la/6 + r2/2
+        wall_apply(stg[0], gw_q);
+        { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp(gw_q, X, f); }
+        lc(lam, 1, acc, 1, Y);
+        wall_vjp(lam);
+    }
+
     // lam (cotangent of q_{n+1}) -> cotangent of q_n for the step whose
     // stage states are stg
     void adjoint_step(const QSet<real> *stg, const StepRec &r, const QSet<real> &lam)
     {
+        if (adj_general && any_wall()) {
+            adjoint_step_walls(stg, r, lam);
+            return;
+        }
         const real rdt = (real)r.dt;
         QSet<real> la = lam;
         if (r.order >= 0) {
@@ -1457,7 +1576,11 @@ struct SolverT : SolverBase {
         const bool stale = sponge && set == 1 && stale_pending;
         gcur_S = stale ? &F.qstale : nullptr;
         if (sponge) zero_set(gl_avg);
-        vjp(set == 1 ? F.q : F.qk, la5, f, stale);
+        // with walls the map cs3d_rhs evaluates is Q -> dt R(W(Q))
+        if (any_wall())
+            vjp_walls(set == 1 ? F.q : F.qk, la5, f.cv, qb5);
+        else
+            vjp(set == 1 ? F.q : F.qk, la5, f, stale);
         download_set(qb5, out);
     }
 

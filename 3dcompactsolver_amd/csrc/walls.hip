@@ -20,7 +20,7 @@
 // pre-update copy there, and a wall point on such a plane writes its new
 // values into that copy as well, since the reference overwrites its
 // primitive arrays at wall points.
-#include "stage_common.hpp"
+#include "adjoint.hpp"
 
 namespace cs3d {
 
@@ -94,6 +94,101 @@ __global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph
     }
     if (sa.stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) qset_store(F.qstale, ip, qw);
 }
+
+// Reverse of k_wall_face (without stale planes) on the cotangent lam, given
+// the state st before the face: at a face point the momenta m = rho u_w and
+// the energy E = rho R T_w/(gamma - 1) + rho |u_w|^2 / 2 were set, so their
+// cotangents pass to the density and, through T_w = sum_m c_m T_m (the
+// calcNeumann weights), to the six inward points' states by T = p/(rho R),
+// p = (gamma - 1)(E - |m|^2/(2 rho)).  One thread per face point: a face
+// point's inward points belong to it alone.
+template <typename real>
+__global__ __launch_bounds__(256) void k_wall_face_vjp(QSet<real> st, QSet<real> lam,
+                                                       Phys<real> ph, StageArgs sa, WallFace w)
+{
+    const int Nx = sa.Nx, Ny = sa.Ny, Nz = sa.Nz;
+    const int na = w.dir == 0 ? Ny : Nx, nb = w.dir == 2 ? Ny : Nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int a = t % na, b = t / na;
+    int ijk[3];
+    if (w.dir == 0) {
+        ijk[0] = w.hi ? Nx - 1 : 0;
+        ijk[1] = a;
+        ijk[2] = b;
+    } else if (w.dir == 1) {
+        ijk[0] = a;
+        ijk[1] = w.hi ? Ny - 1 : 0;
+        ijk[2] = b;
+    } else {
+        ijk[0] = a;
+        ijk[1] = b;
+        ijk[2] = w.hi ? Nz - 1 : 0;
+    }
+    const long sxy = (long)Nx * Ny;
+    const long ip = (long)ijk[2] * sxy + (long)ijk[1] * Nx + ijk[0];
+    const long stride = w.dir == 0 ? 1 : w.dir == 1 ? (long)Nx : sxy;
+    const long in = w.hi ? -stride : stride;
+    const real cn[6] = {real(360) / real(147), real(-450) / real(147), real(400) / real(147),
+                        real(-225) / real(147), real(72) / real(147), real(-10) / real(147)};
+    real Tw = 0;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        real qo[5], ir, U, V, Wv, pp, Tp;
+        qset_load(st, ip + (long)(m + 1) * in, qo);
+        prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], ir, U, V, Wv, pp, Tp);
+        Tw += cn[m] * (ph.T0s + Tp);
+    }
+    real qf[5], l[5];
+    qset_load(st, ip, qf);
+    qset_load(lam, ip, l);
+    const real rho = ph.rho_ref + qf[0];
+    const real u[3] = {(real)w.vel[0], (real)w.vel[1], (real)w.vel[2]};
+    const real R = (real)w.R;
+    l[0] += l[1] * u[0] + l[2] * u[1] + l[3] * u[2] +
+            l[4] * (R * Tw / ph.gm1 + real(0.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    const real gT = l[4] * rho * R / ph.gm1;
+    l[1] = l[2] = l[3] = l[4] = 0;
+    qset_store(lam, ip, l);
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const long o = ip + (long)(m + 1) * in;
+        real qo[5], lo[5], ir, U, V, Wv, pp, Tp;
+        qset_load(st, o, qo);
+        qset_load(lam, o, lo);
+        prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], ir, U, V, Wv, pp, Tp);
+        const real r = ph.rho_ref + qo[0], irr = real(1) / (r * R);
+        const real Tb = gT * cn[m];
+        lo[4] += Tb * ph.gm1 * irr;
+        lo[1] -= Tb * ph.gm1 * U * irr;
+        lo[2] -= Tb * ph.gm1 * V * irr;
+        lo[3] -= Tb * ph.gm1 * Wv * irr;
+        lo[0] += Tb * (ph.gm1 * real(0.5) * (U * U + V * V + Wv * Wv) * irr -
+                       (ph.T0s + Tp) / r);
+        qset_store(lam, o, lo);
+    }
+}
+
+template <typename real>
+hipError_t launch_wall_face_vjp(const QSet<real> &st, const QSet<real> &lam, const Phys<real> &ph,
+                                const StageArgs &sa, const WallFace &w, hipStream_t s)
+{
+    if (w.dir < 0 || w.dir > 2) return hipErrorInvalidValue;
+    const int n = w.dir == 0 ? sa.Nx : w.dir == 1 ? sa.Ny : sa.Nz;
+    if (n < 7) return hipErrorInvalidValue;
+    const long pts = w.dir == 0 ? (long)sa.Ny * sa.Nz
+                                : w.dir == 1 ? (long)sa.Nx * sa.Nz : (long)sa.Nx * sa.Ny;
+    hipLaunchKernelGGL((k_wall_face_vjp<real>), dim3((unsigned)((pts + 255) / 256)), dim3(256), 0,
+                       s, st, lam, ph, sa, w);
+    return hipGetLastError();
+}
+
+template hipError_t launch_wall_face_vjp<float>(const QSet<float> &, const QSet<float> &,
+                                                const Phys<float> &, const StageArgs &,
+                                                const WallFace &, hipStream_t);
+template hipError_t launch_wall_face_vjp<double>(const QSet<double> &, const QSet<double> &,
+                                                 const Phys<double> &, const StageArgs &,
+                                                 const WallFace &, hipStream_t);
 
 template <typename real>
 hipError_t launch_wall_face(const Fields<real> &F, const Phys<real> &ph, const StageArgs &sa,

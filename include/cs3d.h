@@ -214,15 +214,17 @@ int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
 /* Discrete adjoint (no reference counterpart: the reference has no
  * adjoint; these serve gradient-based callers such as optimisation and
  * sensitivity studies).  Periodic or non-periodic lines (the boundary
- * closures transposed), with or without sponge faces; walls and the
- * compensated mode are refused (CS3D_ERR_ARG).
+ * closures transposed), sponge faces or walls (not both together); the
+ * compensated mode and walls beside sponges are refused (CS3D_ERR_ARG).
  *
  * cs3d_rhs_vjp: the transpose of the linearised stage map Q -> dt*RHS(Q)
  * (the map cs3d_rhs evaluates, with the dt of the last pre-step) applied to
  * the cotangents lam_*, at the conserved set `set` (1 = *1, 2 = *k).  With
  * sponges the averages and, at the first stage after a sponge update, the
  * pre-reset values the x derivatives read on the sponge planes are held
- * fixed.  Host fields in XYZ order; a NULL out_* skips that field. */
+ * fixed; with walls the map is Q -> dt*RHS(W(Q)), W the wall correction
+ * cs3d_rhs applies first.  Host fields in XYZ order; a NULL out_* skips
+ * that field. */
 int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *lam_rhoU,
                  const double *lam_rhoV, const double *lam_rhoW, const double *lam_rhoE,
                  double *out_rho, double *out_rhoU, double *out_rhoV, double *out_rhoW,
@@ -240,7 +242,8 @@ int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *l
  * ceil(nsteps/checkpoint) + S*checkpoint + 1 states (S = 4 for RK4, 3 for
  * TVDRK3; 5 fields each; with sponges 2 more per checkpoint and 3 more per
  * step of a segment) and 77 work fields on periodic lines without
- * sponges, 108 on the general path (non-periodic lines), 122 with sponges,
+ * sponges, 108 on the general path (non-periodic lines), 122 with sponges
+ * (with walls 111, and one state per wall face),
  * kept from call to call until cs3d_adjoint_release or cs3d_destroy.
  * The solver is left at the final state.  NULL outputs are skipped. */
 int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double *J, double *gU,

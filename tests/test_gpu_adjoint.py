@@ -135,9 +135,9 @@ def test_ke_gradient_fp32_tracks_fp64():
 
 
 def test_adjoint_rejects_unsupported_configurations():
-    """walls and the compensated mode are not reversed; neither are step
-    counts below one (sponges and non-periodic lines are:
-    test_gpu_adjoint_general.py)"""
+    """the compensated mode is not reversed; neither are step counts below
+    one (sponges, non-periodic lines and walls are:
+    test_gpu_adjoint_general.py, test_gpu_adjoint_walls.py)"""
     N = 16
     kw = adj_kw((N, N, N))
     g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
@@ -147,14 +147,6 @@ def test_adjoint_rejects_unsupported_configurations():
     g.close()
     g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP32, compensated=1, **kw))
     g.set_initial_conditions(*tgv_ic(N, kw["Lx"]))
-    with pytest.raises(cs3d.CS3DError):
-        g.ke_gradient(2, 1)
-    g.close()
-    # adiabatic walls on non-periodic y lines
-    kw = dict(Lx=20.0, Ly=16.0, Lz=8.0, alpha_f=0.49, mu_ref=1e-2, Pr=0.7, cfl=0.5,
-              filter_step=1, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 2, 2, 0, 0))
-    g = cs3d.Solver(cs3d.config(16, 24, 8, precision=cs3d.FP64, **kw))
-    g.set_initial_conditions(*shear_ic(16, 24, 8, 16.0, seed=1))
     with pytest.raises(cs3d.CS3DError):
         g.ke_gradient(2, 1)
 
