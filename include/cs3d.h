@@ -243,7 +243,7 @@ int cs3d_rhs_vjp(cs3d_solver *s, int set, const double *lam_rho, const double *l
  * TVDRK3; 5 fields each; with sponges 2 more per checkpoint and 3 more per
  * step of a segment) and 77 work fields on periodic lines without
  * sponges, 108 on the general path (non-periodic lines), 122 with sponges
- * (with walls 111, and one state per wall face),
+ * (with walls 116, and one state per wall face),
  * kept from call to call until cs3d_adjoint_release or cs3d_destroy.
  * The solver is left at the final state.  NULL outputs are skipped. */
 int cs3d_adjoint_ke_gradient(cs3d_solver *s, int nsteps, int checkpoint, double *J, double *gU,
