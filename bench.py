@@ -24,6 +24,11 @@ adjoint = cost of the kinetic-energy gradient through 40 RK4 steps of the
          (replay + reverse) / forward).
 fp32_compensated = the same FP32 workload in the compensated mode (state,
          accumulator and stage sums carried as FP32 hi + lo pairs).
+shear_layer = the temporal shear layer 512 x 128 x 42 in FP32 (sponges on
+         non-periodic y lines, ragged periodic z lines: the non-uniform
+         line path), forward step time, and its kinetic-energy gradient
+         through 20 steps with a checkpoint every 5 on the adjoint's
+         general path (transposed operators, sponge reverse).
 cpu_baseline = the oracle (FP64 OpenMP restatement of the reference, built
          with the reference's flags) timed on this host's cores at 256^3
          (1 warm-up + 3 steps), with a 128^3 sample beside it.
@@ -162,6 +167,53 @@ def run_adjoint(N, prec, ic, nsteps, checkpoint, fwd_ms):
             "gradient_over_forward": round(total / fwd, 2)}
 
 
+SHEAR_DIMS = (512, 128, 42)
+SHEAR_KW = dict(Lx=172.0, Ly=129.0, Lz=28.2, alpha_f=0.495, mu_ref=0.00375, Pr=0.7, cfl=0.8,
+                filter_step=5, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 1, 1, 0, 0),
+                sponge_frac=(0.25, 0.15, 0.25))
+
+
+def run_shear(steps, warmup, adj_steps=20, adj_checkpoint=5):
+    """the temporal shear layer 512 x 128 x 42 (y non-periodic between sponge
+    layers, z periodic on 42 points: the non-uniform line path) in FP32:
+    forward step time, and the kinetic-energy gradient on the adjoint's
+    general path (dt frozen at the first step's CFL dt; second call timed)"""
+    import ic as icmod
+    dims = SHEAR_DIMS
+    icf = icmod.shear_layer_ic(*dims, Ly=SHEAR_KW["Ly"], delta_u=1.2, delta0=1.0, seed=0)
+    g = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP32, **SHEAR_KW))
+    g.set_initial_conditions(*icf)
+    g.pre_step()
+    dt = g.time()[1]
+    kw = dict(SHEAR_KW, dt_fixed=dt)
+    g.close()
+    g = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP32, **kw))
+    g.set_initial_conditions(*icf)
+    g.step(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    npts = dims[0] * dims[1] * dims[2]
+    out = {"grid": list(dims), "ms_per_step": round(ms, 3),
+           "value": round(npts * 4 / (ms * 1e-3) / 1e6, 1), "unit": "Mpoint-RK-stages/s"}
+    for _ in range(2):
+        g.set_initial_conditions(*icf)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.ke_gradient(adj_steps, adj_checkpoint, outputs=False)
+        g.synchronize()
+        tot = (time.perf_counter() - t0) * 1e3
+    g.close()
+    fwd = adj_steps * ms
+    out["adjoint"] = {"steps": adj_steps, "checkpoint": adj_checkpoint,
+                      "gradient_ms": round(tot, 2), "forward_ms": round(fwd, 2),
+                      "reverse_over_forward": round((tot - fwd) / fwd, 2),
+                      "gradient_over_forward": round(tot / fwd, 2)}
+    return out
+
+
 def roofline(prof, N, prec):
     s = 4 if prec == cs3d.FP32 else 8
     acc = ACCESSES[prec]
@@ -244,6 +296,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--no-compensated", action="store_true",
                     help="skip the compensated FP32 leg")
+    ap.add_argument("--no-shear", action="store_true", help="skip the shear-layer leg")
     ap.add_argument("--adjoint-steps", type=int, default=40)
     ap.add_argument("--adjoint-checkpoint", type=int, default=20,
                     help="steps between adjoint checkpoints; 0: all steps (every stage state "
@@ -296,6 +349,7 @@ def main():
             cmp["ke_rel_err_vs_fp64"] = abs(kec - fp64["kinetic_energy"]) / abs(fp64["kinetic_energy"])
     ck = args.adjoint_checkpoint if args.adjoint_checkpoint > 0 else args.adjoint_steps
     adj = None if args.no_adjoint else run_adjoint(N, cs3d.FP32, ic, args.adjoint_steps, ck, ms32)
+    shear = None if args.no_shear else run_shear(args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_grid, args.cpu_steps)
@@ -326,6 +380,7 @@ def main():
         "fp64": fp64,
         "fp32_compensated": cmp,
         "adjoint": adj,
+        "shear_layer": shear,
         "roofline": roof,
         "hbm_stream_GBps": stream_bw,
         "cpu_baseline": cpu,
