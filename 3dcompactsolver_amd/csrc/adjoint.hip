@@ -1,8 +1,12 @@
-// adjoint.hip -- pointwise kernels of the discrete adjoint of the stage
-// right-hand side (no reference counterpart: the reference has no adjoint).
+// adjoint.hip -- kernels of the discrete adjoint of the stage right-hand
+// side (no reference counterpart: the reference has no adjoint): the fused
+// five line passes of periodic grids without sponges or walls, and the
+// pointwise kernels of the general path (non-periodic lines, sponges,
+// walls), which applies its derivatives with the standalone and transposed
+// operators.
 //
-// The forward right-hand side (CSolver_AWS.cpp:1246-1504, periodic, no
-// sponge) is, with u = m/rho, p = (gamma-1)(E - |m|^2/(2 rho)),
+// The forward right-hand side (CSolver_AWS.cpp:1246-1504, away from sponge
+// and wall planes) is, with u = m/rho, p = (gamma-1)(E - |m|^2/(2 rho)),
 // T = p/(rho R), mu = mu_ref T^0.76, a = dmu/dT = 0.76 mu/T:
 //   R_rho = -sum_j D_j(m_j)
 //   R_i   = -sum_j D_j(m_i u_j + p delta_ij) + Rv_i
@@ -12,9 +16,10 @@
 //            + a (terms linear in grad T times velocity gradients)
 // and QUAD the quadratic form of the velocity gradients.  Every
 // derivative is a compact operator applied to a field; the adjoint applies
-// the transposed operators (periodic D1 is skew-adjoint, D2 self-adjoint)
-// to the cotangent seeds these kernels form, then chains back through the
-// pointwise maps.  The cotangent v of the stage increment dt*R enters
+// the transposed operators (on periodic lines D1 is skew-adjoint and D2
+// self-adjoint, which the fused passes use; the general path applies the
+// transposed operators themselves) to the cotangent seeds these kernels
+// form, then chains back through the pointwise maps.  The cotangent v of the stage increment dt*R enters
 // already multiplied by dt.
 #include "adjoint.hpp"
 

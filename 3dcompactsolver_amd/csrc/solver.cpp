@@ -984,16 +984,18 @@ struct SolverT : SolverBase {
     }
 
     // ------------------------------------------------------------------
-    // Discrete adjoint.  The transpose of the discrete stage map on periodic
-    // lines without sponges, as five line passes per vector-Jacobian
-    // product (adjoint.hpp): the y and z derivative fields of the stage
+    // Discrete adjoint.  The transpose of the discrete stage map.  On
+    // periodic grids without sponges or walls, five line passes per
+    // vector-Jacobian product (adjoint.hpp): the y and z derivative fields of the stage
     // state recomputed, the x derivatives with the cotangent seeds and all
     // x-direction transposed terms, then the z and y transposed terms
     // (periodic D1 is skew-adjoint, D2 and the filter self-adjoint), each
     // pass chaining its share back to the conserved state.  RK4 and TVDRK3
     // are reversed stage by stage with the forward dt sequence frozen; the stage states of
     // each checkpoint segment are replayed with the fused forward stages and
-    // kept.  Cotangents are conserved sets (a quad and the energy).
+    // kept.  Cotangents are conserved sets (a quad and the energy).  Other
+    // grids (non-periodic lines, sponges, walls) take the general path
+    // below (vjp_general).
 
     std::vector<void *> adjalloc;
     bool adj_ready = false;

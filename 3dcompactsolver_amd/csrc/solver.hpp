@@ -39,9 +39,10 @@ struct SolverBase {
     virtual void *stream() = 0;
     virtual void profile_enable(int on) = 0;
     virtual void profile_report(double *ms, int *counts) = 0;
-    // discrete adjoint (periodic lines, no sponge): lam^T d(dt*RHS)/dQ for
-    // the conserved set `set`, and the gradient of the final volume-averaged
-    // kinetic energy after nsteps with respect to the initial velocities
+    // discrete adjoint (not the compensated mode, not walls beside sponges):
+    // lam^T d(dt*RHS)/dQ for the conserved set `set`, and the gradient of
+    // the final volume-averaged kinetic energy after nsteps with respect to
+    // the initial velocities
     virtual void rhs_vjp(int set, const double *const *lam, double *const *out) = 0;
     virtual void ke_gradient(int nsteps, int checkpoint, double *J, double *gU, double *gV,
                              double *gW) = 0;

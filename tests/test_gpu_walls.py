@@ -119,15 +119,26 @@ def test_couette_fp32_tracks_fp64():
     assert max(errs) < 1e-5
 
 
-def test_adjoint_refuses_walls():
-    """the adjoint does not cover walls (here on periodic z lines, which the
-    reference allows, so that the periodic-lines check passes)"""
+def test_adjoint_with_walls_on_periodic_lines_matches_oracle_adjoint():
+    """walls on the faces of periodic z lines (the reference allows them):
+    the gradient through three steps against the numpy adjoint of the
+    oracle (oracle/adjoint.py) from the same initial state"""
+    import adjoint as oa
     dims = (20, 18, 16)
     kw = dict(Lx=2.0, Ly=2.0, Lz=2.0, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
               filter_step=1, rk=4, bc_type=(0, 0, 0), bc_kind=(0, 0, 0, 0, ADIA, MOVE),
-              wall_vel=VEL)
+              wall_vel=VEL, dt_fixed=0.005)
+    ic = random_ic((16, 18, 20), 3, amp=0.1)
     g = cs3d.Solver(cs3d.config(*dims, precision=cs3d.FP64, **kw))
-    g.set_initial_conditions(*random_ic((16, 18, 20), 3, amp=0.1))
-    with pytest.raises(cs3d.CS3DError, match="wall"):
-        g.ke_gradient(1, 1)
+    g.set_initial_conditions(*ic)
+    q0 = g.conserved(1)
+    J, *grad = g.ke_gradient(3, 2)
     g.close()
+    keep = {k: v for k, v in kw.items() if k not in ("cfl", "dt_fixed")}
+    A = oa.Adjoint(po.make_config(*dims, **keep))
+    Jo, *gref = A.ke_gradient(ic, 3, 0.005, 1, q0=q0)
+    assert abs(J - Jo) <= 1e-13 * abs(Jo)
+    errs = [relerr(a, b) for a, b in zip(grad, gref)]
+    print("walls on periodic z, gradient vs oracle adjoint: " + " ".join(f"{e:.2e}" for e in errs),
+          flush=True)
+    assert max(errs) <= 1e-10, errs
