@@ -112,8 +112,11 @@ constexpr size_t LDS_BYTES = 160 * 1024;
 // not fit the LDS) per workgroup, ring margin kr, `planes` values per ring
 // position, and room for stage_per_lane window values per lane
 // (load_windows_lds) in the same LDS; LX = 0 when a line does not fit.
+// pow2: LX rounded down to a power of two (column sweeps: a workgroup's
+// lines are adjacent x points, and 8 or 16 of them fill aligned 32- or
+// 64-byte pieces of a row where 12 or 15 straddle them).
 inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esize,
-                            int stage_per_lane = 0)
+                            int stage_per_lane = 0, bool pow2 = false)
 {
     auto bytes = [&](int LX) {
         const size_t ring = ring_bytes(P, kr, LX, planes, esize);
@@ -123,7 +126,9 @@ inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esiz
     LaunchGeom g;
     g.LX = P > 0 ? max_threads / P : 0;
     if (g.LX > 64) g.LX = 64;
-    while (g.LX > 0 && bytes(g.LX) > LDS_BYTES) g.LX--;
+    if (pow2)
+        while (g.LX & (g.LX - 1)) g.LX &= g.LX - 1;
+    while (g.LX > 0 && bytes(g.LX) > LDS_BYTES) g.LX = pow2 ? g.LX / 2 : g.LX - 1;
     g.threads = g.LX * P;
     g.shm = g.LX > 0 ? bytes(g.LX) : 0;
     return g;

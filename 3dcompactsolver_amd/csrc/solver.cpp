@@ -151,14 +151,14 @@ struct SolverT : SolverBase {
                 if (fx_ok && P != 64) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
             }
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
-                                  SWEEP_STAGE_FIELDS * D1[d].host.M);
+                                  SWEEP_STAGE_FIELDS * D1[d].host.M, d > 0);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             sweepG_small[d] = sweepG[d];
             if (d > 0) {
                 sweepG_small[d] = line_geom(P, kr,
                                             std::min(sweep_max_threads(d, P), sweep_threads<real>() / 2),
                                             SWEEP_PLANES, sizeof(real),
-                                            SWEEP_STAGE_FIELDS * D1[d].host.M);
+                                            SWEEP_STAGE_FIELDS * D1[d].host.M, true);
                 if (sweepG_small[d].LX < 1)
                     throw std::invalid_argument("line too long for the fused sweeps");
             }

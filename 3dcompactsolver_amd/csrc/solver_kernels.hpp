@@ -138,12 +138,17 @@ struct WallFace {
 
 // XCD-aware workgroup order: the hardware hands consecutive workgroups to
 // the 8 XCDs in turn; remapping makes workgroups that share 128-byte lines
-// (neighbouring line batches) run on the same XCD and share its L2.
+// (neighbouring line batches) run on the same XCD and share its L2.  A grid
+// that is not a multiple of 8 keeps the remap on its first 8 * (nb / 8)
+// workgroups (batches of LX lines whose rows straddle cache lines, as with
+// 15 lines of 68 segments, share lines with both neighbours) and runs the
+// last nb % 8 in place.
 __device__ __forceinline__ int xcd_block(int b, int nb)
 {
     const int nx = 8;
-    if (nb < nx || nb % nx) return b;
-    return (b % nx) * (nb / nx) + b / nx;
+    const int n8 = nb - nb % nx;
+    if (n8 < nx || b >= n8) return b;
+    return (b % nx) * (n8 / nx) + b / nx;
 }
 
 // most solves batched through one exchange in a sweep kernel
