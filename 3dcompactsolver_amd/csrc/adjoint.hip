@@ -317,8 +317,11 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
     CS3D_LANE_SETUP(MAP_ROW, d1, sweep_kr(d1, d2), ADJ_X_PLANES)
     (void)per;
     // r: Ux Vx Wx Tx | Uxy Uxz Vxy Wxz | Uxx Vxx Wxx Txx; own values of the
-    // state and of Uy Ty Vy Wy Uz Vz Wz Tz
-    real r[12][M], qo[5][M], yz[8][M];
+    // state and of Uy Ty Vy Wy Uz Vz Wz Tz (held across the first solve
+    // batch on uniform plans; the table variants, whose solves need more
+    // registers, read them again per point)
+    constexpr bool HOLD = UNI;
+    real r[12][M], qo[5][M], yz[HOLD ? 8 : 1][M];
     {
         real pw[4][W], qw[5][W];
         prim_windows(win, a.q, ph, pw, qw);
@@ -335,26 +338,32 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
     {
         real w0[W], w1[W], w2[W], w3[W];
         win.load_pair(a.rec.dYut, w0, w1);
+        if constexpr (HOLD) {
 #pragma unroll
-        for (int t = 0; t < M; t++) {
-            yz[0][t] = w0[HW + t];
-            yz[1][t] = w1[HW + t];
+            for (int t = 0; t < M; t++) {
+                yz[0][t] = w0[HW + t];
+                yz[1][t] = w1[HW + t];
+            }
         }
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[4]);
         win.load_pair(a.rec.dYvw, w0, w1);
+        if constexpr (HOLD) {
 #pragma unroll
-        for (int t = 0; t < M; t++) {
-            yz[2][t] = w0[HW + t];
-            yz[3][t] = w1[HW + t];
+            for (int t = 0; t < M; t++) {
+                yz[2][t] = w0[HW + t];
+                yz[3][t] = w1[HW + t];
+            }
         }
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[6]);
         win.load_quad(a.rec.dZ1, w0, w1, w2, w3);
+        if constexpr (HOLD) {
 #pragma unroll
-        for (int t = 0; t < M; t++) {
-            yz[4][t] = w0[HW + t];
-            yz[5][t] = w1[HW + t];
-            yz[6][t] = w2[HW + t];
-            yz[7][t] = w3[HW + t];
+            for (int t = 0; t < M; t++) {
+                yz[4][t] = w0[HW + t];
+                yz[5][t] = w1[HW + t];
+                yz[6][t] = w2[HW + t];
+                yz[7][t] = w3[HW + t];
+            }
         }
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w0, r[5]);
         rhs<OP_D1, real, M, HW, UNI>(d1, c, w2, r[7]);
@@ -401,10 +410,18 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        real y2[4], z2[4], myz[2];
+        real y2[4], z2[4], myz[2], yt[8];
         pt_quad(win, a.rec.dY2, t, y2);
         pt_quad(win, a.rec.dZ2, t, z2);
         pt_pair(win, a.rec.mYZ, t, myz);
+        if constexpr (HOLD) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) yt[k] = yz[k][t];
+        } else {
+            pt_pair(win, a.rec.dYut, t, yt);
+            pt_pair(win, a.rec.dYvw, t, yt + 2);
+            pt_quad(win, a.rec.dZ1, t, yt + 4);
+        }
         SeedPt<real> p;
         real ir, pp;
         prim(ph, qo[0][t], qo[1][t], qo[2][t], qo[3][t], qo[4][t], ir, p.U, p.V, p.W, pp, p.Tp);
@@ -420,14 +437,14 @@ __global__ __launch_bounds__(ADJ_X_THREADS) void k_adj_x(const OpDev<real> *__re
             p.d2[3 * f + 1] = y2[f];
             p.d2[3 * f + 2] = z2[f];
         }
-        p.d1[1] = yz[0][t];   // Uy
-        p.d1[2] = yz[4][t];   // Uz
-        p.d1[4] = yz[2][t];   // Vy
-        p.d1[5] = yz[5][t];   // Vz
-        p.d1[7] = yz[3][t];   // Wy
-        p.d1[8] = yz[6][t];   // Wz
-        p.d1[10] = yz[1][t];  // Ty
-        p.d1[11] = yz[7][t];  // Tz
+        p.d1[1] = yt[0];   // Uy
+        p.d1[2] = yt[4];   // Uz
+        p.d1[4] = yt[2];   // Vy
+        p.d1[5] = yt[5];   // Vz
+        p.d1[7] = yt[3];   // Wy
+        p.d1[8] = yt[6];   // Wz
+        p.d1[10] = yt[1];  // Ty
+        p.d1[11] = yt[7];  // Tz
         p.mx[0] = r[4][t];    // Uxy
         p.mx[1] = r[5][t];    // Uxz
         p.mx[2] = r[6][t];    // Vxy

@@ -86,7 +86,7 @@ constexpr int ADJ_X_KEEP = 15;
 // kept values
 inline size_t adj_x_lds(int P, int kr, int LX, int M, int esize)
 {
-    const size_t ring = ring_bytes(P, kr, LX, ADJ_X_PLANES, esize);
+    const size_t ring = ring_bytes(P, kr, LX, ADJ_X_PLANES, esize, false);
     const size_t stage = (size_t)ADJ_X_STAGE * M * LX * P * esize;
     return (ring > stage ? ring : stage) + (size_t)ADJ_X_KEEP * M * LX * P * esize;
 }

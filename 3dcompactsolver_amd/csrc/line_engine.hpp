@@ -69,10 +69,14 @@ __host__ __device__ constexpr int inv_mod32(int s)
 // LDS banks: in MAP_COL they are 32 / LX consecutive segments of LX lines, at
 // offsets line * row * S + g * S; in MAP_ROW consecutive segments of one
 // line, at offsets g * S.  A workgroup of one line needs no padding (its
-// lanes are consecutive segments at offsets g * S either way).
-__host__ __device__ inline int ring_row(int P, int KR, int LX, int S)
+// lanes are consecutive segments at offsets g * S either way), and in a
+// MAP_COL workgroup of 32 or more lines a half-wavefront holds one segment
+// of 32 lines, for which any odd row * S will do (col: row odd, at most one
+// position of padding instead of up to 31).
+__host__ __device__ inline int ring_row(int P, int KR, int LX, int S, bool col)
 {
     if (LX == 1) return P + KR;
+    if (col && LX >= 32) return (P + KR) | 1;
     const int want = LX >= 32 ? 1 : 32 / LX;
     const int r = P + KR;
     const int target = (want * inv_mod32(S)) & 31;
@@ -81,20 +85,21 @@ __host__ __device__ inline int ring_row(int P, int KR, int LX, int S)
 
 // Elements of one parity of the exchange buffer, for batches of up to
 // `planes` values per position.
-__host__ __device__ inline int ring_cap(int P, int KR, int LX, int planes)
+__host__ __device__ inline int ring_cap(int P, int KR, int LX, int planes, bool col)
 {
     int best = 0;
     for (int S = 1; S <= (planes | 1); S += 2) {
-        const int e = ring_row(P, KR, LX, S) * S;
+        const int e = ring_row(P, KR, LX, S, col) * S;
         best = e > best ? e : best;
     }
     return LX * best;
 }
 
 // LDS bytes of a workgroup's exchange buffers (two parities).
-__host__ __device__ inline size_t ring_bytes(int P, int KR, int LX, int planes, int esize)
+__host__ __device__ inline size_t ring_bytes(int P, int KR, int LX, int planes, int esize,
+                                             bool col)
 {
-    return 2 * (size_t)ring_cap(P, KR, LX, planes) * esize;
+    return 2 * (size_t)ring_cap(P, KR, LX, planes, col) * esize;
 }
 
 // Workgroup shape of a line-solve launch: LX lines x P segments per
@@ -112,14 +117,15 @@ constexpr size_t LDS_BYTES = 160 * 1024;
 // not fit the LDS) per workgroup, ring margin kr, `planes` values per ring
 // position, and room for stage_per_lane window values per lane
 // (load_windows_lds) in the same LDS; LX = 0 when a line does not fit.
+// map: the lane layout (MAP_COL or MAP_ROW) of the kernel launched.
 // pow2: LX rounded down to a power of two (column sweeps: a workgroup's
 // lines are adjacent x points, and 8 or 16 of them fill aligned 32- or
 // 64-byte pieces of a row where 12 or 15 straddle them).
-inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esize,
+inline LaunchGeom line_geom(int P, int kr, int max_threads, int planes, int esize, int map,
                             int stage_per_lane = 0, bool pow2 = false)
 {
     auto bytes = [&](int LX) {
-        const size_t ring = ring_bytes(P, kr, LX, planes, esize);
+        const size_t ring = ring_bytes(P, kr, LX, planes, esize, map == MAP_COL);
         const size_t stage = (size_t)stage_per_lane * LX * P * esize;
         return ring > stage ? ring : stage;
     };
@@ -160,7 +166,7 @@ __device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restri
     c.P = P;
     c.LX = LX;
     c.KR = KR;
-    c.cap = ring_cap(P, KR, LX, planes);
+    c.cap = ring_cap(P, KR, LX, planes, MAP == MAP_COL);
     c.par = 0;
     c.xb = xb;
     if (op->uniform) {
@@ -737,7 +743,7 @@ __device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 
                                          int K1, const real *__restrict__ w2, int K2, int cyc)
 {
     constexpr int N = N1 + N2, S = N | 1;
-    const int row = ring_row(c.P, c.KR, c.LX, S);
+    const int row = ring_row(c.P, c.KR, c.LX, S, MAP == MAP_COL);
     real *b = c.xb + c.par * c.cap;
     c.par ^= 1;
     const int prim = (c.line * row + (FWD ? c.KR + c.g : c.g)) * S;

@@ -154,12 +154,18 @@ void make_plan(PlanDev<real> &pl, int kind, int periodic, int n, double dh, doub
     hip_check(hipMemcpy(pl.dev, &d, sizeof(d), hipMemcpyHostToDevice), "hipMemcpy(plan)");
     // one workgroup: up to 1024 lanes, at most 64 lines
     const int kc = kind == OP_FILTER ? kc_filter<real>() : kc_deriv<real>();
-    LaunchGeom g = line_geom(h.P, std::max(h.K, kc), 1024, 1, sizeof(real));
+    // (either lane layout: the ring of the larger)
+    LaunchGeom g = line_geom(h.P, std::max(h.K, kc), 1024, 1, sizeof(real), MAP_ROW);
+    if (g.LX >= 1) {
+        const LaunchGeom gc = line_geom(h.P, std::max(h.K, kc), 1024, 1, sizeof(real), MAP_COL);
+        g.shm = std::max(g.shm, gc.shm);
+    }
     if (g.LX < 1) throw std::invalid_argument("too many segments per line");
     if (transpose) {
         // the solved lines (n + 1 values each) go to LDS after the ring
         auto bytes = [&](int LX) {
-            return ring_bytes(h.P, std::max(h.K, kc), LX, 1, sizeof(real)) +
+            return std::max(ring_bytes(h.P, std::max(h.K, kc), LX, 1, sizeof(real), false),
+                            ring_bytes(h.P, std::max(h.K, kc), LX, 1, sizeof(real), true)) +
                    (size_t)LX * (h.n + 1) * sizeof(real);
         };
         while (g.LX > 0 && bytes(g.LX) > LDS_BYTES) g.LX--;

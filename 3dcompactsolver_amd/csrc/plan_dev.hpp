@@ -84,6 +84,31 @@ hipError_t launch_field_opT(const PlanDev<real> &pl, int map, const LineMap &lm,
                             const real *in, real *out, real coef, int accumulate,
                             hipStream_t st);
 
+// Batched operators along one direction (ops_batch.hip): the first- and
+// second-derivative plans d1, d2 of one direction (same segmentation)
+// applied to several fields in one solve batch.  Outputs must be distinct
+// from each other and from every input.
+constexpr int OPB_MAX = 12;
+template <typename real>
+struct OpBatch {
+    const real *in[OPB_MAX];
+    real *out[OPB_MAX];
+    real coef[OPB_MAX];  // transposed launches: weight of each term
+    int acc[OPB_MAX];    // transposed launches: out[f] += (1) or = (0)
+};
+// out[i] = D1(in[i]) for i < n1, out[n1 + i] = D2(in[i]) for i < n2
+// (n1 = 4, 6 or 8; n2 = 4)
+template <typename real>
+hipError_t launch_ops_fwd(const PlanDev<real> &d1, const PlanDev<real> &d2, int map,
+                          const LineMap &lm, int n1, int n2, const OpBatch<real> &b,
+                          hipStream_t st);
+// transposed plans.  n2 = 0 (n1 = 5, 7 or 9): out[i] (+)= coef[i] D1^T(in[i]);
+// n1 = n2 = 4: out[f] (+)= coef[f] D1^T(in[f]) + coef[4 + f] D2^T(in[4 + f])
+template <typename real>
+hipError_t launch_ops_T(const PlanDev<real> &d1, const PlanDev<real> &d2, int map,
+                        const LineMap &lm, int n1, int n2, const OpBatch<real> &b,
+                        hipStream_t st);
+
 // line maps of an Nx*Ny*Nz field in XYZ order (Macros.hpp:8), and of the
 // reference's per-direction layouts YZX / ZXY (Macros.hpp:9-10)
 LineMap xyz_lines(int dir, int Nx, int Ny, int Nz);

@@ -151,13 +151,13 @@ struct SolverT : SolverBase {
                 if (fx_ok && P != 64) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
             }
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
-                                  SWEEP_STAGE_FIELDS * D1[d].host.M, d > 0);
+                                  map_of(d), SWEEP_STAGE_FIELDS * D1[d].host.M, d > 0);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             sweepG_small[d] = sweepG[d];
             if (d > 0) {
                 sweepG_small[d] = line_geom(P, kr,
                                             std::min(sweep_max_threads(d, P), sweep_threads<real>() / 2),
-                                            SWEEP_PLANES, sizeof(real),
+                                            SWEEP_PLANES, sizeof(real), MAP_COL,
                                             SWEEP_STAGE_FIELDS * D1[d].host.M, true);
                 if (sweepG_small[d].LX < 1)
                     throw std::invalid_argument("line too long for the fused sweeps");
@@ -166,7 +166,7 @@ struct SolverT : SolverBase {
             filterG[d] = line_geom(FL[d].host.P, std::max(FL[d].host.K, kc_filter<real>()),
                                    c.compensated ? std::min(filter_max_threads(d), 512)
                                                  : filter_max_threads(d),
-                                   5, sizeof(real));
+                                   5, sizeof(real), map_of(d));
             if (filterG[d].LX < 1) throw std::invalid_argument("line too long for the filter");
             lm[d] = xyz_lines(d, Nx, Ny, Nz);
         }
@@ -1101,7 +1101,7 @@ struct SolverT : SolverBase {
                 const int kr = std::max(std::max(h1.K, h2.K), kc_deriv<real>());
                 LaunchGeom &g = adjG[pass];
                 if (pass == ADJ_X) {
-                    g = line_geom(h1.P, kr, adj_x_threads(h1.P), ADJ_X_PLANES, sizeof(real));
+                    g = line_geom(h1.P, kr, adj_x_threads(h1.P), ADJ_X_PLANES, sizeof(real), MAP_ROW);
                     while (g.LX > 0 && adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) > LDS_BYTES) g.LX--;
                     g.threads = g.LX * h1.P;
                     g.shm = g.LX > 0 ? adj_x_lds(h1.P, kr, g.LX, h1.M, sizeof(real)) : 0;
@@ -1109,10 +1109,10 @@ struct SolverT : SolverBase {
                 else if (pass == ADJ_YREC || pass == ADJ_ZREC)
                     g = line_geom(h1.P, kr,
                                   env_cap("CS3D_ADJ_THREADS", adj_rec_threads<real>(h1.uniform != 0)),
-                                  ADJ_R_PLANES, sizeof(real));
+                                  ADJ_R_PLANES, sizeof(real), MAP_COL);
                 else
                     g = line_geom(h1.P, kr, env_cap("CS3D_ADJ_THREADS", adj_col_threads<real>()),
-                                  ADJ_T_PLANES, sizeof(real));
+                                  ADJ_T_PLANES, sizeof(real), MAP_COL);
                 if (g.LX < 1) throw std::invalid_argument("line too long for the adjoint line kernels");
             }
             arec.dYut = adj_alloc_field(2);
@@ -1165,63 +1165,112 @@ struct SolverT : SolverBase {
         }
         if (sponge) g.resA = gl_avg;
         hc(launch_gadj<real>(GADJ_PRIMS, g, a, ph, N, st), "adjoint prims");
-        // derivative fields: x derivatives of the x block's set
-        for (int f = 0; f < 4; f++) {
-            opF(D1[0], 0, g.pX[f], g.d1[3 * f]);
-            opF(D2[0], 0, g.pX[f], g.d2[3 * f]);
-            for (int d = 1; d < 3; d++) {
-                opF(D1[d], d, g.pY[f], g.d1[3 * f + d]);
-                opF(D2[d], d, g.pY[f], g.d2[3 * f + d]);
+        // derivative fields, one batched launch per direction: x derivatives
+        // of the x block's set; then y, with the mixed Uxy = D_y Ux and
+        // Vxy = D_y Vx; then z, with Uxz, Vyz, Wxz, Wyz = D_z of Ux, Vy, Wx, Wy
+        {
+            OpBatch<real> b{};
+            for (int f = 0; f < 4; f++) {
+                b.in[f] = g.pX[f];
+                b.out[f] = g.d1[3 * f];
+                b.out[4 + f] = g.d2[3 * f];
             }
+            hc(launch_ops_fwd<real>(D1[0], D2[0], map_of(0), lm[0], 4, 4, b, st), "x derivatives");
+            b = OpBatch<real>{};
+            for (int f = 0; f < 4; f++) {
+                b.in[f] = g.pY[f];
+                b.out[f] = g.d1[3 * f + 1];
+                b.out[6 + f] = g.d2[3 * f + 1];
+            }
+            b.in[4] = g.d1[0];
+            b.out[4] = g.mx[0];
+            b.in[5] = g.d1[3];
+            b.out[5] = g.mx[2];
+            hc(launch_ops_fwd<real>(D1[1], D2[1], map_of(1), lm[1], 6, 4, b, st), "y derivatives");
+            b = OpBatch<real>{};
+            for (int f = 0; f < 4; f++) {
+                b.in[f] = g.pY[f];
+                b.out[f] = g.d1[3 * f + 2];
+                b.out[8 + f] = g.d2[3 * f + 2];
+            }
+            const int src[4] = {0, 4, 6, 7}, dst[4] = {1, 3, 4, 5};
+            for (int k = 0; k < 4; k++) {
+                b.in[4 + k] = g.d1[src[k]];
+                b.out[4 + k] = g.mx[dst[k]];
+            }
+            hc(launch_ops_fwd<real>(D1[2], D2[2], map_of(2), lm[2], 8, 4, b, st), "z derivatives");
         }
-        // mixed: Uxy Uxz Vxy Vyz Wxz Wyz = D_y Ux, D_z Ux, D_y Vx, D_z Vy, D_z Wx, D_z Wy
-        opF(D1[1], 1, g.d1[0], g.mx[0]);
-        opF(D1[2], 2, g.d1[0], g.mx[1]);
-        opF(D1[1], 1, g.d1[3], g.mx[2]);
-        opF(D1[2], 2, g.d1[4], g.mx[3]);
-        opF(D1[2], 2, g.d1[6], g.mx[4]);
-        opF(D1[2], 2, g.d1[7], g.mx[5]);
         hc(launch_gadj<real>(GADJ_SEEDS, g, a, ph, N, st), "adjoint seeds");
-        // transposed terms: z, then y, then x.  A shared first-derivative
-        // seed (g1XY for Uy and Vx, g1XZ for Uz and Wx, g1YZ for Vz and Wy)
-        // is used along its first direction before the mixed terms of its
-        // other use are added to it.
+        // transposed terms, two batched launches per direction, z, then y,
+        // then x: the primitive cotangents (b += D1^T of a first-derivative
+        // seed + D2^T of a second-derivative one), then the mixed terms and
+        // the flux cotangents.  A shared first-derivative seed (g1XY for Uy
+        // and Vx, g1XZ for Uz and Wx, g1YZ for Vz and Wy) is read along its
+        // first direction before the mixed terms of its other use are added
+        // to it.
         const real c13 = real(1) / real(3), c43 = real(4) / real(3);
         real *const *G = g.g1;
         real *s0 = g.s[0], *s1 = g.s[1], *s2 = g.s[2], *gq = g.s[3];
-        opT(D1T[2], 2, G[G1_XZ], g.bY[0], 1, 1);
-        opT(D1T[2], 2, G[G1_YZ], g.bY[1], 1, 1);
-        opT(D1T[2], 2, G[G1_WZ], g.bY[2], 1, 1);
-        opT(D1T[2], 2, G[G1_TZ], g.bY[3], 1, 1);
-        opT(D2T[2], 2, s0, g.bY[0], 1, 1);
-        opT(D2T[2], 2, s1, g.bY[1], 1, 1);
-        opT(D2T[2], 2, s2, g.bY[2], c43, 1);
-        opT(D2T[2], 2, gq, g.bY[3], 1, 1);
-        opT(D1T[2], 2, s2, G[G1_VY], c13, 1);  // Vyz
-        opT(D1T[2], 2, s2, G[G1_UX], c13, 1);  // Uxz
-        opT(D1T[2], 2, s0, G[G1_XZ], c13, 1);  // Wxz
-        opT(D1T[2], 2, s1, G[G1_YZ], c13, 1);  // Wyz
-        for (int c = 0; c < 5; c++) opT(D1T[2], 2, c ? g.vs[c] : g.vc[2], g.F[2][c], -1, 0);
-        opT(D1T[1], 1, G[G1_XY], g.bY[0], 1, 1);
-        opT(D1T[1], 1, G[G1_VY], g.bY[1], 1, 1);
-        opT(D1T[1], 1, G[G1_YZ], g.bY[2], 1, 1);
-        opT(D1T[1], 1, G[G1_TY], g.bY[3], 1, 1);
-        opT(D2T[1], 1, s0, g.bY[0], 1, 1);
-        opT(D2T[1], 1, s1, g.bY[1], c43, 1);
-        opT(D2T[1], 1, s2, g.bY[2], 1, 1);
-        opT(D2T[1], 1, gq, g.bY[3], 1, 1);
-        opT(D1T[1], 1, s1, G[G1_UX], c13, 1);  // Uxy
-        opT(D1T[1], 1, s0, G[G1_XY], c13, 1);  // Vxy
-        for (int c = 0; c < 5; c++) opT(D1T[1], 1, c ? g.vs[c] : g.vc[1], g.F[1][c], -1, 0);
-        opT(D1T[0], 0, G[G1_UX], g.bX[0], 1, 1);
-        opT(D1T[0], 0, G[G1_XY], g.bX[1], 1, 1);
-        opT(D1T[0], 0, G[G1_XZ], g.bX[2], 1, 1);
-        opT(D1T[0], 0, G[G1_TX], g.bX[3], 1, 1);
-        opT(D2T[0], 0, s0, g.bX[0], c43, 1);
-        opT(D2T[0], 0, s1, g.bX[1], 1, 1);
-        opT(D2T[0], 0, s2, g.bX[2], 1, 1);
-        opT(D2T[0], 0, gq, g.bX[3], 1, 1);
-        for (int c = 0; c < 5; c++) opT(D1T[0], 0, c ? g.vs[c] : g.vc[0], g.F[0][c], -1, 0);
+        // pair launch: out[f] += D1^T(a[f]) + ca[f] D2^T(e[f])
+        auto pair = [&](int d, real *const *out, real *const (&a)[4], real *const (&e)[4],
+                        const real (&ca)[4]) {
+            OpBatch<real> b{};
+            for (int f = 0; f < 4; f++) {
+                b.in[f] = a[f];
+                b.coef[f] = 1;
+                b.in[4 + f] = e[f];
+                b.coef[4 + f] = ca[f];
+                b.out[f] = out[f];
+                b.acc[f] = 1;
+            }
+            hc(launch_ops_T<real>(D1T[d], D2T[d], map_of(d), lm[d], 4, 4, b, st),
+               "transposed terms");
+        };
+        // list launch: the mixed terms (+= c13 D1^T), then F[d][c] = -D1^T(vs)
+        auto list = [&](int d, int nm, real *const *min, real *const *mout) {
+            OpBatch<real> b{};
+            for (int k = 0; k < nm; k++) {
+                b.in[k] = min[k];
+                b.out[k] = mout[k];
+                b.coef[k] = c13;
+                b.acc[k] = 1;
+            }
+            for (int c = 0; c < 5; c++) {
+                b.in[nm + c] = c ? g.vs[c] : g.vc[d];
+                b.out[nm + c] = g.F[d][c];
+                b.coef[nm + c] = -1;
+                b.acc[nm + c] = 0;
+            }
+            hc(launch_ops_T<real>(D1T[d], D2T[d], map_of(d), lm[d], nm + 5, 0, b, st),
+               "transposed terms");
+        };
+        {
+            real *const a[4] = {G[G1_XZ], G[G1_YZ], G[G1_WZ], G[G1_TZ]};
+            real *const e[4] = {s0, s1, s2, gq};
+            const real ca[4] = {1, 1, c43, 1};
+            pair(2, g.bY, a, e, ca);
+            // Vyz, Uxz, Wxz, Wyz
+            real *const mi[4] = {s2, s2, s0, s1};
+            real *const mo[4] = {G[G1_VY], G[G1_UX], G[G1_XZ], G[G1_YZ]};
+            list(2, 4, mi, mo);
+        }
+        {
+            real *const a[4] = {G[G1_XY], G[G1_VY], G[G1_YZ], G[G1_TY]};
+            real *const e[4] = {s0, s1, s2, gq};
+            const real ca[4] = {1, c43, 1, 1};
+            pair(1, g.bY, a, e, ca);
+            // Uxy, Vxy
+            real *const mi[2] = {s1, s0};
+            real *const mo[2] = {G[G1_UX], G[G1_XY]};
+            list(1, 2, mi, mo);
+        }
+        {
+            real *const a[4] = {G[G1_UX], G[G1_XY], G[G1_XZ], G[G1_TX]};
+            real *const e[4] = {s0, s1, s2, gq};
+            const real ca[4] = {c43, 1, 1, 1};
+            pair(0, g.bX, a, e, ca);
+            list(0, 0, nullptr, nullptr);
+        }
         hc(launch_gadj<real>(GADJ_CHAIN, g, a, ph, N, st), "adjoint chain");
     }
 
