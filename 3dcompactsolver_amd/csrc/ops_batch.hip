@@ -13,11 +13,20 @@
 namespace cs3d {
 
 // most lanes per workgroup: column lines read rows of LX adjacent lines,
-// so the widest workgroups the registers allow
-template <typename real>
+// so the widest workgroups the registers allow; row workgroups hold one
+// line of at most 256 segments (opb_geom)
+template <typename real, int MAP>
 constexpr int opb_threads()
 {
-    return sizeof(real) == 4 ? 1024 : 512;
+    return MAP == MAP_ROW ? 256 : sizeof(real) == 4 ? 1024 : 512;
+}
+
+// solved lines staged in LDS at once by the transposed kernels: batches of
+// more than five go in two groups, which halves the staging area (two
+// workgroups per CU where one would fill the LDS)
+constexpr int opb_stage_group(int N)
+{
+    return N <= 5 ? N : (N + 1) / 2;
 }
 
 // ring margin of a batch over both derivative plans
@@ -30,7 +39,7 @@ __device__ __forceinline__ int opb_kr(const OpDev<real> *d1, const OpDev<real> *
 
 // out[i] = D1(in[i]) for i < N1, out[N1 + i] = D2(in[i]) for i < N2 (N2 <= N1)
 template <typename real, int M, bool UNI, int MAP, int N1, int N2>
-__global__ __launch_bounds__(opb_threads<real>()) void k_ops_fwd(const OpDev<real> *__restrict__ d1,
+__global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_fwd(const OpDev<real> *__restrict__ d1,
                                                                  const OpDev<real> *__restrict__ d2,
                                                                  LineMap lm, OpBatch<real> b)
 {
@@ -67,7 +76,7 @@ __global__ __launch_bounds__(opb_threads<real>()) void k_ops_fwd(const OpDev<rea
 // solved lines of the whole batch are staged in LDS over the exchange ring
 // (after a barrier) for the B^T stencils.
 template <typename real, int M, bool UNI, int MAP, int N1, int N2>
-__global__ __launch_bounds__(opb_threads<real>()) void k_ops_T(const OpDev<real> *__restrict__ d1,
+__global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_T(const OpDev<real> *__restrict__ d1,
                                                                const OpDev<real> *__restrict__ d2,
                                                                const OpTDev<real> *__restrict__ t1,
                                                                const OpTDev<real> *__restrict__ t2,
@@ -89,75 +98,95 @@ __global__ __launch_bounds__(opb_threads<real>()) void k_ops_T(const OpDev<real>
     for (int i = 0; i < N; i++)
 #pragma unroll
         for (int t = 0; t < M; t++) r[i][t] = t < c.len ? ldo(b.in[i], win.mid(t)) : real(0);
-    solve_batch2<N1, N2, kc_deriv<real>(), kc_d2<real>(), real, M, UNI, MAP>(d1, d2, c, r);
-    // every lane is past the last ring read before the staging overwrites it
-    __syncthreads();
-    const int ls = n + 1;  // odd stride across the lines of a column workgroup
+    // the first NP outputs, where accumulated, are read with the inputs, so
+    // that their loads overlap the first ones instead of following the
+    // solve (any later accumulated output is read at the end)
+    constexpr int NO = N2 ? N1 : N;
+    constexpr int NP = NO < 4 ? NO : 4;
+    real ob[NP][M];
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        real *st = xb + (i * LX + c.line) * ls;
+    for (int f = 0; f < NP; f++)
 #pragma unroll
         for (int t = 0; t < M; t++)
-            if (t < c.len) st[c.s + t] = r[i][t];
-    }
-    __syncthreads();
+            ob[f][t] = b.acc[f] && active && t < c.len ? ldo(b.out[f], win.mid(t)) : real(0);
+    solve_batch2<N1, N2, kc_deriv<real>(), kc_d2<real>(), real, M, UNI, MAP>(d1, d2, c, r);
     // B^T of every solve into r[i] (D1 negated: its interior stencil is
-    // antisymmetric), weighted
+    // antisymmetric), weighted.  The solved lines go through LDS in groups
+    // of SG (opb_stage_group), each group staged over the ring after a
+    // barrier (every lane past the last ring read, or the previous group's
+    // window reads).
+    constexpr int SG = opb_stage_group(N);
+    const int ls = n + 1;  // odd stride across the lines of a column workgroup
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        const bool first = i < N1;
-        const OpDev<real> *__restrict__ op = first ? d1 : d2;
-        const OpTDev<real> *__restrict__ tt = first ? t1 : t2;
-        const real *st = xb + (i * LX + c.line) * ls;
-        real w[W], res[M];
+    for (int g0 = 0; g0 < N; g0 += SG) {
+        __syncthreads();
 #pragma unroll
-        for (int t = 0; t < W; t++) {
-            int p = c.s - HW + t;
-            if (d1->cyclic) {
-                p += p < 0 ? n : 0;
-                p -= p >= n ? n : 0;
-            } else {
-                p = p < 0 ? 0 : p >= n ? n - 1 : p;
-            }
-            w[t] = st[p];
+        for (int i = g0; i < g0 + SG && i < N; i++) {
+            real *st = xb + ((i - g0) * LX + c.line) * ls;
+#pragma unroll
+            for (int t = 0; t < M; t++)
+                if (t < c.len) st[c.s + t] = r[i][t];
         }
-        if (first)
-            stencil<OP_D1, real, M, HW>(op, w, res);
-        else
-            stencil<OP_D2, real, M, HW>(op, w, res);
-        if (!d1->cyclic) {
-            const int JB = tt->JB, UB = tt->UB;
+        __syncthreads();
 #pragma unroll
-            for (int t = 0; t < M; t++) {
-                const int j = c.s + t;
-                // (the dense columns hold B^T itself: pre-negated for D1,
-                // whose weight below negates the interior stencil)
-                if (t < c.len && j < JB) {
-                    real a = 0;
-                    for (int u = 0; u < UB; u++) a += tt->BTl[j][u] * st[u];
-                    res[t] = first ? -a : a;
-                } else if (t < c.len && j >= n - JB) {
-                    real a = 0;
-                    for (int u = 0; u < UB; u++) a += tt->BTr[j - (n - JB)][u] * st[n - UB + u];
-                    res[t] = first ? -a : a;
+        for (int i = g0; i < g0 + SG && i < N; i++) {
+            const bool first = i < N1;
+            const OpDev<real> *__restrict__ op = first ? d1 : d2;
+            const OpTDev<real> *__restrict__ tt = first ? t1 : t2;
+            const real *st = xb + ((i - g0) * LX + c.line) * ls;
+            real w[W], res[M];
+#pragma unroll
+            for (int t = 0; t < W; t++) {
+                int p = c.s - HW + t;
+                if (d1->cyclic) {
+                    p += p < 0 ? n : 0;
+                    p -= p >= n ? n : 0;
+                } else {
+                    p = p < 0 ? 0 : p >= n ? n - 1 : p;
+                }
+                w[t] = st[p];
+            }
+            if (first)
+                stencil<OP_D1, real, M, HW>(op, w, res);
+            else
+                stencil<OP_D2, real, M, HW>(op, w, res);
+            if (!d1->cyclic) {
+                const int JB = tt->JB, UB = tt->UB;
+#pragma unroll
+                for (int t = 0; t < M; t++) {
+                    const int j = c.s + t;
+                    // (the dense columns hold B^T itself: pre-negated for
+                    // D1, whose weight below negates the interior stencil)
+                    if (t < c.len && j < JB) {
+                        real a = 0;
+                        for (int u = 0; u < UB; u++) a += tt->BTl[j][u] * st[u];
+                        res[t] = first ? -a : a;
+                    } else if (t < c.len && j >= n - JB) {
+                        real a = 0;
+                        for (int u = 0; u < UB; u++)
+                            a += tt->BTr[j - (n - JB)][u] * st[n - UB + u];
+                        res[t] = first ? -a : a;
+                    }
                 }
             }
-        }
-        const real cf = first ? -b.coef[i] : b.coef[i];
+            const real cf = first ? -b.coef[i] : b.coef[i];
 #pragma unroll
-        for (int t = 0; t < M; t++) r[i][t] = cf * res[t];
+            for (int t = 0; t < M; t++) r[i][t] = cf * res[t];
+        }
     }
-    constexpr int NO = N2 ? N1 : N;
 #pragma unroll
     for (int f = 0; f < NO; f++) {
         real *out = b.out[f];
         const int acc = b.acc[f];
 #pragma unroll
         for (int t = 0; t < M; t++) {
-            const real v = N2 ? r[f][t] + r[N1 + (N2 ? f : 0)][t] : r[f][t];
             if (active && t < c.len) {
                 real *o = (real *)((char *)out + win.mid(t));
-                *o = acc ? *o + v : v;
+                const real v = N2 ? r[f][t] + r[N1 + (N2 ? f : 0)][t] : r[f][t];
+                if (f < NP)
+                    *o = ob[f < NP ? f : 0][t] + v;
+                else
+                    *o = acc ? *o + v : v;
             }
         }
     }
@@ -173,14 +202,15 @@ static LaunchGeom opb_geom(const OpHost &h1, const OpHost &h2, int N, bool trans
     const int P = h1.P;
     auto bytes = [&](int LX) {
         const size_t ring = ring_bytes(P, kr, LX, N, sizeof(real), map == MAP_COL);
-        const size_t stage = trans ? (size_t)N * LX * (h1.n + 1) * sizeof(real) : 0;
+        const size_t stage =
+            trans ? (size_t)opb_stage_group(N) * LX * (h1.n + 1) * sizeof(real) : 0;
         return ring > stage ? ring : stage;
     };
     LaunchGeom g;
     // row lines (contiguous): one line per workgroup where a line fills a
     // wavefront, so that workgroups on a CU run out of phase, as the x sweep
-    const int cap = map == MAP_ROW ? std::min(opb_threads<real>(), std::max(64, P))
-                                   : opb_threads<real>();
+    const int cap = map == MAP_ROW ? std::min(opb_threads<real, MAP_ROW>(), std::max(64, P))
+                                   : opb_threads<real, MAP_COL>();
     g.LX = P > 0 ? std::min(64, cap / P) : 0;
     if (map == MAP_COL)
         while (g.LX & (g.LX - 1)) g.LX &= g.LX - 1;

@@ -303,6 +303,9 @@ def main():
                          "of the run stays in HBM and nothing is recomputed)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="run only the FP64 leg and print its step time (for profiling it)")
+    ap.add_argument("--shear-only", action="store_true",
+                    help="run only the shear-layer leg (forward and its gradient on the "
+                         "adjoint's general path) and print it (for profiling it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 256 MiB device copies (the reference kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -317,6 +320,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.shear_only:
+        print(json.dumps({"shear_layer": run_shear(args.steps, args.warmup)}), flush=True)
+        return
     N = args.grid
     kw, L = tgv_kw(N)
     ic = tgv_ic(N, L)
