@@ -22,8 +22,8 @@ constexpr int opb_threads()
 }
 
 // solved lines staged in LDS at once by the transposed kernels: batches of
-// more than five go in two groups, which halves the staging area (two
-// workgroups per CU where one would fill the LDS)
+// more than five go in two groups, which halves the staging area (long
+// lines keep more lines per workgroup within the LDS)
 constexpr int opb_stage_group(int N)
 {
     return N <= 5 ? N : (N + 1) / 2;
