@@ -158,8 +158,10 @@ def oracle_cfg(dims, kw):
     return po.make_config(Nx, Ny, Nz, **keep)
 
 
-def test_rhs_vjp_matches_oracle_adjoint_fp64():
-    dims = (20, 12, 14)
+@pytest.mark.parametrize("dims", [(20, 12, 14), (18, 10, 14)])
+def test_rhs_vjp_matches_oracle_adjoint_fp64(dims):
+    """(18, 10, 14): periodic lines of unequal segments in every direction,
+    so every pass runs its table variant"""
     Nx, Ny, Nz = dims
     shape = (Nz, Ny, Nx)
     kw = adj_kw(dims, dt_fixed=0.013)
