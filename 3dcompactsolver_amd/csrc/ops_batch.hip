@@ -39,9 +39,9 @@ __device__ __forceinline__ int opb_kr(const OpDev<real> *d1, const OpDev<real> *
 
 // out[i] = D1(in[i]) for i < N1, out[N1 + i] = D2(in[i]) for i < N2 (N2 <= N1)
 template <typename real, int M, bool UNI, int MAP, int N1, int N2>
-__global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_fwd(const OpDev<real> *__restrict__ d1,
-                                                                 const OpDev<real> *__restrict__ d2,
-                                                                 LineMap lm, OpBatch<real> b)
+__global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_fwd(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2, LineMap lm,
+    OpBatch<real> b)
 {
     static_assert(N2 <= N1, "second derivatives of the first inputs");
     extern __shared__ char smem_raw[];
@@ -73,14 +73,13 @@ __global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_fwd(const Op
 // Transposed, PAIR (N1 == N2): out[f] (+)= coef[f] D1^T(in[f])
 // + coef[N1 + f] D2^T(in[N1 + f]); LIST (N2 == 0): out[i] (+)= coef[i]
 // D1^T(in[i]).  acc[f] selects accumulate or overwrite per output.  The
-// solved lines of the whole batch are staged in LDS over the exchange ring
-// (after a barrier) for the B^T stencils.
+// solved lines are staged in LDS over the exchange ring (opb_stage_group
+// at a time) for the B^T stencils.
 template <typename real, int M, bool UNI, int MAP, int N1, int N2>
-__global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_T(const OpDev<real> *__restrict__ d1,
-                                                               const OpDev<real> *__restrict__ d2,
-                                                               const OpTDev<real> *__restrict__ t1,
-                                                               const OpTDev<real> *__restrict__ t2,
-                                                               LineMap lm, OpBatch<real> b)
+__global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_T(
+    const OpDev<real> *__restrict__ d1, const OpDev<real> *__restrict__ d2,
+    const OpTDev<real> *__restrict__ t1, const OpTDev<real> *__restrict__ t2, LineMap lm,
+    OpBatch<real> b)
 {
     static_assert(N2 == 0 || N2 == N1, "a list of D1 terms or pairs of D1 and D2 terms");
     extern __shared__ char smem_raw[];
