@@ -84,7 +84,13 @@ struct SolverT : SolverBase {
         // phases best, as for the x sweep (FP64 filter x sweep 10% faster
         // than at 512 threads)
         if (dir == 0) return env_cap("CS3D_FILTER_X_THREADS", 512, 128);
-        return env_cap("CS3D_FILTER_THREADS", sizeof(real) == 4 ? 1024 : 512);
+        // columns: FP32 at 1024 lanes (32 lines); FP64 at 256 (8 lines,
+        // two workgroups per CU whose phases overlap: 256^3 FP64 filter
+        // sweeps 26% faster than one 512-lane workgroup per CU; FP32 at
+        // 512 lanes made the filter 7% faster but the x sweep after it 1.4%
+        // slower)
+        return env_cap("CS3D_FILTER_THREADS", sizeof(real) == 4 ? 1024 : 512,
+                       sizeof(real) == 4 ? 1024 : 256);
     }
     // the adjoint's x kernels take one line per workgroup, as the x sweep
     static int adj_x_threads(int P)
