@@ -10,7 +10,7 @@ TVDRK3 steps with the filter (RK4.hpp:35-118, TVDRK3.hpp:38-96,
 CSolver_AWS.cpp:1682-1893) and of the sponge update (updateSponge,
 CSolver_AWS.cpp:1937-2031) and of the wall handling (preStepBCHandling,
 :553-826; the wall rows of postStepBCHandling, :1506-1605), on periodic or
-non-periodic lines, with sponge faces or with walls (not both), written
+non-periodic lines, with sponge faces, walls or both, written
 independently of the GPU kernels:
 
 - every compact operator is the dense matrix of the oracle's own line
@@ -35,7 +35,12 @@ independently of the GPU kernels:
   energy with that of p = rho R T_w, T_w = calcNeumann of the six inward
   temperatures, in place (at the first stage the state the RK update
   builds on); the right-hand side on a wall plane keeps only the normal
-  continuity flux term of the last wall face through the point.
+  continuity flux term of the last wall face through the point;
+- walls beside sponges: at the first stage after a sponge update the wall
+  correction acts on the state and on the X set's state alike (the wall
+  temperature from the X set's temperatures, which on the sponge planes
+  are the stale ones; the face values, formed with the state's density,
+  written to both).
 
 Pinned by tests/test_oracle_adjoint.py: the numpy forward stage and the RK4
 and TVDRK3 steps against the C oracle (1e-12; with sponges and non-periodic
@@ -148,8 +153,6 @@ class Adjoint:
         self.shape = (cfg.Nz, cfg.Ny, cfg.Nx)
         kinds = [cfg.bc_kind[f] for f in range(6)]
         is_wall = [k in (po.ADIABATIC_WALL, po.MOVING_ADIABATIC_WALL) for k in kinds]
-        if any(is_wall) and any(k == po.SPONGE for k in kinds):
-            raise ValueError("the numpy adjoint does not cover walls beside sponges")
         lengths = (cfg.Lx, cfg.Ly, cfg.Lz)
         self.rk = cfg.rk
         self.gamma = cfg.gamma
@@ -211,54 +214,77 @@ class Adjoint:
     # ---------------------------------------------------------------- walls
     CN = np.array([360.0, -450.0, 400.0, -225.0, 72.0, -10.0]) / 147.0  # calcNeumann
 
-    def wall_pre(self, q, keep=None):
+    def wall_pre(self, q, keep=None, qs=None, keeps=None):
         """preStepBCHandling on a state (a new list); keep (a list) receives
-        the state before each face"""
+        the state before each face.  With qs (the X set's state at the first
+        stage after a sponge update; equal to q off the sponge planes) the
+        pair: the wall temperature from qs's temperatures, the face values
+        (with q's density) written to both; returns (q, qs), keeps receiving
+        qs before each face"""
         q = [x.copy() for x in q]
+        if qs is not None:
+            qs = [x.copy() for x in qs]
         g = self.gamma
         for d, hi, uw in self.walls:
             if keep is not None:
                 keep.append([x.copy() for x in q])
-            T = self.prims(q)["T"]
+            if keeps is not None:
+                keeps.append([x.copy() for x in qs])
+            T = self.prims(q if qs is None else qs)["T"]
             Tw = sum(c * T[self.plane(d, hi, m + 1)] for m, c in enumerate(self.CN))
             fc = self.plane(d, hi)
             rho = q[0][fc]
             for k in range(3):
                 q[1 + k][fc] = rho * uw[k]
             q[4][fc] = rho * self.R * Tw / (g - 1) + 0.5 * rho * sum(u * u for u in uw)
-        return q
+            if qs is not None:
+                for k in range(5):
+                    qs[k][fc] = q[k][fc]
+        return q if qs is None else (q, qs)
 
-    def wall_pre_vjp(self, q, lam):
+    def wall_pre_vjp(self, q, lam, qs=None, lqs=None):
         """the cotangent of the state before preStepBCHandling from that of
-        the state after it"""
+        the state after it; with qs, those of the pair (q, qs) from (lam,
+        lqs)"""
         if not self.walls:
-            return list(lam)
-        keep = []
-        self.wall_pre(q, keep)
+            return list(lam) if qs is None else (list(lam), list(lqs))
+        keep, keeps = [], []
+        self.wall_pre(q, keep, qs, keeps if qs is not None else None)
         lam = [x.copy() for x in lam]
+        if qs is not None:
+            lqs = [x.copy() for x in lqs]
         g = self.gamma
-        for (d, hi, uw), st in zip(self.walls[::-1], keep[::-1]):
-            v = self.prims(st)
+        for i in range(len(self.walls) - 1, -1, -1):
+            d, hi, uw = self.walls[i]
+            st = keep[i]
+            sT = st if qs is None else keeps[i]   # the temperatures' state
+            lT = lam if qs is None else lqs
+            v = self.prims(sT)
             fc = self.plane(d, hi)
             rho = st[0][fc]
             Tw = sum(c * v["T"][self.plane(d, hi, m + 1)] for m, c in enumerate(self.CN))
-            lE = lam[4][fc].copy()
-            lam[0][fc] += sum(lam[1 + k][fc] * uw[k] for k in range(3)) + lE * (
+            lf = [lam[k][fc].copy() for k in range(5)]
+            if qs is not None:
+                for k in range(5):
+                    lf[k] = lf[k] + lqs[k][fc]
+                    lqs[k][fc] = 0.0
+            lE = lf[4]
+            lam[0][fc] = lf[0] + sum(lf[1 + k] * uw[k] for k in range(3)) + lE * (
                 self.R * Tw / (g - 1) + 0.5 * sum(u * u for u in uw))
             for k in range(1, 5):
                 lam[k][fc] = 0.0
             gT = lE * rho * self.R / (g - 1)
             for m, c in enumerate(self.CN):
                 ip = self.plane(d, hi, m + 1)
-                r = st[0][ip]
-                u = [st[k][ip] / r for k in (1, 2, 3)]
+                r = sT[0][ip]
+                u = [sT[k][ip] / r for k in (1, 2, 3)]
                 Tb = gT * c
-                lam[4][ip] += Tb * (g - 1) / (r * self.R)
+                lT[4][ip] += Tb * (g - 1) / (r * self.R)
                 for k in range(3):
-                    lam[1 + k][ip] -= Tb * (g - 1) * u[k] / (r * self.R)
-                lam[0][ip] += Tb * ((g - 1) * 0.5 * sum(x * x for x in u) / (r * self.R)
-                                    - v["T"][ip] / r)
-        return lam
+                    lT[1 + k][ip] -= Tb * (g - 1) * u[k] / (r * self.R)
+                lT[0][ip] += Tb * ((g - 1) * 0.5 * sum(x * x for x in u) / (r * self.R)
+                                   - v["T"][ip] / r)
+        return lam if qs is None else (lam, lqs)
 
     # ---------------------------------------------------------------- operators
     def op(self, kind, d, f, transpose=False):
@@ -340,14 +366,15 @@ class Adjoint:
             if self.sponge:
                 spg = self.sigma * (A[c] - q[c])
                 acc = acc + (2.0 * spg if c == 4 else spg)
-            out.append(dt * acc * self.zmask)
+            out.append(dt * acc)
         if self.walls:
             # postStepBCHandling: a wall plane keeps -dt D_n(rho u_n)
             for c in range(5):
                 out[c] = np.where(self.rowdir >= 0, 0.0, out[c])
             for d, D in enumerate("XYZ"):
                 out[0] = np.where(self.rowdir == d, -dt * v["cont" + D], out[0])
-        return out
+        # then the sponge face planes are zeroed, wall points on them too
+        return [x * self.zmask for x in out]
 
     # ---------------------------------------------------------------- reverse
     def _chain(self, b, v, names, rho, out):
@@ -383,7 +410,7 @@ class Adjoint:
         w = [dt * self.zmask * np.where(self.rowdir >= 0, 0.0, x) for x in lam]
         for d, D in enumerate("XYZ"):
             if (self.rowdir == d).any():
-                b["cont" + D] = np.where(self.rowdir == d, -dt * lam[0], 0.0)
+                b["cont" + D] = np.where(self.rowdir == d, -dt * lam[0] * self.zmask, 0.0)
         for c, terms in enumerate(RHS_TERMS):
             for coef, fac in terms:
                 cf = self._coef(coef)
@@ -489,7 +516,7 @@ class Adjoint:
         With walls every stage state is wall-corrected first (the first in
         place: the steps build on it)"""
         W = self.wall_pre if self.walls else (lambda x: x)
-        q = W(q)
+        q, qs = self.wall_first(q, qs)
         k1 = self.rhs(q, dt, A, qs)
         k2 = self.rhs(W([a + 0.5 * k for a, k in zip(q, k1)]), dt, A)
         k3 = self.rhs(W([a + 0.5 * k for a, k in zip(q, k2)]), dt, A)
@@ -503,7 +530,7 @@ class Adjoint:
         1/4 (Q2 + k2), q' = 1/3 q + 2/3 (Q3 + k3)) followed by the filter in
         `order` (None: no filter); with walls q, Q2 and Q3 wall-corrected"""
         W = self.wall_pre if self.walls else (lambda x: x)
-        q = W(q)
+        q, qs = self.wall_first(q, qs)
         k1 = self.rhs(q, dt, A, qs)
         Q2 = W([a + k for a, k in zip(q, k1)])
         k2 = self.rhs(Q2, dt, A)
@@ -511,6 +538,24 @@ class Adjoint:
         k3 = self.rhs(Q3, dt, A)
         acc = [a / 3 + 2.0 / 3.0 * b + 2.0 / 3.0 * k for a, b, k in zip(q, Q3, k3)]
         return self.filt(acc, order) if order is not None else acc
+
+    def wall_first(self, q, qs):
+        """the first stage's wall correction: of the state, and of the X
+        set's state with it at the first stage after a sponge update"""
+        if not self.walls:
+            return q, qs
+        if qs is None:
+            return self.wall_pre(q), None
+        return self.wall_pre(q, qs=qs)
+
+    def wall_first_vjp(self, q, qs, lam, lqs):
+        """the reverse of wall_first: cotangents of (q, qs) (lqs None
+        without qs)"""
+        if not self.walls:
+            return lam, lqs
+        if qs is None:
+            return self.wall_pre_vjp(q, lam), None
+        return self.wall_pre_vjp(q, lam, qs, lqs)
 
     def step(self, q, dt, order, A=None, qs=None):
         if self.rk == 4:
@@ -603,8 +648,8 @@ class Adjoint:
         k3/3 + k4/6"""
         W = self.wall_pre if self.walls else (lambda x: x)
         WT = self.wall_pre_vjp
-        q1 = W(q)
-        k1 = self.rhs(q1, dt, A, qs)
+        q1, qs1 = self.wall_first(q, qs)
+        k1 = self.rhs(q1, dt, A, qs1)
         Q2 = [a + 0.5 * k for a, k in zip(q1, k1)]
         k2 = self.rhs(W(Q2), dt, A)
         Q3 = [a + 0.5 * k for a, k in zip(q1, k2)]
@@ -622,10 +667,11 @@ class Adjoint:
         Q2b = WT(Q2, Q2b)
         qb = [a + c for a, c in zip(qb, Q2b)]
         Q1b, qsb, Ab1 = self.rhs_vjp_full(q1, [x / 6 + 0.5 * c for x, c in zip(la, Q2b)], dt, A,
-                                          qs)
+                                          qs1)
         for x in (Ab3, Ab2, Ab1):
             Ab = self._acc(Ab, x)
-        return WT(q, [a + c for a, c in zip(qb, Q1b)]), qsb, Ab
+        lq, lqs = self.wall_first_vjp(q, qs, [a + c for a, c in zip(qb, Q1b)], qsb)
+        return lq, lqs, Ab
 
     def tvdrk3_step_vjp(self, q, dt, order, lam, A=None, qs=None):
         """cotangents of the step's start state (and of qs, A) from that of
@@ -636,8 +682,8 @@ class Adjoint:
         correction of its stage"""
         W = self.wall_pre if self.walls else (lambda x: x)
         WT = self.wall_pre_vjp
-        q1 = W(q)
-        k1 = self.rhs(q1, dt, A, qs)
+        q1, qs1 = self.wall_first(q, qs)
+        k1 = self.rhs(q1, dt, A, qs1)
         Q2 = [a + k for a, k in zip(q1, k1)]
         Q2w = W(Q2)
         k2 = self.rhs(Q2w, dt, A)
@@ -650,8 +696,9 @@ class Adjoint:
         t2 = [0.25 * x for x in Q3b]
         J2, _, Ab2 = self.rhs_vjp_full(Q2w, t2, dt, A)
         Q2b = WT(Q2, [a + c for a, c in zip(t2, J2)])
-        J1, qsb, Ab1 = self.rhs_vjp_full(q1, Q2b, dt, A, qs)
+        J1, qsb, Ab1 = self.rhs_vjp_full(q1, Q2b, dt, A, qs1)
         for x in (Ab2, Ab1):
             Ab = self._acc(Ab, x)
         out = [x / 3 + 0.75 * a + b + c for x, a, b, c in zip(la, Q3b, Q2b, J1)]
-        return WT(q, out), qsb, Ab
+        lq, lqs = self.wall_first_vjp(q, qs, out, qsb)
+        return lq, lqs, Ab

@@ -151,3 +151,87 @@ def test_ke_gradient_with_walls_matches_oracle_finite_differences(case, rk, filt
         fd = (Jp - Jm) / (2 * eps)
         ad = np.vdot(gU, d[0]) + np.vdot(gV, d[1]) + np.vdot(gW, d[2])
         assert abs(ad - fd) <= 1e-7 * abs(fd), (ad, fd)
+
+
+# walls on the x faces beside sponges on the y faces: the wall faces cross
+# the sponge planes, where the first stage after each sponge update reads
+# the stale (pre-reset) temperatures
+WS_DIMS = (16, 20, 6)
+WS_KINDS = (ADIA, MOVE, po.SPONGE, po.SPONGE, 0, 0)
+
+
+def ws_cfg(**kw):
+    base = dict(sponge_frac=(0.25, 0.2, 0.25), sponge_avgT=0.5,
+                wall_vel={1: (0.0, 0.25, 0.1)})
+    base.update(kw)
+    return cfg_for(WS_DIMS, WS_KINDS, (1, 1, 0), **base)
+
+
+@pytest.mark.parametrize("rk,filter_step", [(4, 1), (3, 2)])
+def test_forward_walls_beside_sponges_matches_oracle(rk, filter_step):
+    cfg = ws_cfg(rk=rk, filter_step=filter_step)
+    ic = ic_for(WS_DIMS, 3)
+    dt = 0.01
+    A = oa.Adjoint(cfg)
+    for n in (1, 3):
+        o, q0 = oracle_run(cfg, ic, dt, n)
+        q, Av, _ = A.run(q0, n, dt, filter_step)
+        errs = [relerr(g, r) for g, r in zip(q, o.conserved(1))]
+        errs += [relerr(g, r) for g, r in zip(Av, o.sponge_avg())]
+        o.close()
+        assert max(errs) <= 1e-12, (n, errs)
+
+
+def test_wall_pair_vjp_dot_product():
+    """the first stage's wall correction of the pair (state, pre-reset
+    state): dot-product identity in both inputs; a reverse that sends the
+    wall temperature's cotangent to the state instead of the pre-reset
+    state fails it"""
+    cfg = ws_cfg()
+    A = oa.Adjoint(cfg)
+    shape = A.shape
+    o, q = oracle_run(cfg, ic_for(WS_DIMS, 5), 0.01, 1)
+    qf = o.conserved(1)
+    o.close()
+    rng = np.random.default_rng(6)
+    # a pre-reset state that differs from the state on the sponge planes
+    qs = [np.where(A.onplane, x * (1 + 0.05 * rng.standard_normal(shape)), x) for x in qf]
+    lam = [rng.standard_normal(shape) for _ in range(5)]
+    lqs = [rng.standard_normal(shape) for _ in range(5)]
+    eps = 1e-5
+    dq = [s * smooth(rng.standard_normal(shape), 1) for s in (1e-2, 0.1, 0.1, 0.1, 0.1)]
+    ds = [s * smooth(rng.standard_normal(shape), 1) for s in (1e-2, 0.1, 0.1, 0.1, 0.1)]
+    kp = A.wall_pre([x + eps * d for x, d in zip(qf, dq)], qs=[x + eps * d for x, d in zip(qs, ds)])
+    km = A.wall_pre([x - eps * d for x, d in zip(qf, dq)], qs=[x - eps * d for x, d in zip(qs, ds)])
+    fd = sum(np.vdot(l_, (a - b) / (2 * eps)) for l_, a, b in zip(lam + lqs, kp[0] + kp[1],
+                                                                   km[0] + km[1]))
+    gq, gs = A.wall_pre_vjp(qf, lam, qs, lqs)
+    ad = sum(np.vdot(a, b) for a, b in zip(gq + gs, dq + ds))
+    assert abs(ad - fd) <= 1e-8 * abs(fd), (ad, fd)
+    bq, _ = A.wall_pre_vjp(qf, lam, qf, lqs)  # temperatures of the state
+    bad = sum(np.vdot(a, b) for a, b in zip(bq + gs, dq + ds))
+    assert abs(bad - fd) > 1e-6 * abs(fd), (bad, fd)
+
+
+@pytest.mark.parametrize("rk,filter_step", [(4, 1), (3, 2)])
+def test_ke_gradient_walls_beside_sponges_matches_oracle_finite_differences(rk, filter_step):
+    cfg = ws_cfg(rk=rk, filter_step=filter_step)
+    ic = ic_for(WS_DIMS, 9)
+    dt, nsteps = 0.01, 3
+    A = oa.Adjoint(cfg)
+    o, q0 = oracle_run(cfg, ic, dt, 0)
+    o.close()
+    J, gU, gV, gW = A.ke_gradient(ic, nsteps, dt, filter_step, q0=q0)
+    assert abs(J - run_ke(cfg, ic, dt, nsteps)) <= 1e-13 * abs(J)
+    inner = np.ones(A.shape)
+    inner[A.rowdir >= 0] = 0.0
+    rng = np.random.default_rng(13)
+    eps = 1e-5
+    rho, U, V, W, p = ic
+    for _ in range(2):
+        d = [inner * smooth(rng.standard_normal(U.shape), 1) for _ in range(3)]
+        Jp = run_ke(cfg, (rho, U + eps * d[0], V + eps * d[1], W + eps * d[2], p), dt, nsteps)
+        Jm = run_ke(cfg, (rho, U - eps * d[0], V - eps * d[1], W - eps * d[2], p), dt, nsteps)
+        fd = (Jp - Jm) / (2 * eps)
+        ad = np.vdot(gU, d[0]) + np.vdot(gV, d[1]) + np.vdot(gW, d[2])
+        assert abs(ad - fd) <= 1e-7 * abs(fd), (ad, fd)
