@@ -988,9 +988,12 @@ __global__ __launch_bounds__(256) void k_gadj_chain(GenAdj<real> g, AdjVjp<real>
         }
         if (g.stale) qset_store(g.res_stale, i, rsx);
         if (g.sigma) {
+            // the source is part of the right-hand side that the sponge
+            // planes and the wall rows zero
             real w[5], ra[5];
             qset_load(g.v, i, w);
-            const real mask = on_sponge_plane(g.sa, i) ? real(0) : g.cv;
+            const real mask =
+                on_sponge_plane(g.sa, i) || wall_row_dir(g.sa, i) >= 0 ? real(0) : g.cv;
             qset_load(g.resA, i, ra);
 #pragma unroll
             for (int k = 0; k < 5; k++) {

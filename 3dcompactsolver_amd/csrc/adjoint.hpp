@@ -146,10 +146,13 @@ hipError_t launch_gadj_lincomb(const QSet<real> &out, real ca, const QSet<real> 
 // Reverse of one wall face's correction (walls.hip: k_wall_face): lam, the
 // cotangent of the state after the face, becomes that of the state before
 // it, st (in place); the face's momenta and energy were overwritten, its
-// density kept, and its energy depends on the six inward temperatures.
+// density kept, and its energy depends on the six inward temperatures.  At
+// the first stage after a sponge update the pair: stS the pre-reset copy
+// before the face and lamS its cotangent (both empty otherwise).
 template <typename real>
-hipError_t launch_wall_face_vjp(const QSet<real> &st, const QSet<real> &lam, const Phys<real> &ph,
-                                const StageArgs &sa, const WallFace &w, hipStream_t s);
+hipError_t launch_wall_face_vjp(const QSet<real> &st, const QSet<real> &lam, const QSet<real> &stS,
+                                const QSet<real> &lamS, const Phys<real> &ph, const StageArgs &sa,
+                                const WallFace &w, hipStream_t s);
 
 template <typename real>
 hipError_t launch_adj_ke_seed(const QSet<real> &q, const Phys<real> &ph, const QSet<real> &out,

@@ -1074,6 +1074,10 @@ struct SolverT : SolverBase {
     const QSet<real> *gcur_S = nullptr;
     QSet<real> gw_q{};
     std::vector<QSet<real>> gw_keep;
+    // walls beside sponges: the corrected pre-reset set of a stale first
+    // stage and that set before each face
+    QSet<real> gw_qS{};
+    std::vector<QSet<real>> gw_keepS;
 
     void adj_require()
     {
@@ -1081,8 +1085,6 @@ struct SolverT : SolverBase {
         bool general = sponge || any_wall() || std::getenv("CS3D_ADJ_GENERAL") != nullptr;
         for (int d = 0; d < 3; d++) general |= cfg.bc_type[d] != CS3D_PERIODIC_SOLVE;
         if (cfg.compensated) throw std::invalid_argument("the adjoint does not cover the compensated mode");
-        if (any_wall() && sponge)
-            throw std::invalid_argument("the adjoint does not cover walls beside sponges");
         if (adj_ready) {
             if (general != adj_general) throw std::logic_error("adjoint path changed");
             return;
@@ -1125,6 +1127,12 @@ struct SolverT : SolverBase {
                 gw_q = adj_alloc_set();
                 gw_keep.clear();
                 for (size_t f = 0; f < wall_order().size(); f++) gw_keep.push_back(adj_alloc_set());
+                if (sponge) {
+                    gw_qS = adj_alloc_set();
+                    gw_keepS.clear();
+                    for (size_t f = 0; f < wall_order().size(); f++)
+                        gw_keepS.push_back(adj_alloc_set());
+                }
             }
         } else {
             for (int pass = ADJ_YREC; pass <= ADJ_Y; pass++) {
@@ -1325,8 +1333,12 @@ struct SolverT : SolverBase {
     // The cotangent res = (d(dt R)/dQ)^T lam of the stage map at state Q,
     // with v = dt * lam given; a holds the epilogue (AdjVjp).
     // preStepBCHandling on a copy: dst = W(src), the state before each face
-    // kept in gw_keep for wall_vjp
-    void wall_apply(const QSet<real> &src, const QSet<real> &dst)
+    // kept in gw_keep for wall_vjp.  srcS (a stale first stage): the
+    // pre-reset set is corrected with it into gw_qS, as the forward's first
+    // stage after a sponge update does (the wall temperatures from its
+    // sponge-plane values, the face values written to it there), and kept
+    // before each face in gw_keepS
+    void wall_apply(const QSet<real> &src, const QSet<real> &dst, const QSet<real> *srcS = nullptr)
     {
         copy_set(dst, src);
         Fields<real> Fw = F;
@@ -1335,20 +1347,39 @@ struct SolverT : SolverBase {
         StageArgs sa = base;
         sa.rkStep = 1;
         sa.stale = 0;
+        if (srcS) {
+            copy_set(gw_qS, *srcS);
+            Fw.qstale = gw_qS;
+            sa.stale = 1;
+        }
         const std::vector<int> order = wall_order();
         for (size_t i = 0; i < order.size(); i++) {
             copy_set(gw_keep[i], dst);
+            if (srcS) copy_set(gw_keepS[i], gw_qS);
             hc(launch_wall_face<real>(Fw, ph, sa, wall_face_of(order[i]), st), "walls");
         }
     }
     // lam (in place): from the cotangent of W(src) to that of src, for the
-    // src of the last wall_apply
-    void wall_vjp(const QSet<real> &lam)
+    // src of the last wall_apply; lamS (with a stale wall_apply): the
+    // pre-reset set's cotangent likewise, in place
+    void wall_vjp(const QSet<real> &lam, const QSet<real> *lamS = nullptr)
     {
         const std::vector<int> order = wall_order();
         for (size_t i = order.size(); i-- > 0;)
-            hc(launch_wall_face_vjp<real>(gw_keep[i], lam, ph, base, wall_face_of(order[i]), st),
+            hc(launch_wall_face_vjp<real>(gw_keep[i], lam, lamS ? gw_keepS[i] : QSet<real>{},
+                                          lamS ? *lamS : QSet<real>{}, ph, base,
+                                          wall_face_of(order[i]), st),
                "walls reverse");
+    }
+    // the product at the corrected state of the last wall_apply; stale: its
+    // x block read the corrected pre-reset set gw_qS, whose share of the
+    // cotangent goes to gl_stale
+    void vjp_corrected(const QSet<real> &v, const AdjVjp<real> &f, bool stale)
+    {
+        const QSet<real> *keep = gcur_S;
+        if (stale) gcur_S = &gw_qS;
+        vjp(gw_q, v, f, stale);
+        gcur_S = keep;
     }
     WallFace wall_face_of(int f) const
     {
@@ -1361,15 +1392,18 @@ struct SolverT : SolverBase {
     }
 
     // The product J^T(W(Q)) (cv v) into out, then taken back through the
-    // wall correction: (d(dt R(W(Q)))/dQ)^T v
-    void vjp_walls(const QSet<real> &Q, const QSet<real> &v, real cv, const QSet<real> &out)
+    // wall correction: (d(dt R(W(Q)))/dQ)^T v.  QS (a stale first stage):
+    // the pre-reset set, corrected with Q and held fixed (its cotangent is
+    // left in gl_stale)
+    void vjp_walls(const QSet<real> &Q, const QSet<real> &v, real cv, const QSet<real> &out,
+                   const QSet<real> *QS = nullptr)
     {
-        wall_apply(Q, gw_q);
+        wall_apply(Q, gw_q, QS);
         AdjVjp<real> f{};
         f.cv = cv;
         f.out = out;
-        vjp(gw_q, v, f);
-        wall_vjp(out);
+        vjp_corrected(v, f, QS != nullptr);
+        wall_vjp(out, QS ? &gl_stale : nullptr);
     }
 
     void vjp(const QSet<real> &Q, const QSet<real> &v, AdjVjp<real> a, bool stale = false)
@@ -1507,26 +1541,30 @@ struct SolverT : SolverBase {
         const QSet<real> X = vb5, Y = vb5b, acc = qb5;
         const QSet<real> none{};
         const long n = N;
+        // the first stage after a sponge update corrects the pre-reset set
+        // with the stage state (its planes: gcur_S)
+        const QSet<real> *S = r.stale ? gcur_S : nullptr;
+        if (r.stale && !S) throw std::logic_error("adjoint: stale stage without its planes");
         auto lc = [&](const QSet<real> &o, real ca, const QSet<real> &a, real cb, const QSet<real> &b) {
             hc(launch_gadj_lincomb<real>(o, ca, a, cb, b, n, st), "adjoint combination");
         };
         if (cfg.rk == CS3D_TVDRK3) {
             lc(X, real(2) / 3, la, 0, none);                  // 2/3 la
             wall_apply(stg[2], gw_q);
-            { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp(gw_q, X, f); }
+            { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp_corrected(X, f, false); }
             lc(Y, 1, X, 1, Y);                                // 2/3 la + J3^T
             wall_vjp(Y);                                      // Q3b
             lc(acc, real(1) / 3, la, real(3) / 4, Y);         // la/3 + 3/4 Q3b
             lc(X, real(1) / 4, Y, 0, none);                   // Q3b/4
             wall_apply(stg[1], gw_q);
-            { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp(gw_q, X, f); }
+            { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp_corrected(X, f, false); }
             lc(Y, 1, X, 1, Y);
             wall_vjp(Y);                                      // Q2b
             lc(acc, 1, acc, 1, Y);
-            wall_apply(stg[0], gw_q);
-            { AdjVjp<real> f{}; f.cv = dt; f.out = X; vjp(gw_q, Y, f); }
+            wall_apply(stg[0], gw_q, S);
+            { AdjVjp<real> f{}; f.cv = dt; f.out = X; vjp_corrected(Y, f, S != nullptr); }
             lc(lam, 1, acc, 1, X);
-            wall_vjp(lam);
+            wall_vjp(lam, S ? &gl_stale : nullptr);
             return;
         }
         copy_set(acc, la);
@@ -1539,10 +1577,10 @@ struct SolverT : SolverBase {
         vjp_walls(stg[1], X, dt, Y);                          // r2
         lc(acc, 1, acc, 1, Y);
         lc(X, real(1) / 6, la, real(1) / 2, Y);               // la/6 + r2/2
-        wall_apply(stg[0], gw_q);
-        { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp(gw_q, X, f); }
+        wall_apply(stg[0], gw_q, S);
+        { AdjVjp<real> f{}; f.cv = dt; f.out = Y; vjp_corrected(X, f, S != nullptr); }
         lc(lam, 1, acc, 1, Y);
-        wall_vjp(lam);
+        wall_vjp(lam, S ? &gl_stale : nullptr);
     }
 
     // lam (cotangent of q_{n+1}) -> cotangent of q_n for the step whose
@@ -1663,7 +1701,7 @@ struct SolverT : SolverBase {
         if (sponge) zero_set(gl_avg);
         // with walls the map cs3d_rhs evaluates is Q -> dt R(W(Q))
         if (any_wall())
-            vjp_walls(set == 1 ? F.q : F.qk, la5, f.cv, qb5);
+            vjp_walls(set == 1 ? F.q : F.qk, la5, f.cv, qb5, stale ? &F.qstale : nullptr);
         else
             vjp(set == 1 ? F.q : F.qk, la5, f, stale);
         download_set(qb5, out);

@@ -95,15 +95,20 @@ __global__ __launch_bounds__(256) void k_wall_face(Fields<real> F, Phys<real> ph
     if (sa.stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) qset_store(F.qstale, ip, qw);
 }
 
-// Reverse of k_wall_face (without stale planes) on the cotangent lam, given
-// the state st before the face: at a face point the momenta m = rho u_w and
-// the energy E = rho R T_w/(gamma - 1) + rho |u_w|^2 / 2 were set, so their
-// cotangents pass to the density and, through T_w = sum_m c_m T_m (the
-// calcNeumann weights), to the six inward points' states by T = p/(rho R),
-// p = (gamma - 1)(E - |m|^2/(2 rho)).  One thread per face point: a face
-// point's inward points belong to it alone.
+// Reverse of k_wall_face on the cotangent lam, given the state st before
+// the face: at a face point the momenta m = rho u_w and the energy
+// E = rho R T_w/(gamma - 1) + rho |u_w|^2 / 2 were set, so their cotangents
+// pass to the density and, through T_w = sum_m c_m T_m (the calcNeumann
+// weights), to the six inward points' states by T = p/(rho R),
+// p = (gamma - 1)(E - |m|^2/(2 rho)).  At the first stage after a sponge
+// update (stS set) the face also set the pre-reset copy on the sponge
+// planes, and the inward temperatures on them came from that copy: the
+// copy's cotangent lamS joins lam at a face point on a sponge plane, and
+// the inward terms on sponge planes go to lamS, with stS's values.  One
+// thread per face point: a face point's inward points belong to it alone.
 template <typename real>
 __global__ __launch_bounds__(256) void k_wall_face_vjp(QSet<real> st, QSet<real> lam,
+                                                       QSet<real> stS, QSet<real> lamS,
                                                        Phys<real> ph, StageArgs sa, WallFace w)
 {
     const int Nx = sa.Nx, Ny = sa.Ny, Nz = sa.Nz;
@@ -125,23 +130,40 @@ __global__ __launch_bounds__(256) void k_wall_face_vjp(QSet<real> st, QSet<real>
         ijk[1] = b;
         ijk[2] = w.hi ? Nz - 1 : 0;
     }
+    const bool stale = stS.m != nullptr;
     const long sxy = (long)Nx * Ny;
     const long ip = (long)ijk[2] * sxy + (long)ijk[1] * Nx + ijk[0];
     const long stride = w.dir == 0 ? 1 : w.dir == 1 ? (long)Nx : sxy;
     const long in = w.hi ? -stride : stride;
     const real cn[6] = {real(360) / real(147), real(-450) / real(147), real(400) / real(147),
                         real(-225) / real(147), real(72) / real(147), real(-10) / real(147)};
+    // whether inward point m + 1 read the pre-reset copy
+    auto inward_stale = [&](int m) {
+        int nijk[3] = {ijk[0], ijk[1], ijk[2]};
+        nijk[w.dir] += w.hi ? -(m + 1) : (m + 1);
+        return stale && sponge_plane(sa, nijk[0], nijk[1], nijk[2]);
+    };
     real Tw = 0;
 #pragma unroll
     for (int m = 0; m < 6; m++) {
         real qo[5], ir, U, V, Wv, pp, Tp;
-        qset_load(st, ip + (long)(m + 1) * in, qo);
+        qset_load(inward_stale(m) ? stS : st, ip + (long)(m + 1) * in, qo);
         prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], ir, U, V, Wv, pp, Tp);
         Tw += cn[m] * (ph.T0s + Tp);
     }
     real qf[5], l[5];
     qset_load(st, ip, qf);
     qset_load(lam, ip, l);
+    if (stale && sponge_plane(sa, ijk[0], ijk[1], ijk[2])) {
+        real ls[5];
+        qset_load(lamS, ip, ls);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            l[k] += ls[k];
+            ls[k] = 0;
+        }
+        qset_store(lamS, ip, ls);
+    }
     const real rho = ph.rho_ref + qf[0];
     const real u[3] = {(real)w.vel[0], (real)w.vel[1], (real)w.vel[2]};
     const real R = (real)w.R;
@@ -153,9 +175,12 @@ __global__ __launch_bounds__(256) void k_wall_face_vjp(QSet<real> st, QSet<real>
 #pragma unroll
     for (int m = 0; m < 6; m++) {
         const long o = ip + (long)(m + 1) * in;
+        const bool sp = inward_stale(m);
+        const QSet<real> &so = sp ? stS : st;
+        const QSet<real> &lt = sp ? lamS : lam;
         real qo[5], lo[5], ir, U, V, Wv, pp, Tp;
-        qset_load(st, o, qo);
-        qset_load(lam, o, lo);
+        qset_load(so, o, qo);
+        qset_load(lt, o, lo);
         prim(ph, qo[0], qo[1], qo[2], qo[3], qo[4], ir, U, V, Wv, pp, Tp);
         const real r = ph.rho_ref + qo[0], irr = real(1) / (r * R);
         const real Tb = gT * cn[m];
@@ -165,28 +190,32 @@ __global__ __launch_bounds__(256) void k_wall_face_vjp(QSet<real> st, QSet<real>
         lo[3] -= Tb * ph.gm1 * Wv * irr;
         lo[0] += Tb * (ph.gm1 * real(0.5) * (U * U + V * V + Wv * Wv) * irr -
                        (ph.T0s + Tp) / r);
-        qset_store(lam, o, lo);
+        qset_store(lt, o, lo);
     }
 }
 
 template <typename real>
-hipError_t launch_wall_face_vjp(const QSet<real> &st, const QSet<real> &lam, const Phys<real> &ph,
-                                const StageArgs &sa, const WallFace &w, hipStream_t s)
+hipError_t launch_wall_face_vjp(const QSet<real> &st, const QSet<real> &lam, const QSet<real> &stS,
+                                const QSet<real> &lamS, const Phys<real> &ph, const StageArgs &sa,
+                                const WallFace &w, hipStream_t s)
 {
+    if ((stS.m == nullptr) != (lamS.m == nullptr)) return hipErrorInvalidValue;
     if (w.dir < 0 || w.dir > 2) return hipErrorInvalidValue;
     const int n = w.dir == 0 ? sa.Nx : w.dir == 1 ? sa.Ny : sa.Nz;
     if (n < 7) return hipErrorInvalidValue;
     const long pts = w.dir == 0 ? (long)sa.Ny * sa.Nz
                                 : w.dir == 1 ? (long)sa.Nx * sa.Nz : (long)sa.Nx * sa.Ny;
     hipLaunchKernelGGL((k_wall_face_vjp<real>), dim3((unsigned)((pts + 255) / 256)), dim3(256), 0,
-                       s, st, lam, ph, sa, w);
+                       s, st, lam, stS, lamS, ph, sa, w);
     return hipGetLastError();
 }
 
 template hipError_t launch_wall_face_vjp<float>(const QSet<float> &, const QSet<float> &,
+                                                const QSet<float> &, const QSet<float> &,
                                                 const Phys<float> &, const StageArgs &,
                                                 const WallFace &, hipStream_t);
 template hipError_t launch_wall_face_vjp<double>(const QSet<double> &, const QSet<double> &,
+                                                 const QSet<double> &, const QSet<double> &,
                                                  const Phys<double> &, const StageArgs &,
                                                  const WallFace &, hipStream_t);
 
