@@ -7,6 +7,7 @@
 // instead of 45.  The per-field arithmetic is that of k_field_op and
 // k_field_opT (ops.hip).
 #include <algorithm>
+#include <cstdlib>
 
 #include "plan_dev.hpp"
 
@@ -208,8 +209,12 @@ static LaunchGeom opb_geom(const OpHost &h1, const OpHost &h2, int N, bool trans
     LaunchGeom g;
     // row lines (contiguous): one line per workgroup where a line fills a
     // wavefront, so that workgroups on a CU run out of phase, as the x sweep
-    const int cap = map == MAP_ROW ? std::min(opb_threads<real, MAP_ROW>(), std::max(64, P))
-                                   : opb_threads<real, MAP_COL>();
+    int col = opb_threads<real, MAP_COL>();
+    if (const char *v = std::getenv("CS3D_OPS_BATCH_THREADS")) {
+        const int x = std::atoi(v);
+        if (x >= 64 && x <= col) col = x;
+    }
+    const int cap = map == MAP_ROW ? std::min(opb_threads<real, MAP_ROW>(), std::max(64, P)) : col;
     g.LX = P > 0 ? std::min(64, cap / P) : 0;
     if (map == MAP_COL)
         while (g.LX & (g.LX - 1)) g.LX &= g.LX - 1;
