@@ -9,7 +9,7 @@
 #include <algorithm>
 #include <cstdlib>
 
-#include "plan_dev.hpp"
+#include "stage_common.hpp"
 
 namespace cs3d {
 
@@ -30,14 +30,6 @@ constexpr int opb_stage_group(int N)
     return N <= 5 ? N : (N + 1) / 2;
 }
 
-// ring margin of a batch over both derivative plans
-template <typename real>
-__device__ __forceinline__ int opb_kr(const OpDev<real> *d1, const OpDev<real> *d2)
-{
-    int k = d1->K > d2->K ? d1->K : d2->K;
-    return k > kc_deriv<real>() ? k : kc_deriv<real>();
-}
-
 // out[i] = D1(in[i]) for i < N1, out[N1 + i] = D2(in[i]) for i < N2 (N2 <= N1)
 template <typename real, int M, bool UNI, int MAP, int N1, int N2>
 __global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_fwd(
@@ -50,7 +42,7 @@ __global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_fwd(
     constexpr int HW = 2, W = M + 2 * HW, N = N1 + N2;
     const int P = d1->P;
     const int LX = blockDim.x / P;
-    Lane<real, MAP> c = make_lane<real, MAP>(d1, P, LX, opb_kr(d1, d2), N, xb);
+    Lane<real, MAP> c = make_lane<real, MAP>(d1, P, LX, sweep_kr(d1, d2), N, xb);
     const int L = blockIdx.x * LX + c.line;
     const bool active = L < lm.nlines;
     Win<real, M, HW, vec_windows<real, M, MAP, UNI>()> win;
@@ -88,7 +80,7 @@ __global__ __launch_bounds__((opb_threads<real, MAP>())) void k_ops_T(
     constexpr int HW = 2, W = M + 2 * HW, N = N1 + N2;
     const int P = d1->P, n = d1->n;
     const int LX = blockDim.x / P;
-    Lane<real, MAP> c = make_lane<real, MAP>(d1, P, LX, opb_kr(d1, d2), N, xb);
+    Lane<real, MAP> c = make_lane<real, MAP>(d1, P, LX, sweep_kr(d1, d2), N, xb);
     const int L = blockIdx.x * LX + c.line;
     const bool active = L < lm.nlines;
     Win<real, M, 0, false> win;

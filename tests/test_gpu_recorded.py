@@ -1,8 +1,9 @@
 """The GPU path against the reference's own recorded output: the recorded
 Taylor-Green runs at 64^3 and 128^3 (taylorgreen_vortex/taylorgreen_run64,
-taylorgreen_run128; settings in tests/test_oracle_golden.py), two RK4 steps
-in FP64 through the C ABI, and cs3d_diag_taylor_green against the first two
-rows of their taylorgreen.out (tests/golden/) at the oracle's bounds."""
+taylorgreen_run128; settings in tests/test_oracle_golden.py) through the C
+ABI, cs3d_diag_taylor_green against their taylorgreen.out rows
+(tests/golden/) at the oracle's bounds: two RK4 steps, then FP64 along 1000
+steps at 64^3 and 200 at 128^3, and FP32 along the 1000 steps."""
 import pytest
 
 import cs3d
@@ -22,4 +23,52 @@ def test_gpu_tgv_recorded_rows(N):
         d = g.diag_taylor_green()
         print(f"{N}^3 step {r + 1}: gpu {d}\n            recorded {rows[r]}", flush=True)
         check_row(d, rows[r])
+    g.close()
+
+
+@pytest.mark.parametrize("N,last", [(64, 1000), (128, 200)])
+def test_gpu_tgv_recorded_steps_fp64(N, last):
+    """FP64 along the recorded run: 1000 steps at 64^3 and 200 at 128^3,
+    every recorded step up to 10 and every tenth after, at the oracle's
+    bounds"""
+    from test_oracle_golden import recorded_steps
+    kw, ic, _ = recorded_case(N)
+    rows = recorded_steps(N)
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
+    g.set_initial_conditions(*ic)
+    g.set_diagnostics(True)
+    worst = 0.0
+    for n in range(1, last + 1):
+        g.step(1)
+        if n in rows:
+            d = g.diag_taylor_green()
+            r = rows[n]
+            worst = max(worst, max(abs(d[c] - r[c]) / abs(r[c]) for c in range(1, 6)))
+            # the time is a sum of CFL steps: equal to rounding
+            assert abs(d[0] - r[0]) <= 1e-13 * abs(r[0]), (n, d[0], r[0])
+            check_row((r[0],) + tuple(d[1:]), r)
+    print(f"{N}^3 FP64 {last} steps: largest sum deviation {worst:.2e}", flush=True)
+    g.close()
+
+
+def test_gpu_tgv64_recorded_steps_fp32():
+    """FP32 along the recorded 64^3 run for 1000 steps: kinetic energy,
+    dissipation and enstrophy sums within 7e-7 of the recorded FP64 run
+    (about 3x the measured 2.2e-7)"""
+    from test_oracle_golden import recorded_steps
+    kw, ic, _ = recorded_case(64)
+    rows = recorded_steps(64)
+    g = cs3d.Solver(cs3d.config(64, 64, 64, precision=cs3d.FP32, **kw))
+    g.set_initial_conditions(*ic)
+    g.set_diagnostics(True)
+    worst = [0.0] * 6
+    for n in range(1, 1001):
+        g.step(1)
+        if n in rows:
+            d = g.diag_taylor_green()
+            for c in range(1, 6):
+                worst[c] = max(worst[c], abs(d[c] - rows[n][c]) / abs(rows[n][c]))
+    print("64^3 FP32 1000 steps, largest deviation per column:",
+          " ".join(f"{w:.2e}" for w in worst[1:]), flush=True)
+    assert max(worst[1:4]) <= 7e-7, worst
     g.close()

@@ -59,3 +59,22 @@ def test_tgv_recorded_rows(N):
         s.step()
         check_row(s.diag_tgv(), rows[r])
     s.close()
+
+
+def recorded_steps(N):
+    """{step: recorded row} of the recorded run at N^3 (tests/golden/*_steps.txt)"""
+    a = np.loadtxt(os.path.join(HERE, "golden", f"tgv{N}_taylorgreen_out_steps.txt"))
+    return {int(r[0]): r[1:] for r in a}
+
+
+def test_tgv64_recorded_first_twenty_steps():
+    """the oracle along the recorded 64^3 run, every step to 10 and at 20"""
+    kw, ic, _ = recorded_case(64)
+    rows = recorded_steps(64)
+    s = po.Solver(po.make_config(64, 64, 64, **kw))
+    s.set_ic(*ic)
+    for n in range(1, 21):
+        s.step()
+        if n in rows:
+            check_row(s.diag_tgv(), rows[n])
+    s.close()
