@@ -86,9 +86,10 @@ ACCESSES_PER_STEP = {p: sum(sum(v) for v in a.values()) for p, a in STAGE_ACCESS
 # HBM accesses per point actually moved per launch (reads, writes) at 256^3,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs calibrated on a 256 MiB
 # copy (profiles/hbm_traffic_tgv256_fp32.md, profiles/hbm_traffic_tgv256_fp64.md)
+# (the x sweep's: the mean over its four launches of a step)
 MEASURED_TRAFFIC = {
-    cs3d.FP32: {"x_sweep": (29.25, 8.75), "y_sweep": (5.00, 9.40), "z_sweep": (7.00, 9.44)},
-    cs3d.FP64: {"x_sweep": (29.25, 9.28), "y_sweep": (5.00, 9.45), "z_sweep": (7.00, 9.38)},
+    cs3d.FP32: {"x_sweep": (29.25, 8.75), "y_sweep": (5.00, 9.36), "z_sweep": (7.00, 9.30)},
+    cs3d.FP64: {"x_sweep": (29.25, 8.89), "y_sweep": (5.00, 9.83), "z_sweep": (7.00, 9.40)},
 }
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
 BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP[cs3d.FP32] * 4 / 4,
