@@ -1084,7 +1084,6 @@ struct SolverT : SolverBase {
         need_ic();
         bool general = sponge || any_wall() || std::getenv("CS3D_ADJ_GENERAL") != nullptr;
         for (int d = 0; d < 3; d++) general |= cfg.bc_type[d] != CS3D_PERIODIC_SOLVE;
-        if (cfg.compensated) throw std::invalid_argument("the adjoint does not cover the compensated mode");
         if (adj_ready) {
             if (general != adj_general) throw std::logic_error("adjoint path changed");
             return;
@@ -1739,8 +1738,16 @@ struct SolverT : SolverBase {
         // sponges: per checkpoint the averages and the stale planes, per step
         // of a segment the planes, the averages and the filtered state
         // (fused_step), and the end state's averages and planes
-        std::vector<QSet<real>> ckA, ckS, srec;
-        QSet<real> finA{}, finS{};
+        std::vector<QSet<real>> ckA, ckS, srec, ckL;
+        QSet<real> finA{}, finS{}, finL{};
+        // the compensated mode: the state's low parts at each checkpoint and
+        // at the end, so that the replays retrace the forward pass exactly
+        // (the products themselves take the FP32 states)
+        if (cfg.compensated) {
+            ckL.resize(nseg);
+            for (auto &c : ckL) c = state();
+            finL = state();
+        }
         if (sponge) {
             ckA.resize(nseg);
             ckS.resize(nseg);
@@ -1754,6 +1761,7 @@ struct SolverT : SolverBase {
         auto sponge_rec = [&](int i) { return sponge ? (const QSet<real> *)&srec[3 * (size_t)i] : nullptr; };
         auto save_ck = [&](int c) {
             copy_set(ck[c], F.q);
+            if (cfg.compensated) copy_set(ckL[c], F.qlo);
             if (sponge) {
                 copy_set(ckA[c], F.avg);
                 copy_set(ckS[c], F.qstale);
@@ -1775,6 +1783,7 @@ struct SolverT : SolverBase {
         const bool stale_end = stale_pending;
         if (J) hc(kinetic_energy(J) == CS3D_OK ? hipSuccess : hipErrorUnknown, "kinetic energy");
         copy_set(fin, F.q);
+        if (cfg.compensated) copy_set(finL, F.qlo);
         if (sponge) {
             copy_set(finA, F.avg);
             copy_set(finS, F.qstale);
@@ -1790,6 +1799,7 @@ struct SolverT : SolverBase {
             const int n0 = sgi * K, n1 = std::min(nsteps, n0 + K);
             if (sgi != nseg - 1) {
                 copy_set(F.q, ck[sgi]);
+                if (cfg.compensated) copy_set(F.qlo, ckL[sgi]);
                 if (sponge) {
                     copy_set(F.avg, ckA[sgi]);
                     copy_set(F.qstale, ckS[sgi]);
@@ -1821,6 +1831,7 @@ struct SolverT : SolverBase {
         if (gW) download(gvel[2], gW);
         // back to the final state of the forward pass
         copy_set(F.q, fin);
+        if (cfg.compensated) copy_set(F.qlo, finL);
         if (sponge) {
             copy_set(F.avg, finA);
             copy_set(F.qstale, finS);

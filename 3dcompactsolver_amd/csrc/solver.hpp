@@ -39,7 +39,7 @@ struct SolverBase {
     virtual void *stream() = 0;
     virtual void profile_enable(int on) = 0;
     virtual void profile_report(double *ms, int *counts) = 0;
-    // discrete adjoint (not the compensated mode):
+    // discrete adjoint:
     // lam^T d(dt*RHS)/dQ for the conserved set `set`, and the gradient of
     // the final volume-averaged kinetic energy after nsteps with respect to
     // the initial velocities

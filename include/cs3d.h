@@ -214,8 +214,9 @@ int cs3d_kinetic_energy(cs3d_solver *s, double *ke);
 /* Discrete adjoint (no reference counterpart: the reference has no
  * adjoint; these serve gradient-based callers such as optimisation and
  * sensitivity studies).  Periodic or non-periodic lines (the boundary
- * closures transposed), sponge faces, walls or both; the compensated mode
- * is refused (CS3D_ERR_ARG).
+ * closures transposed), sponge faces, walls or both; in the compensated
+ * FP32 mode the replays retrace the compensated forward pass and the
+ * products take the FP32 states.
  *
  * cs3d_rhs_vjp: the transpose of the linearised stage map Q -> dt*RHS(Q)
  * (the map cs3d_rhs evaluates, with the dt of the last pre-step) applied to

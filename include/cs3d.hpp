@@ -610,7 +610,7 @@ class CSolver_AWS : public AbstractCSolver {
     // NULL skips one) at fixed initial rho and p, states recomputed from a
     // checkpoint every `checkpoint` steps.  rhsVJP gives
     // (d(dt*RHS)/dQ)^T lam at the state (set 1) or stage state (set 2).
-    // Any boundary set; not the compensated mode (cs3d.h).
+    // Any boundary set, either precision, the compensated mode (cs3d.h).
     double keGradient(int nsteps, int checkpoint, double *gU, double *gV, double *gW)
     {
         double J = 0.0;

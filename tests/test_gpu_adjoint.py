@@ -134,10 +134,7 @@ def test_ke_gradient_fp32_tracks_fp64():
         assert err <= 1e-3
 
 
-def test_adjoint_rejects_unsupported_configurations():
-    """the compensated mode is not reversed; neither are step counts below
-    one (sponges, non-periodic lines and walls are:
-    test_gpu_adjoint_general.py, test_gpu_adjoint_walls.py)"""
+def test_adjoint_rejects_step_counts_below_one():
     N = 16
     kw = adj_kw((N, N, N))
     g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
@@ -145,10 +142,35 @@ def test_adjoint_rejects_unsupported_configurations():
     with pytest.raises(cs3d.CS3DError):
         g.ke_gradient(0, 1)
     g.close()
+
+
+@pytest.mark.parametrize("rk", [4, 3])
+def test_ke_gradient_compensated_fp32_tracks_fp64(rk):
+    """the compensated FP32 mode: the replays retrace its forward pass
+    (the state's low parts checkpointed) and the products take the FP32
+    stage states; the gradient against the FP64 one as for plain FP32, and
+    the same J as the mode's own forward run"""
+    N = 32
+    kw = adj_kw((N, N, N), mu_ref=5e-3, dt_fixed=0.01, rk=rk)
+    ic = tgv_ic(N, kw["Lx"], p0=10.0 / 1.4)
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP64, **kw))
+    g.set_initial_conditions(*ic)
+    J64, *g64 = g.ke_gradient(5, 2)
+    g.close()
     g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP32, compensated=1, **kw))
-    g.set_initial_conditions(*tgv_ic(N, kw["Lx"]))
-    with pytest.raises(cs3d.CS3DError):
-        g.ke_gradient(2, 1)
+    g.set_initial_conditions(*ic)
+    J, *gc = g.ke_gradient(5, 2)
+    g.close()
+    g = cs3d.Solver(cs3d.config(N, N, N, precision=cs3d.FP32, compensated=1, **kw))
+    g.set_initial_conditions(*ic)
+    g.step(5)
+    assert J == g.kinetic_energy()
+    g.close()
+    assert abs(J - J64) <= 1e-6 * abs(J64)
+    for a, b in zip(gc, g64):
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
+        print(f"compensated rk{rk} gradient rel err {err:.2e}", flush=True)
+        assert err <= 1e-3
 
 
 def oracle_cfg(dims, kw):
