@@ -64,32 +64,37 @@ __host__ __device__ constexpr int inv_mod32(int s)
 }
 
 // Ring row length (in positions) for P segments, margin KR, LX lines per
-// workgroup and an odd position stride S: at least P + KR, and chosen so that
-// row * S = 32 / LX (mod 32).  Lanes of a half-wavefront then hit distinct
-// LDS banks: in MAP_COL they are 32 / LX consecutive segments of LX lines, at
-// offsets line * row * S + g * S; in MAP_ROW consecutive segments of one
-// line, at offsets g * S.  A workgroup of one line needs no padding (its
-// lanes are consecutive segments at offsets g * S either way), and in a
-// MAP_COL workgroup of 32 or more lines a half-wavefront holds one segment
-// of 32 lines, for which any odd row * S will do (col: row odd, at most one
-// position of padding instead of up to 31).
-__host__ __device__ inline int ring_row(int P, int KR, int LX, int S, bool col)
+// workgroup, an odd position stride S and esize-byte elements: at least
+// P + KR, and chosen so that row * S = B / LX (mod B).  The LDS has 32 banks
+// of 4 bytes and serves a 4-byte read in groups of B = 32 lanes, an 8-byte
+// read in groups of B = 16 (two banks each); the lanes of one group then hit
+// distinct banks: in MAP_COL they are B / LX consecutive segments of LX
+// lines, at offsets line * row * S + g * S; in MAP_ROW consecutive segments
+// of one line, at offsets g * S.  (FP64 with the 4-byte rule, row * S =
+// 32 / LX mod 32, put lines l and l + 4 of an 8-line group on one bank pair:
+// the FP64 y and z sweeps ran 13% slower.)  A workgroup of one line needs
+// no padding (its lanes are consecutive segments at offsets g * S either
+// way), and in a MAP_COL workgroup of 32 or more lines a group holds one
+// segment of B lines, for which any odd row * S will do (col: row odd, at
+// most one position of padding instead of up to B - 1).
+__host__ __device__ inline int ring_row(int P, int KR, int LX, int S, bool col, int esize)
 {
     if (LX == 1) return P + KR;
     if (col && LX >= 32) return (P + KR) | 1;
-    const int want = LX >= 32 ? 1 : 32 / LX;
+    const int bk = esize == 8 ? 16 : 32;
+    const int want = LX >= bk ? 1 : bk / LX;
     const int r = P + KR;
-    const int target = (want * inv_mod32(S)) & 31;
-    return r + (((target - r) % 32) + 32) % 32;
+    const int target = (want * inv_mod32(S)) & (bk - 1);  // S^-1 mod 32 is S^-1 mod 16 too
+    return r + (((target - r) % bk) + bk) % bk;
 }
 
 // Elements of one parity of the exchange buffer, for batches of up to
 // `planes` values per position.
-__host__ __device__ inline int ring_cap(int P, int KR, int LX, int planes, bool col)
+__host__ __device__ inline int ring_cap(int P, int KR, int LX, int planes, bool col, int esize)
 {
     int best = 0;
     for (int S = 1; S <= (planes | 1); S += 2) {
-        const int e = ring_row(P, KR, LX, S, col) * S;
+        const int e = ring_row(P, KR, LX, S, col, esize) * S;
         best = e > best ? e : best;
     }
     return LX * best;
@@ -99,7 +104,7 @@ __host__ __device__ inline int ring_cap(int P, int KR, int LX, int planes, bool 
 __host__ __device__ inline size_t ring_bytes(int P, int KR, int LX, int planes, int esize,
                                              bool col)
 {
-    return 2 * (size_t)ring_cap(P, KR, LX, planes, col) * esize;
+    return 2 * (size_t)ring_cap(P, KR, LX, planes, col, esize) * esize;
 }
 
 // Workgroup shape of a line-solve launch: LX lines x P segments per
@@ -166,7 +171,7 @@ __device__ __forceinline__ Lane<real, MAP> make_lane(const OpDev<real> *__restri
     c.P = P;
     c.LX = LX;
     c.KR = KR;
-    c.cap = ring_cap(P, KR, LX, planes, MAP == MAP_COL);
+    c.cap = ring_cap(P, KR, LX, planes, MAP == MAP_COL, (int)sizeof(real));
     c.par = 0;
     c.xb = xb;
     if (op->uniform) {
@@ -743,7 +748,7 @@ __device__ __forceinline__ void exchange(Lane<real, MAP> &c, const real (&v)[N1 
                                          int K1, const real *__restrict__ w2, int K2, int cyc)
 {
     constexpr int N = N1 + N2, S = N | 1;
-    const int row = ring_row(c.P, c.KR, c.LX, S, MAP == MAP_COL);
+    const int row = ring_row(c.P, c.KR, c.LX, S, MAP == MAP_COL, (int)sizeof(real));
     real *b = c.xb + c.par * c.cap;
     c.par ^= 1;
     const int prim = (c.line * row + (FWD ? c.KR + c.g : c.g)) * S;
