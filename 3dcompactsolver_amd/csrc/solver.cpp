@@ -176,8 +176,9 @@ struct SolverT : SolverBase {
                 fx_ok = D1[0].host.uniform && FLX.host.P == P;
                 if (fx_ok && P != 64) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
             }
-            sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), SWEEP_PLANES, sizeof(real),
-                                  map_of(d), SWEEP_STAGE_FIELDS * D1[d].host.M, d > 0);
+            sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), sweep_planes<real>(d),
+                                  sizeof(real), map_of(d), SWEEP_STAGE_FIELDS * D1[d].host.M,
+                                  d > 0);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             sweepG_small[d] = sweepG[d];
             if (d > 0) {

@@ -153,6 +153,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 
 // most solves batched through one exchange in a sweep kernel
 constexpr int SWEEP_PLANES = 13;
+// the FP64 z sweep on uniform lines solves its two mixed derivatives in the
+// same batch as the other 13 (stage_z.hip)
+template <typename real>
+constexpr bool z_merged(bool uni, bool stale)
+{
+    return sizeof(real) == 8 && uni && !stale;
+}
+template <typename real>
+constexpr int sweep_planes(int dir)
+{
+    return dir == 2 && z_merged<real>(true, false) ? SWEEP_PLANES + 2 : SWEEP_PLANES;
+}
 // most fields whose windows a sweep kernel stages through LDS at once
 constexpr int SWEEP_STAGE_FIELDS = 5;
 

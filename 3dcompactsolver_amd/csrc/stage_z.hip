@@ -10,7 +10,9 @@
 // Uxz and Wxz are taken
 // along x of the stored Uz, Wz in the last sweep (operators along different
 // axes commute).  Solves: one batch of 13 (5 flux + 4 first + 4 second
-// derivatives), then the 2 mixed ones.
+// derivatives), then the 2 mixed ones; FP64 on uniform lines all 15 in one
+// batch (its 8-line ring has the room; the second batch's barriers and the
+// staging of (Vy, Wy) cost 5% of that sweep).
 //
 // STALE (first stage after updateSponge, see stage_y.hip): the reference's
 // Uxz, Wxz are z-derivatives of x-derivatives of the stale primitives; the
@@ -26,8 +28,21 @@ __global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
     Fields<real> F, Phys<real> ph, int use_k, StageArgs sa)
 {
     constexpr int HW = 2, W = M + 2 * HW;
-    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), SWEEP_PLANES)
+    // MRG: the mixed Vyz, Wyz join the first batch (15 solves, one batch
+    // and no staging of (Vy, Wy) fewer); the FP64 ring of 8 lines has room
+    constexpr bool MRG = z_merged<real>(UNI, STALE);
+    CS3D_LANE_SETUP(MAP_COL, d1, sweep_kr(d1, d2), MRG ? SWEEP_PLANES + 2 : SWEEP_PLANES)
     const QSet<real> qs = use_k ? F.qk : F.q;
+    // the solves of the first batch: 0-4 fluxes, 5-8 first derivatives of
+    // T', U, V, W, (MRG: 9-10 Vyz, Wyz,) then their second derivatives
+    constexpr int NB = MRG ? 15 : 13, I2 = NB - 4;
+    real r[NB][M];
+    if constexpr (MRG) {
+        real mw[2][W];
+        win.load_pair(F.dYvw, mw[0], mw[1]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[0], r[9]);
+        rhs<OP_D1, real, M, HW, UNI>(d1, c, mw[1], r[10]);
+    }
     // the conserved windows are read directly: staging them through LDS
     // (as the y sweep does) measured slower here
     real a0[W], a1[W], a2[W], a3[W], a4[W];
@@ -84,7 +99,6 @@ __global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
     // 13 solves from the stage state: the z fluxes (the h_ref W part of the
     // energy flux is added as h_ref Wz), then the first and second
     // z-derivatives of T', U, V, W
-    real r[13][M];
     {
         real f[W];
 #pragma unroll
@@ -105,10 +119,10 @@ __global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
     rhs<OP_D1, real, M, HW, UNI>(d1, c, U, r[6]);
     rhs<OP_D1, real, M, HW, UNI>(d1, c, V, r[7]);
     rhs<OP_D1, real, M, HW, UNI>(d1, c, Wv, r[8]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, Tp, r[9]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[10]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[11]);
-    rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[12]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Tp, r[I2]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, U, r[I2 + 1]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, V, r[I2 + 2]);
+    rhs<OP_D2, real, M, HW, UNI>(d2, c, Wv, r[I2 + 3]);
     real mu[M], Uc[M], Vc[M], Wc[M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
@@ -118,8 +132,9 @@ __global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
         Vc[t] = V[HW + t];
         Wc[t] = Wv[HW + t];
     }
-    static_assert(13 <= SWEEP_PLANES && 2 + NS <= SWEEP_PLANES, "batch wider than the exchange ring");
-    solve_d12<9, 4, real, M, UNI>(d1, d2, c, r);
+    static_assert(NB <= (MRG ? SWEEP_PLANES + 2 : SWEEP_PLANES) && 2 + NS <= SWEEP_PLANES,
+                  "batch wider than the exchange ring");
+    solve_d12<NB - 4, 4, real, M, UNI>(d1, d2, c, r);
     real R0[M], Ru[M], Rv[M], Rw[M], RE[M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
@@ -128,20 +143,28 @@ __global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
         R0[t] = -r[2][t];
         Ru[t] = -r[3][t];
         Rv[t] = -r[4][t];
-        RE[t] += ph.cpPr * (mu[t] * r[9][t]);
+        RE[t] += ph.cpPr * (mu[t] * r[I2][t]);
         RE[t] -= ph.h_ref * r[8][t];
-        Ru[t] += mu[t] * r[10][t];
-        RE[t] += mu[t] * (Uc[t] * r[10][t]);
-        Rv[t] += mu[t] * r[11][t];
-        RE[t] += mu[t] * (Vc[t] * r[11][t]);
-        Rw[t] += mu[t] * (c43 * r[12][t]);
-        RE[t] += mu[t] * (Wc[t] * (c43 * r[12][t]));
+        Ru[t] += mu[t] * r[I2 + 1][t];
+        RE[t] += mu[t] * (Uc[t] * r[I2 + 1][t]);
+        Rv[t] += mu[t] * r[I2 + 2][t];
+        RE[t] += mu[t] * (Vc[t] * r[I2 + 2][t]);
+        Rw[t] += mu[t] * (c43 * r[I2 + 3][t]);
+        RE[t] += mu[t] * (Wc[t] * (c43 * r[I2 + 3][t]));
     }
     if (active) {
         win.template store_pair<true>(F.dZuw, r[6], r[8], c.len);
         win.template store_pair<true>(F.dZvt, r[7], r[5], c.len);
     }
-    {
+    if constexpr (MRG) {
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            Rw[t] += mu[t] * (c13 * r[9][t]);
+            RE[t] += mu[t] * (c13 * (Wc[t] * r[9][t]));
+            Rv[t] += mu[t] * (c13 * r[10][t]);
+            RE[t] += mu[t] * (c13 * (Vc[t] * r[10][t]));
+        }
+    } else {
         // Vyz, Wyz from the stored Vy, Wy (and the stale D_z U, D_z W)
         constexpr int NM = 2 + NS;
         real m[NM][M];
