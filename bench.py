@@ -89,7 +89,7 @@ ACCESSES_PER_STEP = {p: sum(sum(v) for v in a.values()) for p, a in STAGE_ACCESS
 # (the x sweep's: the mean over its four launches of a step)
 MEASURED_TRAFFIC = {
     cs3d.FP32: {"x_sweep": (29.25, 8.75), "y_sweep": (5.00, 9.36), "z_sweep": (7.00, 9.30)},
-    cs3d.FP64: {"x_sweep": (29.25, 8.88), "y_sweep": (5.00, 9.76), "z_sweep": (7.00, 9.43)},
+    cs3d.FP64: {"x_sweep": (29.25, 8.88), "y_sweep": (5.00, 9.75), "z_sweep": (7.00, 9.80)},
 }
 # bytes per point-stage: ACCESSES_PER_STEP accesses per point over 4 stages
 BYTES_PER_POINT_STAGE = {cs3d.FP32: ACCESSES_PER_STEP[cs3d.FP32] * 4 / 4,
