@@ -646,7 +646,7 @@ __device__ __forceinline__ void rhs(const OpDev<real> *__restrict__ op, const La
                                     const real (&w)[M + 2 * HW], real (&r)[M])
 {
     stencil<KIND, real, M, HW>(op, w, r);
-    if (!UNI) boundary_rows<KIND, real, M, HW>(op, c.s, c.len, w, r);
+    if constexpr (!UNI) boundary_rows<KIND, real, M, HW>(op, c.s, c.len, w, r);
 }
 
 // ------------------------------------------------------------ recurrences

@@ -29,6 +29,10 @@ OpHost plan_for(int kind, int periodic, int n, double dh, double alpha_f, bool f
 constexpr int SEG_DERIV = 4;
 constexpr int SEG_FILTER = 8;
 constexpr int SEG_TABLE = 4;
+// uniform 3-point segments for the forward z sweep on periodic z lines whose
+// length divides by 3 but not by 4 (42 in the shear layer), which would
+// otherwise take the table kernels
+constexpr int SEG_DERIV3 = 3;
 
 // Carry terms unrolled by the kernels on uniform plans (any further terms
 // run in a loop), one count per operator: the K its periodic 4-point plan

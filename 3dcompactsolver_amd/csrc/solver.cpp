@@ -109,6 +109,13 @@ struct SolverT : SolverBase {
     // wavefront scan: through the LDS ring its 4-point plan needs 49 terms;
     // CS3D_NO_FILTER_FUSION in the environment turns it off)
     PlanDev<real> FLX;
+    // the forward z sweep's plans with 3-point uniform segments (z3) where the
+    // periodic z lines divide by 3 but not by 4; every other use of the z
+    // derivatives (standalone operators, adjoint) keeps D1[2], D2[2]
+    PlanDev<real> Z31, Z32;
+    bool z3 = false;
+    const PlanDev<real> &sw1(int d) const { return d == 2 && z3 ? Z31 : D1[d]; }
+    const PlanDev<real> &sw2(int d) const { return d == 2 && z3 ? Z32 : D2[d]; }
     bool fx_ok = false;
     LaunchGeom sweepG[3], filterG[3];
     // the column sweeps' geometry for their variants with a smaller
@@ -170,15 +177,22 @@ struct SolverT : SolverBase {
             make_plan(FL[d], OP_FILTER, per, dims[d], dxyz[d], c.alpha_f);
             if (D1[d].host.P != D2[d].host.P || D1[d].host.uniform != D2[d].host.uniform)
                 throw std::invalid_argument("inconsistent derivative plans");
-            const int P = D1[d].host.P;
-            int kr = std::max({D1[d].host.K, D2[d].host.K, kc_deriv<real>()});
+            if (d == 2 && per && dims[2] % SEG_DERIV != 0 && dims[2] % SEG_DERIV3 == 0 &&
+                !std::getenv("CS3D_NO_Z3")) {
+                make_plan(Z31, OP_D1, 1, dims[2], dxyz[2], c.alpha_f, SEG_DERIV3);
+                make_plan(Z32, OP_D2, 1, dims[2], dxyz[2], c.alpha_f, SEG_DERIV3);
+                z3 = Z31.host.uniform && Z32.host.uniform && Z31.host.M == SEG_DERIV3 &&
+                     Z31.host.P == Z32.host.P;
+            }
+            const OpHost &s1 = sw1(d).host, &s2 = sw2(d).host;
+            const int P = s1.P;
+            int kr = std::max({s1.K, s2.K, kc_deriv<real>()});
             if (d == 0 && fx_ok) {
                 fx_ok = D1[0].host.uniform && FLX.host.P == P;
                 if (fx_ok && P != 64) kr = std::max({kr, FLX.host.K, kc_filter_x<real>()});
             }
             sweepG[d] = line_geom(P, kr, sweep_max_threads(d, P), sweep_planes<real>(d),
-                                  sizeof(real), map_of(d), SWEEP_STAGE_FIELDS * D1[d].host.M,
-                                  d > 0);
+                                  sizeof(real), map_of(d), SWEEP_STAGE_FIELDS * s1.M, d > 0);
             if (sweepG[d].LX < 1) throw std::invalid_argument("line too long for the fused sweeps");
             sweepG_small[d] = sweepG[d];
             if (d > 0) {
@@ -190,7 +204,7 @@ struct SolverT : SolverBase {
                 sweepG_small[d] = line_geom(P, kr,
                                             std::min(sweep_max_threads(d, P), sweep_threads<real>() / div),
                                             SWEEP_PLANES, sizeof(real), MAP_COL,
-                                            SWEEP_STAGE_FIELDS * D1[d].host.M, true);
+                                            SWEEP_STAGE_FIELDS * s1.M, true);
                 if (sweepG_small[d].LX < 1)
                     throw std::invalid_argument("line too long for the fused sweeps");
             }
@@ -297,6 +311,8 @@ struct SolverT : SolverBase {
             FL[d].release();
         }
         FLX.release();
+        Z31.release();
+        Z32.release();
         for (int d = 0; d < 3; d++) {
             D1T[d].release();
             D2T[d].release();
@@ -562,11 +578,11 @@ struct SolverT : SolverBase {
         // y, then z, then x (the last sweep assembles and updates)
         for (int d : {1, 2, 0})
             timed(d, [&] {
-                const bool small = d > 0 && col_small<real>(d, D1[d].host.uniform, sa.stale);
+                const OpHost &h = sw1(d).host;
+                const bool small = d > 0 && col_small<real>(d, h.uniform, sa.stale);
                 const LaunchGeom &g = small ? sweepG_small[d] : sweepG[d];
-                hc(launch_sweep<real>(d, D1[d].dev, D2[d].dev, (fuse && d == 0) ? FLX.dev : nullptr,
-                                      D1[d].host.uniform, D1[d].host.M,
-                                      g, lm[d], F, ph, sa, dsc, st),
+                hc(launch_sweep<real>(d, sw1(d).dev, sw2(d).dev, (fuse && d == 0) ? FLX.dev : nullptr,
+                                      h.uniform, h.M, g, lm[d], F, ph, sa, dsc, st),
                    "sweep");
             });
     }

@@ -132,8 +132,9 @@ template <typename real, bool UNI>
 hipError_t launch_ypass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, const StageArgs &sa, hipStream_t st);
+// M: SEG_DERIV, or SEG_DERIV3 on uniform plans
 template <typename real, bool UNI>
-hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
+hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, int M, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, const StageArgs &sa, hipStream_t st);
 // fx non-null: the fused x filter of the step's last stage (stage_x.hip)

@@ -361,7 +361,7 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
                         const Phys<real> &ph, const StageArgs &sa, const StepScalars *sc,
                         hipStream_t st)
 {
-    if (M != SEG_DERIV) return hipErrorInvalidValue;
+    if (M != SEG_DERIV && !(M == SEG_DERIV3 && which == 2 && uni)) return hipErrorInvalidValue;
     if (lg.LX < 1 || lg.threads < 1 || lg.threads > sweep_threads<real>())
         return hipErrorInvalidConfiguration;
     int use_k = sa.rkStep > 1;
@@ -373,8 +373,8 @@ hipError_t launch_sweep(int which, const OpDev<real> *d1, const OpDev<real> *d2,
     if (which == 1)
         return uni ? launch_ypass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
                    : launch_ypass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
-    return uni ? launch_zpass<real, true>(d1, d2, lg, lm, F, ph, use_k, sa, st)
-               : launch_zpass<real, false>(d1, d2, lg, lm, F, ph, use_k, sa, st);
+    return uni ? launch_zpass<real, true>(d1, d2, M, lg, lm, F, ph, use_k, sa, st)
+               : launch_zpass<real, false>(d1, d2, M, lg, lm, F, ph, use_k, sa, st);
 }
 
 template <typename real, bool RED, bool SET, bool CMP = false>

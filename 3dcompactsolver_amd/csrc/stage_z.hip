@@ -210,13 +210,25 @@ __global__ __launch_bounds__((col_threads<real, 2, UNI, STALE>())) void k_zpass(
 }
 
 template <typename real, bool UNI>
-hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const LaunchGeom &lg,
+hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, int M, const LaunchGeom &lg,
                         const LineMap &lm, const Fields<real> &F, const Phys<real> &ph,
                         int use_k, const StageArgs &sa, hipStream_t st)
 {
     dim3 grid((lm.nlines + lg.LX - 1) / lg.LX), block(lg.threads);
     const int cap = sa.stale ? col_threads<real, 2, UNI, true>() : col_threads<real, 2, UNI, false>();
     if (lg.threads > cap) return hipErrorInvalidConfiguration;
+    if constexpr (UNI) {
+        if (M == SEG_DERIV3) {
+            if (sa.stale)
+                hipLaunchKernelGGL((k_zpass<real, SEG_DERIV3, true, true>), grid, block, lg.shm, st,
+                                   d1, d2, lm, F, ph, use_k, sa);
+            else
+                hipLaunchKernelGGL((k_zpass<real, SEG_DERIV3, true, false>), grid, block, lg.shm,
+                                   st, d1, d2, lm, F, ph, use_k, sa);
+            return hipGetLastError();
+        }
+    }
+    if (M != SEG_DERIV) return hipErrorInvalidValue;
     if (sa.stale)
         hipLaunchKernelGGL((k_zpass<real, SEG_DERIV, UNI, true>), grid, block, lg.shm, st, d1, d2,
                            lm, F, ph, use_k, sa);
@@ -227,7 +239,7 @@ hipError_t launch_zpass(const OpDev<real> *d1, const OpDev<real> *d2, const Laun
 }
 
 #define CS3D_INST_Z(real, UNI)                                                                  \
-    template hipError_t launch_zpass<real, UNI>(const OpDev<real> *, const OpDev<real> *,          \
+    template hipError_t launch_zpass<real, UNI>(const OpDev<real> *, const OpDev<real> *, int,     \
                                                 const LaunchGeom &, const LineMap &,               \
                                                 const Fields<real> &, const Phys<real> &, int,     \
                                                 const StageArgs &, hipStream_t);

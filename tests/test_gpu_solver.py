@@ -157,6 +157,27 @@ def test_shear_layer_sponge_nonperiodic(prec):
                      tol * (10 if prec == cs3d.FP64 else 1))
 
 
+@pytest.mark.parametrize("nz", [18, 42])
+@pytest.mark.parametrize("prec", [cs3d.FP64, cs3d.FP32])
+def test_z_lines_of_three_point_segments(prec, nz):
+    """periodic z lines whose length divides by 3 but not by 4 (42 as in the
+    shear layer) run the forward z sweep on uniform 3-point segments (6 and
+    14 segments: the short-line carry loop and the unrolled terms), with
+    sponges on the y faces for the stale first stage after each update"""
+    dims = (32, 24, nz)
+    kw = dict(Lx=8.0, Ly=10.0, Lz=0.25 * nz, alpha_f=0.49, mu_ref=2e-3, Pr=0.7, cfl=0.5,
+              filter_step=2, rk=4, bc_type=(0, 1, 0), bc_kind=(0, 0, 1, 1, 0, 0),
+              sponge_frac=(0.25, 0.15, 0.25))
+    o, g = pair(dims, shear_ic(32, 24, nz, 10.0, 5), prec, **kw)
+    tol = FP64_TOL if prec == cs3d.FP64 else FP32_TOL
+    run_stages(o, g, prec, 4, tol, f"z{nz} prec={prec} step 1")
+    for n in range(2, 5):
+        o.step()
+        g.step(1)
+        check_fields(f"z{nz} prec={prec} step {n}", g.conserved(1), o.conserved(1),
+                     tol * (10 if prec == cs3d.FP64 else 1))
+
+
 def test_x_and_z_sponges_fp64():
     """sponges on x and z faces (x lines crossing a sponge plane)"""
     dims = (24, 16, 16)
