@@ -256,8 +256,8 @@ static bool opb_plans_ok(const PlanDev<real> &d1, const PlanDev<real> &d2, bool 
     const OpHost &a = d1.host, &c = d2.host;
     if (!d1.dev || !d2.dev || (trans && (!d1.tdev || !d2.tdev))) return false;
     return a.kind == OP_D1 && c.kind == OP_D2 && a.n == c.n && a.P == c.P && a.M == c.M &&
-           a.uniform == c.uniform && a.cyclic == c.cyclic && a.M == SEG_DERIV &&
-           (a.uniform || a.M == SEG_TABLE);
+           a.uniform == c.uniform && a.cyclic == c.cyclic &&
+           (a.uniform ? a.M == SEG_DERIV || a.M == SEG_DERIV3 : a.M == SEG_TABLE);
 }
 
 template <typename real>
@@ -269,6 +269,10 @@ hipError_t launch_ops_fwd(const PlanDev<real> &d1, const PlanDev<real> &d2, int 
     if (!opb_plans_ok(d1, d2, false)) return hipErrorInvalidValue;
     const bool uni = d1.host.uniform != 0;
     if (n2 != 4) return hipErrorInvalidValue;
+    // 3-point segments: the z derivatives (8 + 4) only
+    if (uni && d1.host.M == SEG_DERIV3)
+        return n1 == 8 ? launch_fwd<real, SEG_DERIV3, true, 8, 4>(d1, d2, map, lm, b, st)
+                       : hipErrorInvalidValue;
     switch (n1) {
     case 4:
         return uni ? launch_fwd<real, SEG_DERIV, true, 4, 4>(d1, d2, map, lm, b, st)
@@ -291,6 +295,12 @@ hipError_t launch_ops_T(const PlanDev<real> &d1, const PlanDev<real> &d2, int ma
     if (lm.nlines <= 0) return hipSuccess;
     if (!opb_plans_ok(d1, d2, true)) return hipErrorInvalidValue;
     const bool uni = d1.host.uniform != 0;
+    // 3-point segments: the z terms (the pair and the list of nine) only
+    if (uni && d1.host.M == SEG_DERIV3) {
+        if (n1 == 4 && n2 == 4) return launch_T<real, SEG_DERIV3, true, 4, 4>(d1, d2, map, lm, b, st);
+        if (n1 == 9 && n2 == 0) return launch_T<real, SEG_DERIV3, true, 9, 0>(d1, d2, map, lm, b, st);
+        return hipErrorInvalidValue;
+    }
     if (n2 == 4 && n1 == 4)
         return uni ? launch_T<real, SEG_DERIV, true, 4, 4>(d1, d2, map, lm, b, st)
                    : launch_T<real, SEG_TABLE, false, 4, 4>(d1, d2, map, lm, b, st);

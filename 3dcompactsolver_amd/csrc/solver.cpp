@@ -1114,8 +1114,10 @@ struct SolverT : SolverBase {
             const int dims[3] = {Nx, Ny, Nz};
             for (int d = 0; d < 3; d++) {
                 const int per = cfg.bc_type[d] == CS3D_PERIODIC_SOLVE;
-                make_plan(D1T[d], OP_D1, per, dims[d], dxyz[d], cfg.alpha_f, 0, true);
-                make_plan(D2T[d], OP_D2, per, dims[d], dxyz[d], cfg.alpha_f, 0, true);
+                // z on 3-point segments where the forward z sweep has them
+                const int seg = d == 2 && z3 ? SEG_DERIV3 : 0;
+                make_plan(D1T[d], OP_D1, per, dims[d], dxyz[d], cfg.alpha_f, seg, true);
+                make_plan(D2T[d], OP_D2, per, dims[d], dxyz[d], cfg.alpha_f, seg, true);
                 make_plan(FLT[d], OP_FILTER, per, dims[d], dxyz[d], cfg.alpha_f, 0, true);
             }
             GenAdj<real> &g = gadj;
@@ -1256,7 +1258,7 @@ struct SolverT : SolverBase {
                 b.in[4 + k] = g.d1[src[k]];
                 b.out[4 + k] = g.mx[dst[k]];
             }
-            hc(launch_ops_fwd<real>(D1[2], D2[2], map_of(2), lm[2], 8, 4, b, st), "z derivatives");
+            hc(launch_ops_fwd<real>(sw1(2), sw2(2), map_of(2), lm[2], 8, 4, b, st), "z derivatives");
         }
         hc(launch_gadj<real>(GADJ_SEEDS, g, a, ph, N, st), "adjoint seeds");
         // transposed terms, two batched launches per direction, z, then y,
